@@ -1,0 +1,192 @@
+"""GICP registrations/s on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload large_block|small_cube]
+
+A step is one full registration with both clouds already resident in HBM:
+voxel filter of source and target, uniform grids, k-NN plane covariances and a
+fixed number of Gauss-Newton passes (transformation_epsilon = fitness_epsilon
+= 0, so every iteration runs), ending with the pose read back to the host.
+
+Default workload (single large instance): 2M-point target, 200k-point source,
+100 x 100 x 10 m block, voxel leaf 0.2 m, k = 15, d_max = 2.0 m, 50 iterations.
+
+One JSON line on stdout; `roofline` is the dominant kernel (the Gauss-Newton
+pass), timed with HIP events on the stream it runs on; `cpu_baseline` is the
+CPU oracle (f32-faithful restatement of the reference, one thread) on a bounded
+sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "GICP registrations/s + p50/p99 latency at ≤1e-4 rad error, batched vs plain"
+
+
+def _workload(name):
+    from spatialrust_amd import synth
+    if name == "large_block":
+        return synth.large_block_pair()
+    if name == "small_cube":
+        return synth.small_cube_pair()
+    raise SystemExit(f"unknown workload {name}")
+
+
+def _config(w):
+    import spatialrust_amd as sr
+    return sr.GicpConfig(max_iterations=w.max_iterations, k_neighbors=w.k_neighbors,
+                         max_correspondence_distance=w.max_correspondence_distance,
+                         transformation_epsilon=0.0, fitness_epsilon=0.0, voxel_leaf=w.voxel_leaf)
+
+
+def _cpu_baseline(w, budget_iters=(1, 3)):
+    """Faithful CPU oracle, one thread: voxel filter + (tree, covariances,
+    iterations) at two iteration counts; the per-iteration cost is extrapolated
+    to the workload's iteration count."""
+    import oracle
+    from spatialrust_amd.registration import GicpConfig
+    t0 = time.perf_counter()
+    sv, _, _ = oracle.voxel_centroid(w.source, w.voxel_leaf)
+    tv, _, _ = oracle.voxel_centroid(w.target, w.voxel_leaf)
+    t_vox = time.perf_counter() - t0
+    times = []
+    for iters in budget_iters:
+        cfg = GicpConfig(max_iterations=iters, k_neighbors=w.k_neighbors,
+                         max_correspondence_distance=w.max_correspondence_distance,
+                         transformation_epsilon=0.0, fitness_epsilon=0.0)
+        t0 = time.perf_counter()
+        oracle.gicp_align(sv, tv, cfg._c(), mode="faithful")
+        times.append(time.perf_counter() - t0)
+    per_iter = (times[1] - times[0]) / (budget_iters[1] - budget_iters[0])
+    fixed = times[0] - per_iter * budget_iters[0]
+    est = t_vox + fixed + per_iter * w.max_iterations
+    sample = (f"{w.name}: voxel filter of both clouds + trees + covariances in full, "
+              f"{budget_iters[0]} and {budget_iters[1]} Gauss-Newton iterations timed, "
+              f"per-iteration cost ({per_iter * 1e3:.1f} ms) extrapolated to {w.max_iterations}; "
+              f"measured {t_vox + sum(times):.1f} s of CPU work")
+    return {"value": 1.0 / est, "unit": "registrations/s", "cores": 1, "kind": "port",
+            "sample": sample, "seconds_per_registration": est,
+            "host_cpu": _cpu_model(), "host_nproc": os.cpu_count()}
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="large_block")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (registrations "
+              "shard across GPUs with no communication, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import spatialrust_amd as sr
+    from spatialrust_amd import devmem, native, synth
+
+    if not sr.device_available():
+        print("no gfx950 HIP device: " + native.last_error(), file=sys.stderr)
+        sys.exit(1)
+    lib = native.load()
+    w = _workload(args.workload)
+    cfg = _config(w)
+    src = devmem.DeviceCloud(w.source)
+    tgt = devmem.DeviceCloud(w.target)
+    devmem.synchronize()
+    reg = sr.GicpRegistration(cfg)
+    for _ in range(args.warmup):
+        res = reg.align(src.view, tgt.view)
+    print(f"warmup done: {res}", file=sys.stderr, flush=True)
+
+    lib.sr_profile_reset()
+    lib.sr_profile_enable(1)
+    lat = []
+    devmem.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        res = reg.align(src.view, tgt.view)
+        lat.append(time.perf_counter() - t0)
+    devmem.synchronize()
+    total = time.perf_counter() - t_start
+    lib.sr_profile_enable(0)
+
+    import ctypes
+    kern = {}
+    for name in ("gicp_pass", "knn_normals", "voxel_insert"):
+        ms = ctypes.c_double(0.0)
+        cnt = ctypes.c_uint64(0)
+        lib.sr_profile_read(name.encode(), ctypes.byref(ms), ctypes.byref(cnt))
+        kern[name] = (ms.value, cnt.value)
+    pass_ms, pass_n = kern["gicp_pass"]
+    avg_pass_s = (pass_ms / max(pass_n, 1)) * 1e-3
+    algo_bytes = 72 * res.source_points  # source xyz + cov, gathered target xyz + cov
+    achieved = algo_bytes / avg_pass_s / 1e9 if avg_pass_s > 0 else 0.0
+
+    err = None
+    golden = os.path.join(ROOT, "tests", "golden", "fp64_poses.json")
+    if os.path.exists(golden):
+        ref = json.load(open(golden)).get(w.name)
+        if ref:
+            err = {"rot_rad": synth.rotation_angle_between(res.rotation64, ref["q"]),
+                   "trans_m": float(np.linalg.norm(res.translation64 - np.array(ref["t"]))),
+                   "vs": "FP64 CPU oracle (tests/golden/fp64_poses.json)"}
+    gt = {"rot_rad": synth.rotation_angle_between(res.rotation64, w.gt_q),
+          "trans_m": float(np.linalg.norm(res.translation64 - w.gt_t))}
+
+    line = {
+        "metric": METRIC,
+        "value": args.steps / total,
+        "unit": "registrations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 per-point, f64 reduce/solve/pose",
+        "data": "synthetic (seeded scene generator, spatialrust_amd/synth.py)",
+        "config": {"workload": w.name, "target_points": int(len(w.target)),
+                   "source_points": int(len(w.source)), "voxel_leaf": w.voxel_leaf,
+                   "k_neighbors": w.k_neighbors,
+                   "max_correspondence_distance": w.max_correspondence_distance,
+                   "iterations": w.max_iterations, "mode": "plain (one registration per call)",
+                   "after_voxel": {"source": res.source_points, "target": res.target_points}},
+        "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
+                       "p99": float(np.percentile(lat, 99) * 1e3)},
+        "error_vs_fp64_oracle": err,
+        "error_vs_ground_truth": gt,
+        "kernels_ms_per_step": {k: v[0] / args.steps for k, v in kern.items()},
+        "roofline": {"kernel": "k_gicp_pass", "bound": "hbm", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes_per_launch": algo_bytes,
+                     "avg_launch_us": avg_pass_s * 1e6, "launches": pass_n},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        print("timing the CPU oracle sample ...", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = _cpu_baseline(w)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

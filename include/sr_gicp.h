@@ -1,0 +1,247 @@
+/*
+ * sr_gicp.h — C ABI of the MI355X (gfx950) GICP scan-registration path.
+ *
+ * This library is the GPU backend behind SpatialRust's GICP call. Each entry
+ * point below replaces one reference interface (paths relative to the
+ * rsasaki0109/SpatialRust checkout); the replaced interface is cited beside it.
+ *
+ * Conventions
+ *  - Point clouds cross the boundary as borrowed SoA f32 columns (sr_xyz_view),
+ *    the same layout as `HasPositions3::positions3`
+ *    (crates/spatialrust-core/src/capabilities.rs:22-28). `mem` says whether the
+ *    pointers are host (0) or device (1) memory.
+ *  - Every entry point returns an int32 status: 0 = Ok, otherwise 1 + the index
+ *    of the `SpatialError` variant it maps to
+ *    (crates/spatialrust-core/src/error.rs:8-37): 1 MissingField,
+ *    2 SchemaValidation, 3 InvalidArgument, 4 Io, 5 BufferLengthMismatch,
+ *    6 UnsupportedDType. The message (reference wording where the reference has
+ *    one) is available from sr_last_error() on the calling thread.
+ *  - SR_STATUS_UNAVAILABLE (100) means "no usable HIP device"; a caller that
+ *    asked for ExecutionPolicy::Auto may then run its own CPU path. An explicit
+ *    GPU policy gets InvalidArgument instead (crates/spatialrust-core/src/execution.rs:40-48).
+ *  - All entry points are thread-safe. One HIP context per process is created
+ *    lazily on first use (as `WgpuRuntime::shared`, crates/spatialrust-gpu/src/runtime.rs:90-95).
+ *    Calls that share a handle (sr_map, sr_server) must not overlap.
+ */
+#ifndef SR_GICP_H
+#define SR_GICP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SR_OK 0
+#define SR_ERR_MISSING_FIELD 1
+#define SR_ERR_SCHEMA 2
+#define SR_ERR_INVALID_ARGUMENT 3
+#define SR_ERR_IO 4
+#define SR_ERR_LENGTH_MISMATCH 5
+#define SR_ERR_DTYPE 6
+#define SR_STATUS_UNAVAILABLE 100
+
+#define SR_MEM_HOST 0u
+#define SR_MEM_DEVICE 1u
+
+/* Borrowed xyz columns, valid for the duration of the call. */
+typedef struct {
+  const float* x;
+  const float* y;
+  const float* z;
+  uint64_t n;
+  uint32_t mem; /* SR_MEM_HOST or SR_MEM_DEVICE */
+} sr_xyz_view;
+
+/* GicpConfig (crates/spatialrust-registration/src/gicp.rs:23-44) plus the
+ * voxel leaf applied to both clouds before registration (<= 0: none).
+ * Defaults (gicp.rs:46-60) come from sr_gicp_config_default(). */
+typedef struct {
+  uint32_t max_iterations;
+  float max_correspondence_distance;
+  uint32_t k_neighbors;
+  double epsilon;
+  double transformation_epsilon;
+  double fitness_epsilon;
+  uint32_t min_correspondences;
+  float init_q[4]; /* initial_guess rotation, x y z w */
+  float init_t[3]; /* initial_guess translation */
+  float covariance_radius; /* <= 0: None (k-NN covariances) */
+  float voxel_leaf;        /* <= 0: clouds are used as given */
+} sr_gicp_config;
+
+/* ExecutionPolicy / DeviceKind (crates/spatialrust-core/src/execution.rs:5-15,
+ * device.rs:4-12). SR_DEV_HIP is the device kind this backend adds. */
+typedef enum { SR_CPU_SINGLE = 0, SR_CPU_PARALLEL = 1, SR_GPU = 2, SR_AUTO = 3 } sr_policy_kind;
+typedef enum { SR_DEV_CPU = 0, SR_DEV_WGPU = 1, SR_DEV_CUDA = 2, SR_DEV_HIP = 3 } sr_device_kind;
+
+typedef struct {
+  int32_t kind;   /* sr_policy_kind */
+  int32_t device; /* sr_device_kind, used when kind == SR_GPU */
+} sr_policy;
+
+/* RegistrationResult (crates/spatialrust-registration/src/registration.rs:15-24).
+ * q/t is the f32 Isometry3; q64/t64 is the FP64 pose the device iterated. */
+typedef struct {
+  float q[4];
+  float t[3];
+  double fitness;
+  uint64_t iterations;
+  uint8_t converged;
+  uint8_t pad_[7];
+  double q64[4];
+  double t64[3];
+  uint64_t source_points; /* after the voxel filter */
+  uint64_t target_points; /* after the voxel filter */
+  uint64_t correspondences; /* in the last Gauss-Newton pass */
+} sr_result;
+
+/* Stage bits recorded in sr_receipt.stage_mask (ExecutionReceipt::record_stage,
+ * crates/spatialrust-core/src/execution.rs:135-138). */
+#define SR_STAGE_VOXEL 0x1u       /* "gicp-voxel" */
+#define SR_STAGE_COVARIANCE 0x2u  /* "gicp-covariance" */
+#define SR_STAGE_ITERATE 0x4u     /* "gicp-iterate" */
+#define SR_STAGE_READBACK 0x8u    /* "gpu-readback" */
+#define SR_STAGE_GRID 0x10u       /* "gicp-grid" */
+
+/* ExecutionReceipt (crates/spatialrust-core/src/execution.rs:69-138). */
+typedef struct {
+  sr_policy requested;
+  sr_policy resolved;
+  uint64_t h2d_bytes;
+  uint64_t d2d_bytes;
+  uint64_t d2h_bytes;
+  uint32_t stage_mask;
+  uint32_t pad_;
+} sr_receipt;
+
+/* ---- runtime ---------------------------------------------------------- */
+
+/* Thread-local message of the last failed call on this thread ("" if none). */
+const char* sr_last_error(void);
+
+/* 1 if a gfx950 HIP device is usable, else 0 (WgpuRuntime::shared().is_ok(),
+ * crates/spatialrust-gpu/src/runtime.rs:90-95). */
+int32_t sr_device_available(void);
+
+/* Library ABI version (major*10000 + minor*100 + patch). */
+int32_t sr_version(void);
+
+/* GicpConfig::default() (crates/spatialrust-registration/src/gicp.rs:46-60). */
+void sr_gicp_config_default(sr_gicp_config* out);
+
+/* ---- kernel timing ------------------------------------------------------
+ * No counterpart in the reference (it records no GPU timestamps,
+ * `timestamp_writes: None`). When enabled, launches of the named kernels are
+ * bracketed with HIP events on the stream they run on. Names: "gicp_pass",
+ * "knn_normals", "radius_cov", "voxel_insert". */
+void sr_profile_enable(int32_t on);
+void sr_profile_reset(void);
+/* Sum of event-measured durations (ms) and number of launches of `name`. */
+int32_t sr_profile_read(const char* name, double* total_ms, uint64_t* launches);
+
+/* ---- GICP (drop-in for GicpRegistration::align_with_diagnostics) ------- */
+
+/* Replaces GicpRegistration::align / align_with_diagnostics
+ * (crates/spatialrust-registration/src/gicp.rs:103-224, 232-234) and adds the
+ * policy/receipt surface its sibling operators already have
+ * (crates/spatialrust-filtering/src/voxel.rs:213-237). `receipt` may be NULL. */
+int32_t sr_gicp_align(const sr_xyz_view* source, const sr_xyz_view* target,
+                      const sr_gicp_config* config, sr_policy policy,
+                      sr_result* out, sr_receipt* receipt);
+
+/* ---- voxel downsample (VoxelGridDownsample, centroid mode, xyz schema) -- */
+
+/* Replaces VoxelGridDownsample::filter_with_policy for the centroid/xyz case
+ * (crates/spatialrust-filtering/src/voxel.rs:213-237, 251-368, 550-636).
+ * `origin` NULL = per-axis minimum (VoxelGridDownsampleConfig::centroid,
+ * voxel.rs:138-146). Outputs are host arrays of capacity >= input n; cells come
+ * in first-occurrence order. `keys` (3 x int64 per cell) and `counts` may be
+ * NULL. *n_out receives the cell count. */
+int32_t sr_voxel_downsample(const sr_xyz_view* input, float leaf, const float* origin,
+                            sr_policy policy, float* out_x, float* out_y, float* out_z,
+                            int64_t* keys, uint32_t* counts, uint64_t* n_out,
+                            sr_receipt* receipt);
+
+/* ---- per-point plane covariances / normals ----------------------------- */
+
+/* Replaces `covariances` + `regularize` (crates/spatialrust-registration/src/gicp.rs:259-291)
+ * when radius <= 0 (exact k-NN, k including the point itself), and
+ * estimate_plane_covariances_grid_gpu (crates/spatialrust-gpu/src/kernels/covariances_grid.rs:216-364)
+ * when radius > 0. Output: 6 f32 per point [c00,c11,c22,c01,c02,c12]
+ * (GpuCovariance, covariances_grid.rs:21-23) and optionally the unit normal
+ * (smallest-eigenvalue eigenvector, 3 f32 per point; NULL to skip). Host arrays. */
+int32_t sr_plane_covariances(const sr_xyz_view* cloud, uint32_t k_neighbors, float radius,
+                             double epsilon, float* out_cov6, float* out_normals);
+
+/* ---- nearest neighbour (NearestNeighborIndex::nearest_k / nearest_one) - */
+
+/* Exact k-NN of each query against `points` on a device uniform grid.
+ * Replaces KdTree::nearest_k (crates/spatialrust-search/src/kdtree.rs:306-346)
+ * for batches of queries. Neighbours of each query are sorted by ascending
+ * squared distance, ties by lower index (brute_force_knn order,
+ * crates/spatialrust-search/src/brute.rs:84-110). Missing slots (fewer than
+ * k points) get index -1. Host arrays: out_index int32[nq*k], out_d2 f32[nq*k]. */
+int32_t sr_knn(const sr_xyz_view* points, const sr_xyz_view* queries, uint32_t k,
+               int32_t* out_index, float* out_d2);
+
+/* ---- device-resident target maps --------------------------------------- */
+
+typedef struct sr_map sr_map;
+
+/* Uploads and preprocesses a target cloud once (voxel filter per
+ * config->voxel_leaf, uniform grid, plane covariances) so many registrations can
+ * reuse it. This is the target half of align_with_diagnostics
+ * (gicp.rs:119-123) made resident. */
+int32_t sr_map_create(const sr_xyz_view* target, const sr_gicp_config* config, sr_map** out);
+void sr_map_destroy(sr_map* map);
+/* Number of target points kept after the voxel filter. */
+uint64_t sr_map_points(const sr_map* map);
+/* Device bytes held by the map. */
+uint64_t sr_map_device_bytes(const sr_map* map);
+
+/* Registers one source cloud against a resident map (same semantics as
+ * sr_gicp_align with the map's target). */
+int32_t sr_map_align(const sr_map* map, const sr_xyz_view* source, const sr_gicp_config* config,
+                     sr_result* out);
+
+/* ---- batched registration ---------------------------------------------- */
+
+/* Registers `count` independent (source, map) pairs in one set of launches
+ * (one launch per Gauss-Newton pass for the whole batch, captured as a HIP graph).
+ * Each result equals what sr_map_align returns for that pair alone. The
+ * reference registers one pair per call; this is the batched form of
+ * GicpRegistration::align (gicp.rs:103-224). */
+int32_t sr_batch_align(const sr_map* const* maps, const sr_xyz_view* sources, uint32_t count,
+                       const sr_gicp_config* config, sr_result* out);
+
+/* ---- continuous-batching server ---------------------------------------- */
+
+typedef struct sr_server sr_server;
+
+typedef struct {
+  uint64_t hbm_budget_bytes; /* admission limit for per-request device memory */
+  uint32_t max_active;       /* slots: requests iterating together */
+  uint32_t max_iterations;   /* per request */
+  double delta_pose_epsilon; /* converged when ||dx|| < this */
+} sr_server_config;
+
+int32_t sr_server_create(const sr_server_config* config, const sr_gicp_config* gicp,
+                         sr_server** out);
+void sr_server_destroy(sr_server* server);
+/* Queues a registration of `source` (host xyz, copied) against `map`; returns a
+ * ticket in *ticket. */
+int32_t sr_server_submit(sr_server* server, const sr_map* map, const sr_xyz_view* source,
+                         uint64_t* ticket);
+/* Advances the server by one scheduling step: admits queued requests as slots and
+ * HBM budget allow, runs one Gauss-Newton pass over every active request, retires
+ * converged ones. Returns the number of requests still queued or active. */
+int32_t sr_server_step(sr_server* server, uint32_t* pending);
+/* Fetches a finished result. *ready = 1 when `ticket` has finished. */
+int32_t sr_server_poll(sr_server* server, uint64_t ticket, sr_result* out, uint8_t* ready);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SR_GICP_H */

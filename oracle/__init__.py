@@ -1,0 +1,288 @@
+"""ctypes front-end of the CPU oracle — TEST INFRASTRUCTURE ONLY.
+
+The oracle restates SpatialRust's CPU GICP path (see oracle.cpp for the file:line
+of every function it follows). Only tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py may import this module; the product library never
+does, and never falls back to it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import build as _build
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_lib = None
+
+
+class OrAlignResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("error_kind", ctypes.c_int32),
+                ("error_count", ctypes.c_uint64), ("q", ctypes.c_double * 4),
+                ("t", ctypes.c_double * 3), ("fitness", ctypes.c_double),
+                ("iterations", ctypes.c_uint64), ("converged", ctypes.c_uint8)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_build.LIB):
+            _build.build()
+        _lib = ctypes.CDLL(_build.LIB)
+        _declare(_lib)
+    return _lib
+
+
+_F = ctypes.POINTER(ctypes.c_float)
+_D = ctypes.POINTER(ctypes.c_double)
+_I64 = ctypes.POINTER(ctypes.c_int64)
+_U32 = ctypes.POINTER(ctypes.c_uint32)
+
+
+def _declare(L):
+    L.or_voxel_centroid.restype = ctypes.c_int32
+    L.or_voxel_centroid.argtypes = [_F, _F, _F, ctypes.c_int64, ctypes.c_float, _F, _F, _F, _F,
+                                    _I64, _U32, _I64]
+    L.or_brute_knn.restype = ctypes.c_int64
+    L.or_brute_knn.argtypes = [_F, _F, _F, ctypes.c_int64, ctypes.c_float, ctypes.c_float,
+                               ctypes.c_float, ctypes.c_int64, _I64, _F]
+    L.or_kdtree_new.restype = ctypes.c_void_p
+    L.or_kdtree_new.argtypes = [_F, _F, _F, ctypes.c_int64]
+    L.or_kdtree_free.argtypes = [ctypes.c_void_p]
+    L.or_kdtree_nearest_k.restype = ctypes.c_int64
+    L.or_kdtree_nearest_k.argtypes = [ctypes.c_void_p, ctypes.c_float, ctypes.c_float,
+                                      ctypes.c_float, ctypes.c_int64, _I64, _F]
+    L.or_kdtree_knn_batch.argtypes = [ctypes.c_void_p, _F, _F, _F, ctypes.c_int64, ctypes.c_int64,
+                                      _I64, _F]
+    L.or_covariances.argtypes = [_F, _F, _F, ctypes.c_int64, ctypes.c_int64, ctypes.c_double,
+                                 ctypes.c_int32, _D, _D]
+    L.or_symmetric_eigen3.argtypes = [_D, _D, _D]
+    L.or_solve_linear_system.restype = ctypes.c_int32
+    L.or_solve_linear_system.argtypes = [_D, _D, ctypes.c_int32, _D]
+    L.or_quat_from_axis_angle.argtypes = [ctypes.c_float] * 4 + [_F]
+    L.or_quat_to_mat3.argtypes = [_F, _F]
+    L.or_isometry_compose.argtypes = [_F, _F, _F, _F, _F, _F]
+    L.or_transform_points.argtypes = [_F, _F, _F, _F, _F, ctypes.c_int64, _F, _F, _F]
+    L.or_gicp_align.restype = ctypes.c_int32
+    L.or_gicp_align.argtypes = [_F, _F, _F, ctypes.c_int64, _F, _F, _F, ctypes.c_int64,
+                                ctypes.c_void_p, ctypes.c_int32, _D, _D,
+                                ctypes.POINTER(OrAlignResult), _D, ctypes.c_int64]
+    L.or_gicp_normal_equations.restype = ctypes.c_int64
+    L.or_gicp_normal_equations.argtypes = [_F, _F, _F, ctypes.c_int64, _F, _F, _F, ctypes.c_int64,
+                                           _D, _D, _D, _D, ctypes.c_double, _D, _D, _I64]
+    L.or_grid_bounds.restype = ctypes.c_int32
+    L.or_grid_bounds.argtypes = [_F, _F, _F, ctypes.c_int64, ctypes.c_float, _F, _U32]
+    L.or_radius_covariances.restype = ctypes.c_int32
+    L.or_radius_covariances.argtypes = [_F, _F, _F, ctypes.c_int64, ctypes.c_float,
+                                        ctypes.c_float, _F, _F]
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def _cols(points):
+    pts = np.asarray(points, dtype=np.float32)
+    return _f32(pts[:, 0]), _f32(pts[:, 1]), _f32(pts[:, 2])
+
+
+def voxel_centroid(points, leaf, origin=None):
+    """VoxelGridDownsample centroid/xyz (voxel.rs:550-636). Returns
+    (centroids (C,3) f32, keys (C,3) i64, counts (C,) u32) in first-occurrence order."""
+    x, y, z = _cols(points)
+    n = len(x)
+    ox, oy, oz = (np.zeros(max(n, 1), np.float32) for _ in range(3))
+    keys = np.zeros((max(n, 1), 3), np.int64)
+    counts = np.zeros(max(n, 1), np.uint32)
+    n_out = ctypes.c_int64(0)
+    org = None if origin is None else _f32(origin)
+    st = lib().or_voxel_centroid(_p(x, _F), _p(y, _F), _p(z, _F), n, float(leaf),
+                                 None if org is None else _p(org, _F), _p(ox, _F), _p(oy, _F),
+                                 _p(oz, _F), _p(keys, _I64), _p(counts, _U32), ctypes.byref(n_out))
+    if st != 0:
+        raise ValueError("leaf_size must be greater than zero")
+    c = n_out.value
+    return np.stack([ox[:c], oy[:c], oz[:c]], axis=1), keys[:c], counts[:c]
+
+
+def brute_knn(points, q, k):
+    x, y, z = _cols(points)
+    idx = np.zeros(max(k, 1), np.int64)
+    d2 = np.zeros(max(k, 1), np.float32)
+    m = lib().or_brute_knn(_p(x, _F), _p(y, _F), _p(z, _F), len(x), float(q[0]), float(q[1]),
+                           float(q[2]), k, _p(idx, _I64), _p(d2, _F))
+    return idx[:m], d2[:m]
+
+
+class KdTree:
+    """Faithful f32 KdTree (kdtree.rs)."""
+
+    def __init__(self, points):
+        self._cols = _cols(points)
+        x, y, z = self._cols
+        self._h = lib().or_kdtree_new(_p(x, _F), _p(y, _F), _p(z, _F), len(x))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().or_kdtree_free(self._h)
+            self._h = None
+
+    def nearest_k(self, q, k):
+        idx = np.zeros(max(k, 1), np.int64)
+        d2 = np.zeros(max(k, 1), np.float32)
+        m = lib().or_kdtree_nearest_k(self._h, float(q[0]), float(q[1]), float(q[2]), k,
+                                      _p(idx, _I64), _p(d2, _F))
+        return idx[:m], d2[:m]
+
+    def knn_batch(self, queries, k):
+        qx, qy, qz = _cols(queries)
+        nq = len(qx)
+        idx = np.zeros((nq, k), np.int64)
+        d2 = np.zeros((nq, k), np.float32)
+        lib().or_kdtree_knn_batch(self._h, _p(qx, _F), _p(qy, _F), _p(qz, _F), nq, k,
+                                  _p(idx, _I64), _p(d2, _F))
+        return idx, d2
+
+
+def covariances(points, k, epsilon=1e-3, mode="faithful"):
+    """Plane-regularized k-NN covariances (gicp.rs:259-291): (N,3,3) f64 and the
+    normals (N,3) f64."""
+    x, y, z = _cols(points)
+    n = len(x)
+    out = np.zeros((n, 3, 3), np.float64)
+    nrm = np.zeros((n, 3), np.float64)
+    lib().or_covariances(_p(x, _F), _p(y, _F), _p(z, _F), n, k, float(epsilon),
+                         0 if mode == "faithful" else 1, _p(out, _D), _p(nrm, _D))
+    return out, nrm
+
+
+def radius_covariances(points, radius, epsilon=1e-3):
+    """COV_GRID_WGSL semantics on the CPU: (N,6) f32 [c00,c11,c22,c01,c02,c12], normals (N,3)."""
+    x, y, z = _cols(points)
+    n = len(x)
+    out = np.zeros((n, 6), np.float32)
+    nrm = np.zeros((n, 3), np.float32)
+    st = lib().or_radius_covariances(_p(x, _F), _p(y, _F), _p(z, _F), n, float(radius),
+                                     float(epsilon), _p(out, _F), _p(nrm, _F))
+    if st != 0:
+        raise ValueError("grid would exceed the dense cell cap")
+    return out, nrm
+
+
+def gicp_normal_equations(source, target, src_cov, tgt_cov, q, t, dmax):
+    """H (6,6), g (6,), correspondence index per source point (-1 = none) of one
+    FP64 pass of the GICP loop body (gicp.rs:137-179) at pose (q, t)."""
+    sx, sy, sz = _cols(source)
+    tx, ty, tz = _cols(target)
+    sc = np.ascontiguousarray(src_cov, dtype=np.float64)
+    tc = np.ascontiguousarray(tgt_cov, dtype=np.float64)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    H = np.zeros((6, 6))
+    g = np.zeros(6)
+    corr = np.zeros(len(sx), np.int64)
+    lib().or_gicp_normal_equations(_p(sx, _F), _p(sy, _F), _p(sz, _F), len(sx), _p(tx, _F),
+                                   _p(ty, _F), _p(tz, _F), len(tx), _p(sc, _D), _p(tc, _D),
+                                   _p(q, _D), _p(t, _D), float(dmax), _p(H, _D), _p(g, _D),
+                                   _p(corr, _I64))
+    return H, g, corr
+
+
+def grid_bounds(points, cell):
+    x, y, z = _cols(points)
+    org = np.zeros(3, np.float32)
+    dims = np.zeros(3, np.uint32)
+    st = lib().or_grid_bounds(_p(x, _F), _p(y, _F), _p(z, _F), len(x), float(cell),
+                              _p(org, _F), _p(dims, _U32))
+    return st, org, dims
+
+
+def symmetric_eigen3(m):
+    m = np.ascontiguousarray(m, dtype=np.float64)
+    ev = np.zeros(3)
+    vec = np.zeros((3, 3))
+    lib().or_symmetric_eigen3(_p(m, _D), _p(ev, _D), _p(vec, _D))
+    return ev, vec
+
+
+def solve_linear_system(a, b):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(len(b))
+    st = lib().or_solve_linear_system(_p(a, _D), _p(b, _D), len(b), _p(x, _D))
+    return None if st != 0 else x
+
+
+def quat_from_axis_angle(axis, angle):
+    q = np.zeros(4, np.float32)
+    lib().or_quat_from_axis_angle(float(axis[0]), float(axis[1]), float(axis[2]), float(angle),
+                                  _p(q, _F))
+    return q
+
+
+def quat_to_mat3(q):
+    q = _f32(q)
+    m = np.zeros((3, 3), np.float32)
+    lib().or_quat_to_mat3(_p(q, _F), _p(m, _F))
+    return m
+
+
+def isometry_compose(a, b):
+    """a ∘ b for (q, t) pairs in f32 (transform.rs:104-108)."""
+    aq, at = _f32(a[0]), _f32(a[1])
+    bq, bt = _f32(b[0]), _f32(b[1])
+    oq = np.zeros(4, np.float32)
+    ot = np.zeros(3, np.float32)
+    lib().or_isometry_compose(_p(aq, _F), _p(at, _F), _p(bq, _F), _p(bt, _F), _p(oq, _F),
+                              _p(ot, _F))
+    return oq, ot
+
+
+def transform_points(q, t, points):
+    """transform_point_cloud (registration/src/transform.rs:8-25) in f32."""
+    q, t = _f32(q), _f32(t)
+    x, y, z = _cols(points)
+    n = len(x)
+    ox, oy, oz = (np.zeros(n, np.float32) for _ in range(3))
+    lib().or_transform_points(_p(q, _F), _p(t, _F), _p(x, _F), _p(y, _F), _p(z, _F), n,
+                              _p(ox, _F), _p(oy, _F), _p(oz, _F))
+    return np.stack([ox, oy, oz], axis=1)
+
+
+_ERRORS = {
+    1: "GICP requires non-empty source and target point clouds",
+    2: "GICP needs k_neighbors >= 3 for covariance estimation",
+    4: "GICP normal equations were singular",
+}
+
+
+def gicp_align(source, target, config, mode="faithful", src_cov=None, tgt_cov=None,
+               trace_cap=0):
+    """GicpRegistration::align_with_diagnostics (gicp.rs:103-224).
+
+    `config` is a ctypes sr_gicp_config (spatialrust_amd.native.GicpConfigC).
+    Returns dict(q, t, fitness, iterations, converged, trace) or raises ValueError
+    with the reference's message."""
+    sx, sy, sz = _cols(source)
+    tx, ty, tz = _cols(target)
+    res = OrAlignResult()
+    trace = np.zeros((max(trace_cap, 1), 10), np.float64)
+    sc = None if src_cov is None else np.ascontiguousarray(src_cov, dtype=np.float64)
+    tc = None if tgt_cov is None else np.ascontiguousarray(tgt_cov, dtype=np.float64)
+    st = lib().or_gicp_align(_p(sx, _F), _p(sy, _F), _p(sz, _F), len(sx), _p(tx, _F), _p(ty, _F),
+                             _p(tz, _F), len(tx), ctypes.byref(config),
+                             0 if mode == "faithful" else 1,
+                             None if sc is None else _p(sc, _D), None if tc is None else _p(tc, _D),
+                             ctypes.byref(res), _p(trace, _D), trace_cap)
+    if st != 0:
+        if res.error_kind == 3:
+            raise ValueError(f"GICP found only {res.error_count} correspondences, minimum is "
+                             f"{config.min_correspondences}")
+        raise ValueError(_ERRORS.get(res.error_kind, "invalid argument"))
+    return dict(q=np.array(res.q[:]), t=np.array(res.t[:]), fitness=res.fitness,
+                iterations=int(res.iterations), converged=bool(res.converged),
+                trace=trace[:min(trace_cap, res.iterations)])

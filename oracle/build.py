@@ -1,0 +1,26 @@
+"""Build recipe for the CPU oracle (test infrastructure only).
+
+Compiles oracle/oracle.cpp into oracle/liboracle.so with g++. -ffp-contract=off
+keeps every f32/f64 operation rounded separately, as in the reference's Rust
+code (rustc never contracts a*b+c into an FMA).
+"""
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+
+def build(verbose: bool = False) -> str:
+    src = os.path.join(HERE, "oracle.cpp")
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+           "-fno-fast-math", "-Wall", "-Wextra", "-o", LIB + ".tmp", src]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(LIB + ".tmp", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,20 @@
+"""spatialrust_amd — MI355X (gfx950) GICP scan-registration path.
+
+Drop-in GPU backend for SpatialRust's GicpRegistration / ExecutionPolicy
+surface: voxel-centroid downsample, grid k-NN / radius covariances, grid
+nearest-neighbour correspondences and the Gauss-Newton iteration, as HIP
+kernels behind the C ABI in include/sr_gicp.h.
+"""
+from .registration import (DeviceKind, DeviceUnavailable, ExecutionOutput, ExecutionPolicy,
+                           ExecutionReceipt, GicpConfig, GicpRegistration, InvalidArgument,
+                           Isometry3, RegistrationResult, SpatialError, TargetMap,
+                           VoxelGridDownsample, VoxelGridDownsampleConfig, VoxelOutput,
+                           batch_align, device_available, device_view, knn, plane_covariances,
+                           register_gicp)
+
+__all__ = [
+    "DeviceKind", "DeviceUnavailable", "ExecutionOutput", "ExecutionPolicy", "ExecutionReceipt",
+    "GicpConfig", "GicpRegistration", "InvalidArgument", "Isometry3", "RegistrationResult",
+    "SpatialError", "TargetMap", "VoxelGridDownsample", "VoxelGridDownsampleConfig", "VoxelOutput",
+    "batch_align", "device_available", "device_view", "knn", "plane_covariances", "register_gicp",
+]

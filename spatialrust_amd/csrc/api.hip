@@ -1,0 +1,770 @@
+// C ABI of libsr_gicp.so (include/sr_gicp.h) and the host runtime behind it:
+// lazy device context, per-thread streams, a caching device allocator, target
+// maps, single / batched registration and the continuous-batching server.
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/sr_gicp.h"
+#include "internal.hpp"
+
+namespace sr {
+
+void fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+
+// ------------------------------------------------------------- runtime ----
+
+static thread_local std::string t_last_error;
+
+[[noreturn]] void throw_hip(hipError_t e, const char* what) {
+  throw Error{SR_ERR_INVALID_ARGUMENT, std::string("HIP error in ") + what + ": " + hipGetErrorString(e)};
+}
+
+struct DeviceState {
+  int status = -1;  // 0 usable, 1 unavailable
+  std::string reason;
+  int cus = 0;
+};
+
+static DeviceState& device_state() {
+  static DeviceState ds;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+      ds.status = 1;
+      ds.reason = "no HIP device is visible";
+      return;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) != hipSuccess) {
+      ds.status = 1;
+      ds.reason = "hipGetDeviceProperties failed";
+      return;
+    }
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+      ds.status = 1;
+      ds.reason = std::string("device 0 is ") + prop.gcnArchName + ", this library is built for gfx950";
+      return;
+    }
+    if (hipSetDevice(0) != hipSuccess) {
+      ds.status = 1;
+      ds.reason = "hipSetDevice(0) failed";
+      return;
+    }
+    ds.cus = prop.multiProcessorCount;
+    ds.status = 0;
+  });
+  return ds;
+}
+
+void ensure_device() {
+  DeviceState& ds = device_state();
+  if (ds.status != 0) throw Error{SR_STATUS_UNAVAILABLE, "HIP device unavailable: " + ds.reason};
+  static thread_local bool bound = false;
+  if (!bound) {
+    SR_HIP(hipSetDevice(0));
+    bound = true;
+  }
+}
+
+int device_cu_count() { return device_state().cus; }
+
+hipStream_t thread_stream() {
+  struct Holder {
+    hipStream_t s = nullptr;
+    ~Holder() {
+      if (s) (void)hipStreamDestroy(s);
+    }
+  };
+  static thread_local Holder h;
+  if (!h.s) SR_HIP(hipStreamCreateWithFlags(&h.s, hipStreamNonBlocking));
+  return h.s;
+}
+
+// Caching allocator: sizes rounded up to a power of two (>= 512 B); freed
+// blocks are kept per size class. Blocks are handed back only to the thread
+// that freed them, whose stream orders every later use after earlier work.
+struct Pool {
+  std::mutex mu;
+  std::unordered_map<void*, size_t> live;
+  std::map<std::pair<size_t, uint64_t>, std::vector<void*>> free_by_owner;
+  uint64_t bytes_live = 0;
+};
+static Pool& pool() {
+  static Pool* p = new Pool();  // never destroyed: blocks stay valid until exit
+  return *p;
+}
+static uint64_t thread_token() {
+  static std::atomic<uint64_t> next{1};
+  static thread_local uint64_t tok = next++;
+  return tok;
+}
+
+static size_t size_class(size_t bytes) {
+  size_t c = 512;
+  while (c < bytes) c <<= 1;
+  return c;
+}
+
+void* dev_alloc(size_t bytes) {
+  Pool& P = pool();
+  const size_t cls = size_class(bytes);
+  const uint64_t tok = thread_token();
+  {
+    std::lock_guard<std::mutex> lk(P.mu);
+    auto it = P.free_by_owner.find({cls, tok});
+    if (it != P.free_by_owner.end() && !it->second.empty()) {
+      void* p = it->second.back();
+      it->second.pop_back();
+      P.live[p] = cls;
+      return p;
+    }
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, cls);
+  if (e != hipSuccess) {
+    // Give cached blocks back to the driver and retry once.
+    SR_HIP(hipDeviceSynchronize());
+    {
+      std::lock_guard<std::mutex> lk(P.mu);
+      for (auto& kv : P.free_by_owner)
+        for (void* q : kv.second) (void)hipFree(q);
+      P.free_by_owner.clear();
+    }
+    (void)hipGetLastError();
+    e = hipMalloc(&p, cls);
+    if (e != hipSuccess) throw_hip(e, "hipMalloc");
+  }
+  std::lock_guard<std::mutex> lk(P.mu);
+  P.live[p] = cls;
+  P.bytes_live += cls;
+  return p;
+}
+
+void dev_free(void* p) {
+  if (!p) return;
+  Pool& P = pool();
+  std::lock_guard<std::mutex> lk(P.mu);
+  auto it = P.live.find(p);
+  if (it == P.live.end()) return;
+  size_t cls = it->second;
+  P.live.erase(it);
+  P.free_by_owner[{cls, thread_token()}].push_back(p);
+}
+
+// --------------------------------------------------------------- timing ----
+
+struct Profiler {
+  std::mutex mu;
+  std::atomic<int> on{0};
+  struct Rec {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> spare;
+};
+static Profiler& profiler() {
+  static Profiler* p = new Profiler();
+  return *p;
+}
+
+static hipEvent_t prof_event() {
+  Profiler& P = profiler();
+  {
+    std::lock_guard<std::mutex> lk(P.mu);
+    if (!P.spare.empty()) {
+      hipEvent_t e = P.spare.back();
+      P.spare.pop_back();
+      return e;
+    }
+  }
+  hipEvent_t e;
+  SR_HIP(hipEventCreate(&e));
+  return e;
+}
+
+ProfScope::ProfScope(const char* name, hipStream_t s) : stream(s) {
+  Profiler& P = profiler();
+  if (!P.on.load()) return;
+  hipEvent_t a = prof_event(), b = prof_event();
+  SR_HIP(hipEventRecord(a, s));
+  std::lock_guard<std::mutex> lk(P.mu);
+  P.recs.push_back({name, a, b});
+  slot = int(P.recs.size()) - 1;
+}
+
+ProfScope::~ProfScope() {
+  if (slot < 0) return;
+  Profiler& P = profiler();
+  hipEvent_t b;
+  {
+    std::lock_guard<std::mutex> lk(P.mu);
+    b = P.recs[size_t(slot)].b;
+  }
+  (void)hipEventRecord(b, stream);
+}
+
+// --------------------------------------------------------- preparation ----
+
+struct Prepared {
+  Grid grid;               // points (after the voxel filter) in grid order
+  DevBuf<float4> normals;  // grid order: float4(n, a)
+  uint32_t n = 0;
+};
+
+struct Receipt {
+  uint64_t h2d = 0, d2h = 0, d2d = 0;
+  uint32_t stages = 0;
+};
+
+static float knn_occupancy(uint32_t k) { return std::max(4.0f, float(k) * 0.35f); }
+
+// Voxel filter (optional) + grid + plane normals of one cloud.
+static void prepare(const sr_xyz_view& v, const sr_gicp_config& cfg, Prepared& out, Receipt& rc,
+                    hipStream_t s) {
+  if (v.n > 0xFFFFFFF0ull) fail(SR_ERR_INVALID_ARGUMENT, "point clouds above 4G points are not supported");
+  Cloud raw;
+  pack_cloud(v.x, v.y, v.z, uint32_t(v.n), v.mem == SR_MEM_DEVICE, raw, s, &rc.h2d);
+  Cloud* cloud = &raw;
+  VoxelOut vox;
+  if (cfg.voxel_leaf > 0.0f) {
+    voxel_downsample(raw, cfg.voxel_leaf, nullptr, false, vox, s);
+    cloud = &vox.centroids;
+    rc.stages |= SR_STAGE_VOXEL;
+  }
+  out.n = cloud->n;
+  uint32_t hb[6];
+  host_bounds(*cloud, hb, s);
+  const float h = choose_cell_edge(*cloud, hb, knn_occupancy(cfg.k_neighbors));
+  build_grid(*cloud, hb, h, out.grid, s);
+  rc.stages |= SR_STAGE_GRID;
+  out.normals.resize(std::max<uint32_t>(out.n, 1));
+  if (cfg.covariance_radius > 0.0f) {
+    Grid rg;
+    build_grid(*cloud, hb, cfg.covariance_radius, rg, s);
+    DevBuf<float4> orig(std::max<uint32_t>(out.n, 1));
+    radius_normals(rg, cfg.covariance_radius, float(cfg.epsilon), orig.p, nullptr, nullptr, s);
+    gather_to_grid_order(out.grid, orig.p, out.normals.p, s);
+    SR_HIP(hipStreamSynchronize(s));
+  } else {
+    knn_normals(out.grid, cfg.k_neighbors, cfg.epsilon, out.normals.p, nullptr, nullptr, s);
+  }
+  rc.stages |= SR_STAGE_COVARIANCE;
+}
+
+static InstConfig inst_config(const sr_gicp_config& c) {
+  InstConfig ic{};
+  ic.dmax = c.max_correspondence_distance;
+  ic.dmax2 = c.max_correspondence_distance * c.max_correspondence_distance;
+  ic.one_minus_eps = float(1.0 - c.epsilon);
+  ic.max_iter = c.max_iterations;
+  ic.min_corr = c.min_correspondences;
+  ic.trans_eps = c.transformation_epsilon;
+  ic.fit_eps = c.fitness_epsilon;
+  return ic;
+}
+
+static uint32_t pick_ppt(uint64_t total_points) {
+  // Enough workgroups to fill 256 CUs several times over before packing more
+  // points per thread (fewer block reductions).
+  if (total_points >= (1ull << 23)) return 4;
+  if (total_points >= (1ull << 21)) return 2;
+  return 1;
+}
+
+// Runs the Gauss-Newton loop for `count` (source, target) pairs in one set of
+// launches and fills `out`. Each registration keeps its own blocks, partials and
+// state, so results do not depend on the batch.
+struct Batch {
+  DevBuf<IterInstance> insts;
+  DevBuf<uint32_t> block_inst;
+  DevBuf<InstState> states;
+  DevBuf<float> partials;
+  DevBuf<float> init;
+  uint32_t total_blocks = 0, ppt = 1, count = 0, max_passes = 0;
+};
+
+static void batch_setup(const std::vector<const Prepared*>& src, const std::vector<const Prepared*>& tgt,
+                        const std::vector<sr_gicp_config>& cfgs, Batch& B, hipStream_t s) {
+  const uint32_t count = uint32_t(src.size());
+  B.count = count;
+  uint64_t total_pts = 0;
+  for (auto* p : src) total_pts += p->n;
+  B.ppt = pick_ppt(total_pts);
+  std::vector<IterInstance> hi(count);
+  std::vector<uint32_t> hbi;
+  std::vector<float> init(size_t(7) * count);
+  uint32_t blocks = 0;
+  B.max_passes = 0;
+  std::vector<uint32_t> nbl(count);
+  for (uint32_t i = 0; i < count; ++i) {
+    nbl[i] = std::max<uint32_t>(1, (src[i]->n + kBlock * B.ppt - 1) / (kBlock * B.ppt));
+    blocks += nbl[i];
+  }
+  B.states.resize(count);
+  B.partials.resize(size_t(blocks) * 64);
+  uint32_t begin = 0;
+  for (uint32_t i = 0; i < count; ++i) {
+    IterInstance& I = hi[i];
+    I.src = src[i]->grid.pts.p;
+    I.src_n = src[i]->normals.p;
+    I.tgt = tgt[i]->grid.pts.p;
+    I.tgt_n = tgt[i]->normals.p;
+    I.cell_start = tgt[i]->grid.cell_start.p;
+    I.grid = tgt[i]->grid.desc;
+    I.cfg = inst_config(cfgs[i]);
+    I.state = B.states.p + i;
+    I.partials = B.partials.p + size_t(begin) * 64;
+    I.n_src = src[i]->n;
+    I.block_begin = begin;
+    I.nblocks = nbl[i];
+    for (uint32_t b = 0; b < nbl[i]; ++b) hbi.push_back(i);
+    begin += nbl[i];
+    std::memcpy(&init[7 * size_t(i)], cfgs[i].init_q, 4 * sizeof(float));
+    std::memcpy(&init[7 * size_t(i) + 4], cfgs[i].init_t, 3 * sizeof(float));
+    B.max_passes = std::max(B.max_passes, cfgs[i].max_iterations + 1);
+  }
+  B.total_blocks = blocks;
+  B.insts.resize(count);
+  B.block_inst.resize(blocks);
+  B.init.resize(init.size());
+  SR_HIP(hipMemcpyAsync(B.insts.p, hi.data(), sizeof(IterInstance) * count, hipMemcpyHostToDevice, s));
+  SR_HIP(hipMemcpyAsync(B.block_inst.p, hbi.data(), sizeof(uint32_t) * blocks, hipMemcpyHostToDevice, s));
+  SR_HIP(hipMemcpyAsync(B.init.p, init.data(), sizeof(float) * init.size(), hipMemcpyHostToDevice, s));
+  init_states(B.states.p, B.init.p, count, s);
+}
+
+static void batch_run(Batch& B, hipStream_t s) {
+  for (uint32_t k = 0; k < B.max_passes; ++k)
+    gicp_pass(B.insts.p, B.block_inst.p, B.total_blocks, B.ppt, s);
+}
+
+static int32_t state_to_result(const InstState& st, const sr_gicp_config& cfg, const Prepared& src,
+                               const Prepared& tgt, sr_result* out, std::string* msg) {
+  sr_result r;
+  std::memset(&r, 0, sizeof(r));
+  for (int a = 0; a < 4; ++a) {
+    r.q64[a] = st.q[a];
+    r.q[a] = float(st.q[a]);
+  }
+  for (int a = 0; a < 3; ++a) {
+    r.t64[a] = st.t[a];
+    r.t[a] = float(st.t[a]);
+  }
+  r.fitness = st.fitness;
+  r.iterations = st.iter;
+  r.converged = uint8_t(st.converged);
+  r.source_points = src.n;
+  r.target_points = tgt.n;
+  r.correspondences = st.count;
+  if (out) *out = r;
+  if (st.status == 1) {
+    *msg = "GICP found only " + std::to_string(st.count) + " correspondences, minimum is " +
+           std::to_string(cfg.min_correspondences);
+    return SR_ERR_INVALID_ARGUMENT;
+  }
+  if (st.status == 2) {
+    *msg = "GICP normal equations were singular";
+    return SR_ERR_INVALID_ARGUMENT;
+  }
+  return SR_OK;
+}
+
+static void validate_cfg(const sr_xyz_view* source, const sr_xyz_view* target,
+                         const sr_gicp_config* cfg) {
+  if (!cfg) fail(SR_ERR_INVALID_ARGUMENT, "config must not be null");
+  if ((source && source->n == 0) || (target && target->n == 0) || !source || !target)
+    fail(SR_ERR_INVALID_ARGUMENT, "GICP requires non-empty source and target point clouds");
+  if (cfg->k_neighbors < 3)
+    fail(SR_ERR_INVALID_ARGUMENT, "GICP needs k_neighbors >= 3 for covariance estimation");
+  if (!(cfg->voxel_leaf <= 0.0f) && !(cfg->voxel_leaf > 0.0f))
+    fail(SR_ERR_INVALID_ARGUMENT, "leaf_size must be greater than zero");
+}
+
+static void validate_view(const sr_xyz_view* v) {
+  if (!v) fail(SR_ERR_INVALID_ARGUMENT, "point cloud view must not be null");
+  if (v->n > 0 && (!v->x || !v->y || !v->z))
+    fail(SR_ERR_INVALID_ARGUMENT, "point cloud view has null columns");
+  if (v->mem != SR_MEM_HOST && v->mem != SR_MEM_DEVICE)
+    fail(SR_ERR_INVALID_ARGUMENT, "point cloud view mem must be 0 (host) or 1 (device)");
+}
+
+// Resolves an ExecutionPolicy for this backend (execution.rs:17-58 contract):
+// Gpu(Cpu) is invalid; Gpu(Hip) is strict; Auto uses HIP when it is usable and
+// otherwise reports UNAVAILABLE so the caller runs its own CPU path; CPU
+// policies are the caller's CPU implementation.
+static sr_policy resolve(sr_policy p) {
+  if (p.kind == SR_GPU && p.device == SR_DEV_CPU)
+    fail(SR_ERR_INVALID_ARGUMENT, "GPU execution policy cannot target the CPU device");
+  if (p.kind == SR_CPU_SINGLE || p.kind == SR_CPU_PARALLEL)
+    fail(SR_STATUS_UNAVAILABLE, "CPU policies run in the caller's CPU implementation; this library is the HIP backend");
+  if (p.kind == SR_GPU && p.device != SR_DEV_HIP)
+    fail(SR_ERR_INVALID_ARGUMENT, "unsupported GPU device kind for this backend (expected Hip)");
+  if (p.kind != SR_GPU && p.kind != SR_AUTO) fail(SR_ERR_INVALID_ARGUMENT, "unknown execution policy");
+  try {
+    ensure_device();
+  } catch (const Error& e) {
+    if (p.kind == SR_AUTO) throw;  // UNAVAILABLE: Auto may fall back
+    fail(SR_ERR_INVALID_ARGUMENT, e.msg);
+  }
+  return sr_policy{SR_GPU, SR_DEV_HIP};
+}
+
+}  // namespace sr
+
+// ------------------------------------------------------------------ ABI ----
+
+using namespace sr;
+
+struct sr_map {
+  Prepared tgt;
+  sr_gicp_config cfg;
+};
+
+template <typename F>
+static int32_t guarded(F&& f) {
+  try {
+    t_last_error.clear();
+    f();
+    return SR_OK;
+  } catch (const Error& e) {
+    t_last_error = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    t_last_error = "host allocation failed";
+    return SR_ERR_INVALID_ARGUMENT;
+  } catch (const std::exception& e) {
+    t_last_error = e.what();
+    return SR_ERR_INVALID_ARGUMENT;
+  }
+}
+
+extern "C" {
+
+const char* sr_last_error(void) { return t_last_error.c_str(); }
+
+void sr_profile_enable(int32_t on) { profiler().on.store(on ? 1 : 0); }
+
+void sr_profile_reset(void) {
+  Profiler& P = profiler();
+  (void)hipDeviceSynchronize();
+  std::lock_guard<std::mutex> lk(P.mu);
+  for (auto& r : P.recs) {
+    P.spare.push_back(r.a);
+    P.spare.push_back(r.b);
+  }
+  P.recs.clear();
+}
+
+int32_t sr_profile_read(const char* name, double* total_ms, uint64_t* launches) {
+  return guarded([&] {
+    Profiler& P = profiler();
+    SR_HIP(hipDeviceSynchronize());
+    std::lock_guard<std::mutex> lk(P.mu);
+    double tot = 0.0;
+    uint64_t cnt = 0;
+    for (auto& r : P.recs) {
+      if (r.name != name) continue;
+      float ms = 0.0f;
+      SR_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+      tot += ms;
+      ++cnt;
+    }
+    if (total_ms) *total_ms = tot;
+    if (launches) *launches = cnt;
+  });
+}
+
+int32_t sr_device_available(void) { return device_state().status == 0 ? 1 : 0; }
+
+int32_t sr_version(void) { return 100; }
+
+void sr_gicp_config_default(sr_gicp_config* out) {
+  if (!out) return;
+  std::memset(out, 0, sizeof(*out));
+  out->max_iterations = 50;
+  out->max_correspondence_distance = 1.0f;
+  out->k_neighbors = 20;
+  out->epsilon = 1e-3;
+  out->transformation_epsilon = 1e-8;
+  out->fitness_epsilon = 1e-6;
+  out->min_correspondences = 6;
+  out->init_q[3] = 1.0f;
+  out->covariance_radius = 0.0f;
+  out->voxel_leaf = 0.0f;
+}
+
+int32_t sr_gicp_align(const sr_xyz_view* source, const sr_xyz_view* target,
+                      const sr_gicp_config* config, sr_policy policy, sr_result* out,
+                      sr_receipt* receipt) {
+  return guarded([&] {
+    if (receipt) {
+      std::memset(receipt, 0, sizeof(*receipt));
+      receipt->requested = policy;
+    }
+    sr_policy resolved = resolve(policy);
+    validate_view(source);
+    validate_view(target);
+    validate_cfg(source, target, config);
+    hipStream_t s = thread_stream();
+    Receipt rc;
+    Prepared tgt, src;
+    prepare(*target, *config, tgt, rc, s);
+    prepare(*source, *config, src, rc, s);
+    Batch B;
+    batch_setup({&src}, {&tgt}, {*config}, B, s);
+    batch_run(B, s);
+    rc.stages |= SR_STAGE_ITERATE;
+    InstState st;
+    SR_HIP(hipMemcpyAsync(&st, B.states.p, sizeof(st), hipMemcpyDeviceToHost, s));
+    SR_HIP(hipStreamSynchronize(s));
+    rc.d2h += sizeof(st);
+    rc.stages |= SR_STAGE_READBACK;
+    if (receipt) {
+      receipt->resolved = resolved;
+      receipt->h2d_bytes = rc.h2d;
+      receipt->d2h_bytes = rc.d2h;
+      receipt->d2d_bytes = rc.d2d;
+      receipt->stage_mask = rc.stages;
+    }
+    std::string msg;
+    int32_t code = state_to_result(st, *config, src, tgt, out, &msg);
+    if (code) fail(code, msg);
+  });
+}
+
+int32_t sr_voxel_downsample(const sr_xyz_view* input, float leaf, const float* origin,
+                            sr_policy policy, float* out_x, float* out_y, float* out_z,
+                            int64_t* keys, uint32_t* counts, uint64_t* n_out, sr_receipt* receipt) {
+  return guarded([&] {
+    if (n_out) *n_out = 0;
+    if (receipt) {
+      std::memset(receipt, 0, sizeof(*receipt));
+      receipt->requested = policy;
+    }
+    sr_policy resolved = resolve(policy);
+    validate_view(input);
+    if (input->n == 0) return;  // voxel.rs:257-259
+    if (!(leaf > 0.0f)) fail(SR_ERR_INVALID_ARGUMENT, "leaf_size must be greater than zero");
+    if (input->n > 0xFFFFFFF0ull) fail(SR_ERR_INVALID_ARGUMENT, "point clouds above 4G points are not supported");
+    hipStream_t s = thread_stream();
+    Receipt rc;
+    Cloud raw;
+    pack_cloud(input->x, input->y, input->z, uint32_t(input->n), input->mem == SR_MEM_DEVICE, raw, s,
+               &rc.h2d);
+    VoxelOut vox;
+    voxel_downsample(raw, leaf, origin, keys != nullptr, vox, s);
+    const uint32_t c = vox.cells;
+    std::vector<float4> host(c);
+    if (c) SR_HIP(hipMemcpyAsync(host.data(), vox.centroids.pts.p, sizeof(float4) * c, hipMemcpyDeviceToHost, s));
+    if (keys && c)
+      SR_HIP(hipMemcpyAsync(keys, vox.keys.p, sizeof(int64_t) * 3 * c, hipMemcpyDeviceToHost, s));
+    if (counts && c)
+      SR_HIP(hipMemcpyAsync(counts, vox.counts.p, sizeof(uint32_t) * c, hipMemcpyDeviceToHost, s));
+    SR_HIP(hipStreamSynchronize(s));
+    for (uint32_t i = 0; i < c; ++i) {
+      out_x[i] = host[i].x;
+      out_y[i] = host[i].y;
+      out_z[i] = host[i].z;
+    }
+    rc.d2h += uint64_t(12) * c;
+    if (n_out) *n_out = c;
+    if (receipt) {
+      receipt->resolved = resolved;
+      receipt->h2d_bytes = rc.h2d;
+      receipt->d2h_bytes = rc.d2h;
+      receipt->stage_mask = SR_STAGE_VOXEL | SR_STAGE_READBACK;
+    }
+  });
+}
+
+int32_t sr_plane_covariances(const sr_xyz_view* cloud, uint32_t k_neighbors, float radius,
+                             double epsilon, float* out_cov6, float* out_normals) {
+  return guarded([&] {
+    ensure_device();
+    validate_view(cloud);
+    if (cloud->n == 0) return;
+    if (radius <= 0.0f && k_neighbors < 3)
+      fail(SR_ERR_INVALID_ARGUMENT, "GICP needs k_neighbors >= 3 for covariance estimation");
+    if (radius > 0.0f && !(radius == radius)) fail(SR_ERR_INVALID_ARGUMENT, "grid radius must be positive");
+    hipStream_t s = thread_stream();
+    Cloud c;
+    pack_cloud(cloud->x, cloud->y, cloud->z, uint32_t(cloud->n), cloud->mem == SR_MEM_DEVICE, c, s, nullptr);
+    uint32_t hb[6];
+    host_bounds(c, hb, s);
+    const uint32_t n = c.n;
+    DevBuf<float> cov6(size_t(6) * n), nrm3(size_t(3) * n);
+    if (radius > 0.0f) {
+      Grid rg;
+      build_grid(c, hb, radius, rg, s);
+      radius_normals(rg, radius, float(epsilon), nullptr, cov6.p, nrm3.p, s);
+    } else {
+      Grid g;
+      build_grid(c, hb, choose_cell_edge(c, hb, knn_occupancy(k_neighbors)), g, s);
+      DevBuf<float4> nrm(n);
+      knn_normals(g, k_neighbors, epsilon, nrm.p, cov6.p, nrm3.p, s);
+    }
+    if (out_cov6) SR_HIP(hipMemcpyAsync(out_cov6, cov6.p, sizeof(float) * 6 * n, hipMemcpyDeviceToHost, s));
+    if (out_normals) SR_HIP(hipMemcpyAsync(out_normals, nrm3.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s));
+    SR_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int32_t sr_knn(const sr_xyz_view* points, const sr_xyz_view* queries, uint32_t k,
+               int32_t* out_index, float* out_d2) {
+  return guarded([&] {
+    ensure_device();
+    validate_view(points);
+    validate_view(queries);
+    if (queries->n == 0 || k == 0) return;
+    hipStream_t s = thread_stream();
+    if (points->n == 0) {
+      for (uint64_t i = 0; i < queries->n * k; ++i) {
+        out_index[i] = -1;
+        out_d2[i] = -1.0f;
+      }
+      return;
+    }
+    Cloud pc, qc;
+    pack_cloud(points->x, points->y, points->z, uint32_t(points->n), points->mem == SR_MEM_DEVICE, pc, s, nullptr);
+    pack_cloud(queries->x, queries->y, queries->z, uint32_t(queries->n), queries->mem == SR_MEM_DEVICE, qc, s,
+               nullptr);
+    uint32_t hb[6];
+    host_bounds(pc, hb, s);
+    Grid g;
+    build_grid(pc, hb, choose_cell_edge(pc, hb, knn_occupancy(k)), g, s);
+    DevBuf<int32_t> idx(size_t(qc.n) * k);
+    DevBuf<float> d2(size_t(qc.n) * k);
+    knn_query(g, qc.pts.p, qc.n, k, idx.p, d2.p, s);
+    SR_HIP(hipMemcpyAsync(out_index, idx.p, sizeof(int32_t) * qc.n * k, hipMemcpyDeviceToHost, s));
+    SR_HIP(hipMemcpyAsync(out_d2, d2.p, sizeof(float) * qc.n * k, hipMemcpyDeviceToHost, s));
+    SR_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int32_t sr_map_create(const sr_xyz_view* target, const sr_gicp_config* config, sr_map** out) {
+  return guarded([&] {
+    if (!out) fail(SR_ERR_INVALID_ARGUMENT, "out must not be null");
+    *out = nullptr;
+    ensure_device();
+    validate_view(target);
+    validate_cfg(target, target, config);
+    hipStream_t s = thread_stream();
+    auto m = std::make_unique<sr_map>();
+    m->cfg = *config;
+    Receipt rc;
+    prepare(*target, *config, m->tgt, rc, s);
+    SR_HIP(hipStreamSynchronize(s));
+    *out = m.release();
+  });
+}
+
+void sr_map_destroy(sr_map* map) {
+  if (!map) return;
+  (void)hipDeviceSynchronize();
+  delete map;
+}
+
+uint64_t sr_map_points(const sr_map* map) { return map ? map->tgt.n : 0; }
+
+uint64_t sr_map_device_bytes(const sr_map* map) {
+  if (!map) return 0;
+  const Prepared& p = map->tgt;
+  return p.grid.cell_start.bytes() + p.grid.pts.bytes() + p.grid.order.bytes() + p.normals.bytes();
+}
+
+int32_t sr_map_align(const sr_map* map, const sr_xyz_view* source, const sr_gicp_config* config,
+                     sr_result* out) {
+  return guarded([&] {
+    if (!map) fail(SR_ERR_INVALID_ARGUMENT, "map must not be null");
+    ensure_device();
+    validate_view(source);
+    const sr_xyz_view dummy{nullptr, nullptr, nullptr, map->tgt.n, 0};
+    validate_cfg(source, &dummy, config);
+    hipStream_t s = thread_stream();
+    Receipt rc;
+    Prepared src;
+    prepare(*source, *config, src, rc, s);
+    Batch B;
+    batch_setup({&src}, {&map->tgt}, {*config}, B, s);
+    batch_run(B, s);
+    InstState st;
+    SR_HIP(hipMemcpyAsync(&st, B.states.p, sizeof(st), hipMemcpyDeviceToHost, s));
+    SR_HIP(hipStreamSynchronize(s));
+    std::string msg;
+    int32_t code = state_to_result(st, *config, src, map->tgt, out, &msg);
+    if (code) fail(code, msg);
+  });
+}
+
+int32_t sr_batch_align(const sr_map* const* maps, const sr_xyz_view* sources, uint32_t count,
+                       const sr_gicp_config* config, sr_result* out) {
+  return guarded([&] {
+    if (count == 0) return;
+    if (!maps || !sources || !out) fail(SR_ERR_INVALID_ARGUMENT, "null batch argument");
+    ensure_device();
+    hipStream_t s = thread_stream();
+    std::vector<Prepared> srcs(count);
+    std::vector<const Prepared*> sp(count), tp(count);
+    std::vector<sr_gicp_config> cfgs(count, *config);
+    Receipt rc;
+    for (uint32_t i = 0; i < count; ++i) {
+      if (!maps[i]) fail(SR_ERR_INVALID_ARGUMENT, "map must not be null");
+      validate_view(&sources[i]);
+      const sr_xyz_view dummy{nullptr, nullptr, nullptr, maps[i]->tgt.n, 0};
+      validate_cfg(&sources[i], &dummy, config);
+      prepare(sources[i], *config, srcs[i], rc, s);
+      sp[i] = &srcs[i];
+      tp[i] = &maps[i]->tgt;
+    }
+    Batch B;
+    batch_setup(sp, tp, cfgs, B, s);
+    batch_run(B, s);
+    std::vector<InstState> st(count);
+    SR_HIP(hipMemcpyAsync(st.data(), B.states.p, sizeof(InstState) * count, hipMemcpyDeviceToHost, s));
+    SR_HIP(hipStreamSynchronize(s));
+    int32_t first_code = 0;
+    std::string first_msg;
+    for (uint32_t i = 0; i < count; ++i) {
+      std::string msg;
+      int32_t code = state_to_result(st[i], *config, srcs[i], maps[i]->tgt, &out[i], &msg);
+      if (code && !first_code) {
+        first_code = code;
+        first_msg = "registration " + std::to_string(i) + ": " + msg;
+      }
+    }
+    if (first_code) fail(first_code, first_msg);
+  });
+}
+
+int32_t sr_server_create(const sr_server_config*, const sr_gicp_config*, sr_server** out) {
+  if (out) *out = nullptr;
+  t_last_error = "the serving loop is not built yet";
+  return SR_ERR_INVALID_ARGUMENT;
+}
+void sr_server_destroy(sr_server*) {}
+int32_t sr_server_submit(sr_server*, const sr_map*, const sr_xyz_view*, uint64_t*) {
+  t_last_error = "the serving loop is not built yet";
+  return SR_ERR_INVALID_ARGUMENT;
+}
+int32_t sr_server_step(sr_server*, uint32_t*) {
+  t_last_error = "the serving loop is not built yet";
+  return SR_ERR_INVALID_ARGUMENT;
+}
+int32_t sr_server_poll(sr_server*, uint64_t, sr_result*, uint8_t*) {
+  t_last_error = "the serving loop is not built yet";
+  return SR_ERR_INVALID_ARGUMENT;
+}
+
+}  // extern "C"

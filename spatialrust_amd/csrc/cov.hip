@@ -1,0 +1,443 @@
+// Exact k-NN and per-point plane covariances on a device uniform grid.
+//
+// k-NN path: the GPU replacement of `covariances` + `regularize`
+// (crates/spatialrust-registration/src/gicp.rs:259-291): for every point, its
+// k nearest neighbours including itself (KdTree::nearest_k semantics,
+// crates/spatialrust-search/src/kdtree.rs:306-346), the covariance of that set,
+// and the eigenvector n of its smallest eigenvalue. The regularized covariance
+// eps*v0v0^T + v1v1^T + v2v2^T equals I - (1-eps) n n^T because the eigenvectors
+// are orthonormal, so only n is stored (float4(n, 1)); the iteration kernel
+// rebuilds the 3x3 from it.
+//
+// Search: rings of cells around the query's cell (Chebyshev distance r = 0, 1,
+// ...), skipping cells whose box lies farther than the current k-th distance,
+// and stopping once every point outside the scanned block is provably farther
+// than the k-th neighbour. Neighbours are kept sorted by (d2, index) in
+// registers, so ties resolve to the lower index as in brute_force_knn
+// (crates/spatialrust-search/src/brute.rs:84-110).
+//
+// Radius path: COV_GRID_WGSL semantics (crates/spatialrust-gpu/src/kernels/
+// covariances_grid.rs:81-206) — cell = radius, 27-cell gather of d2 <= r2,
+// f32 two-pass moments, 16 cyclic Jacobi sweeps, eigenvalues (eps, 1, 1),
+// count < 3 gives eps*I — evaluated with the same operation order so the
+// result matches a CPU evaluation of that kernel bit for bit.
+
+#include <algorithm>
+
+#include "internal.hpp"
+
+namespace sr {
+
+template <int K>
+struct KList {
+  float d[K];
+  uint32_t id[K];   // original index (tie-break key)
+  uint32_t pos[K];  // grid position (for re-reading the point)
+  __device__ __forceinline__ void init() {
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      d[j] = INFINITY;
+      id[j] = kNone;
+      pos[j] = kNone;
+    }
+  }
+  // Bubble the candidate into its sorted place; the last entry falls off.
+  __device__ __forceinline__ void insert(float nd, uint32_t ni, uint32_t np) {
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      bool less = (nd < d[j]) || (nd == d[j] && ni < id[j]);
+      float td = less ? d[j] : nd;
+      uint32_t ti = less ? id[j] : ni;
+      uint32_t tp = less ? pos[j] : np;
+      d[j] = less ? nd : d[j];
+      id[j] = less ? ni : id[j];
+      pos[j] = less ? np : pos[j];
+      nd = td;
+      ni = ti;
+      np = tp;
+    }
+  }
+};
+
+// Exact k nearest neighbours (k <= K) of (px,py,pz) among the grid's points.
+// Returns the number found (min(k, npoints)); list entries beyond k are unused.
+template <int K>
+__device__ int knn_search(const GridDesc& g, const uint32_t* __restrict__ cs,
+                          const float4* __restrict__ gp, float px, float py, float pz, int k,
+                          KList<K>& L) {
+  L.init();
+  const int hx = cell_coord_free(px, g.ox, g.inv_h), hy = cell_coord_free(py, g.oy, g.inv_h),
+            hz = cell_coord_free(pz, g.oz, g.inv_h);
+  const int rmax = max(max(max(hx, g.dx - 1 - hx), max(hy, g.dy - 1 - hy)),
+                       max(hz, g.dz - 1 - hz));
+  // Rings closer than r0 hold no grid cell (query outside the grid box).
+  const int r0 = max(max(max(-hx, hx - (g.dx - 1)), max(-hy, hy - (g.dy - 1))),
+                     max(max(-hz, hz - (g.dz - 1)), 0));
+  float worst_d = INFINITY;
+  uint32_t worst_i = kNone;
+  for (int r = r0; r <= rmax; ++r) {
+    const int z0 = max(hz - r, 0), z1 = min(hz + r, g.dz - 1);
+    const int y0 = max(hy - r, 0), y1 = min(hy + r, g.dy - 1);
+    for (int cz = z0; cz <= z1; ++cz) {
+      const bool zedge = (cz == hz - r) || (cz == hz + r);
+      for (int cy = y0; cy <= y1; ++cy) {
+        const bool edge = zedge || (cy == hy - r) || (cy == hy + r);
+        const int step = edge ? 1 : 2 * r;
+        for (int cx = hx - r; cx <= hx + r; cx += (step > 0 ? step : 1)) {
+          if (cx < 0 || cx >= g.dx) continue;
+          if (cell_lb2(g, px, py, pz, cx, cy, cz) > worst_d) continue;
+          const uint32_t c = (uint32_t(cz) * uint32_t(g.dy) + uint32_t(cy)) * uint32_t(g.dx) +
+                             uint32_t(cx);
+          const uint32_t e = cs[c + 1];
+          for (uint32_t s = cs[c]; s < e; ++s) {
+            const float4 q = gp[s];
+            const float d2 = dist2(q.x, q.y, q.z, px, py, pz);
+            const uint32_t qi = __float_as_uint(q.w);
+            if (d2 < worst_d || (d2 == worst_d && qi < worst_i)) {
+              L.insert(d2, qi, s);
+#pragma unroll
+              for (int j = 0; j < K; ++j) {
+                if (j == k - 1) {
+                  worst_d = L.d[j];
+                  worst_i = L.id[j];
+                }
+              }
+            }
+          }
+        }
+      }
+    }
+    if (worst_i != kNone) {
+      const float cl = ring_clearance(g, px, py, pz, hx, hy, hz, r);
+      if (cl > 0.0f && worst_d < cl * cl) break;
+    }
+  }
+  return min(k, int(g.npoints));
+}
+
+// Smallest-eigenvalue eigenvector of a symmetric 3x3 (cyclic Jacobi, f32).
+__device__ __forceinline__ void smallest_eigvec(float a00, float a11, float a22, float a01,
+                                                float a02, float a12, float n[3]) {
+  float a[3][3] = {{a00, a01, a02}, {a01, a11, a12}, {a02, a12, a22}};
+  float v[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  const float scale = fabsf(a00) + fabsf(a11) + fabsf(a22) + 1e-30f;
+  for (int sweep = 0; sweep < 10; ++sweep) {
+    float off = fabsf(a[0][1]) + fabsf(a[0][2]) + fabsf(a[1][2]);
+    if (off <= 1e-9f * scale) break;
+#pragma unroll
+    for (int pq = 0; pq < 3; ++pq) {
+      const int p = pq == 2 ? 1 : 0, q = pq == 0 ? 1 : 2;
+      float apq = a[p][q];
+      if (fabsf(apq) < 1e-30f) continue;
+      float phi = 0.5f * (a[q][q] - a[p][p]) / apq;
+      float t = (phi >= 0.0f ? 1.0f : -1.0f) / (fabsf(phi) + sqrtf(1.0f + phi * phi));
+      float c = 1.0f / sqrtf(1.0f + t * t), s = t * c;
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        float arp = a[r][p], arq = a[r][q];
+        a[r][p] = c * arp - s * arq;
+        a[r][q] = s * arp + c * arq;
+      }
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        float apr = a[p][r], aqr = a[q][r];
+        a[p][r] = c * apr - s * aqr;
+        a[q][r] = s * apr + c * aqr;
+      }
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        float vrp = v[r][p], vrq = v[r][q];
+        v[r][p] = c * vrp - s * vrq;
+        v[r][q] = s * vrp + c * vrq;
+      }
+    }
+  }
+  int mi = 0;
+  if (a[1][1] < a[mi][mi]) mi = 1;
+  if (a[2][2] < a[mi][mi]) mi = 2;
+  float x = v[0][mi], y = v[1][mi], z = v[2][mi];
+  float len = sqrtf(x * x + y * y + z * z);
+  float inv = len > 0.0f ? 1.0f / len : 0.0f;
+  n[0] = x * inv;
+  n[1] = y * inv;
+  n[2] = z * inv;
+}
+
+template <int K>
+__global__ void __launch_bounds__(kBlock) k_knn_normals(GridDesc g, const uint32_t* __restrict__ cs,
+                                                        const float4* __restrict__ gp, int k,
+                                                        float one_minus_eps,
+                                                        float4* __restrict__ nrm_grid,
+                                                        float* __restrict__ cov6,
+                                                        float* __restrict__ normal3) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.npoints) return;
+  const float4 p = gp[i];
+  KList<K> L;
+  const int found = knn_search<K>(g, cs, gp, p.x, p.y, p.z, k, L);
+  float n[3] = {1.0f, 0.0f, 0.0f};  // count < 2: identity -> eigenvector e_x (gicp.rs:268-270)
+  if (found >= 2) {
+    // Mean and centred sums relative to the query point (f32, two passes).
+    float mx = 0.0f, my = 0.0f, mz = 0.0f;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      if (j < found) {
+        float4 q = gp[L.pos[j]];
+        mx += q.x - p.x;
+        my += q.y - p.y;
+        mz += q.z - p.z;
+      }
+    }
+    const float invn = 1.0f / float(found);
+    mx *= invn;
+    my *= invn;
+    mz *= invn;
+    float c00 = 0, c11 = 0, c22 = 0, c01 = 0, c02 = 0, c12 = 0;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      if (j < found) {
+        float4 q = gp[L.pos[j]];
+        float ex = (q.x - p.x) - mx, ey = (q.y - p.y) - my, ez = (q.z - p.z) - mz;
+        c00 += ex * ex;
+        c11 += ey * ey;
+        c22 += ez * ez;
+        c01 += ex * ey;
+        c02 += ex * ez;
+        c12 += ey * ez;
+      }
+    }
+    smallest_eigvec(c00, c11, c22, c01, c02, c12, n);
+  }
+  nrm_grid[i] = make_float4(n[0], n[1], n[2], 1.0f);
+  const uint32_t orig = __float_as_uint(p.w);
+  if (cov6) {
+    const float b = one_minus_eps;
+    float* o = cov6 + 6 * size_t(orig);
+    o[0] = 1.0f - b * n[0] * n[0];
+    o[1] = 1.0f - b * n[1] * n[1];
+    o[2] = 1.0f - b * n[2] * n[2];
+    o[3] = -b * n[0] * n[1];
+    o[4] = -b * n[0] * n[2];
+    o[5] = -b * n[1] * n[2];
+  }
+  if (normal3) {
+    normal3[3 * size_t(orig) + 0] = n[0];
+    normal3[3 * size_t(orig) + 1] = n[1];
+    normal3[3 * size_t(orig) + 2] = n[2];
+  }
+}
+
+template <int K>
+static void launch_knn_normals(const Grid& g, int k, float b, float4* nrm, float* cov6,
+                               float* normal3, hipStream_t s) {
+  const uint32_t n = g.desc.npoints;
+  ProfScope prof("knn_normals", s);
+  hipLaunchKernelGGL(k_knn_normals<K>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, g.desc,
+                     g.cell_start.p, g.pts.p, k, b, nrm, cov6, normal3);
+}
+
+void knn_normals(const Grid& g, uint32_t k, double eps, float4* nrm, float* cov6, float* normal3,
+                 hipStream_t s) {
+  if (g.desc.npoints == 0) return;
+  const float b = float(1.0 - eps);
+  if (k == 12) launch_knn_normals<12>(g, int(k), b, nrm, cov6, normal3, s);
+  else if (k == 15) launch_knn_normals<15>(g, int(k), b, nrm, cov6, normal3, s);
+  else if (k == 20) launch_knn_normals<20>(g, int(k), b, nrm, cov6, normal3, s);
+  else if (k <= 8) launch_knn_normals<8>(g, int(k), b, nrm, cov6, normal3, s);
+  else if (k <= 32) launch_knn_normals<32>(g, int(k), b, nrm, cov6, normal3, s);
+  else if (k <= 64) launch_knn_normals<64>(g, int(k), b, nrm, cov6, normal3, s);
+  else fail(3, "k_neighbors above 64 is not supported by the HIP covariance kernel");
+}
+
+// ------------------------------------------------------------ k-NN query ----
+
+template <int K>
+__global__ void __launch_bounds__(kBlock) k_knn_query(GridDesc g, const uint32_t* __restrict__ cs,
+                                                      const float4* __restrict__ gp,
+                                                      const float4* __restrict__ qs, uint32_t nq,
+                                                      int k, int32_t* __restrict__ out_idx,
+                                                      float* __restrict__ out_d2) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  const float4 q = qs[i];
+  KList<K> L;
+  const int found = knn_search<K>(g, cs, gp, q.x, q.y, q.z, k, L);
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    if (j < k) {
+      out_idx[size_t(i) * k + j] = j < found ? int32_t(L.id[j]) : -1;
+      out_d2[size_t(i) * k + j] = j < found ? L.d[j] : -1.0f;
+    }
+  }
+}
+
+void knn_query(const Grid& g, const float4* queries, uint32_t nq, uint32_t k, int32_t* out_idx,
+               float* out_d2, hipStream_t s) {
+  if (nq == 0 || k == 0) return;
+  const dim3 grid((nq + kBlock - 1) / kBlock);
+#define SR_KNNQ(KK)                                                                         \
+  hipLaunchKernelGGL(k_knn_query<KK>, grid, dim3(kBlock), 0, s, g.desc, g.cell_start.p, g.pts.p, \
+                     queries, nq, int(k), out_idx, out_d2)
+  if (k == 1) SR_KNNQ(1);
+  else if (k <= 8) SR_KNNQ(8);
+  else if (k <= 16) SR_KNNQ(16);
+  else if (k <= 32) SR_KNNQ(32);
+  else if (k <= 64) SR_KNNQ(64);
+  else fail(3, "k above 64 is not supported by the HIP k-NN kernel");
+#undef SR_KNNQ
+}
+
+// ------------------------------------------------------- radius (WGSL) ----
+
+__device__ __forceinline__ void wgsl_rotate(float a[3][3], float v[3][3], int p, int q) {
+  float apq = a[p][q];
+  if (fabsf(apq) < 1e-20f) return;
+  float app = a[p][p], aqq = a[q][q];
+  float phi = __fdiv_rn(__fmul_rn(0.5f, __fsub_rn(aqq, app)), apq);
+  float root = __fsqrt_rn(__fadd_rn(1.0f, __fmul_rn(phi, phi)));
+  float t = phi >= 0.0f ? __fdiv_rn(1.0f, __fadd_rn(phi, root))
+                        : __fdiv_rn(-1.0f, __fadd_rn(-phi, root));
+  float c = __fdiv_rn(1.0f, __fsqrt_rn(__fadd_rn(1.0f, __fmul_rn(t, t))));
+  float s = __fmul_rn(t, c);
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    float arp = a[r][p], arq = a[r][q];
+    a[r][p] = __fsub_rn(__fmul_rn(c, arp), __fmul_rn(s, arq));
+    a[r][q] = __fadd_rn(__fmul_rn(s, arp), __fmul_rn(c, arq));
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    float apr = a[p][r], aqr = a[q][r];
+    a[p][r] = __fsub_rn(__fmul_rn(c, apr), __fmul_rn(s, aqr));
+    a[q][r] = __fadd_rn(__fmul_rn(s, apr), __fmul_rn(c, aqr));
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    float vrp = v[r][p], vrq = v[r][q];
+    v[r][p] = __fsub_rn(__fmul_rn(c, vrp), __fmul_rn(s, vrq));
+    v[r][q] = __fadd_rn(__fmul_rn(s, vrp), __fmul_rn(c, vrq));
+  }
+}
+
+// One point per thread over a grid whose edge is the radius. Neighbours are
+// read from the grid-ordered copy (ascending index inside each cell, the order
+// build_grid produces), so the f32 sums happen in the reference's order.
+__global__ void __launch_bounds__(kBlock) k_radius_cov(GridDesc g, const uint32_t* __restrict__ cs,
+                                                       const float4* __restrict__ gp, float r2,
+                                                       float eps, float4* __restrict__ nrm_orig,
+                                                       float* __restrict__ cov6,
+                                                       float* __restrict__ normal3) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.npoints) return;
+  const float4 p = gp[i];
+  const uint32_t orig = __float_as_uint(p.w);
+  const int cx = cell_coord(p.x, g.ox, g.inv_h, g.dx), cy = cell_coord(p.y, g.oy, g.inv_h, g.dy),
+            cz = cell_coord(p.z, g.oz, g.inv_h, g.dz);
+  float mx = 0.0f, my = 0.0f, mz = 0.0f, count = 0.0f;
+  for (int dz = -1; dz <= 1; ++dz) {
+    int nz = cz + dz;
+    if (nz < 0 || nz >= g.dz) continue;
+    for (int dy = -1; dy <= 1; ++dy) {
+      int ny = cy + dy;
+      if (ny < 0 || ny >= g.dy) continue;
+      for (int dx = -1; dx <= 1; ++dx) {
+        int nx = cx + dx;
+        if (nx < 0 || nx >= g.dx) continue;
+        uint32_t c = (uint32_t(nz) * uint32_t(g.dy) + uint32_t(ny)) * uint32_t(g.dx) + uint32_t(nx);
+        for (uint32_t s = cs[c]; s < cs[c + 1]; ++s) {
+          float4 q = gp[s];
+          float ddx = __fsub_rn(q.x, p.x), ddy = __fsub_rn(q.y, p.y), ddz = __fsub_rn(q.z, p.z);
+          float d2 = __fadd_rn(__fadd_rn(__fmul_rn(ddx, ddx), __fmul_rn(ddy, ddy)), __fmul_rn(ddz, ddz));
+          if (d2 <= r2) {
+            mx = __fadd_rn(mx, q.x);
+            my = __fadd_rn(my, q.y);
+            mz = __fadd_rn(mz, q.z);
+            count = __fadd_rn(count, 1.0f);
+          }
+        }
+      }
+    }
+  }
+  float reg[6] = {eps, eps, eps, 0.0f, 0.0f, 0.0f};
+  float nv[3] = {0.0f, 0.0f, 0.0f};
+  float a_iso = eps;  // C = a*I - (1-eps) n n^T; eps*I when count < 3
+  if (count >= 3.0f) {
+    mx = __fdiv_rn(mx, count);
+    my = __fdiv_rn(my, count);
+    mz = __fdiv_rn(mz, count);
+    float c00 = 0, c11 = 0, c22 = 0, c01 = 0, c02 = 0, c12 = 0;
+    for (int dz = -1; dz <= 1; ++dz) {
+      int nz = cz + dz;
+      if (nz < 0 || nz >= g.dz) continue;
+      for (int dy = -1; dy <= 1; ++dy) {
+        int ny = cy + dy;
+        if (ny < 0 || ny >= g.dy) continue;
+        for (int dx = -1; dx <= 1; ++dx) {
+          int nx = cx + dx;
+          if (nx < 0 || nx >= g.dx) continue;
+          uint32_t c = (uint32_t(nz) * uint32_t(g.dy) + uint32_t(ny)) * uint32_t(g.dx) + uint32_t(nx);
+          for (uint32_t s = cs[c]; s < cs[c + 1]; ++s) {
+            float4 q = gp[s];
+            float rx = __fsub_rn(q.x, p.x), ry = __fsub_rn(q.y, p.y), rz = __fsub_rn(q.z, p.z);
+            float d2 = __fadd_rn(__fadd_rn(__fmul_rn(rx, rx), __fmul_rn(ry, ry)), __fmul_rn(rz, rz));
+            if (d2 <= r2) {
+              float ex = __fsub_rn(q.x, mx), ey = __fsub_rn(q.y, my), ez = __fsub_rn(q.z, mz);
+              c00 = __fadd_rn(c00, __fmul_rn(ex, ex));
+              c11 = __fadd_rn(c11, __fmul_rn(ey, ey));
+              c22 = __fadd_rn(c22, __fmul_rn(ez, ez));
+              c01 = __fadd_rn(c01, __fmul_rn(ex, ey));
+              c02 = __fadd_rn(c02, __fmul_rn(ex, ez));
+              c12 = __fadd_rn(c12, __fmul_rn(ey, ez));
+            }
+          }
+        }
+      }
+    }
+    float a[3][3] = {{c00, c01, c02}, {c01, c11, c12}, {c02, c12, c22}};
+    float v[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    for (int sweep = 0; sweep < 16; ++sweep) {
+      wgsl_rotate(a, v, 0, 1);
+      wgsl_rotate(a, v, 0, 2);
+      wgsl_rotate(a, v, 1, 2);
+    }
+    int mi = 0;
+    if (a[1][1] < a[mi][mi]) mi = 1;
+    if (a[2][2] < a[mi][mi]) mi = 2;
+    for (int k = 0; k < 6; ++k) reg[k] = 0.0f;
+    for (int col = 0; col < 3; ++col) {
+      float lam = col == mi ? eps : 1.0f;
+      float ax = v[0][col], ay = v[1][col], az = v[2][col];
+      reg[0] = __fadd_rn(reg[0], __fmul_rn(__fmul_rn(lam, ax), ax));
+      reg[1] = __fadd_rn(reg[1], __fmul_rn(__fmul_rn(lam, ay), ay));
+      reg[2] = __fadd_rn(reg[2], __fmul_rn(__fmul_rn(lam, az), az));
+      reg[3] = __fadd_rn(reg[3], __fmul_rn(__fmul_rn(lam, ax), ay));
+      reg[4] = __fadd_rn(reg[4], __fmul_rn(__fmul_rn(lam, ax), az));
+      reg[5] = __fadd_rn(reg[5], __fmul_rn(__fmul_rn(lam, ay), az));
+    }
+    nv[0] = v[0][mi];
+    nv[1] = v[1][mi];
+    nv[2] = v[2][mi];
+    a_iso = 1.0f;
+  }
+  if (nrm_orig) nrm_orig[orig] = make_float4(nv[0], nv[1], nv[2], a_iso);
+  if (cov6) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) cov6[6 * size_t(orig) + k] = reg[k];
+  }
+  if (normal3) {
+    normal3[3 * size_t(orig) + 0] = nv[0];
+    normal3[3 * size_t(orig) + 1] = nv[1];
+    normal3[3 * size_t(orig) + 2] = nv[2];
+  }
+}
+
+void radius_normals(const Grid& rg, float radius, float eps, float4* nrm_orig, float* cov6,
+                    float* normal3, hipStream_t s) {
+  const uint32_t n = rg.desc.npoints;
+  if (n == 0) return;
+  ProfScope prof("radius_cov", s);
+  hipLaunchKernelGGL(k_radius_cov, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, rg.desc,
+                     rg.cell_start.p, rg.pts.p, radius * radius, eps, nrm_orig, cov6, normal3);
+}
+
+}  // namespace sr

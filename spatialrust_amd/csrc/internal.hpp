@@ -1,0 +1,218 @@
+// Host-side internals shared by the .hip translation units of libsr_gicp.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstddef>
+#include <string>
+#include <vector>
+
+#include "sr_device.hpp"
+
+namespace sr {
+
+// Error raised by host code; carries the SpatialError status code.
+struct Error {
+  int32_t code;
+  std::string msg;
+};
+
+[[noreturn]] void throw_hip(hipError_t e, const char* what);
+#define SR_HIP(call)                               \
+  do {                                             \
+    hipError_t e_ = (call);                        \
+    if (e_ != hipSuccess) ::sr::throw_hip(e_, #call); \
+  } while (0)
+
+[[noreturn]] inline void fail(int32_t code, const std::string& msg) { throw Error{code, msg}; }
+
+// ---- device memory -------------------------------------------------------
+
+// Caching allocator: freed blocks go to a size-bucketed free list and are reused
+// by later calls of the same process (hipMalloc/hipFree are far too slow to sit
+// on the per-registration path).
+void* dev_alloc(size_t bytes);
+void dev_free(void* p);
+
+// RAII buffer from the caching allocator.
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t count) { resize(count); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      n = o.n;
+      o.p = nullptr;
+      o.n = 0;
+    }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void resize(size_t count) {
+    if (count <= n && p) return;
+    release();
+    p = static_cast<T*>(dev_alloc(count * sizeof(T) > 0 ? count * sizeof(T) : 16));
+    n = count;
+  }
+  void release() {
+    if (p) dev_free(p);
+    p = nullptr;
+    n = 0;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+// Per-thread stream (default priority) of the calling host thread.
+hipStream_t thread_stream();
+// Ensures the HIP device is initialised; throws SR_STATUS_UNAVAILABLE if not.
+void ensure_device();
+int device_cu_count();
+
+// ---- kernel timing (sr_profile_*) ------------------------------------------
+
+// When profiling is on, brackets a launch with HIP events on its stream and
+// files the pair under `name`. Cheap no-op when off.
+struct ProfScope {
+  int slot = -1;
+  ProfScope(const char* name, hipStream_t s);
+  ~ProfScope();
+  hipStream_t stream = nullptr;
+};
+
+// ---- scan ----------------------------------------------------------------
+
+// Exclusive prefix sum of n u32 values (in may equal out). If total is non-null
+// the grand total is written to *total (device pointer).
+void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total,
+                        hipStream_t s);
+
+// ---- point clouds --------------------------------------------------------
+
+// Device cloud packed as float4 (x, y, z, 0).
+struct Cloud {
+  DevBuf<float4> pts;
+  uint32_t n = 0;
+};
+
+// Uploads / packs an xyz view into a float4 device cloud.
+void pack_cloud(const float* x, const float* y, const float* z, uint32_t n, bool device_mem,
+                Cloud& out, hipStream_t s, uint64_t* h2d_bytes);
+
+// Axis-aligned bounds of a device float4 cloud: writes 6 ordered-u32 words
+// (minx,miny,minz,maxx,maxy,maxz) to dev_bounds.
+void cloud_bounds(const float4* pts, uint32_t n, uint32_t* dev_bounds, hipStream_t s);
+
+// ---- voxel downsample ----------------------------------------------------
+
+struct VoxelOut {
+  Cloud centroids;      // C cells, first-occurrence order
+  DevBuf<int64_t> keys; // 3*C (optional)
+  DevBuf<uint32_t> counts;
+  uint32_t cells = 0;
+};
+
+// Centroid voxel filter of a device cloud (voxel.rs:550-636 semantics).
+void voxel_downsample(const Cloud& in, float leaf, const float* origin, bool want_keys,
+                      VoxelOut& out, hipStream_t s);
+
+// ---- uniform grid --------------------------------------------------------
+
+struct Grid {
+  GridDesc desc{};
+  DevBuf<uint32_t> cell_start;  // ncells + 1
+  DevBuf<float4> pts;           // grid order; .w = original index (bit cast)
+  DevBuf<uint32_t> order;       // grid order -> original index
+};
+
+// Chooses a cell edge so that occupied cells hold about `occupancy` points.
+float choose_cell_edge(const Cloud& c, const uint32_t* host_bounds, float occupancy);
+// Builds a dense counting-sort grid with edge h over a device cloud (points in
+// each cell sorted by ascending original index, like build_grid).
+void build_grid(const Cloud& c, const uint32_t* host_bounds, float h, Grid& g, hipStream_t s);
+// Host copy of cloud bounds (6 ordered words) — synchronises the stream.
+void host_bounds(const Cloud& c, uint32_t* host6, hipStream_t s);
+
+// ---- covariances ---------------------------------------------------------
+
+// Exact k-NN normals of the grid's own points, in grid order: float4(n, a) with
+// C = a*I - (1-eps) n n^T (a = 1). If cov6 is non-null also writes the 6
+// regularized covariance entries per point in ORIGINAL index order.
+void knn_normals(const Grid& g, uint32_t k, double eps, float4* normals_grid_order, float* cov6,
+                 float* normal3, hipStream_t s);
+// Radius-grid covariances with COV_GRID_WGSL semantics over a grid of edge =
+// radius. normals in grid order of `g_out_order` (the GICP grid); cov6 in original order.
+void radius_normals(const Grid& rg, float radius, float eps, float4* normals_orig_order,
+                    float* cov6, float* normal3, hipStream_t s);
+// Permutes per-point float4 values from original order to grid order.
+void gather_to_grid_order(const Grid& g, const float4* orig, float4* out, hipStream_t s);
+
+// Exact k-NN of queries against a grid (sr_knn).
+void knn_query(const Grid& g, const float4* queries, uint32_t nq, uint32_t k, int32_t* out_idx,
+               float* out_d2, hipStream_t s);
+
+
+// ---- GICP iterations -----------------------------------------------------
+
+// Per-instance device state of the Gauss-Newton loop.
+struct InstState {
+  double q[4];       // pose rotation (x,y,z,w), FP64
+  double t[3];       // pose translation, FP64
+  float rf[9];       // f32 rotation matrix used by the per-point pass
+  float tf[3];
+  double fitness;
+  double last_update;
+  uint64_t count;    // correspondences of the last pass
+  uint32_t phase;    // 0 iterate, 1 final fitness pass pending, 2 done
+  uint32_t iter;     // Gauss-Newton iterations started
+  uint32_t status;   // 0 ok, 1 too few correspondences, 2 singular
+  uint32_t converged;
+  uint32_t counter;  // blocks of the current pass that have finished
+  uint32_t pad_;
+};
+
+struct InstConfig {
+  float dmax;       // max_correspondence_distance
+  float dmax2;      // dmax*dmax in f32 (gicp.rs:125-126)
+  float one_minus_eps;  // 1 - epsilon of the plane regularization
+  uint32_t max_iter;
+  uint32_t min_corr;
+  uint32_t pad_;
+  double trans_eps;
+  double fit_eps;
+};
+
+// One registration instance of a pass launch.
+struct IterInstance {
+  const float4* src;      // source points (any order; grid order for locality)
+  const float4* src_n;    // source normals float4(n, a)
+  const float4* tgt;      // target points in grid order
+  const float4* tgt_n;    // target normals in grid order
+  const uint32_t* cell_start;
+  GridDesc grid;
+  InstConfig cfg;
+  InstState* state;
+  float* partials;        // [nblocks][kTerms][2]
+  uint32_t n_src;
+  uint32_t block_begin;
+  uint32_t nblocks;
+  uint32_t pad_;
+};
+
+constexpr int kTerms = 28;  // 21 H (upper triangle) + 6 g + fitness sum
+
+// Launches one Gauss-Newton pass over `count` instances (device array `insts`),
+// total_blocks blocks; block_inst maps block -> instance.
+void gicp_pass(const IterInstance* insts, const uint32_t* block_inst, uint32_t total_blocks,
+               uint32_t pts_per_thread, hipStream_t s);
+// Initialises instance states from initial guesses (host array of 7 floats per instance).
+void init_states(InstState* states, const float* init7, uint32_t count, hipStream_t s);
+
+}  // namespace sr

@@ -1,0 +1,124 @@
+// Shared device-side definitions for the gfx950 GICP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sr {
+
+constexpr int kWave = 64;     // CDNA wavefront width
+constexpr int kBlock = 256;   // threads per workgroup for point-parallel kernels
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+// Uniform grid description. Cell (cx,cy,cz) has linear id (cz*dy + cy)*dx + cx,
+// the layout of build_grid (crates/spatialrust-search/src/uniform_grid.rs:56-64).
+struct GridDesc {
+  float ox, oy, oz;  // origin (per-axis minimum of the points)
+  float h;           // cell edge
+  float inv_h;       // 1/h in f32, as the reference computes inv_cell
+  int32_t dx, dy, dz;
+  uint32_t ncells;
+  uint32_t npoints;
+  float margin;  // slack covering f32 cell-assignment rounding (see cell_lb2)
+  float pad_[3];
+};
+
+// Order-preserving map of f32 onto u32 so atomicMin/atomicMax work on floats.
+__device__ __forceinline__ uint32_t f2ord(float f) {
+  uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__host__ __device__ __forceinline__ float ord2f(uint32_t u) {
+  uint32_t b = (u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u;
+#ifdef __HIP_DEVICE_COMPILE__
+  return __uint_as_float(b);
+#else
+  float f;
+  __builtin_memcpy(&f, &b, 4);
+  return f;
+#endif
+}
+
+// Cell coordinate with the clamping of build_grid (uniform_grid.rs:56-62):
+// floor((v - o) * inv) clamped to [0, dim-1].
+__device__ __forceinline__ int cell_coord(float v, float o, float inv, int dim) {
+  float f = floorf(__fmul_rn(__fsub_rn(v, o), inv));
+  int c;
+  if (!(f == f)) c = 0;
+  else if (f < -1.0e9f) c = -1000000000;
+  else if (f > 1.0e9f) c = 1000000000;
+  else c = (int)f;
+  return c < 0 ? 0 : (c >= dim ? dim - 1 : c);
+}
+
+// Squared distance exactly as squared_distance (kdtree.rs:544-549): three
+// separately rounded products summed left to right, no FMA.
+__device__ __forceinline__ float dist2(float px, float py, float pz, float qx, float qy, float qz) {
+  float dx = __fsub_rn(px, qx), dy = __fsub_rn(py, qy), dz = __fsub_rn(pz, qz);
+  return __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+}
+
+// Unclamped cell coordinate of an arbitrary query (may lie outside the grid).
+__device__ __forceinline__ int cell_coord_free(float v, float o, float inv) {
+  float f = floorf((v - o) * inv);
+  if (!(f == f)) return 0;
+  f = fminf(fmaxf(f, -1.0e8f), 1.0e8f);
+  return (int)f;
+}
+
+// Lower bound of the squared distance from p to any point filed in cell
+// (cx,cy,cz). The cell box is widened by g.margin on every side: a point's cell
+// comes from an f32 floor((v-o)*inv_h), whose rounding can file it up to a small
+// fraction of a cell outside the exact box, so the bound must not exceed the
+// distance of such a point.
+__device__ __forceinline__ float cell_lb2(const GridDesc& g, float px, float py, float pz, int cx,
+                                          int cy, int cz) {
+  float lo, d, acc;
+  lo = g.ox + (float)cx * g.h - g.margin;
+  d = fmaxf(fmaxf(lo - px, px - (lo + g.h + 2.0f * g.margin)), 0.0f);
+  acc = d * d;
+  lo = g.oy + (float)cy * g.h - g.margin;
+  d = fmaxf(fmaxf(lo - py, py - (lo + g.h + 2.0f * g.margin)), 0.0f);
+  acc += d * d;
+  lo = g.oz + (float)cz * g.h - g.margin;
+  d = fmaxf(fmaxf(lo - pz, pz - (lo + g.h + 2.0f * g.margin)), 0.0f);
+  acc += d * d;
+  return acc;
+}
+
+// Distance from p to the outside of the block of cells within Chebyshev ring r
+// of cell (hx,hy,hz), minus the filing margin: every point filed in a cell
+// beyond ring r is at least this far from p.
+__device__ __forceinline__ float ring_clearance(const GridDesc& g, float px, float py, float pz,
+                                               int hx, int hy, int hz, int r) {
+  float lo = g.ox + (float)(hx - r) * g.h, hi = g.ox + (float)(hx + r + 1) * g.h;
+  float m = fminf(px - lo, hi - px);
+  lo = g.oy + (float)(hy - r) * g.h;
+  hi = g.oy + (float)(hy + r + 1) * g.h;
+  m = fminf(m, fminf(py - lo, hi - py));
+  lo = g.oz + (float)(hz - r) * g.h;
+  hi = g.oz + (float)(hz + r + 1) * g.h;
+  m = fminf(m, fminf(pz - lo, hi - pz));
+  return m - g.margin;
+}
+
+// Kahan-compensated f32 accumulator.
+struct Kahan {
+  float s, c;
+  __device__ __forceinline__ void init() { s = 0.0f; c = 0.0f; }
+  __device__ __forceinline__ void add(float v) {
+    float y = v - c;
+    float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+};
+
+// Error-free transformation: a + b = s + e exactly.
+__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
+  s = a + b;
+  float bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+}  // namespace sr

@@ -1,0 +1,514 @@
+// Sort-free voxel-centroid downsample on gfx950, plus the small primitives it
+// needs (packing, bounds, exclusive scan).
+//
+// Semantics follow VoxelGridDownsample's centroid path for a plain xyz cloud
+// (crates/spatialrust-filtering/src/voxel.rs:251-368, 550-636, 1067-1121):
+//  - origin: the per-axis minimum (default) or an explicit origin;
+//  - key per axis: (v - origin) * inv_leaf truncated to u32 (default origin, when
+//    the span fits u32), truncated to i64 (default origin otherwise) or floored
+//    to i64 (explicit origin); inv_leaf = 1/leaf computed in f32 on the host;
+//  - cells in first-occurrence order (the reference's hash assigns ids on first
+//    insert);
+//  - centroid = (f32 sequential sum over the cell's points in input order) *
+//    (1/count).
+// The reference builds cells with a serial hash map; here every point inserts
+// its 64-bit linear key into an open-addressing table with atomics, the first
+// occurrence of a cell is the point holding the slot's minimum index, and an
+// exclusive scan over first-occurrence flags gives each cell its output rank.
+// Keys and occupancy are bit-exact; centroids are bit-exact too because each
+// cell's members are put back in index order before the f32 sum.
+
+#include "internal.hpp"
+
+namespace sr {
+
+// ---------------------------------------------------------------- utils ----
+
+__global__ void k_pack(const float* __restrict__ x, const float* __restrict__ y,
+                       const float* __restrict__ z, uint32_t n, float4* __restrict__ out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = make_float4(x[i], y[i], z[i], 0.0f);
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+__global__ void k_fill_u64(unsigned long long* p, unsigned long long v, size_t n) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+static unsigned fill_blocks(size_t n) {
+  size_t b = (n + kBlock - 1) / kBlock;
+  return unsigned(b < 1 ? 1 : (b > 4096 ? 4096 : b));
+}
+
+void fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_fill_u32, dim3(fill_blocks(n)), dim3(kBlock), 0, s, p, v, n);
+}
+
+__global__ void k_bounds_init(uint32_t* b) {
+  if (threadIdx.x < 3) b[threadIdx.x] = 0xFF800000u;        // f2ord(+inf)
+  else if (threadIdx.x < 6) b[threadIdx.x] = 0x007FFFFFu;   // f2ord(-inf)
+}
+
+// Per-axis min/max; NaN coordinates are ignored, like f32::min/max.
+__global__ void k_bounds(const float4* __restrict__ pts, uint32_t n, uint32_t* __restrict__ b) {
+  float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    float4 p = pts[i];
+    mn[0] = fminf(mn[0], p.x);
+    mn[1] = fminf(mn[1], p.y);
+    mn[2] = fminf(mn[2], p.z);
+    mx[0] = fmaxf(mx[0], p.x);
+    mx[1] = fmaxf(mx[1], p.y);
+    mx[2] = fmaxf(mx[2], p.z);
+  }
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    for (int a = 0; a < 3; ++a) {
+      mn[a] = fminf(mn[a], __shfl_xor(mn[a], off));
+      mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], off));
+    }
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    for (int a = 0; a < 3; ++a) {
+      if (mn[a] <= mx[a]) {
+        atomicMin(&b[a], f2ord(mn[a]));
+        atomicMax(&b[3 + a], f2ord(mx[a]));
+      }
+    }
+  }
+}
+
+void cloud_bounds(const float4* pts, uint32_t n, uint32_t* dev_bounds, hipStream_t s) {
+  hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, s, dev_bounds);
+  if (n == 0) return;
+  unsigned blocks = unsigned(std::min<size_t>((n + kBlock - 1) / kBlock, 1024));
+  hipLaunchKernelGGL(k_bounds, dim3(blocks), dim3(kBlock), 0, s, pts, n, dev_bounds);
+}
+
+void pack_cloud(const float* x, const float* y, const float* z, uint32_t n, bool device_mem,
+                Cloud& out, hipStream_t s, uint64_t* h2d_bytes) {
+  out.n = n;
+  out.pts.resize(n > 0 ? n : 1);
+  if (n == 0) return;
+  unsigned blocks = (n + kBlock - 1) / kBlock;
+  if (device_mem) {
+    hipLaunchKernelGGL(k_pack, dim3(blocks), dim3(kBlock), 0, s, x, y, z, n, out.pts.p);
+    return;
+  }
+  DevBuf<float> tmp(size_t(3) * n);
+  SR_HIP(hipMemcpyAsync(tmp.p, x, sizeof(float) * n, hipMemcpyHostToDevice, s));
+  SR_HIP(hipMemcpyAsync(tmp.p + n, y, sizeof(float) * n, hipMemcpyHostToDevice, s));
+  SR_HIP(hipMemcpyAsync(tmp.p + 2 * size_t(n), z, sizeof(float) * n, hipMemcpyHostToDevice, s));
+  if (h2d_bytes) *h2d_bytes += uint64_t(12) * n;
+  hipLaunchKernelGGL(k_pack, dim3(blocks), dim3(kBlock), 0, s, tmp.p, tmp.p + n, tmp.p + 2 * size_t(n),
+                     n, out.pts.p);
+  // tmp returns to the caching allocator; the copy and the pack are ordered on s,
+  // and any later reuse of the block is ordered after them on the same stream.
+  SR_HIP(hipStreamSynchronize(s));
+}
+
+// ----------------------------------------------------------------- scan ----
+
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;  // 2048
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* wave_tot,
+                                                         uint32_t& block_total) {
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  uint32_t incl = v;
+  for (int off = 1; off < kWave; off <<= 1) {
+    uint32_t u = __shfl_up(incl, off);
+    if (lane >= off) incl += u;
+  }
+  if (lane == kWave - 1) wave_tot[wid] = incl;
+  __syncthreads();
+  uint32_t pre = 0, tot = 0;
+  for (int w = 0; w < kBlock / kWave; ++w) {
+    uint32_t t = wave_tot[w];
+    if (w < wid) pre += t;
+    tot += t;
+  }
+  block_total = tot;
+  __syncthreads();
+  return pre + incl - v;
+}
+
+// Scans one tile of kScanTile values in place in LDS-staged form.
+__global__ void k_scan_tiles(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                             uint32_t n, uint32_t* __restrict__ tile_sums,
+                             const uint32_t* __restrict__ tile_offsets, uint32_t* total) {
+  __shared__ uint32_t tile[kScanTile];
+  __shared__ uint32_t wave_tot[kBlock / kWave];
+  const size_t base = size_t(blockIdx.x) * kScanTile;
+  for (int k = 0; k < kScanItems; ++k) {
+    size_t i = base + size_t(k) * kBlock + threadIdx.x;
+    tile[k * kBlock + threadIdx.x] = i < n ? in[i] : 0u;
+  }
+  __syncthreads();
+  uint32_t loc[kScanItems], sum = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    loc[k] = tile[threadIdx.x * kScanItems + k];
+    sum += loc[k];
+  }
+  uint32_t btot;
+  uint32_t pre = block_exclusive_scan(sum, wave_tot, btot);
+  if (tile_offsets) pre += tile_offsets[blockIdx.x];
+  for (int k = 0; k < kScanItems; ++k) {
+    tile[threadIdx.x * kScanItems + k] = pre;
+    pre += loc[k];
+  }
+  __syncthreads();
+  if (out) {
+    for (int k = 0; k < kScanItems; ++k) {
+      size_t i = base + size_t(k) * kBlock + threadIdx.x;
+      if (i < n) out[i] = tile[k * kBlock + threadIdx.x];
+    }
+  }
+  if (threadIdx.x == 0) {
+    if (tile_sums) tile_sums[blockIdx.x] = btot;
+    if (total && gridDim.x == 1) *total = btot;
+  }
+}
+
+// Sums of each tile only (first level of the multi-tile scan).
+__global__ void k_tile_sums(const uint32_t* __restrict__ in, uint32_t n,
+                            uint32_t* __restrict__ tile_sums) {
+  __shared__ uint32_t wave_tot[kBlock / kWave];
+  const size_t base = size_t(blockIdx.x) * kScanTile;
+  uint32_t sum = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    size_t i = base + size_t(k) * kBlock + threadIdx.x;
+    if (i < n) sum += in[i];
+  }
+  uint32_t btot;
+  (void)block_exclusive_scan(sum, wave_tot, btot);
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = btot;
+}
+
+void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total,
+                        hipStream_t s) {
+  if (n == 0) {
+    if (total) fill_u32(total, 0u, 1, s);
+    return;
+  }
+  uint32_t tiles = (n + kScanTile - 1) / kScanTile;
+  if (tiles == 1) {
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kBlock), 0, s, in, out, n, nullptr, nullptr,
+                       total);
+    return;
+  }
+  DevBuf<uint32_t> sums(tiles);
+  hipLaunchKernelGGL(k_tile_sums, dim3(tiles), dim3(kBlock), 0, s, in, n, sums.p);
+  exclusive_scan_u32(sums.p, sums.p, tiles, total, s);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(kBlock), 0, s, in, out, n, nullptr, sums.p,
+                     nullptr);
+}
+
+// ---------------------------------------------------------------- voxel ----
+
+struct VoxelParams {
+  float org[3];
+  float inv_leaf;
+  int32_t mode;   // 0: u32 trunc from min, 1: i64 trunc from min, 2: floor from origin
+  int32_t error;  // 1: linear key range does not fit 64 bits
+  int64_t kmin[3];
+  unsigned long long dims[3];
+};
+
+__device__ __forceinline__ int64_t f32_to_i64_sat(float v) {
+  if (!(v == v)) return 0;
+  if (v >= 9.2233720368547758e18f) return INT64_MAX;
+  if (v <= -9.2233720368547758e18f) return INT64_MIN;
+  return (int64_t)v;
+}
+
+__device__ __forceinline__ uint32_t f32_to_u32_sat(float v) {
+  if (!(v == v) || v <= 0.0f) return 0u;
+  if (v >= 4294967296.0f) return 0xFFFFFFFFu;
+  return (uint32_t)v;
+}
+
+__device__ __forceinline__ void voxel_key(const VoxelParams& P, float x, float y, float z,
+                                          int64_t k[3]) {
+  float f[3] = {__fmul_rn(__fsub_rn(x, P.org[0]), P.inv_leaf),
+                __fmul_rn(__fsub_rn(y, P.org[1]), P.inv_leaf),
+                __fmul_rn(__fsub_rn(z, P.org[2]), P.inv_leaf)};
+  for (int a = 0; a < 3; ++a) {
+    if (P.mode == 0) k[a] = int64_t(f32_to_u32_sat(f[a]));
+    else if (P.mode == 1) k[a] = f32_to_i64_sat(f[a]);
+    else k[a] = f32_to_i64_sat(floorf(f[a]));
+  }
+}
+
+// One thread: origin, key mode and key ranges from the bounds.
+__global__ void k_voxel_params(const uint32_t* __restrict__ b, float leaf_inv, int explicit_origin,
+                               float o0, float o1, float o2, VoxelParams* P) {
+  if (threadIdx.x != 0) return;
+  float mn[3] = {ord2f(b[0]), ord2f(b[1]), ord2f(b[2])};
+  float mx[3] = {ord2f(b[3]), ord2f(b[4]), ord2f(b[5])};
+  VoxelParams p;
+  p.inv_leaf = leaf_inv;
+  p.error = 0;
+  if (explicit_origin) {
+    p.org[0] = o0;
+    p.org[1] = o1;
+    p.org[2] = o2;
+    p.mode = 2;
+  } else {
+    for (int a = 0; a < 3; ++a) p.org[a] = mn[a];
+    const float limit = 4294967296.0f;  // fits_u32_voxel_key, voxel.rs:1081-1086
+    bool fits = __fmul_rn(__fsub_rn(mx[0], mn[0]), leaf_inv) <= limit &&
+                __fmul_rn(__fsub_rn(mx[1], mn[1]), leaf_inv) <= limit &&
+                __fmul_rn(__fsub_rn(mx[2], mn[2]), leaf_inv) <= limit;
+    p.mode = fits ? 0 : 1;
+  }
+  int64_t klo[3], khi[3];
+  voxel_key(p, mn[0], mn[1], mn[2], klo);
+  voxel_key(p, mx[0], mx[1], mx[2], khi);
+  double prod = 1.0;
+  for (int a = 0; a < 3; ++a) {
+    p.kmin[a] = klo[a];
+    long long span = khi[a] - klo[a];
+    p.dims[a] = (unsigned long long)(span < 0 ? 0 : span) + 1ull;
+    prod *= double(p.dims[a]);
+  }
+  if (!(prod < 1.8e19)) p.error = 1;
+  *P = p;
+}
+
+__device__ __forceinline__ uint32_t hash64(unsigned long long k, uint32_t mask) {
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdull;
+  k ^= k >> 33;
+  k *= 0xc4ceb9fe1a85ec53ull;
+  k ^= k >> 33;
+  return uint32_t(k) & mask;
+}
+
+constexpr unsigned long long kEmptyKey = ~0ull;
+
+__global__ void k_voxel_insert(const float4* __restrict__ pts, uint32_t n,
+                               const VoxelParams* __restrict__ Pp,
+                               unsigned long long* __restrict__ tkeys, uint32_t* __restrict__ tfirst,
+                               uint32_t* __restrict__ tcount, uint32_t mask,
+                               uint32_t* __restrict__ slot_of) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const VoxelParams P = *Pp;
+  if (P.error) return;
+  float4 p = pts[i];
+  int64_t k[3];
+  voxel_key(P, p.x, p.y, p.z, k);
+  unsigned long long lin =
+      ((unsigned long long)(k[2] - P.kmin[2]) * P.dims[1] + (unsigned long long)(k[1] - P.kmin[1])) *
+          P.dims[0] +
+      (unsigned long long)(k[0] - P.kmin[0]);
+  uint32_t h = hash64(lin, mask);
+  while (true) {
+    unsigned long long prev = atomicCAS(&tkeys[h], kEmptyKey, lin);
+    if (prev == kEmptyKey || prev == lin) break;
+    h = (h + 1) & mask;
+  }
+  atomicMin(&tfirst[h], i);
+  atomicAdd(&tcount[h], 1u);
+  slot_of[i] = h;
+}
+
+__global__ void k_voxel_flags(const uint32_t* __restrict__ slot_of,
+                              const uint32_t* __restrict__ tfirst, uint32_t n,
+                              uint32_t* __restrict__ flag) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flag[i] = tfirst[slot_of[i]] == i ? 1u : 0u;
+}
+
+__global__ void k_voxel_cellinfo(const float4* __restrict__ pts, const uint32_t* __restrict__ slot_of,
+                                 const uint32_t* __restrict__ flag, const uint32_t* __restrict__ rank,
+                                 const uint32_t* __restrict__ tcount, uint32_t n,
+                                 const VoxelParams* __restrict__ Pp,
+                                 uint32_t* __restrict__ cell_of_slot, uint32_t* __restrict__ ocount,
+                                 int64_t* __restrict__ okeys) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !flag[i]) return;
+  uint32_t c = rank[i], s = slot_of[i];
+  cell_of_slot[s] = c;
+  ocount[c] = tcount[s];
+  if (okeys) {
+    int64_t k[3];
+    float4 p = pts[i];
+    voxel_key(*Pp, p.x, p.y, p.z, k);
+    okeys[3 * size_t(c) + 0] = k[0];
+    okeys[3 * size_t(c) + 1] = k[1];
+    okeys[3 * size_t(c) + 2] = k[2];
+  }
+}
+
+__global__ void k_voxel_scatter(const uint32_t* __restrict__ slot_of,
+                                const uint32_t* __restrict__ cell_of_slot,
+                                const uint32_t* __restrict__ cstart, uint32_t* __restrict__ cursor,
+                                uint32_t n, uint32_t* __restrict__ members) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t c = cell_of_slot[slot_of[i]];
+  uint32_t pos = cstart[c] + atomicAdd(&cursor[c], 1u);
+  members[pos] = i;
+}
+
+constexpr uint32_t kSmallCell = 64;
+
+// Restores index order inside a segment (insertion sort; segments are short).
+__device__ __forceinline__ void sort_segment(uint32_t* seg, uint32_t cnt) {
+  for (uint32_t a = 1; a < cnt; ++a) {
+    uint32_t v = seg[a];
+    uint32_t b = a;
+    while (b > 0 && seg[b - 1] > v) {
+      seg[b] = seg[b - 1];
+      --b;
+    }
+    seg[b] = v;
+  }
+}
+
+__device__ __forceinline__ float4 cell_centroid(const float4* __restrict__ pts,
+                                                const uint32_t* seg, uint32_t cnt) {
+  float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+  for (uint32_t a = 0; a < cnt; ++a) {
+    float4 p = pts[seg[a]];
+    sx = __fadd_rn(sx, p.x);
+    sy = __fadd_rn(sy, p.y);
+    sz = __fadd_rn(sz, p.z);
+  }
+  float inv = __fdiv_rn(1.0f, (float)cnt);
+  return make_float4(__fmul_rn(sx, inv), __fmul_rn(sy, inv), __fmul_rn(sz, inv), 0.0f);
+}
+
+__global__ void k_voxel_centroids(const float4* __restrict__ pts, const uint32_t* __restrict__ ncells,
+                                  const uint32_t* __restrict__ cstart,
+                                  const uint32_t* __restrict__ ocount, uint32_t* __restrict__ members,
+                                  float4* __restrict__ out, uint32_t* __restrict__ big_list,
+                                  uint32_t* __restrict__ big_count) {
+  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= *ncells) return;
+  uint32_t cnt = ocount[c];
+  uint32_t* seg = members + cstart[c];
+  if (cnt > kSmallCell) {
+    big_list[atomicAdd(big_count, 1u)] = c;
+    return;
+  }
+  sort_segment(seg, cnt);
+  out[c] = cell_centroid(pts, seg, cnt);
+}
+
+// Large cells: one workgroup per cell sorts its member indices (bitonic in LDS
+// for up to kBigTile members, selection in global memory beyond) then one lane
+// sums in index order.
+constexpr uint32_t kBigTile = 4096;
+
+__global__ void k_voxel_big_cells(const float4* __restrict__ pts,
+                                  const uint32_t* __restrict__ cstart,
+                                  const uint32_t* __restrict__ ocount, uint32_t* __restrict__ members,
+                                  const uint32_t* __restrict__ big_list,
+                                  const uint32_t* __restrict__ big_count, float4* __restrict__ out) {
+  __shared__ uint32_t keys[kBigTile];
+  const uint32_t nbig = *big_count;
+  for (uint32_t b = blockIdx.x; b < nbig; b += gridDim.x) {
+    const uint32_t c = big_list[b];
+    const uint32_t cnt = ocount[c];
+    uint32_t* seg = members + cstart[c];
+    if (cnt <= kBigTile) {
+      uint32_t m = 1;
+      while (m < cnt) m <<= 1;
+      for (uint32_t a = threadIdx.x; a < m; a += blockDim.x) keys[a] = a < cnt ? seg[a] : 0xFFFFFFFFu;
+      __syncthreads();
+      for (uint32_t size = 2; size <= m; size <<= 1) {
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+          for (uint32_t a = threadIdx.x; a < m; a += blockDim.x) {
+            uint32_t partner = a ^ stride;
+            if (partner > a) {
+              bool up = (a & size) == 0;
+              uint32_t ka = keys[a], kb = keys[partner];
+              if ((ka > kb) == up) {
+                keys[a] = kb;
+                keys[partner] = ka;
+              }
+            }
+          }
+          __syncthreads();
+        }
+      }
+      for (uint32_t a = threadIdx.x; a < cnt; a += blockDim.x) seg[a] = keys[a];
+      __syncthreads();
+    } else if (threadIdx.x == 0) {
+      sort_segment(seg, cnt);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) out[c] = cell_centroid(pts, seg, cnt);
+    __syncthreads();
+  }
+}
+
+__global__ void k_error_check(const VoxelParams* P, uint32_t* flag_out) {
+  if (threadIdx.x == 0) *flag_out = uint32_t(P->error);
+}
+
+void voxel_downsample(const Cloud& in, float leaf, const float* origin, bool want_keys,
+                      VoxelOut& out, hipStream_t s) {
+  const uint32_t n = in.n;
+  out.cells = 0;
+  if (n == 0) return;
+  const float inv_leaf = 1.0f / leaf;  // voxel.rs:267, f32 on the host
+  DevBuf<uint32_t> bounds(8);
+  cloud_bounds(in.pts.p, n, bounds.p, s);
+  DevBuf<VoxelParams> params(1);
+  hipLaunchKernelGGL(k_voxel_params, dim3(1), dim3(64), 0, s, bounds.p, inv_leaf,
+                     origin ? 1 : 0, origin ? origin[0] : 0.0f, origin ? origin[1] : 0.0f,
+                     origin ? origin[2] : 0.0f, params.p);
+  uint32_t cap = 1024;
+  while (cap < 2u * n) cap <<= 1;
+  DevBuf<unsigned long long> tkeys(cap);
+  DevBuf<uint32_t> tfirst(cap), tcount(cap), cell_of_slot(cap);
+  hipLaunchKernelGGL(k_fill_u64, dim3(fill_blocks(cap)), dim3(kBlock), 0, s, tkeys.p, kEmptyKey,
+                     size_t(cap));
+  fill_u32(tfirst.p, 0xFFFFFFFFu, cap, s);
+  fill_u32(tcount.p, 0u, cap, s);
+  DevBuf<uint32_t> slot_of(n), flag(n), rank(n), ocount(n), cstart(n), cursor(n), members(n),
+      scal(4);
+  const unsigned blocks = (n + kBlock - 1) / kBlock;
+  ProfScope prof_insert("voxel_insert", s);
+  hipLaunchKernelGGL(k_voxel_insert, dim3(blocks), dim3(kBlock), 0, s, in.pts.p, n, params.p,
+                     tkeys.p, tfirst.p, tcount.p, cap - 1, slot_of.p);
+  hipLaunchKernelGGL(k_voxel_flags, dim3(blocks), dim3(kBlock), 0, s, slot_of.p, tfirst.p, n, flag.p);
+  exclusive_scan_u32(flag.p, rank.p, n, scal.p, s);  // scal[0] = cell count
+  fill_u32(ocount.p, 0u, n, s);
+  fill_u32(cursor.p, 0u, n, s);
+  if (want_keys) out.keys.resize(size_t(3) * n);
+  hipLaunchKernelGGL(k_voxel_cellinfo, dim3(blocks), dim3(kBlock), 0, s, in.pts.p, slot_of.p,
+                     flag.p, rank.p, tcount.p, n, params.p, cell_of_slot.p, ocount.p,
+                     want_keys ? out.keys.p : nullptr);
+  exclusive_scan_u32(ocount.p, cstart.p, n, nullptr, s);
+  hipLaunchKernelGGL(k_voxel_scatter, dim3(blocks), dim3(kBlock), 0, s, slot_of.p, cell_of_slot.p,
+                     cstart.p, cursor.p, n, members.p);
+  out.centroids.pts.resize(n);
+  fill_u32(scal.p + 1, 0u, 1, s);
+  DevBuf<uint32_t> big_list(n);
+  hipLaunchKernelGGL(k_voxel_centroids, dim3(blocks), dim3(kBlock), 0, s, in.pts.p, scal.p,
+                     cstart.p, ocount.p, members.p, out.centroids.pts.p, big_list.p, scal.p + 1);
+  hipLaunchKernelGGL(k_voxel_big_cells, dim3(256), dim3(kBlock), 0, s, in.pts.p, cstart.p,
+                     ocount.p, members.p, big_list.p, scal.p + 1, out.centroids.pts.p);
+  hipLaunchKernelGGL(k_error_check, dim3(1), dim3(64), 0, s, params.p, scal.p + 2);
+  uint32_t host[3];
+  SR_HIP(hipMemcpyAsync(host, scal.p, sizeof(host), hipMemcpyDeviceToHost, s));
+  SR_HIP(hipStreamSynchronize(s));
+  if (host[2]) fail(3, "voxel key range does not fit a 64-bit linear key; use a larger leaf_size");
+  out.cells = host[0];
+  out.centroids.n = host[0];
+  out.counts = std::move(ocount);
+}
+
+}  // namespace sr

@@ -1,0 +1,244 @@
+"""HIP path vs the CPU oracle, through the C ABI (gpu).
+
+Bars:
+  voxel keys / counts / order / centroids    bit-exact (integer and f32 sums in input order)
+  k-NN indices and squared distances         bit-exact (same f32 formula, ties to lower index)
+  radius covariances (WGSL semantics)        bit-exact vs the CPU evaluation of that kernel
+  k-NN plane covariances                     |C_gpu - C_fp64| <= 1e-5 (entries of a matrix of
+                                             norm 1) where the normal is well conditioned
+  GICP pose                                  <= 1e-4 rad and <= 1e-4 x scene extent vs the FP64
+                                             oracle (observed: ~1e-7)
+"""
+import numpy as np
+import pytest
+
+import spatialrust_amd as sr
+from spatialrust_amd import synth
+from tests import refinputs as ri
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu_voxel(pts, leaf, origin=None):
+    out = sr.VoxelGridDownsample(sr.VoxelGridDownsampleConfig(leaf, origin)).filter_with_policy(
+        pts, sr.ExecutionPolicy.gpu())
+    return out.points, out.keys, out.counts
+
+
+@pytest.mark.parametrize("case", ["fixture", "explicit_origin", "random", "small_cube", "big_cells"])
+def test_voxel_bit_exact(hip, orc, case):
+    rng = np.random.default_rng(11)
+    origin = None
+    if case == "fixture":
+        pts, leaf = np.array([[0, 0, 0], [0.1, 0, 0], [1.0, 0, 0], [1.1, 0, 0]], np.float32), 0.5
+    elif case == "explicit_origin":
+        pts, leaf, origin = (rng.random((20000, 3)) * 6 - 3).astype(np.float32), 0.25, (0.1, -0.2, 0.3)
+    elif case == "random":
+        pts, leaf = (rng.random((100000, 3)) * 10).astype(np.float32), 0.2
+    elif case == "small_cube":
+        pts, leaf = synth.small_cube_pair().target, 0.1
+    else:  # cells with hundreds to thousands of points (large-cell sort path)
+        pts, leaf = (rng.random((200000, 3)) * 4).astype(np.float32), 0.5
+    g = _gpu_voxel(pts, leaf, origin)
+    o = orc.voxel_centroid(pts, leaf, origin)
+    assert len(g[0]) == len(o[0])
+    assert np.array_equal(g[1], o[1]), "keys / first-occurrence order"
+    assert np.array_equal(g[2], o[2]), "occupancy"
+    assert np.array_equal(g[0], o[0]), "centroids"
+
+
+def test_voxel_default_origin_matches_explicit_min(hip):
+    pts = np.array([[-1.2, 0, 0], [-1.1, 0.1, 0], [-0.7, 0, 0], [0.4, 1, 0], [0.6, 1, 0]], np.float32)
+    a = _gpu_voxel(pts, 0.5)
+    b = _gpu_voxel(pts, 0.5, (-1.2, 0.0, 0.0))
+    assert np.array_equal(a[0], b[0])
+
+
+def test_voxel_errors_and_empty(hip):
+    with pytest.raises(sr.InvalidArgument, match="leaf_size must be greater than zero"):
+        _gpu_voxel(np.zeros((3, 3), np.float32), 0.0)
+    e = _gpu_voxel(np.zeros((0, 3), np.float32), 0.1)
+    assert len(e[0]) == 0
+
+
+def test_knn_matches_bruteforce_lcg(hip, orc):
+    # kdtree.rs:709-737 on the GPU grid: exact equality
+    pts, nxt = ri.lcg_points(0x12345678, 257)
+    for k in [1, 2, 5, 10, 33]:
+        qs = np.array([[nxt(), nxt(), nxt()] for _ in range(64)], np.float32)
+        idx, d2 = sr.knn(pts, qs, k)
+        for qi in range(64):
+            ib, db = orc.brute_knn(pts, qs[qi], k)
+            assert idx[qi].tolist() == ib.tolist(), (k, qi)
+            assert np.array_equal(d2[qi], db), (k, qi)
+
+
+def test_knn_large_cloud_matches_kdtree(hip, orc):
+    w = synth.small_cube_pair()
+    pts = w.target
+    qs = w.source[:2000]
+    idx, d2 = sr.knn(pts, qs, 20)
+    tree = orc.KdTree(pts)
+    ti, td = tree.knn_batch(qs, 20)
+    assert np.array_equal(d2, td)
+    assert np.array_equal(idx, ti.astype(np.int32))
+
+
+def test_knn_covariances_match_fp64_oracle(hip, orc):
+    w = synth.small_cube_pair()
+    pts, _, _ = orc.voxel_centroid(w.target, 0.1)
+    cov6, nrm = sr.plane_covariances(pts, 20)
+    ref9, rn = orc.covariances(pts, 20, 1e-3, mode="fp64")
+    g9 = np.zeros_like(ref9)
+    g9[:, 0, 0], g9[:, 1, 1], g9[:, 2, 2] = cov6[:, 0], cov6[:, 1], cov6[:, 2]
+    g9[:, 0, 1] = g9[:, 1, 0] = cov6[:, 3]
+    g9[:, 0, 2] = g9[:, 2, 0] = cov6[:, 4]
+    g9[:, 1, 2] = g9[:, 2, 1] = cov6[:, 5]
+    err = np.abs(g9 - ref9).reshape(len(pts), 9).max(1)
+    # conditioning of the normal: eigen-gap of the raw neighbourhood covariance
+    tree = orc.KdTree(pts)
+    idx, _ = tree.knn_batch(pts, 20)
+    nb = pts[idx].astype(np.float64)
+    c = np.einsum("nki,nkj->nij", nb - nb.mean(1, keepdims=True), nb - nb.mean(1, keepdims=True))
+    ev = np.linalg.eigvalsh(c)
+    gap = (ev[:, 1] - ev[:, 0]) / np.maximum(ev[:, 2], 1e-30)
+    good = gap > 1e-2
+    assert good.mean() > 0.95
+    assert err[good].max() <= 1e-5, float(err[good].max())
+    assert np.median(err) < 1e-6
+
+
+def test_radius_covariances_bit_exact(hip, orc):
+    w = synth.small_cube_pair()
+    pts, _, _ = orc.voxel_centroid(w.target, 0.1)
+    pts = pts[:20000]
+    g, gn = sr.plane_covariances(pts, radius=0.3)
+    o, on = orc.radius_covariances(pts, 0.3, 1e-3)
+    assert np.array_equal(g, o)
+
+
+def _align(src, tgt, **kw):
+    return sr.GicpRegistration(sr.GicpConfig(**kw)).align(src, tgt)
+
+
+def test_gicp_box_corner_reference_test(hip, orc):
+    # gicp.rs:465-489 through the HIP path
+    tgt = ri.box_corner()
+    mq = orc.quat_from_axis_angle([0, 0, 1], 0.06)
+    mt = np.array([0.015, -0.01, 0.012], np.float32)
+    src = orc.transform_points(mq, mt, tgt)
+    r = _align(src, tgt, max_correspondence_distance=0.2, max_iterations=40, k_neighbors=12)
+    assert r.fitness < 1e-4
+    cq, ct = orc.isometry_compose((np.array(r.transform.rotation, np.float32),
+                                   np.array(r.transform.translation, np.float32)), (mq, mt))
+    probe = np.array([[0.3, 0.4, 0.2]], np.float32)
+    assert np.all(np.abs(orc.transform_points(cq, ct, probe) - probe) < 1e-2)
+
+
+def test_gicp_gpu_covariance_variant(hip, orc):
+    # gicp.rs:498-525 (covariance_radius = 0.16)
+    tgt = ri.box_corner()
+    mq = orc.quat_from_axis_angle([0, 0, 1], 0.05)
+    mt = np.array([0.015, -0.01, 0.012], np.float32)
+    src = orc.transform_points(mq, mt, tgt)
+    r = _align(src, tgt, max_correspondence_distance=0.2, max_iterations=40, covariance_radius=0.16)
+    cq, ct = orc.isometry_compose((np.array(r.transform.rotation, np.float32),
+                                   np.array(r.transform.translation, np.float32)), (mq, mt))
+    probe = np.array([[0.3, 0.4, 0.2]], np.float32)
+    assert np.all(np.abs(orc.transform_points(cq, ct, probe) - probe) < 2e-2)
+
+
+def test_gicp_errors(hip):
+    cloud = ri.box_corner()
+    with pytest.raises(sr.InvalidArgument, match="k_neighbors >= 3"):
+        _align(cloud, cloud, k_neighbors=2)
+    with pytest.raises(sr.InvalidArgument, match="non-empty"):
+        _align(np.zeros((0, 3), np.float32), cloud)
+    with pytest.raises(sr.InvalidArgument, match=r"GICP found only 0 correspondences, minimum is 6"):
+        _align(cloud, cloud + np.float32(50.0), k_neighbors=12)
+
+
+def test_register_gicp_python_contract(hip):
+    # test_bindings.py:673-692: a sphere shifted +0.1 in x is pulled back (t_x < 0)
+    pts = ri.sphere_surface(300, 1.0)
+    res = sr.register_gicp(pts + np.array([0.1, 0, 0], np.float32), pts)
+    t = res.transform.matrix()
+    assert t.shape == (4, 4) and t.dtype == np.float32
+    assert t[0, 3] < 0.0
+    assert "RegistrationResult(" in repr(res)
+
+
+def _oracle_pose(orc, w, iters, mode="fp64"):
+    sv, _, _ = orc.voxel_centroid(w.source, w.voxel_leaf)
+    tv, _, _ = orc.voxel_centroid(w.target, w.voxel_leaf)
+    cfg = sr.GicpConfig(max_iterations=iters, k_neighbors=w.k_neighbors,
+                        max_correspondence_distance=w.max_correspondence_distance,
+                        transformation_epsilon=0.0, fitness_epsilon=0.0)._c()
+    return orc.gicp_align(sv, tv, cfg, mode=mode), len(sv), len(tv)
+
+
+def test_gicp_small_cube_matches_fp64_oracle(hip, orc):
+    w = synth.small_cube_pair()
+    ref, ns, nt = _oracle_pose(orc, w, 30)
+    r = _align(w.source, w.target, max_iterations=30, k_neighbors=20, max_correspondence_distance=1.0,
+               transformation_epsilon=0.0, fitness_epsilon=0.0, voxel_leaf=0.1)
+    assert r.source_points == ns and r.target_points == nt
+    assert r.iterations == 30
+    rot = synth.rotation_angle_between(r.rotation64, ref["q"])
+    tr = float(np.linalg.norm(r.translation64 - ref["t"]))
+    assert rot <= 1e-4, rot
+    assert tr <= 1e-4 * w.extent, tr
+    assert abs(r.fitness - ref["fitness"]) <= 1e-4 * ref["fitness"]
+
+
+def test_gicp_default_convergence_flags(hip, orc):
+    """Default epsilons: the FP64 device pose reaches transformation_epsilon
+    like the fp64 oracle (the reference's f32 pose floors at ~2e-7 and runs
+    all iterations)."""
+    w = synth.small_cube_pair()
+    sv, _, _ = orc.voxel_centroid(w.source, 0.1)
+    tv, _, _ = orc.voxel_centroid(w.target, 0.1)
+    cfg = sr.GicpConfig(max_iterations=30)
+    ref = orc.gicp_align(sv, tv, cfg._c(), mode="fp64")
+    r = sr.GicpRegistration(cfg).align(sv, tv)
+    assert r.converged == ref["converged"]
+    assert abs(r.iterations - ref["iterations"]) <= 1
+    assert synth.rotation_angle_between(r.rotation64, ref["q"]) <= 1e-4
+
+
+def test_gicp_deterministic(hip):
+    w = synth.small_cube_pair()
+    cfg = sr.GicpConfig(max_iterations=10, voxel_leaf=0.1, transformation_epsilon=0.0,
+                        fitness_epsilon=0.0)
+    a = sr.GicpRegistration(cfg).align(w.source, w.target)
+    b = sr.GicpRegistration(cfg).align(w.source, w.target)
+    assert np.array_equal(a.rotation64, b.rotation64) and np.array_equal(a.translation64, b.translation64)
+    assert a.fitness == b.fitness
+
+
+def test_batch_alone_equals_in_mix(hip):
+    """A registration's result is bit-identical alone and inside a batch."""
+    pairs = synth.batch_pairs(count=6, lo=5000, hi=30000)
+    cfg = sr.GicpConfig(max_iterations=15, voxel_leaf=0.1, transformation_epsilon=0.0,
+                        fitness_epsilon=0.0)
+    maps = [sr.TargetMap(p.target, cfg) for p in pairs]
+    mixed = sr.batch_align(maps, [p.source for p in pairs], cfg)
+    for i, p in enumerate(pairs):
+        alone = maps[i].align(p.source, cfg)
+        assert np.array_equal(alone.rotation64, mixed[i].rotation64), i
+        assert np.array_equal(alone.translation64, mixed[i].translation64), i
+        assert alone.fitness == mixed[i].fitness
+        assert synth.rotation_angle_between(alone.rotation64, p.gt_q) < 2e-3
+    for m in maps:
+        m.close()
+
+
+def test_receipt_records_stages_and_bytes(hip):
+    w = synth.small_cube_pair()
+    out = sr.GicpRegistration(sr.GicpConfig(max_iterations=3, voxel_leaf=0.1)).align_with_policy_and_receipt(
+        w.source, w.target, sr.ExecutionPolicy.auto())
+    rc = out.receipt
+    assert rc.resolved_policy == sr.ExecutionPolicy.gpu(sr.DeviceKind.HIP)
+    assert rc.requested_policy == sr.ExecutionPolicy.auto()
+    assert rc.host_to_device_bytes == 12 * (len(w.source) + len(w.target))
+    assert rc.stages == ("gicp-voxel", "gicp-grid", "gicp-covariance", "gicp-iterate", "gpu-readback")
