@@ -93,6 +93,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="large_block")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--pass-times", action="store_true",
+                    help="print each Gauss-Newton pass duration of the last registration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (registrations "
@@ -137,6 +139,12 @@ def main():
         lib.sr_profile_read(name.encode(), ctypes.byref(ms), ctypes.byref(cnt))
         kern[name] = (ms.value, cnt.value)
     pass_ms, pass_n = kern["gicp_pass"]
+    if args.pass_times and pass_n:
+        each = (ctypes.c_double * pass_n)()
+        lib.sr_profile_read_each(b"gicp_pass", each, pass_n)
+        per = w.max_iterations + 1
+        last = [round(each[i] * 1e3, 1) for i in range(pass_n - per, pass_n)]
+        print(f"per-pass us (last registration): {last}", file=sys.stderr, flush=True)
     avg_pass_s = (pass_ms / max(pass_n, 1)) * 1e-3
     algo_bytes = 72 * res.source_points  # source xyz + cov, gathered target xyz + cov
     achieved = algo_bytes / avg_pass_s / 1e9 if avg_pass_s > 0 else 0.0

@@ -139,6 +139,9 @@ void sr_profile_enable(int32_t on);
 void sr_profile_reset(void);
 /* Sum of event-measured durations (ms) and number of launches of `name`. */
 int32_t sr_profile_read(const char* name, double* total_ms, uint64_t* launches);
+/* Individual durations (ms) of up to `cap` launches of `name`, in launch order;
+ * returns the number recorded (or -1 on a HIP error). */
+int64_t sr_profile_read_each(const char* name, double* out_ms, uint64_t cap);
 
 /* ---- GICP (drop-in for GicpRegistration::align_with_diagnostics) ------- */
 

@@ -232,7 +232,7 @@ static float knn_occupancy(uint32_t k) { return std::max(4.0f, float(k) * 0.35f)
 
 // Voxel filter (optional) + grid + plane normals of one cloud.
 static void prepare(const sr_xyz_view& v, const sr_gicp_config& cfg, Prepared& out, Receipt& rc,
-                    hipStream_t s) {
+                    hipStream_t s, bool is_target) {
   if (v.n > 0xFFFFFFF0ull) fail(SR_ERR_INVALID_ARGUMENT, "point clouds above 4G points are not supported");
   Cloud raw;
   pack_cloud(v.x, v.y, v.z, uint32_t(v.n), v.mem == SR_MEM_DEVICE, raw, s, &rc.h2d);
@@ -248,6 +248,7 @@ static void prepare(const sr_xyz_view& v, const sr_gicp_config& cfg, Prepared& o
   host_bounds(*cloud, hb, s);
   const float h = choose_cell_edge(*cloud, hb, knn_occupancy(cfg.k_neighbors));
   build_grid(*cloud, hb, h, out.grid, s);
+  if (is_target) build_empty_radius(out.grid, s);
   rc.stages |= SR_STAGE_GRID;
   out.normals.resize(std::max<uint32_t>(out.n, 1));
   if (cfg.covariance_radius > 0.0f) {
@@ -322,6 +323,7 @@ static void batch_setup(const std::vector<const Prepared*>& src, const std::vect
     I.tgt = tgt[i]->grid.pts.p;
     I.tgt_n = tgt[i]->normals.p;
     I.cell_start = tgt[i]->grid.cell_start.p;
+    I.empty_radius = tgt[i]->grid.empty_radius.p;
     I.grid = tgt[i]->grid.desc;
     I.cfg = inst_config(cfgs[i]);
     I.state = B.states.p + i;
@@ -486,6 +488,23 @@ int32_t sr_profile_read(const char* name, double* total_ms, uint64_t* launches) 
   });
 }
 
+int64_t sr_profile_read_each(const char* name, double* out_ms, uint64_t cap) {
+  Profiler& P = profiler();
+  if (hipDeviceSynchronize() != hipSuccess) return -1;
+  std::lock_guard<std::mutex> lk(P.mu);
+  uint64_t cnt = 0;
+  for (auto& r : P.recs) {
+    if (r.name != name) continue;
+    if (cnt < cap && out_ms) {
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, r.a, r.b) != hipSuccess) return -1;
+      out_ms[cnt] = ms;
+    }
+    ++cnt;
+  }
+  return int64_t(cnt);
+}
+
 int32_t sr_device_available(void) { return device_state().status == 0 ? 1 : 0; }
 
 int32_t sr_version(void) { return 100; }
@@ -520,8 +539,8 @@ int32_t sr_gicp_align(const sr_xyz_view* source, const sr_xyz_view* target,
     hipStream_t s = thread_stream();
     Receipt rc;
     Prepared tgt, src;
-    prepare(*target, *config, tgt, rc, s);
-    prepare(*source, *config, src, rc, s);
+    prepare(*target, *config, tgt, rc, s, true);
+    prepare(*source, *config, src, rc, s, false);
     Batch B;
     batch_setup({&src}, {&tgt}, {*config}, B, s);
     batch_run(B, s);
@@ -664,7 +683,7 @@ int32_t sr_map_create(const sr_xyz_view* target, const sr_gicp_config* config, s
     auto m = std::make_unique<sr_map>();
     m->cfg = *config;
     Receipt rc;
-    prepare(*target, *config, m->tgt, rc, s);
+    prepare(*target, *config, m->tgt, rc, s, true);
     SR_HIP(hipStreamSynchronize(s));
     *out = m.release();
   });
@@ -695,7 +714,7 @@ int32_t sr_map_align(const sr_map* map, const sr_xyz_view* source, const sr_gicp
     hipStream_t s = thread_stream();
     Receipt rc;
     Prepared src;
-    prepare(*source, *config, src, rc, s);
+    prepare(*source, *config, src, rc, s, false);
     Batch B;
     batch_setup({&src}, {&map->tgt}, {*config}, B, s);
     batch_run(B, s);
@@ -724,7 +743,7 @@ int32_t sr_batch_align(const sr_map* const* maps, const sr_xyz_view* sources, ui
       validate_view(&sources[i]);
       const sr_xyz_view dummy{nullptr, nullptr, nullptr, maps[i]->tgt.n, 0};
       validate_cfg(&sources[i], &dummy, config);
-      prepare(sources[i], *config, srcs[i], rc, s);
+      prepare(sources[i], *config, srcs[i], rc, s, false);
       sp[i] = &srcs[i];
       tp[i] = &maps[i]->tgt;
     }

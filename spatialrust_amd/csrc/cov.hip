@@ -76,37 +76,24 @@ __device__ int knn_search(const GridDesc& g, const uint32_t* __restrict__ cs,
   float worst_d = INFINITY;
   uint32_t worst_i = kNone;
   for (int r = r0; r <= rmax; ++r) {
-    const int z0 = max(hz - r, 0), z1 = min(hz + r, g.dz - 1);
-    const int y0 = max(hy - r, 0), y1 = min(hy + r, g.dy - 1);
-    for (int cz = z0; cz <= z1; ++cz) {
-      const bool zedge = (cz == hz - r) || (cz == hz + r);
-      for (int cy = y0; cy <= y1; ++cy) {
-        const bool edge = zedge || (cy == hy - r) || (cy == hy + r);
-        const int step = edge ? 1 : 2 * r;
-        for (int cx = hx - r; cx <= hx + r; cx += (step > 0 ? step : 1)) {
-          if (cx < 0 || cx >= g.dx) continue;
-          if (cell_lb2(g, px, py, pz, cx, cy, cz) > worst_d) continue;
-          const uint32_t c = (uint32_t(cz) * uint32_t(g.dy) + uint32_t(cy)) * uint32_t(g.dx) +
-                             uint32_t(cx);
-          const uint32_t e = cs[c + 1];
-          for (uint32_t s = cs[c]; s < e; ++s) {
-            const float4 q = gp[s];
-            const float d2 = dist2(q.x, q.y, q.z, px, py, pz);
-            const uint32_t qi = __float_as_uint(q.w);
-            if (d2 < worst_d || (d2 == worst_d && qi < worst_i)) {
-              L.insert(d2, qi, s);
+    scan_ring(g, px, py, pz, hx, hy, hz, r, worst_d, [&](uint32_t c) {
+      const uint32_t e = cs[c + 1];
+      for (uint32_t s = cs[c]; s < e; ++s) {
+        const float4 q = gp[s];
+        const float d2 = dist2(q.x, q.y, q.z, px, py, pz);
+        const uint32_t qi = __float_as_uint(q.w);
+        if (d2 < worst_d || (d2 == worst_d && qi < worst_i)) {
+          L.insert(d2, qi, s);
 #pragma unroll
-              for (int j = 0; j < K; ++j) {
-                if (j == k - 1) {
-                  worst_d = L.d[j];
-                  worst_i = L.id[j];
-                }
-              }
+          for (int j = 0; j < K; ++j) {
+            if (j == k - 1) {
+              worst_d = L.d[j];
+              worst_i = L.id[j];
             }
           }
         }
       }
-    }
+    });
     if (worst_i != kNone) {
       const float cl = ring_clearance(g, px, py, pz, hx, hy, hz, r);
       if (cl > 0.0f && worst_d < cl * cl) break;
@@ -294,10 +281,12 @@ __device__ __forceinline__ void wgsl_rotate(float a[3][3], float v[3][3], int p,
   if (fabsf(apq) < 1e-20f) return;
   float app = a[p][p], aqq = a[q][q];
   float phi = __fdiv_rn(__fmul_rn(0.5f, __fsub_rn(aqq, app)), apq);
-  float root = __fsqrt_rn(__fadd_rn(1.0f, __fmul_rn(phi, phi)));
+  // sqrtf, not __fsqrt_rn: on gfx950 the latter is not correctly rounded, while
+  // sqrtf is under the default correctly-rounded divide/sqrt mode.
+  float root = sqrtf(__fadd_rn(1.0f, __fmul_rn(phi, phi)));
   float t = phi >= 0.0f ? __fdiv_rn(1.0f, __fadd_rn(phi, root))
                         : __fdiv_rn(-1.0f, __fadd_rn(-phi, root));
-  float c = __fdiv_rn(1.0f, __fsqrt_rn(__fadd_rn(1.0f, __fmul_rn(t, t))));
+  float c = __fdiv_rn(1.0f, sqrtf(__fadd_rn(1.0f, __fmul_rn(t, t))));
   float s = __fmul_rn(t, c);
 #pragma unroll
   for (int r = 0; r < 3; ++r) {

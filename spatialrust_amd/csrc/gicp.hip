@@ -60,8 +60,16 @@ __device__ __forceinline__ void refresh_f32_pose(InstState* st) {
   double R[3][3];
   q_to_mat64(st->q, R);
   for (int a = 0; a < 3; ++a)
-    for (int b = 0; b < 3; ++b) st->rf[3 * a + b] = float(R[a][b]);
-  for (int a = 0; a < 3; ++a) st->tf[a] = float(st->t[a]);
+    for (int b = 0; b < 3; ++b) {
+      const float hi = float(R[a][b]);
+      st->rf[3 * a + b] = hi;
+      st->rlo[3 * a + b] = float(R[a][b] - double(hi));
+    }
+  for (int a = 0; a < 3; ++a) {
+    const float hi = float(st->t[a]);
+    st->tf[a] = hi;
+    st->tlo[a] = float(st->t[a] - double(hi));
+  }
 }
 
 __global__ void k_init_states(InstState* states, const float* __restrict__ init7, uint32_t count) {
@@ -205,6 +213,7 @@ __device__ __noinline__ void finalize(InstState* st, const InstConfig& cfg, cons
 // Nearest target point of p within dmax (exact; ties to the lower index).
 // Returns the grid position or kNone.
 __device__ __forceinline__ uint32_t nearest_within(const GridDesc& g, const uint32_t* __restrict__ cs,
+                                                   const uint8_t* __restrict__ empty,
                                                    const float4* __restrict__ tp, float px, float py,
                                                    float pz, float dmax, float dmax2,
                                                    float& best_d) {
@@ -221,37 +230,33 @@ __device__ __forceinline__ uint32_t nearest_within(const GridDesc& g, const uint
   const int hx = cell_coord_free(px, g.ox, g.inv_h), hy = cell_coord_free(py, g.oy, g.inv_h),
             hz = cell_coord_free(pz, g.oz, g.inv_h);
   const int rlim = int(dmax * g.inv_h) + 2;
-  for (int r = 0; r <= rlim; ++r) {
-    const float thr = fminf(best_d, dmax2);
-    const int z0 = max(hz - r, 0), z1 = min(hz + r, g.dz - 1);
-    const int y0 = max(hy - r, 0), y1 = min(hy + r, g.dy - 1);
-    for (int cz = z0; cz <= z1; ++cz) {
-      const bool zedge = (cz == hz - r) || (cz == hz + r);
-      for (int cy = y0; cy <= y1; ++cy) {
-        const bool edge = zedge || (cy == hy - r) || (cy == hy + r);
-        const int step = (edge || r == 0) ? 1 : 2 * r;
-        for (int cx = hx - r; cx <= hx + r; cx += step) {
-          if (cx < 0 || cx >= g.dx) continue;
-          if (cell_lb2(g, px, py, pz, cx, cy, cz) > fminf(best_d, dmax2)) continue;
-          const uint32_t c = (uint32_t(cz) * uint32_t(g.dy) + uint32_t(cy)) * uint32_t(g.dx) +
-                             uint32_t(cx);
-          const uint32_t e = cs[c + 1];
-          for (uint32_t s = cs[c]; s < e; ++s) {
-            const float4 q = tp[s];
-            const float d2 = dist2(px, py, pz, q.x, q.y, q.z);
-            const uint32_t qi = __float_as_uint(q.w);
-            if (d2 <= dmax2 && (d2 < best_d || (d2 == best_d && qi < best_id))) {
-              best_d = d2;
-              best_id = qi;
-              best_pos = s;
-            }
-          }
+  int rstart = 0;
+  if (empty && hx >= 0 && hx < g.dx && hy >= 0 && hy < g.dy && hz >= 0 && hz < g.dz) {
+    // Rings closer than the empty radius hold no point; a point in ring e is at
+    // least (e - 1) cells away.
+    const int e = empty[(uint32_t(hz) * uint32_t(g.dy) + uint32_t(hy)) * uint32_t(g.dx) + uint32_t(hx)];
+    if (float(e - 1) * g.h - g.margin > dmax) return kNone;
+    rstart = e;
+  }
+  // thr = min(best so far, dmax^2): cells and rows beyond it are never touched.
+  float thr = dmax2;
+  for (int r = rstart; r <= rlim; ++r) {
+    scan_ring(g, px, py, pz, hx, hy, hz, r, thr, [&](uint32_t c) {
+      const uint32_t e = cs[c + 1];
+      for (uint32_t s = cs[c]; s < e; ++s) {
+        const float4 q = tp[s];
+        const float d2 = dist2(px, py, pz, q.x, q.y, q.z);
+        const uint32_t qi = __float_as_uint(q.w);
+        if (d2 <= dmax2 && (d2 < best_d || (d2 == best_d && qi < best_id))) {
+          best_d = d2;
+          best_id = qi;
+          best_pos = s;
+          thr = d2;
         }
       }
-    }
-    (void)thr;
+    });
     const float cl = ring_clearance(g, px, py, pz, hx, hy, hz, r);
-    if (cl > 0.0f && fminf(best_d, dmax2) < cl * cl) break;
+    if (cl > 0.0f && thr < cl * cl) break;
   }
   return best_pos;
 }
@@ -325,11 +330,17 @@ __global__ void __launch_bounds__(kBlock) k_gicp_pass(const IterInstance* __rest
   const GridDesc g = I.grid;
   const InstConfig cfg = I.cfg;
   const float beta = cfg.one_minus_eps, eps = 1.0f - cfg.one_minus_eps;
-  float R[9], T[3];
+  float R[9], T[3], Rl[9], Tl[3];
 #pragma unroll
-  for (int a = 0; a < 9; ++a) R[a] = st->rf[a];
+  for (int a = 0; a < 9; ++a) {
+    R[a] = st->rf[a];
+    Rl[a] = st->rlo[a];
+  }
 #pragma unroll
-  for (int a = 0; a < 3; ++a) T[a] = st->tf[a];
+  for (int a = 0; a < 3; ++a) {
+    T[a] = st->tf[a];
+    Tl[a] = st->tlo[a];
+  }
 
   // Plain f32 per-thread sums (a thread holds only ppt points); compensation
   // happens in the block reduction below.
@@ -342,12 +353,19 @@ __global__ void __launch_bounds__(kBlock) k_gicp_pass(const IterInstance* __rest
     const uint32_t i = (lb * ppt + k) * kBlock + threadIdx.x;
     if (i >= I.n_src) break;
     const float4 p = I.src[i];
-    // Isometry3::transform_point in f32 (transform.rs:126-128, mat.rs:46-52).
-    const float px = R[0] * p.x + R[1] * p.y + R[2] * p.z + T[0];
-    const float py = R[3] * p.x + R[4] * p.y + R[5] * p.z + T[1];
-    const float pz = R[6] * p.x + R[7] * p.y + R[8] * p.z + T[2];
+    // p' = R p + t (Isometry3::transform_point, transform.rs:126-128) with the
+    // FP64 pose split into f32 high and low parts: the low part removes the
+    // systematic error of an f32-rounded pose, which would otherwise floor the
+    // Gauss-Newton update at ~1e-7 (the reference's own f32 pose has that floor).
+    const float lx = fmaf(Rl[0], p.x, fmaf(Rl[1], p.y, fmaf(Rl[2], p.z, Tl[0])));
+    const float ly = fmaf(Rl[3], p.x, fmaf(Rl[4], p.y, fmaf(Rl[5], p.z, Tl[1])));
+    const float lz = fmaf(Rl[6], p.x, fmaf(Rl[7], p.y, fmaf(Rl[8], p.z, Tl[2])));
+    const float px = fmaf(R[0], p.x, fmaf(R[1], p.y, fmaf(R[2], p.z, T[0] + lx)));
+    const float py = fmaf(R[3], p.x, fmaf(R[4], p.y, fmaf(R[5], p.z, T[1] + ly)));
+    const float pz = fmaf(R[6], p.x, fmaf(R[7], p.y, fmaf(R[8], p.z, T[2] + lz)));
     float d2;
-    const uint32_t j = nearest_within(g, I.cell_start, I.tgt, px, py, pz, cfg.dmax, cfg.dmax2, d2);
+    const uint32_t j =
+        nearest_within(g, I.cell_start, I.empty_radius, I.tgt, px, py, pz, cfg.dmax, cfg.dmax2, d2);
     if (j == kNone) continue;
     const float4 q = I.tgt[j];
     const float4 nt = I.tgt_n[j];
@@ -453,40 +471,56 @@ __global__ void __launch_bounds__(kBlock) k_gicp_pass(const IterInstance* __rest
   __syncthreads();
   if (!s_last) return;
   __threadfence();
-  __shared__ double dsum[kTerms][8];
-  __shared__ unsigned long long dcnt[8];
-  const uint32_t nb = I.nblocks;
-  if (threadIdx.x < kTerms * 8) {
-    const int t = threadIdx.x >> 3, ch = threadIdx.x & 7;
-    const uint32_t b0 = uint32_t((uint64_t(nb) * ch) / 8), b1 = uint32_t((uint64_t(nb) * (ch + 1)) / 8);
-    double s = 0.0;
-    for (uint32_t bb = b0; bb < b1; ++bb) {
-      const float* pp = I.partials + size_t(bb) * 64;
-      s += double(__builtin_nontemporal_load(pp + 2 * t)) +
-           double(__builtin_nontemporal_load(pp + 2 * t + 1));
-    }
-    dsum[t][ch] = s;
-  } else if (threadIdx.x < kTerms * 8 + 8) {
-    const int ch = threadIdx.x - kTerms * 8;
-    const uint32_t b0 = uint32_t((uint64_t(nb) * ch) / 8), b1 = uint32_t((uint64_t(nb) * (ch + 1)) / 8);
-    unsigned long long s = 0;
-    for (uint32_t bb = b0; bb < b1; ++bb)
-      s += __builtin_nontemporal_load(
-          reinterpret_cast<const unsigned long long*>(I.partials + size_t(bb) * 64) + 30);
-    dcnt[ch] = s;
-  }
+  // FP64 sum of the block partials. Every thread takes blocks tid, tid+256, ...
+  // (many loads in flight), then a fixed xor-butterfly per wave and a fixed
+  // wave order: the same order on every run, whatever the launch mix.
+  __shared__ double wsum[kBlock / kWave][4];
+  __shared__ unsigned long long wcnt2[kBlock / kWave];
   __shared__ double dtot[kTerms];
   __shared__ double work[64];
-  __syncthreads();
-  if (threadIdx.x < kTerms) {
-    double s = 0.0;
-    for (int ch = 0; ch < 8; ++ch) s += dsum[threadIdx.x][ch];
-    dtot[threadIdx.x] = s;
+  const uint32_t nb = I.nblocks;
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  for (int t0 = 0; t0 < kTerms; t0 += 4) {
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+    for (uint32_t bb = threadIdx.x; bb < nb; bb += kBlock) {
+      const float4* pp = reinterpret_cast<const float4*>(I.partials + size_t(bb) * 64 + 2 * t0);
+      const float4 a = pp[0], c = pp[1];
+      v0 += double(a.x) + double(a.y);
+      v1 += double(a.z) + double(a.w);
+      v2 += double(c.x) + double(c.y);
+      v3 += double(c.z) + double(c.w);
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+      v0 += __shfl_xor(v0, off);
+      v1 += __shfl_xor(v1, off);
+      v2 += __shfl_xor(v2, off);
+      v3 += __shfl_xor(v3, off);
+    }
+    if (lane == 0) {
+      wsum[wid][0] = v0;
+      wsum[wid][1] = v1;
+      wsum[wid][2] = v2;
+      wsum[wid][3] = v3;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+      double s = 0.0;
+      for (int w = 0; w < kBlock / kWave; ++w) s += wsum[w][threadIdx.x];
+      dtot[t0 + threadIdx.x] = s;
+    }
+    __syncthreads();
+  }
+  {
+    unsigned long long cs = 0;
+    for (uint32_t bb = threadIdx.x; bb < nb; bb += kBlock)
+      cs += reinterpret_cast<const unsigned long long*>(I.partials + size_t(bb) * 64)[30];
+    for (int off = kWave / 2; off > 0; off >>= 1) cs += __shfl_xor(cs, off);
+    if (lane == 0) wcnt2[wid] = cs;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     unsigned long long cnt = 0;
-    for (int ch = 0; ch < 8; ++ch) cnt += dcnt[ch];
+    for (int w = 0; w < kBlock / kWave; ++w) cnt += wcnt2[w];
     finalize(st, cfg, dtot, dtot + 21, dtot[27], cnt, work);
     st->counter = 0;
     __threadfence();

@@ -60,15 +60,22 @@ __global__ void k_grid_count(const float4* __restrict__ pts, uint32_t n, GridDes
                              uint32_t* __restrict__ counts, uint32_t* __restrict__ cid,
                              uint32_t* __restrict__ nonempty) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  float4 p = pts[i];
-  int cx = cell_coord(p.x, g.ox, g.inv_h, g.dx);
-  int cy = cell_coord(p.y, g.oy, g.inv_h, g.dy);
-  int cz = cell_coord(p.z, g.oz, g.inv_h, g.dz);
-  uint32_t c = (uint32_t(cz) * uint32_t(g.dy) + uint32_t(cy)) * uint32_t(g.dx) + uint32_t(cx);
-  uint32_t old = atomicAdd(&counts[c], 1u);
-  if (cid) cid[i] = c;
-  if (nonempty && old == 0) atomicAdd(nonempty, 1u);
+  bool first = false;
+  if (i < n) {
+    float4 p = pts[i];
+    int cx = cell_coord(p.x, g.ox, g.inv_h, g.dx);
+    int cy = cell_coord(p.y, g.oy, g.inv_h, g.dy);
+    int cz = cell_coord(p.z, g.oz, g.inv_h, g.dz);
+    uint32_t c = (uint32_t(cz) * uint32_t(g.dy) + uint32_t(cy)) * uint32_t(g.dx) + uint32_t(cx);
+    uint32_t old = atomicAdd(&counts[c], 1u);
+    if (cid) cid[i] = c;
+    first = old == 0;
+  }
+  if (nonempty) {
+    // one atomic per wave instead of one per newly touched cell
+    const unsigned long long m = __ballot(first);
+    if ((threadIdx.x & (kWave - 1)) == 0 && m) atomicAdd(nonempty, uint32_t(__popcll(m)));
+  }
 }
 
 __global__ void k_grid_scatter(const uint32_t* __restrict__ cid, uint32_t n,
@@ -261,6 +268,45 @@ void build_grid(const Cloud& c, const uint32_t* hb, float h, Grid& g, hipStream_
   hipLaunchKernelGGL(k_grid_gather, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, c.pts.p,
                      g.order.p, n, g.pts.p);
   // counts/cid/cursor/big return to the cache; ordering on s keeps reuse safe.
+}
+
+// Chebyshev distance transform of occupancy, separable: along one axis
+// d'(c) = min_k max(|k|, d(c + k e_axis)) for |k| <= cap; three passes give the
+// min over the cube. Values saturate at cap + 1 ("at least cap + 1").
+__global__ void k_empty_init(const uint32_t* __restrict__ cs, uint32_t ncells,
+                             uint8_t* __restrict__ out) {
+  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < ncells) out[c] = cs[c + 1] > cs[c] ? 0 : uint8_t(kEmptyCap + 1);
+}
+
+__global__ void k_empty_pass(const uint8_t* __restrict__ in, uint8_t* __restrict__ out, GridDesc g,
+                             int axis) {
+  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.ncells) return;
+  const uint32_t dx = uint32_t(g.dx), dy = uint32_t(g.dy);
+  const int cx = int(c % dx), cy = int((c / dx) % dy), cz = int(c / (dx * dy));
+  const int coord = axis == 0 ? cx : (axis == 1 ? cy : cz);
+  const int dim = axis == 0 ? g.dx : (axis == 1 ? g.dy : g.dz);
+  const int stride = axis == 0 ? 1 : (axis == 1 ? g.dx : g.dx * g.dy);
+  int best = in[c];
+  for (int k = 1; k <= kEmptyCap && k < best; ++k) {
+    if (coord - k >= 0) best = min(best, max(k, int(in[int64_t(c) - int64_t(k) * stride])));
+    if (coord + k < dim) best = min(best, max(k, int(in[int64_t(c) + int64_t(k) * stride])));
+  }
+  out[c] = uint8_t(best);
+}
+
+void build_empty_radius(Grid& g, hipStream_t s) {
+  const uint32_t nc = g.desc.ncells;
+  g.empty_radius.resize(std::max<uint32_t>(nc, 1));
+  if (nc == 0) return;
+  DevBuf<uint8_t> tmp(nc);
+  const dim3 grid((nc + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(k_empty_init, grid, dim3(kBlock), 0, s, g.cell_start.p, nc, g.empty_radius.p);
+  hipLaunchKernelGGL(k_empty_pass, grid, dim3(kBlock), 0, s, g.empty_radius.p, tmp.p, g.desc, 0);
+  hipLaunchKernelGGL(k_empty_pass, grid, dim3(kBlock), 0, s, tmp.p, g.empty_radius.p, g.desc, 1);
+  hipLaunchKernelGGL(k_empty_pass, grid, dim3(kBlock), 0, s, g.empty_radius.p, tmp.p, g.desc, 2);
+  SR_HIP(hipMemcpyAsync(g.empty_radius.p, tmp.p, nc, hipMemcpyDeviceToDevice, s));
 }
 
 }  // namespace sr

@@ -130,7 +130,14 @@ struct Grid {
   DevBuf<uint32_t> cell_start;  // ncells + 1
   DevBuf<float4> pts;           // grid order; .w = original index (bit cast)
   DevBuf<uint32_t> order;       // grid order -> original index
+  DevBuf<uint8_t> empty_radius; // per cell: Chebyshev distance (cells) to the nearest
+                                // occupied cell, capped at kEmptyCap + 1 (optional)
 };
+
+constexpr int kEmptyCap = 14;
+
+// Fills g.empty_radius (Chebyshev distance transform of cell occupancy, capped).
+void build_empty_radius(Grid& g, hipStream_t s);
 
 // Chooses a cell edge so that occupied cells hold about `occupancy` points.
 float choose_cell_edge(const Cloud& c, const uint32_t* host_bounds, float occupancy);
@@ -165,8 +172,10 @@ void knn_query(const Grid& g, const float4* queries, uint32_t nq, uint32_t k, in
 struct InstState {
   double q[4];       // pose rotation (x,y,z,w), FP64
   double t[3];       // pose translation, FP64
-  float rf[9];       // f32 rotation matrix used by the per-point pass
+  float rf[9];       // f32 rotation matrix used by the per-point pass (high part)
   float tf[3];
+  float rlo[9];      // low parts: R = rf + rlo, t = tf + tlo to ~2^-48 relative
+  float tlo[3];
   double fitness;
   double last_update;
   uint64_t count;    // correspondences of the last pass
@@ -196,6 +205,7 @@ struct IterInstance {
   const float4* tgt;      // target points in grid order
   const float4* tgt_n;    // target normals in grid order
   const uint32_t* cell_start;
+  const uint8_t* empty_radius;  // target grid's empty-radius field (may be null)
   GridDesc grid;
   InstConfig cfg;
   InstState* state;

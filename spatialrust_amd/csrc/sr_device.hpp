@@ -102,6 +102,60 @@ __device__ __forceinline__ float ring_clearance(const GridDesc& g, float px, flo
   return m - g.margin;
 }
 
+// Distance from v to the widened interval of cell c along one axis (0 inside).
+__device__ __forceinline__ float axis_gap(float v, float o, float h, float m, int c) {
+  const float lo = o + (float)c * h - m;
+  return fmaxf(fmaxf(lo - v, v - (lo + h + 2.0f * m)), 0.0f);
+}
+
+// Visits every cell of Chebyshev ring r around (hx,hy,hz) that lies inside the
+// grid and can hold a point within sqrt(thr) of p (thr is re-read as `visit`
+// tightens it). Rows of cells are clipped to the sphere before any cell is
+// touched, so a wide ring costs about the cells inside the sphere rather than
+// the whole (2r+1)^3 shell. visit(c) scans cell c.
+template <typename Visit>
+__device__ __forceinline__ void scan_ring(const GridDesc& g, float px, float py, float pz, int hx,
+                                          int hy, int hz, int r, const float& thr, Visit&& visit) {
+  const int z0 = max(hz - r, 0), z1 = min(hz + r, g.dz - 1);
+  const int y0 = max(hy - r, 0), y1 = min(hy + r, g.dy - 1);
+  for (int cz = z0; cz <= z1; ++cz) {
+    const float gz = axis_gap(pz, g.oz, g.h, g.margin, cz);
+    const float gz2 = gz * gz;
+    if (gz2 > thr) continue;
+    const bool zedge = (cz == hz - r) || (cz == hz + r);
+    for (int cy = y0; cy <= y1; ++cy) {
+      const float gy = axis_gap(py, g.oy, g.h, g.margin, cy);
+      const float gyz = gz2 + gy * gy;
+      if (gyz > thr) continue;
+      const uint32_t row = (uint32_t(cz) * uint32_t(g.dy) + uint32_t(cy)) * uint32_t(g.dx);
+      if (zedge || cy == hy - r || cy == hy + r) {
+        int x0 = max(hx - r, 0), x1 = min(hx + r, g.dx - 1);
+        if (thr < INFINITY) {
+          const float rx = sqrtf(thr - gyz);
+          x0 = max(x0, cell_coord_free(px - rx - g.margin, g.ox, g.inv_h) - 1);
+          x1 = min(x1, cell_coord_free(px + rx + g.margin, g.ox, g.inv_h) + 1);
+        }
+        for (int cx = x0; cx <= x1; ++cx) {
+          const float gx = axis_gap(px, g.ox, g.h, g.margin, cx);
+          if (gyz + gx * gx > thr) continue;
+          visit(row + uint32_t(cx));
+        }
+      } else {
+        // interior row of the ring: only its two end cells are new
+        const int ends[2] = {hx - r, hx + r};
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const int cx = ends[e];
+          if (cx < 0 || cx >= g.dx) continue;
+          const float gx = axis_gap(px, g.ox, g.h, g.margin, cx);
+          if (gyz + gx * gx > thr) continue;
+          visit(row + uint32_t(cx));
+        }
+      }
+    }
+  }
+}
+
 // Kahan-compensated f32 accumulator.
 struct Kahan {
   float s, c;

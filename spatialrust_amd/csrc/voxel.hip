@@ -57,7 +57,10 @@ __global__ void k_bounds_init(uint32_t* b) {
   else if (threadIdx.x < 6) b[threadIdx.x] = 0x007FFFFFu;   // f2ord(-inf)
 }
 
-// Per-axis min/max; NaN coordinates are ignored, like f32::min/max.
+// Per-axis min/max; NaN coordinates are ignored, like f32::min/max. Each block
+// reduces its stride of points in registers, across the wave by shuffles and
+// across waves in LDS, then issues one atomic per axis (few blocks, so the
+// atomics on the six words never queue up).
 __global__ void k_bounds(const float4* __restrict__ pts, uint32_t n, uint32_t* __restrict__ b) {
   float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -75,12 +78,25 @@ __global__ void k_bounds(const float4* __restrict__ pts, uint32_t n, uint32_t* _
       mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], off));
     }
   }
+  __shared__ float smn[kBlock / kWave][3], smx[kBlock / kWave][3];
+  const int wid = threadIdx.x / kWave;
   if ((threadIdx.x & (kWave - 1)) == 0) {
     for (int a = 0; a < 3; ++a) {
-      if (mn[a] <= mx[a]) {
-        atomicMin(&b[a], f2ord(mn[a]));
-        atomicMax(&b[3 + a], f2ord(mx[a]));
-      }
+      smn[wid][a] = mn[a];
+      smx[wid][a] = mx[a];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int a = threadIdx.x;
+    float lo = smn[0][a], hi = smx[0][a];
+    for (int w = 1; w < kBlock / kWave; ++w) {
+      lo = fminf(lo, smn[w][a]);
+      hi = fmaxf(hi, smx[w][a]);
+    }
+    if (lo <= hi) {
+      atomicMin(&b[a], f2ord(lo));
+      atomicMax(&b[3 + a], f2ord(hi));
     }
   }
 }
@@ -88,7 +104,7 @@ __global__ void k_bounds(const float4* __restrict__ pts, uint32_t n, uint32_t* _
 void cloud_bounds(const float4* pts, uint32_t n, uint32_t* dev_bounds, hipStream_t s) {
   hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, s, dev_bounds);
   if (n == 0) return;
-  unsigned blocks = unsigned(std::min<size_t>((n + kBlock - 1) / kBlock, 1024));
+  unsigned blocks = unsigned(std::min<size_t>((n + kBlock * 8 - 1) / (kBlock * 8), 512));
   hipLaunchKernelGGL(k_bounds, dim3(blocks), dim3(kBlock), 0, s, pts, n, dev_bounds);
 }
 

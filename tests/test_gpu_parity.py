@@ -114,6 +114,9 @@ def test_radius_covariances_bit_exact(hip, orc):
     pts = pts[:20000]
     g, gn = sr.plane_covariances(pts, radius=0.3)
     o, on = orc.radius_covariances(pts, 0.3, 1e-3)
+    rows = np.any(g != o, axis=1)
+    print(f"radius covariances: {rows.sum()} of {len(g)} rows differ, max |diff| "
+          f"{np.abs(g - o).max():.3e}", flush=True)
     assert np.array_equal(g, o)
 
 
