@@ -93,6 +93,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="large_block")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--search-stats", action="store_true",
+                    help="count grid-search work (rings/cells/points per query) on one extra run")
     ap.add_argument("--pass-times", action="store_true",
                     help="print each Gauss-Newton pass duration of the last registration")
     args = ap.parse_args()
@@ -116,7 +118,7 @@ def main():
     reg = sr.GicpRegistration(cfg)
     for _ in range(args.warmup):
         res = reg.align(src.view, tgt.view)
-    print(f"warmup done: {res}", file=sys.stderr, flush=True)
+        print(f"warmup: {res}", file=sys.stderr, flush=True)
 
     lib.sr_profile_reset()
     lib.sr_profile_enable(1)
@@ -133,7 +135,7 @@ def main():
 
     import ctypes
     kern = {}
-    for name in ("gicp_pass", "knn_normals", "voxel_insert"):
+    for name in ("gicp_pass", "gicp_finalize", "knn_normals", "voxel_insert"):
         ms = ctypes.c_double(0.0)
         cnt = ctypes.c_uint64(0)
         lib.sr_profile_read(name.encode(), ctypes.byref(ms), ctypes.byref(cnt))
@@ -148,6 +150,17 @@ def main():
     avg_pass_s = (pass_ms / max(pass_n, 1)) * 1e-3
     algo_bytes = 72 * res.source_points  # source xyz + cov, gathered target xyz + cov
     achieved = algo_bytes / avg_pass_s / 1e9 if avg_pass_s > 0 else 0.0
+
+    if args.search_stats:
+        lib.sr_profile_enable(2)
+        reg.align(src.view, tgt.view)
+        lib.sr_profile_enable(0)
+        for name in ("knn_normals", "gicp_pass"):
+            st = (ctypes.c_uint64 * 4)()
+            lib.sr_profile_read_stats(name.encode(), st)
+            q = max(st[0], 1)
+            print(f"search stats {name}: queries {st[0]} rings/q {st[1] / q:.2f} "
+                  f"cells/q {st[2] / q:.2f} points/q {st[3] / q:.2f}", file=sys.stderr, flush=True)
 
     err = None
     golden = os.path.join(ROOT, "tests", "golden", "fp64_poses.json")

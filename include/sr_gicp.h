@@ -142,6 +142,10 @@ int32_t sr_profile_read(const char* name, double* total_ms, uint64_t* launches);
 /* Individual durations (ms) of up to `cap` launches of `name`, in launch order;
  * returns the number recorded (or -1 on a HIP error). */
 int64_t sr_profile_read_each(const char* name, double* out_ms, uint64_t cap);
+/* With sr_profile_enable(2), the grid searches also count their work; reads and
+ * clears {queries, rings, cells visited, points tested} of "knn_normals" or
+ * "gicp_pass". */
+int32_t sr_profile_read_stats(const char* name, uint64_t* out4);
 
 /* ---- GICP (drop-in for GicpRegistration::align_with_diagnostics) ------- */
 

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -173,6 +174,7 @@ struct Profiler {
   };
   std::vector<Rec> recs;
   std::vector<hipEvent_t> spare;
+  std::map<std::string, SearchStats*> stats;  // device counters (search-statistics mode)
 };
 static Profiler& profiler() {
   static Profiler* p = new Profiler();
@@ -192,6 +194,19 @@ static hipEvent_t prof_event() {
   hipEvent_t e;
   SR_HIP(hipEventCreate(&e));
   return e;
+}
+
+SearchStats* prof_stats(const char* name) {
+  Profiler& P = profiler();
+  if (P.on.load() != 2) return nullptr;
+  std::lock_guard<std::mutex> lk(P.mu);
+  auto it = P.stats.find(name);
+  if (it != P.stats.end()) return it->second;
+  SearchStats* d = nullptr;
+  SR_HIP(hipMalloc(&d, sizeof(SearchStats)));
+  SR_HIP(hipMemset(d, 0, sizeof(SearchStats)));
+  P.stats[name] = d;
+  return d;
 }
 
 ProfScope::ProfScope(const char* name, hipStream_t s) : stream(s) {
@@ -228,7 +243,21 @@ struct Receipt {
   uint32_t stages = 0;
 };
 
-static float knn_occupancy(uint32_t k) { return std::max(4.0f, float(k) * 0.35f); }
+// Tuning overrides (environment, read once): SR_GRID_OCCUPANCY = target points
+// per occupied grid cell, SR_GICP_PPT = source points per thread in a pass.
+static float env_float(const char* name, float dflt) {
+  const char* v = std::getenv(name);
+  if (!v || !*v) return dflt;
+  char* end = nullptr;
+  const float f = std::strtof(v, &end);
+  return (end && end != v && f > 0.0f) ? f : dflt;
+}
+
+static float knn_occupancy(uint32_t k) {
+  static const float forced = env_float("SR_GRID_OCCUPANCY", 0.0f);
+  if (forced > 0.0f) return forced;
+  return std::max(4.0f, float(k) * 0.35f);
+}
 
 // Voxel filter (optional) + grid + plane normals of one cloud.
 static void prepare(const sr_xyz_view& v, const sr_gicp_config& cfg, Prepared& out, Receipt& rc,
@@ -277,6 +306,8 @@ static InstConfig inst_config(const sr_gicp_config& c) {
 }
 
 static uint32_t pick_ppt(uint64_t total_points) {
+  static const float forced = env_float("SR_GICP_PPT", 0.0f);
+  if (forced >= 1.0f) return uint32_t(forced);
   // Enough workgroups to fill 256 CUs several times over before packing more
   // points per thread (fewer block reductions).
   if (total_points >= (1ull << 23)) return 4;
@@ -349,7 +380,7 @@ static void batch_setup(const std::vector<const Prepared*>& src, const std::vect
 
 static void batch_run(Batch& B, hipStream_t s) {
   for (uint32_t k = 0; k < B.max_passes; ++k)
-    gicp_pass(B.insts.p, B.block_inst.p, B.total_blocks, B.ppt, s);
+    gicp_pass(B.insts.p, B.block_inst.p, B.total_blocks, B.count, B.ppt, s);
 }
 
 static int32_t state_to_result(const InstState& st, const sr_gicp_config& cfg, const Prepared& src,
@@ -456,7 +487,25 @@ extern "C" {
 
 const char* sr_last_error(void) { return t_last_error.c_str(); }
 
-void sr_profile_enable(int32_t on) { profiler().on.store(on ? 1 : 0); }
+void sr_profile_enable(int32_t on) { profiler().on.store(on == 2 ? 2 : (on ? 1 : 0)); }
+
+int32_t sr_profile_read_stats(const char* name, uint64_t* out4) {
+  return guarded([&] {
+    Profiler& P = profiler();
+    SR_HIP(hipDeviceSynchronize());
+    std::lock_guard<std::mutex> lk(P.mu);
+    auto it = P.stats.find(name);
+    SearchStats h{0, 0, 0, 0};
+    if (it != P.stats.end()) {
+      SR_HIP(hipMemcpy(&h, it->second, sizeof(h), hipMemcpyDeviceToHost));
+      SR_HIP(hipMemset(it->second, 0, sizeof(h)));
+    }
+    out4[0] = h.queries;
+    out4[1] = h.rings;
+    out4[2] = h.cells;
+    out4[3] = h.points;
+  });
+}
 
 void sr_profile_reset(void) {
   Profiler& P = profiler();

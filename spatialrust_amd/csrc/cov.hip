@@ -64,8 +64,9 @@ struct KList {
 template <int K>
 __device__ int knn_search(const GridDesc& g, const uint32_t* __restrict__ cs,
                           const float4* __restrict__ gp, float px, float py, float pz, int k,
-                          KList<K>& L) {
+                          KList<K>& L, SearchStats* stats = nullptr) {
   L.init();
+  uint32_t n_rings = 0, n_cells = 0, n_pts = 0;
   const int hx = cell_coord_free(px, g.ox, g.inv_h), hy = cell_coord_free(py, g.oy, g.inv_h),
             hz = cell_coord_free(pz, g.oz, g.inv_h);
   const int rmax = max(max(max(hx, g.dx - 1 - hx), max(hy, g.dy - 1 - hy)),
@@ -75,30 +76,52 @@ __device__ int knn_search(const GridDesc& g, const uint32_t* __restrict__ cs,
                      max(max(-hz, hz - (g.dz - 1)), 0));
   float worst_d = INFINITY;
   uint32_t worst_i = kNone;
-  for (int r = r0; r <= rmax; ++r) {
-    scan_ring(g, px, py, pz, hx, hy, hz, r, worst_d, [&](uint32_t c) {
-      const uint32_t e = cs[c + 1];
-      for (uint32_t s = cs[c]; s < e; ++s) {
-        const float4 q = gp[s];
-        const float d2 = dist2(q.x, q.y, q.z, px, py, pz);
-        const uint32_t qi = __float_as_uint(q.w);
-        if (d2 < worst_d || (d2 == worst_d && qi < worst_i)) {
-          L.insert(d2, qi, s);
+  auto consider = [&](uint32_t s, const float4& q) {
+    const float d2 = dist2(q.x, q.y, q.z, px, py, pz);
+    const uint32_t qi = __float_as_uint(q.w);
+    if (d2 < worst_d || (d2 == worst_d && qi < worst_i)) {
+      L.insert(d2, qi, s);
 #pragma unroll
-          for (int j = 0; j < K; ++j) {
-            if (j == k - 1) {
-              worst_d = L.d[j];
-              worst_i = L.id[j];
-            }
-          }
+      for (int j = 0; j < K; ++j) {
+        if (j == k - 1) {
+          worst_d = L.d[j];
+          worst_i = L.id[j];
         }
       }
+    }
+  };
+  int rfirst = r0;
+  if (r0 == 0) {
+    // Rings 0 and 1 as nine contiguous point ranges, then rings >= 2 only if
+    // the k-th distance is not yet certified by the block's clearance.
+    ++n_rings;
+    scan_block27(g, cs, gp, hx, hy, hz, [&](uint32_t s, const float4& q) {
+      ++n_pts;
+      consider(s, q);
+    });
+    if (worst_i != kNone) {
+      const float cl = ring_clearance(g, px, py, pz, hx, hy, hz, 1);
+      if (cl > 0.0f && worst_d < cl * cl) {
+        stats_add(stats, n_rings, n_cells, n_pts);
+        return min(k, int(g.npoints));
+      }
+    }
+    rfirst = 2;
+  }
+  for (int r = rfirst; r <= rmax; ++r) {
+    ++n_rings;
+    scan_ring(g, px, py, pz, hx, hy, hz, r, worst_d, [&](uint32_t c) {
+      const uint32_t hi = cs[c + 1];
+      ++n_cells;
+      n_pts += hi - cs[c];
+      for (uint32_t s = cs[c]; s < hi; ++s) consider(s, gp[s]);
     });
     if (worst_i != kNone) {
       const float cl = ring_clearance(g, px, py, pz, hx, hy, hz, r);
       if (cl > 0.0f && worst_d < cl * cl) break;
     }
   }
+  stats_add(stats, n_rings, n_cells, n_pts);
   return min(k, int(g.npoints));
 }
 
@@ -156,12 +179,13 @@ __global__ void __launch_bounds__(kBlock) k_knn_normals(GridDesc g, const uint32
                                                         float one_minus_eps,
                                                         float4* __restrict__ nrm_grid,
                                                         float* __restrict__ cov6,
-                                                        float* __restrict__ normal3) {
+                                                        float* __restrict__ normal3,
+                                                        SearchStats* __restrict__ stats) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= g.npoints) return;
   const float4 p = gp[i];
   KList<K> L;
-  const int found = knn_search<K>(g, cs, gp, p.x, p.y, p.z, k, L);
+  const int found = knn_search<K>(g, cs, gp, p.x, p.y, p.z, k, L, stats);
   float n[3] = {1.0f, 0.0f, 0.0f};  // count < 2: identity -> eigenvector e_x (gicp.rs:268-270)
   if (found >= 2) {
     // Mean and centred sums relative to the query point (f32, two passes).
@@ -220,7 +244,7 @@ static void launch_knn_normals(const Grid& g, int k, float b, float4* nrm, float
   const uint32_t n = g.desc.npoints;
   ProfScope prof("knn_normals", s);
   hipLaunchKernelGGL(k_knn_normals<K>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, g.desc,
-                     g.cell_start.p, g.pts.p, k, b, nrm, cov6, normal3);
+                     g.cell_start.p, g.pts.p, k, b, nrm, cov6, normal3, prof_stats("knn_normals"));
 }
 
 void knn_normals(const Grid& g, uint32_t k, double eps, float4* nrm, float* cov6, float* normal3,

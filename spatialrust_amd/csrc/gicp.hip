@@ -11,11 +11,10 @@
 //  - C_t, C_s are stored as unit normals: C = a I - (1-eps) n n^T (a = 1, or
 //    a = eps with n = 0 for the radius path's eps*I), so M has a closed form
 //    (no 3x3 adjugate, which in f32 would lose most digits at cond(C) ~ 1e3);
-//  - per-point terms in f32; each thread sums its points with Kahan
-//    compensation, the block reduces them in a fixed order with compensated
-//    sums, and the last block of each registration adds the block partials in
-//    FP64 in block order — so a registration's result does not depend on what
-//    else shares the launch;
+//  - per-point terms in f32; the workgroup reduces its threads' sums in a fixed
+//    order with Kahan-compensated chunks and a TwoSum combine, writing (hi, lo)
+//    pairs; the finalize kernel adds the block partials in FP64 in a fixed order
+//    — so a registration's result does not depend on what else shares the launch;
 //  - the 6x6 solve is an FP64 LDL^T (the reference runs Gaussian elimination
 //    with partial pivoting, linalg.rs:12-53; both stop on pivots below 1e-12);
 //  - the pose is held in FP64 and updated with the reference's SO(3) x R^3
@@ -24,8 +23,11 @@
 //  - the fitness test of iteration k (a second nearest-neighbour pass in the
 //    reference, gicp.rs:203-208, 330-357) is taken from pass k+1, which visits
 //    the same transformed points against the same target.
-// The finalize step runs in the last workgroup to finish each registration, so
-// a pass is a single launch and the host never waits between iterations.
+// A pass is two launches — the point-parallel pass and a one-workgroup-per-
+// registration finalize — and the host never waits between iterations. (A
+// last-workgroup-finalizes scheme needs a device-scope fence in every
+// workgroup, which on gfx950 writes back and invalidates the XCD's L2 and cost
+// more than the extra launch.)
 
 #include <cfloat>
 
@@ -135,7 +137,7 @@ __device__ bool ldlt_solve(const double* Hp, const double* g, double* x, double*
 }
 
 // Gauss-Newton bookkeeping of one registration after a pass (one thread).
-__device__ __noinline__ void finalize(InstState* st, const InstConfig& cfg, const double* Hp,
+__device__ __forceinline__ void finalize(InstState* st, const InstConfig& cfg, const double* Hp,
                                       const double* g, double fit_sum, unsigned long long cnt,
                                       double* w) {
   const double F = cnt > 0 ? fit_sum / double(cnt) : DBL_MAX;
@@ -216,48 +218,76 @@ __device__ __forceinline__ uint32_t nearest_within(const GridDesc& g, const uint
                                                    const uint8_t* __restrict__ empty,
                                                    const float4* __restrict__ tp, float px, float py,
                                                    float pz, float dmax, float dmax2,
-                                                   float& best_d) {
+                                                   float& best_d, SearchStats* stats) {
   best_d = INFINITY;
   uint32_t best_pos = kNone, best_id = kNone;
+  uint32_t n_rings = 0, n_cells = 0, n_pts = 0;
   // Quick reject: query farther than dmax from the grid's box.
   {
     float bx = fmaxf(fmaxf(g.ox - px, px - (g.ox + g.h * g.dx)), 0.0f);
     float by = fmaxf(fmaxf(g.oy - py, py - (g.oy + g.h * g.dy)), 0.0f);
     float bz = fmaxf(fmaxf(g.oz - pz, pz - (g.oz + g.h * g.dz)), 0.0f);
     float lb = bx * bx + by * by + bz * bz;
-    if (lb > dmax2 + 2.0f * dmax * g.margin + g.margin * g.margin) return kNone;
+    if (lb > dmax2 + 2.0f * dmax * g.margin + g.margin * g.margin) {
+      stats_add(stats, 0, 0, 0);
+      return kNone;
+    }
   }
   const int hx = cell_coord_free(px, g.ox, g.inv_h), hy = cell_coord_free(py, g.oy, g.inv_h),
             hz = cell_coord_free(pz, g.oz, g.inv_h);
   const int rlim = int(dmax * g.inv_h) + 2;
   int rstart = 0;
-  if (empty && hx >= 0 && hx < g.dx && hy >= 0 && hy < g.dy && hz >= 0 && hz < g.dz) {
+  const bool inside = hx >= 0 && hx < g.dx && hy >= 0 && hy < g.dy && hz >= 0 && hz < g.dz;
+  int e = 0;
+  if (empty && inside) {
     // Rings closer than the empty radius hold no point; a point in ring e is at
     // least (e - 1) cells away.
-    const int e = empty[(uint32_t(hz) * uint32_t(g.dy) + uint32_t(hy)) * uint32_t(g.dx) + uint32_t(hx)];
-    if (float(e - 1) * g.h - g.margin > dmax) return kNone;
+    e = empty[(uint32_t(hz) * uint32_t(g.dy) + uint32_t(hy)) * uint32_t(g.dx) + uint32_t(hx)];
+    if (float(e - 1) * g.h - g.margin > dmax) {
+      stats_add(stats, 0, 0, 0);
+      return kNone;
+    }
     rstart = e;
   }
   // thr = min(best so far, dmax^2): cells and rows beyond it are never touched.
   float thr = dmax2;
+  auto consider = [&](uint32_t s, const float4& q) {
+    const float d2 = dist2(px, py, pz, q.x, q.y, q.z);
+    const uint32_t qi = __float_as_uint(q.w);
+    if (d2 <= dmax2 && (d2 < best_d || (d2 == best_d && qi < best_id))) {
+      best_d = d2;
+      best_id = qi;
+      best_pos = s;
+      thr = d2;
+    }
+  };
+  if (inside && e <= 1) {
+    // Rings 0 and 1 as nine contiguous point ranges; most queries of a
+    // converging registration end here.
+    ++n_rings;
+    scan_block27(g, cs, tp, hx, hy, hz, [&](uint32_t s, const float4& q) {
+      ++n_pts;
+      consider(s, q);
+    });
+    const float cl = ring_clearance(g, px, py, pz, hx, hy, hz, 1);
+    if (cl > 0.0f && thr < cl * cl) {
+      stats_add(stats, n_rings, n_cells, n_pts);
+      return best_pos;
+    }
+    rstart = 2;
+  }
   for (int r = rstart; r <= rlim; ++r) {
+    ++n_rings;
     scan_ring(g, px, py, pz, hx, hy, hz, r, thr, [&](uint32_t c) {
-      const uint32_t e = cs[c + 1];
-      for (uint32_t s = cs[c]; s < e; ++s) {
-        const float4 q = tp[s];
-        const float d2 = dist2(px, py, pz, q.x, q.y, q.z);
-        const uint32_t qi = __float_as_uint(q.w);
-        if (d2 <= dmax2 && (d2 < best_d || (d2 == best_d && qi < best_id))) {
-          best_d = d2;
-          best_id = qi;
-          best_pos = s;
-          thr = d2;
-        }
-      }
+      const uint32_t hi = cs[c + 1];
+      ++n_cells;
+      n_pts += hi - cs[c];
+      for (uint32_t s = cs[c]; s < hi; ++s) consider(s, tp[s]);
     });
     const float cl = ring_clearance(g, px, py, pz, hx, hy, hz, r);
     if (cl > 0.0f && thr < cl * cl) break;
   }
+  stats_add(stats, n_rings, n_cells, n_pts);
   return best_pos;
 }
 
@@ -314,7 +344,8 @@ __device__ __forceinline__ void combined_inverse(float ux, float uy, float uz, f
 
 __global__ void __launch_bounds__(kBlock) k_gicp_pass(const IterInstance* __restrict__ insts,
                                                       const uint32_t* __restrict__ block_inst,
-                                                      uint32_t ppt) {
+                                                      uint32_t ppt,
+                                                      SearchStats* __restrict__ stats) {
   // XCD-aware block order: the hardware deals consecutive workgroup ids round
   // robin over the 8 XCDs; give each XCD a contiguous range of logical blocks so
   // neighbouring source tiles (which touch the same target cells) share an L2.
@@ -365,7 +396,8 @@ __global__ void __launch_bounds__(kBlock) k_gicp_pass(const IterInstance* __rest
     const float pz = fmaf(R[6], p.x, fmaf(R[7], p.y, fmaf(R[8], p.z, T[2] + lz)));
     float d2;
     const uint32_t j =
-        nearest_within(g, I.cell_start, I.empty_radius, I.tgt, px, py, pz, cfg.dmax, cfg.dmax2, d2);
+        nearest_within(g, I.cell_start, I.empty_radius, I.tgt, px, py, pz, cfg.dmax, cfg.dmax2, d2,
+                       stats);
     if (j == kNone) continue;
     const float4 q = I.tgt[j];
     const float4 nt = I.tgt_n[j];
@@ -425,10 +457,12 @@ __global__ void __launch_bounds__(kBlock) k_gicp_pass(const IterInstance* __rest
   }
 
   // ---- block reduction (fixed order, compensated) -------------------------
-  __shared__ float red[kTerms][kBlock];
+  // Rows padded by 8 floats and chunks read interleaved (element e*8 + ch), so
+  // the 64 lanes of a wave spread over the LDS banks.
+  constexpr int kRow = kBlock + 8;
+  __shared__ float red[kTerms][kRow];
   __shared__ float part_hi[kTerms][8], part_lo[kTerms][8];
   __shared__ unsigned long long wcount[kBlock / kWave];
-  __shared__ int s_last;
 #pragma unroll
   for (int t = 0; t < kTerms; ++t) red[t][threadIdx.x] = acc[t];
   unsigned long long c64 = count;
@@ -439,11 +473,14 @@ __global__ void __launch_bounds__(kBlock) k_gicp_pass(const IterInstance* __rest
     const int t = threadIdx.x >> 3, ch = threadIdx.x & 7;
     Kahan k;
     k.init();
-    for (int e = 0; e < kBlock / 8; ++e) k.add(red[t][ch * (kBlock / 8) + e]);
+    for (int e = 0; e < kBlock / 8; ++e) k.add(red[t][e * 8 + ch]);
     part_hi[t][ch] = k.s;
     part_lo[t][ch] = -k.c;
   }
   __syncthreads();
+  // Block partial: 28 (hi, lo) pairs + u64 count. The finalize kernel of this
+  // pass reads them after the kernel boundary, so no device-scope fence (an L2
+  // write-back + invalidate on every XCD) is needed here.
   float* out = I.partials + size_t(lb) * 64;
   if (threadIdx.x < kTerms) {
     const int t = threadIdx.x;
@@ -464,74 +501,78 @@ __global__ void __launch_bounds__(kBlock) k_gicp_pass(const IterInstance* __rest
     for (int w = 0; w < kBlock / kWave; ++w) tot += wcount[w];
     reinterpret_cast<unsigned long long*>(out)[30] = tot;  // floats 60..61
   }
-  // ---- last workgroup of this registration finalizes --------------------
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) s_last = atomicAdd(&st->counter, 1u) == I.nblocks - 1;
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  // FP64 sum of the block partials. Every thread takes blocks tid, tid+256, ...
-  // (many loads in flight), then a fixed xor-butterfly per wave and a fixed
-  // wave order: the same order on every run, whatever the launch mix.
-  __shared__ double wsum[kBlock / kWave][4];
-  __shared__ unsigned long long wcnt2[kBlock / kWave];
+}
+
+// One workgroup per registration: FP64 sum of its block partials, then the
+// Gauss-Newton bookkeeping, solve and pose update on one thread. Every thread
+// takes blocks tid, tid+256, ... (many loads in flight), then a fixed
+// xor-butterfly per wave and a fixed wave order: the same order on every run,
+// whatever the launch mix.
+__global__ void __launch_bounds__(kBlock) k_gicp_finalize(const IterInstance* __restrict__ insts,
+                                                          uint32_t count) {
+  const uint32_t inst_id = blockIdx.x;
+  if (inst_id >= count) return;
+  const IterInstance& I = insts[inst_id];
+  InstState* st = I.state;
+  if (st->phase == 2) return;
+  const InstConfig cfg = I.cfg;
+  __shared__ double wsum[kBlock / kWave][kTerms];
+  __shared__ unsigned long long wcnt[kBlock / kWave];
   __shared__ double dtot[kTerms];
   __shared__ double work[64];
   const uint32_t nb = I.nblocks;
   const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-  for (int t0 = 0; t0 < kTerms; t0 += 4) {
-    double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
-    for (uint32_t bb = threadIdx.x; bb < nb; bb += kBlock) {
-      const float4* pp = reinterpret_cast<const float4*>(I.partials + size_t(bb) * 64 + 2 * t0);
-      const float4 a = pp[0], c = pp[1];
-      v0 += double(a.x) + double(a.y);
-      v1 += double(a.z) + double(a.w);
-      v2 += double(c.x) + double(c.y);
-      v3 += double(c.z) + double(c.w);
+  // All 28 terms of a block partial are loaded together (14 float4 + the
+  // count), so each thread has its whole share of loads in flight at once.
+  double v[kTerms];
+#pragma unroll
+  for (int t = 0; t < kTerms; ++t) v[t] = 0.0;
+  unsigned long long cs = 0;
+  for (uint32_t bb = threadIdx.x; bb < nb; bb += kBlock) {
+    const float4* pp = reinterpret_cast<const float4*>(I.partials + size_t(bb) * 64);
+    float4 q[kTerms / 2];
+#pragma unroll
+    for (int j = 0; j < kTerms / 2; ++j) q[j] = pp[j];
+    cs += reinterpret_cast<const unsigned long long*>(pp)[30];
+#pragma unroll
+    for (int j = 0; j < kTerms / 2; ++j) {
+      v[2 * j] += double(q[j].x) + double(q[j].y);
+      v[2 * j + 1] += double(q[j].z) + double(q[j].w);
     }
-    for (int off = kWave / 2; off > 0; off >>= 1) {
-      v0 += __shfl_xor(v0, off);
-      v1 += __shfl_xor(v1, off);
-      v2 += __shfl_xor(v2, off);
-      v3 += __shfl_xor(v3, off);
-    }
-    if (lane == 0) {
-      wsum[wid][0] = v0;
-      wsum[wid][1] = v1;
-      wsum[wid][2] = v2;
-      wsum[wid][3] = v3;
-    }
-    __syncthreads();
-    if (threadIdx.x < 4) {
-      double s = 0.0;
-      for (int w = 0; w < kBlock / kWave; ++w) s += wsum[w][threadIdx.x];
-      dtot[t0 + threadIdx.x] = s;
-    }
-    __syncthreads();
   }
-  {
-    unsigned long long cs = 0;
-    for (uint32_t bb = threadIdx.x; bb < nb; bb += kBlock)
-      cs += reinterpret_cast<const unsigned long long*>(I.partials + size_t(bb) * 64)[30];
-    for (int off = kWave / 2; off > 0; off >>= 1) cs += __shfl_xor(cs, off);
-    if (lane == 0) wcnt2[wid] = cs;
+#pragma unroll
+  for (int t = 0; t < kTerms; ++t)
+    for (int off = kWave / 2; off > 0; off >>= 1) v[t] += __shfl_xor(v[t], off);
+  for (int off = kWave / 2; off > 0; off >>= 1) cs += __shfl_xor(cs, off);
+  if (lane == 0) {
+#pragma unroll
+    for (int t = 0; t < kTerms; ++t) wsum[wid][t] = v[t];
+    wcnt[wid] = cs;
+  }
+  __syncthreads();
+  if (threadIdx.x < kTerms) {
+    double sacc = 0.0;
+    for (int w = 0; w < kBlock / kWave; ++w) sacc += wsum[w][threadIdx.x];
+    dtot[threadIdx.x] = sacc;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     unsigned long long cnt = 0;
-    for (int w = 0; w < kBlock / kWave; ++w) cnt += wcnt2[w];
+    for (int w = 0; w < kBlock / kWave; ++w) cnt += wcnt[w];
     finalize(st, cfg, dtot, dtot + 21, dtot[27], cnt, work);
-    st->counter = 0;
-    __threadfence();
   }
 }
 
 void gicp_pass(const IterInstance* insts, const uint32_t* block_inst, uint32_t total_blocks,
-               uint32_t ppt, hipStream_t s) {
+               uint32_t count, uint32_t ppt, hipStream_t s) {
   if (total_blocks == 0) return;
-  ProfScope prof("gicp_pass", s);
-  hipLaunchKernelGGL(k_gicp_pass, dim3(total_blocks), dim3(kBlock), 0, s, insts, block_inst, ppt);
+  {
+    ProfScope prof("gicp_pass", s);
+    hipLaunchKernelGGL(k_gicp_pass, dim3(total_blocks), dim3(kBlock), 0, s, insts, block_inst, ppt,
+                       prof_stats("gicp_pass"));
+  }
+  ProfScope prof("gicp_finalize", s);
+  hipLaunchKernelGGL(k_gicp_finalize, dim3(count), dim3(kBlock), 0, s, insts, count);
 }
 
 }  // namespace sr

@@ -54,6 +54,11 @@ static void grid_desc(const uint32_t* hb, float h, uint32_t npoints, GridDesc& d
   // box corners o + c*h relative to the coordinate magnitudes.
   int32_t dmax = std::max(dims[0], std::max(dims[1], dims[2]));
   d.margin = h * std::max(1e-4f, float(dmax) * 1e-6f) + extent * 1e-6f;
+  d.wpr = (uint32_t(d.dx) + 31) / 32;
+  d.bits = nullptr;
+  d.wps = (uint32_t(d.dy) + 31) / 32;
+  d.rows = nullptr;
+  d.pad2_ = 0;
 }
 
 __global__ void k_grid_count(const float4* __restrict__ pts, uint32_t n, GridDesc g,
@@ -163,6 +168,41 @@ __global__ void k_sort_big_cells(const uint32_t* __restrict__ start, uint32_t* _
   }
 }
 
+// Occupancy bitmap: one thread per 32-cell word of an x-row.
+__global__ void k_grid_bits(const uint32_t* __restrict__ cs, GridDesc g, uint32_t nwords,
+                            uint32_t* __restrict__ bits) {
+  const uint32_t wi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (wi >= nwords) return;
+  const uint32_t row = wi / g.wpr, w = wi % g.wpr;
+  const uint32_t base = row * uint32_t(g.dx);
+  uint32_t m = 0;
+  for (uint32_t b = 0; b < 32; ++b) {
+    const uint32_t cx = w * 32 + b;
+    if (cx >= uint32_t(g.dx)) break;
+    if (cs[base + cx + 1] > cs[base + cx]) m |= 1u << b;
+  }
+  bits[wi] = m;
+}
+
+// Row bitmap: one thread per 32-row word of a z-slab; bit set when any cell of
+// the x-row is occupied.
+__global__ void k_grid_rowbits(const uint32_t* __restrict__ bits, GridDesc g, uint32_t nwords,
+                               uint32_t* __restrict__ rows) {
+  const uint32_t wi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (wi >= nwords) return;
+  const uint32_t cz = wi / g.wps, wy = wi % g.wps;
+  uint32_t m = 0;
+  for (uint32_t b = 0; b < 32; ++b) {
+    const uint32_t cy = wy * 32 + b;
+    if (cy >= uint32_t(g.dy)) break;
+    const uint32_t* rb = bits + (size_t(cz) * uint32_t(g.dy) + cy) * g.wpr;
+    uint32_t any = 0;
+    for (uint32_t w = 0; w < g.wpr; ++w) any |= rb[w];
+    if (any) m |= 1u << b;
+  }
+  rows[wi] = m;
+}
+
 __global__ void k_grid_gather(const float4* __restrict__ pts, const uint32_t* __restrict__ order,
                               uint32_t n, float4* __restrict__ out) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -256,6 +296,20 @@ void build_grid(const Cloud& c, const uint32_t* hb, float h, Grid& g, hipStream_
                        g.desc, counts.p, cid.p, (uint32_t*)nullptr);
   }
   exclusive_scan_u32(counts.p, g.cell_start.p, nc + 1, nullptr, s);
+  {
+    g.desc.wpr = (uint32_t(g.desc.dx) + 31) / 32;
+    const uint64_t nwords = uint64_t(g.desc.wpr) * uint64_t(g.desc.dy) * uint64_t(g.desc.dz);
+    g.occ.resize(std::max<uint64_t>(nwords, 1));
+    g.desc.bits = g.occ.p;
+    hipLaunchKernelGGL(k_grid_bits, dim3(uint32_t((nwords + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                       g.cell_start.p, g.desc, uint32_t(nwords), g.occ.p);
+    g.desc.wps = (uint32_t(g.desc.dy) + 31) / 32;
+    const uint64_t nrw = uint64_t(g.desc.wps) * uint64_t(g.desc.dz);
+    g.rowocc.resize(std::max<uint64_t>(nrw, 1));
+    g.desc.rows = g.rowocc.p;
+    hipLaunchKernelGGL(k_grid_rowbits, dim3(uint32_t((nrw + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                       g.occ.p, g.desc, uint32_t(nrw), g.rowocc.p);
+  }
   if (n == 0) return;
   hipLaunchKernelGGL(k_grid_scatter, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, cid.p, n,
                      g.cell_start.p, cursor.p, g.order.p);

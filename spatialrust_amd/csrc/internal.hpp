@@ -87,6 +87,10 @@ struct ProfScope {
   hipStream_t stream = nullptr;
 };
 
+// Device counters for the named search kernel when search statistics are on
+// (sr_profile_enable(2)); nullptr otherwise.
+SearchStats* prof_stats(const char* name);
+
 // ---- scan ----------------------------------------------------------------
 
 // Exclusive prefix sum of n u32 values (in may equal out). If total is non-null
@@ -132,6 +136,8 @@ struct Grid {
   DevBuf<uint32_t> order;       // grid order -> original index
   DevBuf<uint8_t> empty_radius; // per cell: Chebyshev distance (cells) to the nearest
                                 // occupied cell, capped at kEmptyCap + 1 (optional)
+  DevBuf<uint32_t> occ;         // occupancy bitmap (desc.bits points here)
+  DevBuf<uint32_t> rowocc;      // row bitmap (desc.rows points here)
 };
 
 constexpr int kEmptyCap = 14;
@@ -221,7 +227,7 @@ constexpr int kTerms = 28;  // 21 H (upper triangle) + 6 g + fitness sum
 // Launches one Gauss-Newton pass over `count` instances (device array `insts`),
 // total_blocks blocks; block_inst maps block -> instance.
 void gicp_pass(const IterInstance* insts, const uint32_t* block_inst, uint32_t total_blocks,
-               uint32_t pts_per_thread, hipStream_t s);
+               uint32_t count, uint32_t pts_per_thread, hipStream_t s);
 // Initialises instance states from initial guesses (host array of 7 floats per instance).
 void init_states(InstState* states, const float* init7, uint32_t count, hipStream_t s);
 

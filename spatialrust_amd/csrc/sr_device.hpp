@@ -10,6 +10,20 @@ constexpr int kWave = 64;     // CDNA wavefront width
 constexpr int kBlock = 256;   // threads per workgroup for point-parallel kernels
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 
+// Optional search counters (profiling): summed over all queries of a launch.
+struct SearchStats {
+  unsigned long long queries, rings, cells, points;
+};
+
+__device__ __forceinline__ void stats_add(SearchStats* st, uint32_t rings, uint32_t cells,
+                                          uint32_t points) {
+  if (!st) return;
+  atomicAdd(&st->queries, 1ull);
+  atomicAdd(&st->rings, (unsigned long long)rings);
+  atomicAdd(&st->cells, (unsigned long long)cells);
+  atomicAdd(&st->points, (unsigned long long)points);
+}
+
 // Uniform grid description. Cell (cx,cy,cz) has linear id (cz*dy + cy)*dx + cx,
 // the layout of build_grid (crates/spatialrust-search/src/uniform_grid.rs:56-64).
 struct GridDesc {
@@ -20,7 +34,11 @@ struct GridDesc {
   uint32_t ncells;
   uint32_t npoints;
   float margin;  // slack covering f32 cell-assignment rounding (see cell_lb2)
-  float pad_[3];
+  uint32_t wpr;  // 32-bit words per x-row of the occupancy bitmap
+  const uint32_t* bits;  // occupancy bitmap: row (cz*dy+cy) has wpr words, bit cx
+  const uint32_t* rows;  // row bitmap: slab cz has wps words, bit cy = row has a point
+  uint32_t wps;          // 32-bit words per z-slab of the row bitmap
+  uint32_t pad2_;
 };
 
 // Order-preserving map of f32 onto u32 so atomicMin/atomicMax work on floats.
@@ -108,50 +126,115 @@ __device__ __forceinline__ float axis_gap(float v, float o, float h, float m, in
   return fmaxf(fmaxf(lo - v, v - (lo + h + 2.0f * m)), 0.0f);
 }
 
-// Visits every cell of Chebyshev ring r around (hx,hy,hz) that lies inside the
-// grid and can hold a point within sqrt(thr) of p (thr is re-read as `visit`
-// tightens it). Rows of cells are clipped to the sphere before any cell is
-// touched, so a wide ring costs about the cells inside the sphere rather than
-// the whole (2r+1)^3 shell. visit(c) scans cell c.
+// Visits every occupied cell of Chebyshev ring r around (hx,hy,hz) that lies
+// inside the grid and can hold a point within sqrt(thr) of p (thr is re-read as
+// `visit` tightens it). Slabs and rows outside the sphere are cut analytically
+// and each row's cells come from the occupancy bitmap, so empty cells cost
+// nothing; only occupied cells within the sphere reach visit(c).
 template <typename Visit>
 __device__ __forceinline__ void scan_ring(const GridDesc& g, float px, float py, float pz, int hx,
                                           int hy, int hz, int r, const float& thr, Visit&& visit) {
   const int z0 = max(hz - r, 0), z1 = min(hz + r, g.dz - 1);
-  const int y0 = max(hy - r, 0), y1 = min(hy + r, g.dy - 1);
   for (int cz = z0; cz <= z1; ++cz) {
     const float gz = axis_gap(pz, g.oz, g.h, g.margin, cz);
     const float gz2 = gz * gz;
     if (gz2 > thr) continue;
     const bool zedge = (cz == hz - r) || (cz == hz + r);
-    for (int cy = y0; cy <= y1; ++cy) {
-      const float gy = axis_gap(py, g.oy, g.h, g.margin, cy);
-      const float gyz = gz2 + gy * gy;
-      if (gyz > thr) continue;
-      const uint32_t row = (uint32_t(cz) * uint32_t(g.dy) + uint32_t(cy)) * uint32_t(g.dx);
-      if (zedge || cy == hy - r || cy == hy + r) {
-        int x0 = max(hx - r, 0), x1 = min(hx + r, g.dx - 1);
-        if (thr < INFINITY) {
-          const float rx = sqrtf(thr - gyz);
-          x0 = max(x0, cell_coord_free(px - rx - g.margin, g.ox, g.inv_h) - 1);
-          x1 = min(x1, cell_coord_free(px + rx + g.margin, g.ox, g.inv_h) + 1);
-        }
-        for (int cx = x0; cx <= x1; ++cx) {
-          const float gx = axis_gap(px, g.ox, g.h, g.margin, cx);
-          if (gyz + gx * gx > thr) continue;
-          visit(row + uint32_t(cx));
-        }
-      } else {
-        // interior row of the ring: only its two end cells are new
-        const int ends[2] = {hx - r, hx + r};
+    int y0 = max(hy - r, 0), y1 = min(hy + r, g.dy - 1);
+    if (thr < INFINITY) {
+      // rows whose y-gap fits in the remaining radius (+1 cell of slack covers
+      // the approximate sqrt and the filing margin)
+      const float ry = __builtin_amdgcn_sqrtf(thr - gz2) + g.margin;
+      y0 = max(y0, cell_coord_free(py - ry, g.oy, g.inv_h) - 1);
+      y1 = min(y1, cell_coord_free(py + ry, g.oy, g.inv_h) + 1);
+    }
+    if (y0 > y1) continue;
+    const uint32_t* sb = g.rows + size_t(cz) * g.wps;
+    for (int wy = y0 >> 5; wy <= (y1 >> 5); ++wy) {
+      const int ylo = max(y0 - wy * 32, 0), yhi = min(y1 - wy * 32, 31);
+      uint32_t rm = sb[wy] & (0xFFFFFFFFu >> (31 - yhi)) & (0xFFFFFFFFu << ylo);
+      while (rm) {  // occupied rows only
+        const int cy = wy * 32 + (__ffs(rm) - 1);
+        rm &= rm - 1;
+        const float gy = axis_gap(py, g.oy, g.h, g.margin, cy);
+        const float gyz = gz2 + gy * gy;
+        if (gyz > thr) continue;
+        const uint32_t rowi = uint32_t(cz) * uint32_t(g.dy) + uint32_t(cy);
+        const uint32_t* rb = g.bits + size_t(rowi) * g.wpr;
+        const uint32_t cbase = rowi * uint32_t(g.dx);
+        if (zedge || cy == hy - r || cy == hy + r) {
+          int x0 = max(hx - r, 0), x1 = min(hx + r, g.dx - 1);
+          if (thr < INFINITY) {
+            const float rx = __builtin_amdgcn_sqrtf(thr - gyz) + g.margin;
+            x0 = max(x0, cell_coord_free(px - rx, g.ox, g.inv_h) - 1);
+            x1 = min(x1, cell_coord_free(px + rx, g.ox, g.inv_h) + 1);
+          }
+          if (x0 > x1) continue;
+          for (int w = x0 >> 5; w <= (x1 >> 5); ++w) {
+            const int lo = max(x0 - w * 32, 0), hi = min(x1 - w * 32, 31);
+            uint32_t m = rb[w] & (0xFFFFFFFFu >> (31 - hi)) & (0xFFFFFFFFu << lo);
+            while (m) {
+              const int cx = w * 32 + (__ffs(m) - 1);
+              m &= m - 1;
+              const float gx = axis_gap(px, g.ox, g.h, g.margin, cx);
+              if (gyz + gx * gx > thr) continue;
+              visit(cbase + uint32_t(cx));
+            }
+          }
+        } else {
+          // interior row of the ring: only its two end cells are new
+          const int ends[2] = {hx - r, hx + r};
 #pragma unroll
-        for (int e = 0; e < 2; ++e) {
-          const int cx = ends[e];
-          if (cx < 0 || cx >= g.dx) continue;
-          const float gx = axis_gap(px, g.ox, g.h, g.margin, cx);
-          if (gyz + gx * gx > thr) continue;
-          visit(row + uint32_t(cx));
+          for (int e = 0; e < 2; ++e) {
+            const int cx = ends[e];
+            if (cx < 0 || cx >= g.dx) continue;
+            if (!((rb[cx >> 5] >> (cx & 31)) & 1u)) continue;
+            const float gx = axis_gap(px, g.ox, g.h, g.margin, cx);
+            if (gyz + gx * gx > thr) continue;
+            visit(cbase + uint32_t(cx));
+          }
         }
       }
+    }
+  }
+}
+
+// Visits every point filed in the 3x3x3 block of cells around (hx,hy,hz)
+// (rings 0 and 1). In grid order the three cells of a block row are adjacent,
+// so the block is nine contiguous point ranges; all eighteen range bounds are
+// loaded up front (independent loads) and the points are then streamed, which
+// keeps the chain of dependent memory round trips at two.
+template <typename VisitPoint>
+__device__ __forceinline__ void scan_block27(const GridDesc& g, const uint32_t* __restrict__ cs,
+                                             const float4* __restrict__ pts, int hx, int hy, int hz,
+                                             VisitPoint&& visit) {
+  const int x0 = max(hx - 1, 0), x1 = min(hx + 1, g.dx - 1);
+  if (x0 > x1) return;
+  uint32_t lo[9], hi[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    const int cz = hz - 1 + k / 3, cy = hy - 1 + k % 3;
+    lo[k] = hi[k] = 0;
+    if (cz >= 0 && cz < g.dz && cy >= 0 && cy < g.dy) {
+      const uint32_t base = (uint32_t(cz) * uint32_t(g.dy) + uint32_t(cy)) * uint32_t(g.dx);
+      lo[k] = cs[base + uint32_t(x0)];
+      hi[k] = cs[base + uint32_t(x1) + 1];
+    }
+  }
+  // Four independent loads per step (indices clamped into the range, extra
+  // lanes of the step skipped) so a wave keeps several requests in flight.
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    for (uint32_t s = lo[k]; s < hi[k]; s += 4) {
+      const uint32_t last = hi[k] - 1;
+      const float4 q0 = pts[s];
+      const float4 q1 = pts[min(s + 1, last)];
+      const float4 q2 = pts[min(s + 2, last)];
+      const float4 q3 = pts[min(s + 3, last)];
+      visit(s, q0);
+      if (s + 1 <= last) visit(s + 1, q1);
+      if (s + 2 <= last) visit(s + 2, q2);
+      if (s + 3 <= last) visit(s + 3, q3);
     }
   }
 }

@@ -84,6 +84,7 @@ SIGNATURES = {
                                ctypes.POINTER(ctypes.c_uint64)]),
     "sr_profile_read_each": (ctypes.c_int64, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                               ctypes.c_uint64]),
+    "sr_profile_read_stats": (_I32, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint64)]),
     "sr_gicp_align": (_I32, [ctypes.POINTER(XyzView), ctypes.POINTER(XyzView),
                              ctypes.POINTER(GicpConfigC), PolicyC, ctypes.POINTER(ResultC),
                              ctypes.POINTER(ReceiptC)]),
