@@ -76,6 +76,20 @@ def _cpu_baseline(w, budget_iters=(1, 3)):
             "host_cpu": _cpu_model(), "host_nproc": os.cpu_count()}
 
 
+def _traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    counter summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
+    calibration), or None when no summary exists."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}.{workload}.pmc.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def _cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -97,6 +111,9 @@ def main():
                     help="count grid-search work (rings/cells/points per query) on one extra run")
     ap.add_argument("--pass-times", action="store_true",
                     help="print each Gauss-Newton pass duration of the last registration")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="run one device-to-device copy of this many MiB first (a plain copy "
+                         "kernel of known size for calibrating hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (registrations "
@@ -110,6 +127,8 @@ def main():
         print("no gfx950 HIP device: " + native.last_error(), file=sys.stderr)
         sys.exit(1)
     lib = native.load()
+    if args.copy_mib > 0:
+        devmem.copy_calibration(args.copy_mib << 20)
     w = _workload(args.workload)
     cfg = _config(w)
     src = devmem.DeviceCloud(w.source)
@@ -199,7 +218,8 @@ def main():
         "kernels_ms_per_step": {k: v[0] / args.steps for k, v in kern.items()},
         "roofline": {"kernel": "k_gicp_pass", "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes_per_launch": algo_bytes,
+                     "traffic": _traffic("k_gicp_pass", w.name),
+                     "algorithmic_bytes_per_launch": algo_bytes,
                      "avg_launch_us": avg_pass_s * 1e6, "launches": pass_n},
         "cpu_baseline": None,
     }

@@ -215,7 +215,7 @@ int32_t sr_map_align(const sr_map* map, const sr_xyz_view* source, const sr_gicp
 /* ---- batched registration ---------------------------------------------- */
 
 /* Registers `count` independent (source, map) pairs in one set of launches
- * (one launch per Gauss-Newton pass for the whole batch, captured as a HIP graph).
+ * (one launch per Gauss-Newton pass for the whole batch).
  * Each result equals what sr_map_align returns for that pair alone. The
  * reference registers one pair per call; this is the batched form of
  * GicpRegistration::align (gicp.rs:103-224). */

@@ -62,5 +62,19 @@ class DeviceCloud:
             c.close()
 
 
+def copy_calibration(nbytes: int):
+    """One device-to-device hipMemcpy of `nbytes` (the runtime's plain copy
+    kernel): a known-size reference for hardware byte counters."""
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    _ok(_rt().hipMalloc(ctypes.byref(a), nbytes), "hipMalloc")
+    _ok(_rt().hipMalloc(ctypes.byref(b), nbytes), "hipMalloc")
+    try:
+        _ok(_rt().hipMemcpy(b, a, nbytes, 3), "hipMemcpy D2D")
+        synchronize()
+    finally:
+        _rt().hipFree(a)
+        _rt().hipFree(b)
+
+
 def synchronize():
     _ok(_rt().hipDeviceSynchronize(), "hipDeviceSynchronize")

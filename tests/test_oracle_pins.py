@@ -356,3 +356,49 @@ def test_gicp_recovers_ground_truth_small_cube(orc):
     rf = orc.gicp_align(sv, tv, cfg, mode="faithful")
     assert synth.rotation_angle_between(rf["q"], r64["q"]) < 1e-5
     assert np.linalg.norm(rf["t"] - r64["t"]) < 1e-4
+
+
+def test_grid_bounds_invariants(orc):
+    # uniform_grid.rs:9-36: origin = per-axis min, every point's cell index is in
+    # range, the extreme point lands in the last cell, and the 64M-cell cap errors.
+    rng = np.random.default_rng(7)
+    for scale, cell in ((1.0, 0.07), (10.0, 0.5), (3.0, 1.0)):
+        pts = (rng.standard_normal((3000, 3)) * scale).astype(np.float32)
+        st, org, dims = orc.grid_bounds(pts, cell)
+        assert st == 0
+        assert np.array_equal(org, pts.min(0))
+        inv = np.float32(1.0) / np.float32(cell)
+        idx = np.floor((pts - org) * inv).astype(np.int64)
+        assert idx.min() >= 0 and np.all(idx.max(0) == dims.astype(np.int64) - 1)
+    st, _, _ = orc.grid_bounds(np.array([[0, 0, 0], [1000, 1000, 1000]], np.float32), 0.01)
+    assert st != 0
+
+
+def test_radius_covariances_match_bruteforce_neighbourhoods(orc):
+    # The 27-cell grid gather (covariances_grid.rs:81-206) must see exactly the
+    # brute-force set {j : |p_i - p_j|^2 <= r^2}; an FP64 covariance of that set,
+    # regularized to I - (1-eps) n n^T (eps I below 3 neighbours), must match.
+    rng = np.random.default_rng(11)
+    plane = np.c_[rng.uniform(0, 3, (1500, 2)), rng.normal(0, 0.01, 1500)]
+    blob = rng.normal([1.5, 1.5, 1.0], 0.3, (300, 3))
+    pts = np.r_[plane, blob, [[9.0, 9.0, 9.0]]].astype(np.float32)
+    r, eps = 0.25, 1e-3
+    cov, _ = orc.radius_covariances(pts, r, eps)
+    d = pts[:, None, :] - pts[None, :, :]
+    d2 = (d * d).sum(-1, dtype=np.float32)
+    checked = 0
+    for i in range(len(pts)):
+        nb = pts[d2[i] <= np.float32(r * r)].astype(np.float64)
+        if len(nb) < 3:
+            assert np.allclose(cov[i], [eps, eps, eps, 0, 0, 0], atol=1e-7), i
+            continue
+        c = np.cov(nb.T, bias=False)
+        w, v = np.linalg.eigh(c)
+        if w[1] - w[0] < 1e-2 * w[2]:
+            continue  # normal ill-conditioned: any f32 solver may pick another
+        n = v[:, 0]
+        m = np.eye(3) - (1 - eps) * np.outer(n, n)
+        want = [m[0, 0], m[1, 1], m[2, 2], m[0, 1], m[0, 2], m[1, 2]]
+        assert np.allclose(cov[i], want, atol=2e-4), (i, cov[i], want)
+        checked += 1
+    assert checked > 1000
