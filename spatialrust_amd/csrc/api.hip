@@ -20,8 +20,6 @@
 
 namespace sr {
 
-void fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
-
 // ------------------------------------------------------------- runtime ----
 
 static thread_local std::string t_last_error;
@@ -324,6 +322,7 @@ struct Batch {
   DevBuf<InstState> states;
   DevBuf<float> partials;
   DevBuf<float> init;
+  DevBuf<uint32_t> match;
   uint32_t total_blocks = 0, ppt = 1, count = 0, max_passes = 0;
 };
 
@@ -346,7 +345,10 @@ static void batch_setup(const std::vector<const Prepared*>& src, const std::vect
   }
   B.states.resize(count);
   B.partials.resize(size_t(blocks) * 64);
+  B.match.resize(std::max<uint64_t>(total_pts, 1));
+  fill_u32(B.match.p, kNone, total_pts, s);
   uint32_t begin = 0;
+  uint64_t src_begin = 0;
   for (uint32_t i = 0; i < count; ++i) {
     IterInstance& I = hi[i];
     I.src = src[i]->grid.pts.p;
@@ -355,6 +357,8 @@ static void batch_setup(const std::vector<const Prepared*>& src, const std::vect
     I.tgt_n = tgt[i]->normals.p;
     I.cell_start = tgt[i]->grid.cell_start.p;
     I.empty_radius = tgt[i]->grid.empty_radius.p;
+    I.match = B.match.p + src_begin;
+    src_begin += src[i]->n;
     I.grid = tgt[i]->grid.desc;
     I.cfg = inst_config(cfgs[i]);
     I.state = B.states.p + i;
@@ -489,21 +493,22 @@ const char* sr_last_error(void) { return t_last_error.c_str(); }
 
 void sr_profile_enable(int32_t on) { profiler().on.store(on == 2 ? 2 : (on ? 1 : 0)); }
 
-int32_t sr_profile_read_stats(const char* name, uint64_t* out4) {
+int32_t sr_profile_read_stats(const char* name, uint64_t* out8) {
   return guarded([&] {
     Profiler& P = profiler();
     SR_HIP(hipDeviceSynchronize());
     std::lock_guard<std::mutex> lk(P.mu);
     auto it = P.stats.find(name);
-    SearchStats h{0, 0, 0, 0};
+    SearchStats h{};
     if (it != P.stats.end()) {
       SR_HIP(hipMemcpy(&h, it->second, sizeof(h), hipMemcpyDeviceToHost));
       SR_HIP(hipMemset(it->second, 0, sizeof(h)));
     }
-    out4[0] = h.queries;
-    out4[1] = h.rings;
-    out4[2] = h.cells;
-    out4[3] = h.points;
+    out8[0] = h.queries;
+    out8[1] = h.rings;
+    out8[2] = h.cells;
+    out8[3] = h.points;
+    for (int o = 0; o < 4; ++o) out8[4 + o] = h.outcome[o];
   });
 }
 

@@ -102,7 +102,7 @@ __device__ int knn_search(const GridDesc& g, const uint32_t* __restrict__ cs,
     if (worst_i != kNone) {
       const float cl = ring_clearance(g, px, py, pz, hx, hy, hz, 1);
       if (cl > 0.0f && worst_d < cl * cl) {
-        stats_add(stats, n_rings, n_cells, n_pts);
+        stats_add(stats, n_rings, n_cells, n_pts, kBlockDone);
         return min(k, int(g.npoints));
       }
     }
@@ -110,18 +110,16 @@ __device__ int knn_search(const GridDesc& g, const uint32_t* __restrict__ cs,
   }
   for (int r = rfirst; r <= rmax; ++r) {
     ++n_rings;
-    scan_ring(g, px, py, pz, hx, hy, hz, r, worst_d, [&](uint32_t c) {
-      const uint32_t hi = cs[c + 1];
-      ++n_cells;
-      n_pts += hi - cs[c];
-      for (uint32_t s = cs[c]; s < hi; ++s) consider(s, gp[s]);
+    scan_ring<1>(g, cs, gp, px, py, pz, hx, hy, hz, r, worst_d, n_cells, [&](uint32_t s, const float4& q) {
+      ++n_pts;
+      consider(s, q);
     });
     if (worst_i != kNone) {
       const float cl = ring_clearance(g, px, py, pz, hx, hy, hz, r);
       if (cl > 0.0f && worst_d < cl * cl) break;
     }
   }
-  stats_add(stats, n_rings, n_cells, n_pts);
+  stats_add(stats, n_rings, n_cells, n_pts, kRingFound);
   return min(k, int(g.npoints));
 }
 

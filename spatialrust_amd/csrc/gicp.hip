@@ -213,11 +213,13 @@ __device__ __forceinline__ void finalize(InstState* st, const InstConfig& cfg, c
 }
 
 // Nearest target point of p within dmax (exact; ties to the lower index).
-// Returns the grid position or kNone.
+// Returns the grid position or kNone. `prev` (grid position or kNone) is the
+// previous pass's correspondence of this source point: a real candidate, so it
+// seeds the best and the search radius without changing the answer.
 __device__ __forceinline__ uint32_t nearest_within(const GridDesc& g, const uint32_t* __restrict__ cs,
                                                    const uint8_t* __restrict__ empty,
                                                    const float4* __restrict__ tp, float px, float py,
-                                                   float pz, float dmax, float dmax2,
+                                                   float pz, float dmax, float dmax2, uint32_t prev,
                                                    float& best_d, SearchStats* stats) {
   best_d = INFINITY;
   uint32_t best_pos = kNone, best_id = kNone;
@@ -229,7 +231,7 @@ __device__ __forceinline__ uint32_t nearest_within(const GridDesc& g, const uint
     float bz = fmaxf(fmaxf(g.oz - pz, pz - (g.oz + g.h * g.dz)), 0.0f);
     float lb = bx * bx + by * by + bz * bz;
     if (lb > dmax2 + 2.0f * dmax * g.margin + g.margin * g.margin) {
-      stats_add(stats, 0, 0, 0);
+      stats_add(stats, 0, 0, 0, kRejected);
       return kNone;
     }
   }
@@ -240,11 +242,11 @@ __device__ __forceinline__ uint32_t nearest_within(const GridDesc& g, const uint
   const bool inside = hx >= 0 && hx < g.dx && hy >= 0 && hy < g.dy && hz >= 0 && hz < g.dz;
   int e = 0;
   if (empty && inside) {
-    // Rings closer than the empty radius hold no point; a point in ring e is at
-    // least (e - 1) cells away.
+    // Rings closer than the empty radius hold no point, so every point lies
+    // outside the box of rings 0..e-1: its clearance bounds the distance.
     e = empty[(uint32_t(hz) * uint32_t(g.dy) + uint32_t(hy)) * uint32_t(g.dx) + uint32_t(hx)];
-    if (float(e - 1) * g.h - g.margin > dmax) {
-      stats_add(stats, 0, 0, 0);
+    if (e > 0 && ring_clearance(g, px, py, pz, hx, hy, hz, e - 1) > dmax) {
+      stats_add(stats, 0, 0, 0, kRejected);
       return kNone;
     }
     rstart = e;
@@ -261,33 +263,32 @@ __device__ __forceinline__ uint32_t nearest_within(const GridDesc& g, const uint
       thr = d2;
     }
   };
+  if (prev != kNone) consider(prev, tp[prev]);
   if (inside && e <= 1) {
-    // Rings 0 and 1 as nine contiguous point ranges; most queries of a
-    // converging registration end here.
+    // Rings 0 and 1 as nine contiguous point ranges, cut to the sphere of the
+    // seeded radius; most queries of a converging registration end here.
     ++n_rings;
-    scan_block27(g, cs, tp, hx, hy, hz, [&](uint32_t s, const float4& q) {
+    scan_block27_within(g, cs, tp, px, py, pz, hx, hy, hz, thr, [&](uint32_t s, const float4& q) {
       ++n_pts;
       consider(s, q);
     });
     const float cl = ring_clearance(g, px, py, pz, hx, hy, hz, 1);
     if (cl > 0.0f && thr < cl * cl) {
-      stats_add(stats, n_rings, n_cells, n_pts);
+      stats_add(stats, n_rings, n_cells, n_pts, kBlockDone);
       return best_pos;
     }
     rstart = 2;
   }
   for (int r = rstart; r <= rlim; ++r) {
     ++n_rings;
-    scan_ring(g, px, py, pz, hx, hy, hz, r, thr, [&](uint32_t c) {
-      const uint32_t hi = cs[c + 1];
-      ++n_cells;
-      n_pts += hi - cs[c];
-      for (uint32_t s = cs[c]; s < hi; ++s) consider(s, tp[s]);
+    scan_ring(g, cs, tp, px, py, pz, hx, hy, hz, r, thr, n_cells, [&](uint32_t s, const float4& q) {
+      ++n_pts;
+      consider(s, q);
     });
     const float cl = ring_clearance(g, px, py, pz, hx, hy, hz, r);
     if (cl > 0.0f && thr < cl * cl) break;
   }
-  stats_add(stats, n_rings, n_cells, n_pts);
+  stats_add(stats, n_rings, n_cells, n_pts, best_pos == kNone ? kRingNone : kRingFound);
   return best_pos;
 }
 
@@ -395,9 +396,9 @@ __global__ void __launch_bounds__(kBlock) k_gicp_pass(const IterInstance* __rest
     const float py = fmaf(R[3], p.x, fmaf(R[4], p.y, fmaf(R[5], p.z, T[1] + ly)));
     const float pz = fmaf(R[6], p.x, fmaf(R[7], p.y, fmaf(R[8], p.z, T[2] + lz)));
     float d2;
-    const uint32_t j =
-        nearest_within(g, I.cell_start, I.empty_radius, I.tgt, px, py, pz, cfg.dmax, cfg.dmax2, d2,
-                       stats);
+    const uint32_t j = nearest_within(g, I.cell_start, I.empty_radius, I.tgt, px, py, pz, cfg.dmax,
+                                      cfg.dmax2, I.match[i], d2, stats);
+    I.match[i] = j;
     if (j == kNone) continue;
     const float4 q = I.tgt[j];
     const float4 nt = I.tgt_n[j];

@@ -16,8 +16,6 @@
 
 namespace sr {
 
-void fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
-
 constexpr uint64_t kMaxCells = 64000000ull;  // MAX_UNIFORM_GRID_CELLS, uniform_grid.rs:6
 
 void host_bounds(const Cloud& c, uint32_t* host6, hipStream_t s) {

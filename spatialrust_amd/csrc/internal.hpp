@@ -91,6 +91,9 @@ struct ProfScope {
 // (sr_profile_enable(2)); nullptr otherwise.
 SearchStats* prof_stats(const char* name);
 
+// Fills n u32 words with v.
+void fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+
 // ---- scan ----------------------------------------------------------------
 
 // Exclusive prefix sum of n u32 values (in may equal out). If total is non-null
@@ -212,6 +215,8 @@ struct IterInstance {
   const float4* tgt_n;    // target normals in grid order
   const uint32_t* cell_start;
   const uint8_t* empty_radius;  // target grid's empty-radius field (may be null)
+  uint32_t* match;        // per source point: target grid position matched by the
+                          // previous pass (kNone: none); seeds the next search
   GridDesc grid;
   InstConfig cfg;
   InstState* state;

@@ -11,17 +11,23 @@ constexpr int kBlock = 256;   // threads per workgroup for point-parallel kernel
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 
 // Optional search counters (profiling): summed over all queries of a launch.
+// Work counters of a grid search (search-statistics mode). outcome[]: queries
+// rejected without a scan, finished in the 3x3x3 block, finished in the ring
+// search with a neighbour, finished in the ring search without one.
+enum { kRejected = 0, kBlockDone = 1, kRingFound = 2, kRingNone = 3 };
 struct SearchStats {
   unsigned long long queries, rings, cells, points;
+  unsigned long long outcome[4];
 };
 
 __device__ __forceinline__ void stats_add(SearchStats* st, uint32_t rings, uint32_t cells,
-                                          uint32_t points) {
+                                          uint32_t points, int outcome) {
   if (!st) return;
   atomicAdd(&st->queries, 1ull);
   atomicAdd(&st->rings, (unsigned long long)rings);
   atomicAdd(&st->cells, (unsigned long long)cells);
   atomicAdd(&st->points, (unsigned long long)points);
+  atomicAdd(&st->outcome[outcome], 1ull);
 }
 
 // Uniform grid description. Cell (cx,cy,cz) has linear id (cz*dy + cy)*dx + cx,
@@ -126,14 +132,48 @@ __device__ __forceinline__ float axis_gap(float v, float o, float h, float m, in
   return fmaxf(fmaxf(lo - v, v - (lo + h + 2.0f * m)), 0.0f);
 }
 
-// Visits every occupied cell of Chebyshev ring r around (hx,hy,hz) that lies
-// inside the grid and can hold a point within sqrt(thr) of p (thr is re-read as
-// `visit` tightens it). Slabs and rows outside the sphere are cut analytically
-// and each row's cells come from the occupancy bitmap, so empty cells cost
-// nothing; only occupied cells within the sphere reach visit(c).
-template <typename Visit>
-__device__ __forceinline__ void scan_ring(const GridDesc& g, float px, float py, float pz, int hx,
-                                          int hy, int hz, int r, const float& thr, Visit&& visit) {
+// Visits the points of N contiguous ranges [lo[k], hi[k]) of a grid-ordered
+// array. With W = 4, four independent loads per step (indices clamped into the
+// range, the extra lanes of a step skipped) keep several requests in flight per
+// lane; W = 1 visits one point per step, for visitors too large to inline four
+// times.
+template <int W = 4, int N, typename VisitPoint>
+__device__ __forceinline__ void stream_ranges(const uint32_t (&lo)[N], const uint32_t (&hi)[N],
+                                              const float4* __restrict__ pts, VisitPoint&& visit) {
+  static_assert(W == 1 || W == 4, "stream width");
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    if (W == 1) {
+      for (uint32_t s = lo[k]; s < hi[k]; ++s) visit(s, pts[s]);
+      continue;
+    }
+    for (uint32_t s = lo[k]; s < hi[k]; s += 4) {
+      const uint32_t last = hi[k] - 1;
+      const float4 q0 = pts[s];
+      const float4 q1 = pts[min(s + 1, last)];
+      const float4 q2 = pts[min(s + 2, last)];
+      const float4 q3 = pts[min(s + 3, last)];
+      visit(s, q0);
+      if (s + 1 <= last) visit(s + 1, q1);
+      if (s + 2 <= last) visit(s + 2, q2);
+      if (s + 3 <= last) visit(s + 3, q3);
+    }
+  }
+}
+
+// Visits every point filed in Chebyshev ring r around (hx,hy,hz) whose cell can
+// hold a point within sqrt(thr) of p (thr is re-read as `visit` tightens it).
+// Slabs and rows outside the sphere are cut analytically and empty rows are
+// skipped through the slab's row bitmap. In grid order a row's cells are
+// adjacent, so a face row of the ring is one contiguous point range (its x span
+// cut to the sphere) and an interior row contributes its two end cells; each
+// row's range bounds are independent loads and its points are streamed.
+// `rows` counts the rows scanned; W is the streaming width (stream_ranges).
+template <int W = 4, typename VisitPoint>
+__device__ __forceinline__ void scan_ring(const GridDesc& g, const uint32_t* __restrict__ cs,
+                                          const float4* __restrict__ pts, float px, float py,
+                                          float pz, int hx, int hy, int hz, int r, const float& thr,
+                                          uint32_t& rows, VisitPoint&& visit) {
   const int z0 = max(hz - r, 0), z1 = min(hz + r, g.dz - 1);
   for (int cz = z0; cz <= z1; ++cz) {
     const float gz = axis_gap(pz, g.oz, g.h, g.margin, cz);
@@ -159,9 +199,8 @@ __device__ __forceinline__ void scan_ring(const GridDesc& g, float px, float py,
         const float gy = axis_gap(py, g.oy, g.h, g.margin, cy);
         const float gyz = gz2 + gy * gy;
         if (gyz > thr) continue;
-        const uint32_t rowi = uint32_t(cz) * uint32_t(g.dy) + uint32_t(cy);
-        const uint32_t* rb = g.bits + size_t(rowi) * g.wpr;
-        const uint32_t cbase = rowi * uint32_t(g.dx);
+        const uint32_t cbase = (uint32_t(cz) * uint32_t(g.dy) + uint32_t(cy)) * uint32_t(g.dx);
+        uint32_t lo[2] = {0, 0}, hi[2] = {0, 0};
         if (zedge || cy == hy - r || cy == hy + r) {
           int x0 = max(hx - r, 0), x1 = min(hx + r, g.dx - 1);
           if (thr < INFINITY) {
@@ -170,17 +209,8 @@ __device__ __forceinline__ void scan_ring(const GridDesc& g, float px, float py,
             x1 = min(x1, cell_coord_free(px + rx, g.ox, g.inv_h) + 1);
           }
           if (x0 > x1) continue;
-          for (int w = x0 >> 5; w <= (x1 >> 5); ++w) {
-            const int lo = max(x0 - w * 32, 0), hi = min(x1 - w * 32, 31);
-            uint32_t m = rb[w] & (0xFFFFFFFFu >> (31 - hi)) & (0xFFFFFFFFu << lo);
-            while (m) {
-              const int cx = w * 32 + (__ffs(m) - 1);
-              m &= m - 1;
-              const float gx = axis_gap(px, g.ox, g.h, g.margin, cx);
-              if (gyz + gx * gx > thr) continue;
-              visit(cbase + uint32_t(cx));
-            }
-          }
+          lo[0] = cs[cbase + uint32_t(x0)];
+          hi[0] = cs[cbase + uint32_t(x1) + 1];
         } else {
           // interior row of the ring: only its two end cells are new
           const int ends[2] = {hx - r, hx + r};
@@ -188,12 +218,14 @@ __device__ __forceinline__ void scan_ring(const GridDesc& g, float px, float py,
           for (int e = 0; e < 2; ++e) {
             const int cx = ends[e];
             if (cx < 0 || cx >= g.dx) continue;
-            if (!((rb[cx >> 5] >> (cx & 31)) & 1u)) continue;
             const float gx = axis_gap(px, g.ox, g.h, g.margin, cx);
             if (gyz + gx * gx > thr) continue;
-            visit(cbase + uint32_t(cx));
+            lo[e] = cs[cbase + uint32_t(cx)];
+            hi[e] = cs[cbase + uint32_t(cx) + 1];
           }
         }
+        ++rows;
+        stream_ranges<W>(lo, hi, pts, visit);
       }
     }
   }
@@ -221,22 +253,44 @@ __device__ __forceinline__ void scan_block27(const GridDesc& g, const uint32_t* 
       hi[k] = cs[base + uint32_t(x1) + 1];
     }
   }
-  // Four independent loads per step (indices clamped into the range, extra
-  // lanes of the step skipped) so a wave keeps several requests in flight.
+  stream_ranges(lo, hi, pts, visit);
+}
+
+// scan_block27 restricted to the cells that can hold a point within sqrt(thr)
+// of p, for a query inside the grid (hx,hy,hz = its cell): a block row whose
+// (y,z) gap exceeds thr is skipped, and each kept row is cut to the cells whose
+// x gap still fits (the query's own column always stays). Gaps use the filing
+// margin, so no point within sqrt(thr) is ever cut.
+template <typename VisitPoint>
+__device__ __forceinline__ void scan_block27_within(const GridDesc& g, const uint32_t* __restrict__ cs,
+                                                    const float4* __restrict__ pts, float px,
+                                                    float py, float pz, int hx, int hy, int hz,
+                                                    float thr, VisitPoint&& visit) {
+  const float gxl = axis_gap(px, g.ox, g.h, g.margin, hx - 1);
+  const float gxr = axis_gap(px, g.ox, g.h, g.margin, hx + 1);
+  float gy2[3], gz2[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float gy = axis_gap(py, g.oy, g.h, g.margin, hy - 1 + k);
+    const float gz = axis_gap(pz, g.oz, g.h, g.margin, hz - 1 + k);
+    gy2[k] = gy * gy;
+    gz2[k] = gz * gz;
+  }
+  uint32_t lo[9], hi[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
-    for (uint32_t s = lo[k]; s < hi[k]; s += 4) {
-      const uint32_t last = hi[k] - 1;
-      const float4 q0 = pts[s];
-      const float4 q1 = pts[min(s + 1, last)];
-      const float4 q2 = pts[min(s + 2, last)];
-      const float4 q3 = pts[min(s + 3, last)];
-      visit(s, q0);
-      if (s + 1 <= last) visit(s + 1, q1);
-      if (s + 2 <= last) visit(s + 2, q2);
-      if (s + 3 <= last) visit(s + 3, q3);
+    const int cz = hz - 1 + k / 3, cy = hy - 1 + k % 3;
+    const float gyz = gz2[k / 3] + gy2[k % 3];
+    lo[k] = hi[k] = 0;
+    if (cz >= 0 && cz < g.dz && cy >= 0 && cy < g.dy && gyz <= thr) {
+      const int xa = (hx > 0 && gyz + gxl * gxl <= thr) ? hx - 1 : hx;
+      const int xb = (hx + 1 < g.dx && gyz + gxr * gxr <= thr) ? hx + 1 : hx;
+      const uint32_t base = (uint32_t(cz) * uint32_t(g.dy) + uint32_t(cy)) * uint32_t(g.dx);
+      lo[k] = cs[base + uint32_t(xa)];
+      hi[k] = cs[base + uint32_t(xb) + 1];
     }
   }
+  stream_ranges(lo, hi, pts, visit);
 }
 
 // Kahan-compensated f32 accumulator.
