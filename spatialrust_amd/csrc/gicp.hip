@@ -237,8 +237,6 @@ __device__ __forceinline__ uint32_t nearest_within(const GridDesc& g, const uint
   }
   const int hx = cell_coord_free(px, g.ox, g.inv_h), hy = cell_coord_free(py, g.oy, g.inv_h),
             hz = cell_coord_free(pz, g.oz, g.inv_h);
-  const int rlim = int(dmax * g.inv_h) + 2;
-  int rstart = 0;
   const bool inside = hx >= 0 && hx < g.dx && hy >= 0 && hy < g.dy && hz >= 0 && hz < g.dz;
   int e = 0;
   if (empty && inside) {
@@ -249,7 +247,6 @@ __device__ __forceinline__ uint32_t nearest_within(const GridDesc& g, const uint
       stats_add(stats, 0, 0, 0, kRejected);
       return kNone;
     }
-    rstart = e;
   }
   // thr = min(best so far, dmax^2): cells and rows beyond it are never touched.
   float thr = dmax2;
@@ -277,17 +274,15 @@ __device__ __forceinline__ uint32_t nearest_within(const GridDesc& g, const uint
       stats_add(stats, n_rings, n_cells, n_pts, kBlockDone);
       return best_pos;
     }
-    rstart = 2;
   }
-  for (int r = rstart; r <= rlim; ++r) {
-    ++n_rings;
-    scan_ring(g, cs, tp, px, py, pz, hx, hy, hz, r, thr, n_cells, [&](uint32_t s, const float4& q) {
-      ++n_pts;
-      consider(s, q);
-    });
-    const float cl = ring_clearance(g, px, py, pz, hx, hy, hz, r);
-    if (cl > 0.0f && thr < cl * cl) break;
-  }
+  // Not certified by the block: every row of the sphere of the current radius,
+  // each once (the block's cells excluded when it was scanned).
+  ++n_rings;
+  scan_ball(g, cs, tp, px, py, pz, hx, hy, hz, thr, inside && e <= 1, n_cells,
+            [&](uint32_t s, const float4& q) {
+              ++n_pts;
+              consider(s, q);
+            });
   stats_add(stats, n_rings, n_cells, n_pts, best_pos == kNone ? kRingNone : kRingFound);
   return best_pos;
 }
