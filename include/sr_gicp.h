@@ -158,6 +158,21 @@ int32_t sr_gicp_align(const sr_xyz_view* source, const sr_xyz_view* target,
                       const sr_gicp_config* config, sr_policy policy,
                       sr_result* out, sr_receipt* receipt);
 
+/* One correspondence-and-accumulate pass of the Gauss-Newton loop at a given
+ * FP64 pose (q = x y z w, t), without the solve: the loop body of
+ * GicpRegistration::align_with_diagnostics (crates/spatialrust-registration/src/gicp.rs:137-179)
+ * as the device runs it. Per source point, the index of its target
+ * correspondence (the nearest target point within max_correspondence_distance,
+ * ties to the lower index) or -1; the FP64 totals H (6x6 row-major, full
+ * symmetric), g (6), the correspondence count and the sum of squared
+ * correspondence distances. Clouds are used as given (voxel_leaf is ignored);
+ * covariances follow config (k_neighbors or covariance_radius). Any output may
+ * be NULL. For checking the device pass against a CPU evaluation. */
+int32_t sr_gicp_normal_equations(const sr_xyz_view* source, const sr_xyz_view* target,
+                                 const sr_gicp_config* config, const double* q, const double* t,
+                                 int32_t* out_match, double* out_H, double* out_g,
+                                 uint64_t* out_count, double* out_d2_sum);
+
 /* ---- voxel downsample (VoxelGridDownsample, centroid mode, xyz schema) -- */
 
 /* Replaces VoxelGridDownsample::filter_with_policy for the centroid/xyz case

@@ -321,13 +321,14 @@ struct Batch {
   DevBuf<uint32_t> block_inst;
   DevBuf<InstState> states;
   DevBuf<float> partials;
-  DevBuf<float> init;
+  DevBuf<double> init;
   DevBuf<uint32_t> match;
   uint32_t total_blocks = 0, ppt = 1, count = 0, max_passes = 0;
 };
 
 static void batch_setup(const std::vector<const Prepared*>& src, const std::vector<const Prepared*>& tgt,
-                        const std::vector<sr_gicp_config>& cfgs, Batch& B, hipStream_t s) {
+                        const std::vector<sr_gicp_config>& cfgs, Batch& B, hipStream_t s,
+                        const double* pose7 = nullptr) {
   const uint32_t count = uint32_t(src.size());
   B.count = count;
   uint64_t total_pts = 0;
@@ -335,7 +336,7 @@ static void batch_setup(const std::vector<const Prepared*>& src, const std::vect
   B.ppt = pick_ppt(total_pts);
   std::vector<IterInstance> hi(count);
   std::vector<uint32_t> hbi;
-  std::vector<float> init(size_t(7) * count);
+  std::vector<double> init(size_t(7) * count);
   uint32_t blocks = 0;
   B.max_passes = 0;
   std::vector<uint32_t> nbl(count);
@@ -368,8 +369,10 @@ static void batch_setup(const std::vector<const Prepared*>& src, const std::vect
     I.nblocks = nbl[i];
     for (uint32_t b = 0; b < nbl[i]; ++b) hbi.push_back(i);
     begin += nbl[i];
-    std::memcpy(&init[7 * size_t(i)], cfgs[i].init_q, 4 * sizeof(float));
-    std::memcpy(&init[7 * size_t(i) + 4], cfgs[i].init_t, 3 * sizeof(float));
+    for (int a = 0; a < 4; ++a) init[7 * size_t(i) + a] = double(cfgs[i].init_q[a]);
+    for (int a = 0; a < 3; ++a) init[7 * size_t(i) + 4 + a] = double(cfgs[i].init_t[a]);
+    if (pose7)
+      for (int a = 0; a < 7; ++a) init[7 * size_t(i) + a] = pose7[7 * size_t(i) + a];
     B.max_passes = std::max(B.max_passes, cfgs[i].max_iterations + 1);
   }
   B.total_blocks = blocks;
@@ -378,7 +381,7 @@ static void batch_setup(const std::vector<const Prepared*>& src, const std::vect
   B.init.resize(init.size());
   SR_HIP(hipMemcpyAsync(B.insts.p, hi.data(), sizeof(IterInstance) * count, hipMemcpyHostToDevice, s));
   SR_HIP(hipMemcpyAsync(B.block_inst.p, hbi.data(), sizeof(uint32_t) * blocks, hipMemcpyHostToDevice, s));
-  SR_HIP(hipMemcpyAsync(B.init.p, init.data(), sizeof(float) * init.size(), hipMemcpyHostToDevice, s));
+  SR_HIP(hipMemcpyAsync(B.init.p, init.data(), sizeof(double) * init.size(), hipMemcpyHostToDevice, s));
   init_states(B.states.p, B.init.p, count, s);
 }
 
@@ -614,6 +617,58 @@ int32_t sr_gicp_align(const sr_xyz_view* source, const sr_xyz_view* target,
     std::string msg;
     int32_t code = state_to_result(st, *config, src, tgt, out, &msg);
     if (code) fail(code, msg);
+  });
+}
+
+int32_t sr_gicp_normal_equations(const sr_xyz_view* source, const sr_xyz_view* target,
+                                 const sr_gicp_config* config, const double* q, const double* t,
+                                 int32_t* out_match, double* out_H, double* out_g,
+                                 uint64_t* out_count, double* out_d2_sum) {
+  return guarded([&] {
+    resolve(sr_policy{SR_GPU, SR_DEV_HIP});
+    validate_view(source);
+    validate_view(target);
+    validate_cfg(source, target, config);
+    if (!q || !t) fail(SR_ERR_INVALID_ARGUMENT, "pose must not be null");
+    sr_gicp_config cfg = *config;
+    cfg.voxel_leaf = 0.0f;  // clouds as given, so indices refer to the caller's points
+    hipStream_t s = thread_stream();
+    Receipt rc;
+    Prepared tgt, src;
+    prepare(*target, cfg, tgt, rc, s, true);
+    prepare(*source, cfg, src, rc, s, false);
+    double pose[7] = {q[0], q[1], q[2], q[3], t[0], t[1], t[2]};
+    Batch B;
+    batch_setup({&src}, {&tgt}, {cfg}, B, s, pose);
+    gicp_pass(B.insts.p, B.block_inst.p, B.total_blocks, 1, B.ppt, s, false);
+    std::vector<float> part(size_t(B.total_blocks) * 64);
+    std::vector<uint32_t> match(src.n), sorder(src.n), torder(tgt.n);
+    SR_HIP(hipMemcpyAsync(part.data(), B.partials.p, part.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+    SR_HIP(hipMemcpyAsync(match.data(), B.match.p, match.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    SR_HIP(hipMemcpyAsync(sorder.data(), src.grid.order.p, sorder.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    SR_HIP(hipMemcpyAsync(torder.data(), tgt.grid.order.p, torder.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    SR_HIP(hipStreamSynchronize(s));
+    // Block partials in block order, each term as its (hi, lo) pair, in FP64:
+    // the same sums the finalize kernel forms.
+    double tot[kTerms] = {};
+    uint64_t cnt = 0;
+    for (uint32_t b = 0; b < B.total_blocks; ++b) {
+      const float* pb = part.data() + size_t(b) * 64;
+      for (int k = 0; k < kTerms; ++k) tot[k] += double(pb[2 * k]) + double(pb[2 * k + 1]);
+      uint64_t c;
+      std::memcpy(&c, pb + 60, sizeof(c));
+      cnt += c;
+    }
+    if (out_H)
+      for (int a = 0; a < 6; ++a)
+        for (int b = 0; b < 6; ++b) out_H[6 * a + b] = tot[a <= b ? ut(a, b) : ut(b, a)];
+    if (out_g)
+      for (int a = 0; a < 6; ++a) out_g[a] = tot[21 + a];
+    if (out_count) *out_count = cnt;
+    if (out_d2_sum) *out_d2_sum = tot[27];
+    if (out_match)
+      for (uint32_t i = 0; i < src.n; ++i)
+        out_match[sorder[i]] = match[i] == kNone ? -1 : int32_t(torder[match[i]]);
   });
 }
 

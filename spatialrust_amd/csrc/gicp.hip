@@ -74,12 +74,12 @@ __device__ __forceinline__ void refresh_f32_pose(InstState* st) {
   }
 }
 
-__global__ void k_init_states(InstState* states, const float* __restrict__ init7, uint32_t count) {
+__global__ void k_init_states(InstState* states, const double* __restrict__ init7, uint32_t count) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
   InstState s;
-  for (int a = 0; a < 4; ++a) s.q[a] = double(init7[7 * i + a]);
-  for (int a = 0; a < 3; ++a) s.t[a] = double(init7[7 * i + 4 + a]);
+  for (int a = 0; a < 4; ++a) s.q[a] = init7[7 * i + a];
+  for (int a = 0; a < 3; ++a) s.t[a] = init7[7 * i + 4 + a];
   s.fitness = DBL_MAX;
   s.last_update = 0.0;
   s.count = 0;
@@ -93,14 +93,11 @@ __global__ void k_init_states(InstState* states, const float* __restrict__ init7
   states[i] = s;
 }
 
-void init_states(InstState* states, const float* init7_dev, uint32_t count, hipStream_t s) {
+void init_states(InstState* states, const double* init7_dev, uint32_t count, hipStream_t s) {
   if (count == 0) return;
   hipLaunchKernelGGL(k_init_states, dim3((count + 127) / 128), dim3(128), 0, s, states, init7_dev,
                      count);
 }
-
-// Upper-triangle index of (a, b), a <= b, in a 6x6 symmetric matrix.
-__host__ __device__ constexpr int ut(int a, int b) { return a * 6 - a * (a - 1) / 2 + (b - a); }
 
 // FP64 LDL^T solve of H x = -g (H symmetric, packed upper). false if a pivot
 // falls below 1e-12 — the singularity threshold of solve_linear_system.
@@ -560,13 +557,14 @@ __global__ void __launch_bounds__(kBlock) k_gicp_finalize(const IterInstance* __
 }
 
 void gicp_pass(const IterInstance* insts, const uint32_t* block_inst, uint32_t total_blocks,
-               uint32_t count, uint32_t ppt, hipStream_t s) {
+               uint32_t count, uint32_t ppt, hipStream_t s, bool finalize) {
   if (total_blocks == 0) return;
   {
     ProfScope prof("gicp_pass", s);
     hipLaunchKernelGGL(k_gicp_pass, dim3(total_blocks), dim3(kBlock), 0, s, insts, block_inst, ppt,
                        prof_stats("gicp_pass"));
   }
+  if (!finalize) return;
   ProfScope prof("gicp_finalize", s);
   hipLaunchKernelGGL(k_gicp_finalize, dim3(count), dim3(kBlock), 0, s, insts, count);
 }

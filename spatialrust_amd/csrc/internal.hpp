@@ -229,11 +229,17 @@ struct IterInstance {
 
 constexpr int kTerms = 28;  // 21 H (upper triangle) + 6 g + fitness sum
 
+// Upper-triangle index of (a, b), a <= b, in a 6x6 symmetric matrix: the order
+// of the 21 H terms in a block partial.
+__host__ __device__ constexpr int ut(int a, int b) { return a * 6 - a * (a - 1) / 2 + (b - a); }
+
 // Launches one Gauss-Newton pass over `count` instances (device array `insts`),
-// total_blocks blocks; block_inst maps block -> instance.
+// total_blocks blocks; block_inst maps block -> instance. Without `finalize`
+// only the per-block partial sums are written (no solve, no state change).
 void gicp_pass(const IterInstance* insts, const uint32_t* block_inst, uint32_t total_blocks,
-               uint32_t count, uint32_t pts_per_thread, hipStream_t s);
-// Initialises instance states from initial guesses (host array of 7 floats per instance).
-void init_states(InstState* states, const float* init7, uint32_t count, hipStream_t s);
+               uint32_t count, uint32_t pts_per_thread, hipStream_t s, bool finalize = true);
+// Initialises instance states from poses (device array of 7 doubles per
+// instance: quaternion x y z w, translation).
+void init_states(InstState* states, const double* init7, uint32_t count, hipStream_t s);
 
 }  // namespace sr

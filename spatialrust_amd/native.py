@@ -88,6 +88,8 @@ SIGNATURES = {
     "sr_gicp_align": (_I32, [ctypes.POINTER(XyzView), ctypes.POINTER(XyzView),
                              ctypes.POINTER(GicpConfigC), PolicyC, ctypes.POINTER(ResultC),
                              ctypes.POINTER(ReceiptC)]),
+    "sr_gicp_normal_equations": (_I32, [ctypes.POINTER(XyzView), ctypes.POINTER(XyzView),
+                                        ctypes.POINTER(GicpConfigC), _P, _P, _P, _P, _P, _P, _P]),
     "sr_voxel_downsample": (_I32, [ctypes.POINTER(XyzView), ctypes.c_float, _P, PolicyC, _P, _P, _P,
                                    _P, _P, ctypes.POINTER(ctypes.c_uint64),
                                    ctypes.POINTER(ReceiptC)]),

@@ -333,6 +333,28 @@ def knn(points, queries, k: int):
     return idx[:nq, :k], d2[:nq, :k]
 
 
+def gicp_normal_equations(source, target, config: "GicpConfig", q, t):
+    """One device correspondence-and-accumulate pass at the FP64 pose (q, t)
+    (gicp.rs:137-179, no solve): (match (N,) int32 with -1 for none,
+    H (6,6), g (6,), count, sum of squared distances)."""
+    sv, keep_s = _view(source)
+    tv, keep_t = _view(target)
+    n = int(sv.n)
+    match = np.zeros(max(n, 1), np.int32)
+    H = np.zeros((6, 6))
+    g = np.zeros(6)
+    cnt = ctypes.c_uint64(0)
+    d2 = ctypes.c_double(0.0)
+    qa = np.ascontiguousarray(q, dtype=np.float64)
+    ta = np.ascontiguousarray(t, dtype=np.float64)
+    _check(_n.load().sr_gicp_normal_equations(
+        ctypes.byref(sv), ctypes.byref(tv), ctypes.byref(config._c()), qa.ctypes.data,
+        ta.ctypes.data, match.ctypes.data, H.ctypes.data, g.ctypes.data, ctypes.addressof(cnt),
+        ctypes.addressof(d2)))
+    del keep_s, keep_t
+    return match[:n], H, g, int(cnt.value), float(d2.value)
+
+
 class TargetMap:
     """A target cloud preprocessed once and kept on the device (sr_map)."""
 

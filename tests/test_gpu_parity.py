@@ -245,3 +245,43 @@ def test_receipt_records_stages_and_bytes(hip):
     assert rc.requested_policy == sr.ExecutionPolicy.auto()
     assert rc.host_to_device_bytes == 12 * (len(w.source) + len(w.target))
     assert rc.stages == ("gicp-voxel", "gicp-grid", "gicp-covariance", "gicp-iterate", "gpu-readback")
+
+
+def _fp64_pose_distance2(src, tgt, q, t):
+    R = synth.quat_to_mat(np.asarray(q, np.float64))
+    p = src.astype(np.float64) @ R.T + np.asarray(t, np.float64)
+    return p, tgt.astype(np.float64)
+
+
+@pytest.mark.parametrize("pose", ["identity", "ground_truth"])
+def test_pass_matches_fp64_oracle(hip, orc, pose):
+    """One device pass (correspondences + H, g) vs the FP64 oracle of the loop
+    body (gicp.rs:137-179) with FP64 k-NN covariances.
+
+    Correspondences: equal, except where the FP64 evaluation itself is within
+    f32 rounding of a tie (nearest two candidates within 1e-5 m; coordinates
+    reach 20 m, where an f32 ulp is 1.9e-6 m) or of the d_max gate; H within
+    1e-4 and g within 1e-3 relative (Frobenius / 2-norm; the device's f32
+    per-point terms and f32 normals vs FP64 3x3 covariances)."""
+    w = synth.small_cube_pair(n_source=20000, n_target=30000)
+    cfg = sr.GicpConfig(k_neighbors=20, max_correspondence_distance=1.0)
+    q, t = ((0.0, 0.0, 0.0, 1.0), (0.0, 0.0, 0.0)) if pose == "identity" else (w.gt_q, w.gt_t)
+    match, H, g, count, d2sum = sr.gicp_normal_equations(w.source, w.target, cfg, q, t)
+    sc, _ = orc.covariances(w.source, 20, 1e-3, mode="fp64")
+    tc, _ = orc.covariances(w.target, 20, 1e-3, mode="fp64")
+    Ho, go, corr = orc.gicp_normal_equations(w.source, w.target, sc, tc, q, t, 1.0)
+    p, T = _fp64_pose_distance2(w.source, w.target, q, t)
+    diff = np.nonzero(match != corr)[0]
+    assert len(diff) <= max(3, len(match) // 10000), len(diff)
+    for i in diff:
+        d = np.sort(((T - p[i]) ** 2).sum(1))
+        near_tie = np.sqrt(d[1]) - np.sqrt(d[0]) <= 1e-5
+        near_gate = abs(np.sqrt(d[0]) - 1.0) <= 1e-5
+        assert near_tie or near_gate, (i, match[i], corr[i], d[:2])
+    assert count == int((match >= 0).sum())
+    on = match >= 0
+    want_d2 = ((p[on] - T[match[on]]) ** 2).sum(1).sum()
+    assert abs(d2sum - want_d2) <= 1e-5 * want_d2
+    assert np.linalg.norm(H - Ho) <= 1e-4 * np.linalg.norm(Ho)
+    assert np.linalg.norm(g - go) <= 1e-3 * np.linalg.norm(go)
+    assert np.allclose(H, H.T)
