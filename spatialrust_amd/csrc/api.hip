@@ -316,9 +316,21 @@ static uint32_t pick_ppt(uint64_t total_points) {
 // Runs the Gauss-Newton loop for `count` (source, target) pairs in one set of
 // launches and fills `out`. Each registration keeps its own blocks, partials and
 // state, so results do not depend on the batch.
-struct Batch {
+// Launch descriptors live in per-thread buffers that only ever grow, so the
+// kernel arguments of a batch of a given shape stay the same from call to call
+// and its captured pass loop (batch_run) can be replayed.
+struct LoopArgs {
   DevBuf<IterInstance> insts;
   DevBuf<uint32_t> block_inst;
+};
+static LoopArgs& thread_loop_args() {
+  static thread_local LoopArgs* a = new LoopArgs();  // lives until the thread ends
+  return *a;
+}
+
+struct Batch {
+  DevBuf<IterInstance>& insts = thread_loop_args().insts;
+  DevBuf<uint32_t>& block_inst = thread_loop_args().block_inst;
   DevBuf<InstState> states;
   DevBuf<float> partials;
   DevBuf<double> init;
@@ -385,9 +397,57 @@ static void batch_setup(const std::vector<const Prepared*>& src, const std::vect
   init_states(B.states.p, B.init.p, count, s);
 }
 
+// The Gauss-Newton loop of a batch: max_passes pairs of (pass, finalize)
+// launches. Outside profiling the loop is captured once per launch shape as a
+// HIP graph and replayed: its kernel arguments are the per-thread descriptor
+// buffers and sizes, while everything a registration changes (clouds, states,
+// partials) is read through the descriptors at run time. The cache is per host
+// thread, since a graph replays on that thread's stream, and every entry point
+// synchronises before it returns, so no cached graph is in flight when one is
+// evicted.
+struct GraphEntry {
+  const void* insts;
+  const void* block_inst;
+  uint32_t blocks, count, ppt, passes;
+  hipGraphExec_t exec;
+};
+
 static void batch_run(Batch& B, hipStream_t s) {
-  for (uint32_t k = 0; k < B.max_passes; ++k)
-    gicp_pass(B.insts.p, B.block_inst.p, B.total_blocks, B.count, B.ppt, s);
+  if (profiler().on.load() != 0) {  // eager, so each launch can be timed
+    for (uint32_t k = 0; k < B.max_passes; ++k)
+      gicp_pass(B.insts.p, B.block_inst.p, B.total_blocks, B.count, B.ppt, s);
+    return;
+  }
+  static thread_local std::vector<GraphEntry>* cache = new std::vector<GraphEntry>();
+  hipGraphExec_t exec = nullptr;
+  for (const GraphEntry& e : *cache)
+    if (e.insts == B.insts.p && e.block_inst == B.block_inst.p && e.blocks == B.total_blocks &&
+        e.count == B.count && e.ppt == B.ppt && e.passes == B.max_passes)
+      exec = e.exec;
+  if (!exec) {
+    SR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipGraph_t graph = nullptr;
+    try {
+      for (uint32_t k = 0; k < B.max_passes; ++k)
+        gicp_pass(B.insts.p, B.block_inst.p, B.total_blocks, B.count, B.ppt, s);
+    } catch (...) {
+      (void)hipStreamEndCapture(s, &graph);
+      if (graph) (void)hipGraphDestroy(graph);
+      throw;
+    }
+    SR_HIP(hipStreamEndCapture(s, &graph));
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    SR_HIP(e);
+    if (cache->size() >= 8) {
+      SR_HIP(hipStreamSynchronize(s));
+      (void)hipGraphExecDestroy(cache->front().exec);
+      cache->erase(cache->begin());
+    }
+    cache->push_back(GraphEntry{B.insts.p, B.block_inst.p, B.total_blocks, B.count, B.ppt,
+                                B.max_passes, exec});
+  }
+  SR_HIP(hipGraphLaunch(exec, s));
 }
 
 static int32_t state_to_result(const InstState& st, const sr_gicp_config& cfg, const Prepared& src,
