@@ -238,6 +238,14 @@ int32_t sr_map_align(const sr_map* map, const sr_xyz_view* source, const sr_gicp
 int32_t sr_batch_align(const sr_map* const* maps, const sr_xyz_view* sources, uint32_t count,
                        const sr_gicp_config* config, sr_result* out);
 
+/* Registers `count` independent (source, target) pairs: every cloud is
+ * preprocessed (voxel filter, grid, covariances) on a pool of worker streams,
+ * then one set of launches runs the Gauss-Newton passes of all pairs. Each
+ * result equals sr_gicp_align of that pair alone. The batched form of
+ * GicpRegistration::align (crates/spatialrust-registration/src/gicp.rs:103-224). */
+int32_t sr_batch_align_pairs(const sr_xyz_view* sources, const sr_xyz_view* targets,
+                             uint32_t count, const sr_gicp_config* config, sr_result* out);
+
 /* ---- continuous-batching server ---------------------------------------- */
 
 typedef struct sr_server sr_server;

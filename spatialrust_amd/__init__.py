@@ -9,13 +9,13 @@ from .registration import (DeviceKind, DeviceUnavailable, ExecutionOutput, Execu
                            ExecutionReceipt, GicpConfig, GicpRegistration, InvalidArgument,
                            Isometry3, RegistrationResult, SpatialError, TargetMap,
                            VoxelGridDownsample, VoxelGridDownsampleConfig, VoxelOutput,
-                           batch_align, device_available, device_view, gicp_normal_equations, knn,
+                           batch_align, batch_align_pairs, device_available, device_view, gicp_normal_equations, knn,
                            plane_covariances, register_gicp)
 
 __all__ = [
     "DeviceKind", "DeviceUnavailable", "ExecutionOutput", "ExecutionPolicy", "ExecutionReceipt",
     "GicpConfig", "GicpRegistration", "InvalidArgument", "Isometry3", "RegistrationResult",
     "SpatialError", "TargetMap", "VoxelGridDownsample", "VoxelGridDownsampleConfig", "VoxelOutput",
-    "batch_align", "device_available", "device_view", "gicp_normal_equations", "knn",
+    "batch_align", "batch_align_pairs", "device_available", "device_view", "gicp_normal_equations", "knn",
     "plane_covariances", "register_gicp",
 ]

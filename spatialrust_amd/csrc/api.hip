@@ -5,6 +5,10 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <condition_variable>
+#include <exception>
+#include <functional>
+#include <thread>
 #include <cstdlib>
 #include <cstring>
 #include <deque>
@@ -303,14 +307,105 @@ static InstConfig inst_config(const sr_gicp_config& c) {
   return ic;
 }
 
-static uint32_t pick_ppt(uint64_t total_points) {
+static uint32_t pick_ppt(uint64_t points) {
   static const float forced = env_float("SR_GICP_PPT", 0.0f);
   if (forced >= 1.0f) return uint32_t(forced);
   // Enough workgroups to fill 256 CUs several times over before packing more
-  // points per thread (fewer block reductions).
-  if (total_points >= (1ull << 23)) return 4;
-  if (total_points >= (1ull << 21)) return 2;
+  // points per thread (fewer block reductions). Chosen from one registration's
+  // own size, so a registration sums the same way alone and inside a batch.
+  if (points >= (1ull << 23)) return 4;
+  if (points >= (1ull << 21)) return 2;
   return 1;
+}
+
+// Persistent preprocessing workers. Small clouds cannot fill the GPU one at a
+// time and their preparation waits on a few host reads (cell counts, bounds),
+// so many clouds are prepared concurrently: each worker thread has its own
+// stream and its own allocator free lists, and the threads live as long as
+// the process so those free lists keep being reused.
+class Workers {
+ public:
+  static Workers& get() {
+    static Workers* w = new Workers(4);  // one per default hardware queue
+    return *w;
+  }
+  // Runs fn(i, stream) for i in [0, n) on the workers, each worker's stream
+  // synchronised before run returns; rethrows the first error.
+  void run(size_t n, const std::function<void(size_t, hipStream_t)>& fn) {
+    if (n == 0) return;
+    std::unique_lock<std::mutex> call(call_mu_);  // one batch job at a time
+    std::unique_lock<std::mutex> lk(mu_);
+    job_ = &fn;
+    n_ = n;
+    next_ = 0;
+    busy_ = nthreads_;
+    err_ = nullptr;
+    ++gen_;
+    cv_.notify_all();
+    done_cv_.wait(lk, [&] { return busy_ == 0; });
+    job_ = nullptr;
+    if (err_) std::rethrow_exception(err_);
+  }
+
+ private:
+  explicit Workers(unsigned n) : nthreads_(n) {
+    for (unsigned i = 0; i < n; ++i) std::thread([this] { loop(); }).detach();
+  }
+  void loop() {
+    uint64_t seen = 0;
+    for (;;) {
+      const std::function<void(size_t, hipStream_t)>* job;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return gen_ != seen; });
+        seen = gen_;
+        job = job_;
+      }
+      std::exception_ptr err;
+      try {
+        hipStream_t s = thread_stream();
+        for (;;) {
+          size_t i;
+          {
+            std::lock_guard<std::mutex> lk(mu_);
+            if (next_ >= n_ || err_) break;
+            i = next_++;
+          }
+          (*job)(i, s);
+        }
+        SR_HIP(hipStreamSynchronize(s));
+      } catch (...) {
+        err = std::current_exception();
+      }
+      std::lock_guard<std::mutex> lk(mu_);
+      if (err && !err_) err_ = err;
+      if (--busy_ == 0) done_cv_.notify_all();
+    }
+  }
+  const unsigned nthreads_;
+  std::mutex call_mu_, mu_;
+  std::condition_variable cv_, done_cv_;
+  const std::function<void(size_t, hipStream_t)>* job_ = nullptr;
+  size_t n_ = 0, next_ = 0;
+  unsigned busy_ = 0;
+  uint64_t gen_ = 0;
+  std::exception_ptr err_;
+};
+
+// Prepares many clouds on the workers; out[i] receives views[i] prepared.
+static void prepare_many(const std::vector<const sr_xyz_view*>& views,
+                         const std::vector<sr_gicp_config>& cfgs, std::vector<Prepared>& out,
+                         bool is_target, Receipt& rc) {
+  std::mutex mu;
+  Workers::get().run(views.size(), [&](size_t i, hipStream_t s) {
+    Receipt local;
+    prepare(*views[i], cfgs[i], out[i], local, s, is_target);
+    std::lock_guard<std::mutex> lk(mu);
+    rc.h2d += local.h2d;
+    rc.d2h += local.d2h;
+    rc.d2d += local.d2d;
+    rc.stages |= local.stages;
+  });
 }
 
 // Runs the Gauss-Newton loop for `count` (source, target) pairs in one set of
@@ -335,7 +430,7 @@ struct Batch {
   DevBuf<float> partials;
   DevBuf<double> init;
   DevBuf<uint32_t> match;
-  uint32_t total_blocks = 0, ppt = 1, count = 0, max_passes = 0;
+  uint32_t total_blocks = 0, count = 0, max_passes = 0;
 };
 
 static void batch_setup(const std::vector<const Prepared*>& src, const std::vector<const Prepared*>& tgt,
@@ -345,7 +440,6 @@ static void batch_setup(const std::vector<const Prepared*>& src, const std::vect
   B.count = count;
   uint64_t total_pts = 0;
   for (auto* p : src) total_pts += p->n;
-  B.ppt = pick_ppt(total_pts);
   std::vector<IterInstance> hi(count);
   std::vector<uint32_t> hbi;
   std::vector<double> init(size_t(7) * count);
@@ -353,7 +447,8 @@ static void batch_setup(const std::vector<const Prepared*>& src, const std::vect
   B.max_passes = 0;
   std::vector<uint32_t> nbl(count);
   for (uint32_t i = 0; i < count; ++i) {
-    nbl[i] = std::max<uint32_t>(1, (src[i]->n + kBlock * B.ppt - 1) / (kBlock * B.ppt));
+    const uint32_t ppt = pick_ppt(src[i]->n);
+    nbl[i] = std::max<uint32_t>(1, (src[i]->n + kBlock * ppt - 1) / (kBlock * ppt));
     blocks += nbl[i];
   }
   B.states.resize(count);
@@ -379,6 +474,7 @@ static void batch_setup(const std::vector<const Prepared*>& src, const std::vect
     I.n_src = src[i]->n;
     I.block_begin = begin;
     I.nblocks = nbl[i];
+    I.ppt = pick_ppt(src[i]->n);
     for (uint32_t b = 0; b < nbl[i]; ++b) hbi.push_back(i);
     begin += nbl[i];
     for (int a = 0; a < 4; ++a) init[7 * size_t(i) + a] = double(cfgs[i].init_q[a]);
@@ -408,28 +504,28 @@ static void batch_setup(const std::vector<const Prepared*>& src, const std::vect
 struct GraphEntry {
   const void* insts;
   const void* block_inst;
-  uint32_t blocks, count, ppt, passes;
+  uint32_t blocks, count, passes;
   hipGraphExec_t exec;
 };
 
 static void batch_run(Batch& B, hipStream_t s) {
   if (profiler().on.load() != 0) {  // eager, so each launch can be timed
     for (uint32_t k = 0; k < B.max_passes; ++k)
-      gicp_pass(B.insts.p, B.block_inst.p, B.total_blocks, B.count, B.ppt, s);
+      gicp_pass(B.insts.p, B.block_inst.p, B.total_blocks, B.count, s);
     return;
   }
   static thread_local std::vector<GraphEntry>* cache = new std::vector<GraphEntry>();
   hipGraphExec_t exec = nullptr;
   for (const GraphEntry& e : *cache)
     if (e.insts == B.insts.p && e.block_inst == B.block_inst.p && e.blocks == B.total_blocks &&
-        e.count == B.count && e.ppt == B.ppt && e.passes == B.max_passes)
+        e.count == B.count && e.passes == B.max_passes)
       exec = e.exec;
   if (!exec) {
     SR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     hipGraph_t graph = nullptr;
     try {
       for (uint32_t k = 0; k < B.max_passes; ++k)
-        gicp_pass(B.insts.p, B.block_inst.p, B.total_blocks, B.count, B.ppt, s);
+        gicp_pass(B.insts.p, B.block_inst.p, B.total_blocks, B.count, s);
     } catch (...) {
       (void)hipStreamEndCapture(s, &graph);
       if (graph) (void)hipGraphDestroy(graph);
@@ -444,7 +540,7 @@ static void batch_run(Batch& B, hipStream_t s) {
       (void)hipGraphExecDestroy(cache->front().exec);
       cache->erase(cache->begin());
     }
-    cache->push_back(GraphEntry{B.insts.p, B.block_inst.p, B.total_blocks, B.count, B.ppt,
+    cache->push_back(GraphEntry{B.insts.p, B.block_inst.p, B.total_blocks, B.count,
                                 B.max_passes, exec});
   }
   SR_HIP(hipGraphLaunch(exec, s));
@@ -700,7 +796,7 @@ int32_t sr_gicp_normal_equations(const sr_xyz_view* source, const sr_xyz_view* t
     double pose[7] = {q[0], q[1], q[2], q[3], t[0], t[1], t[2]};
     Batch B;
     batch_setup({&src}, {&tgt}, {cfg}, B, s, pose);
-    gicp_pass(B.insts.p, B.block_inst.p, B.total_blocks, 1, B.ppt, s, false);
+    gicp_pass(B.insts.p, B.block_inst.p, B.total_blocks, 1, s, false);
     std::vector<float> part(size_t(B.total_blocks) * 64);
     std::vector<uint32_t> match(src.n), sorder(src.n), torder(tgt.n);
     SR_HIP(hipMemcpyAsync(part.data(), B.partials.p, part.size() * sizeof(float), hipMemcpyDeviceToHost, s));
@@ -905,6 +1001,7 @@ int32_t sr_batch_align(const sr_map* const* maps, const sr_xyz_view* sources, ui
     hipStream_t s = thread_stream();
     std::vector<Prepared> srcs(count);
     std::vector<const Prepared*> sp(count), tp(count);
+    std::vector<const sr_xyz_view*> views(count);
     std::vector<sr_gicp_config> cfgs(count, *config);
     Receipt rc;
     for (uint32_t i = 0; i < count; ++i) {
@@ -912,10 +1009,11 @@ int32_t sr_batch_align(const sr_map* const* maps, const sr_xyz_view* sources, ui
       validate_view(&sources[i]);
       const sr_xyz_view dummy{nullptr, nullptr, nullptr, maps[i]->tgt.n, 0};
       validate_cfg(&sources[i], &dummy, config);
-      prepare(sources[i], *config, srcs[i], rc, s, false);
+      views[i] = &sources[i];
       sp[i] = &srcs[i];
       tp[i] = &maps[i]->tgt;
     }
+    prepare_many(views, cfgs, srcs, false, rc);
     Batch B;
     batch_setup(sp, tp, cfgs, B, s);
     batch_run(B, s);
@@ -927,6 +1025,51 @@ int32_t sr_batch_align(const sr_map* const* maps, const sr_xyz_view* sources, ui
     for (uint32_t i = 0; i < count; ++i) {
       std::string msg;
       int32_t code = state_to_result(st[i], *config, srcs[i], maps[i]->tgt, &out[i], &msg);
+      if (code && !first_code) {
+        first_code = code;
+        first_msg = "registration " + std::to_string(i) + ": " + msg;
+      }
+    }
+    if (first_code) fail(first_code, first_msg);
+  });
+}
+
+int32_t sr_batch_align_pairs(const sr_xyz_view* sources, const sr_xyz_view* targets,
+                             uint32_t count, const sr_gicp_config* config, sr_result* out) {
+  return guarded([&] {
+    if (count == 0) return;
+    if (!sources || !targets || !out || !config) fail(SR_ERR_INVALID_ARGUMENT, "null batch argument");
+    ensure_device();
+    for (uint32_t i = 0; i < count; ++i) {
+      validate_view(&sources[i]);
+      validate_view(&targets[i]);
+      validate_cfg(&sources[i], &targets[i], config);
+    }
+    hipStream_t s = thread_stream();
+    std::vector<Prepared> srcs(count), tgts(count);
+    std::vector<const sr_xyz_view*> sv(count), tv(count);
+    std::vector<const Prepared*> sp(count), tp(count);
+    std::vector<sr_gicp_config> cfgs(count, *config);
+    for (uint32_t i = 0; i < count; ++i) {
+      sv[i] = &sources[i];
+      tv[i] = &targets[i];
+      sp[i] = &srcs[i];
+      tp[i] = &tgts[i];
+    }
+    Receipt rc;
+    prepare_many(tv, cfgs, tgts, true, rc);
+    prepare_many(sv, cfgs, srcs, false, rc);
+    Batch B;
+    batch_setup(sp, tp, cfgs, B, s);
+    batch_run(B, s);
+    std::vector<InstState> st(count);
+    SR_HIP(hipMemcpyAsync(st.data(), B.states.p, sizeof(InstState) * count, hipMemcpyDeviceToHost, s));
+    SR_HIP(hipStreamSynchronize(s));
+    int32_t first_code = 0;
+    std::string first_msg;
+    for (uint32_t i = 0; i < count; ++i) {
+      std::string msg;
+      int32_t code = state_to_result(st[i], *config, srcs[i], tgts[i], &out[i], &msg);
       if (code && !first_code) {
         first_code = code;
         first_msg = "registration " + std::to_string(i) + ": " + msg;

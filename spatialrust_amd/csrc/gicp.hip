@@ -337,7 +337,6 @@ __device__ __forceinline__ void combined_inverse(float ux, float uy, float uz, f
 
 __global__ void __launch_bounds__(kBlock) k_gicp_pass(const IterInstance* __restrict__ insts,
                                                       const uint32_t* __restrict__ block_inst,
-                                                      uint32_t ppt,
                                                       SearchStats* __restrict__ stats) {
   // XCD-aware block order: the hardware deals consecutive workgroup ids round
   // robin over the 8 XCDs; give each XCD a contiguous range of logical blocks so
@@ -373,6 +372,7 @@ __global__ void __launch_bounds__(kBlock) k_gicp_pass(const IterInstance* __rest
   for (int t = 0; t < kTerms; ++t) acc[t] = 0.0f;
   uint32_t count = 0;
 
+  const uint32_t ppt = I.ppt;
   for (uint32_t k = 0; k < ppt; ++k) {
     const uint32_t i = (lb * ppt + k) * kBlock + threadIdx.x;
     if (i >= I.n_src) break;
@@ -557,11 +557,11 @@ __global__ void __launch_bounds__(kBlock) k_gicp_finalize(const IterInstance* __
 }
 
 void gicp_pass(const IterInstance* insts, const uint32_t* block_inst, uint32_t total_blocks,
-               uint32_t count, uint32_t ppt, hipStream_t s, bool finalize) {
+               uint32_t count, hipStream_t s, bool finalize) {
   if (total_blocks == 0) return;
   {
     ProfScope prof("gicp_pass", s);
-    hipLaunchKernelGGL(k_gicp_pass, dim3(total_blocks), dim3(kBlock), 0, s, insts, block_inst, ppt,
+    hipLaunchKernelGGL(k_gicp_pass, dim3(total_blocks), dim3(kBlock), 0, s, insts, block_inst,
                        prof_stats("gicp_pass"));
   }
   if (!finalize) return;

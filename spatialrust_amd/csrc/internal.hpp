@@ -224,7 +224,8 @@ struct IterInstance {
   uint32_t n_src;
   uint32_t block_begin;
   uint32_t nblocks;
-  uint32_t pad_;
+  uint32_t ppt;           // source points per thread (from this instance's size only,
+                          // so its sums do not depend on the rest of a batch)
 };
 
 constexpr int kTerms = 28;  // 21 H (upper triangle) + 6 g + fitness sum
@@ -237,7 +238,7 @@ __host__ __device__ constexpr int ut(int a, int b) { return a * 6 - a * (a - 1) 
 // total_blocks blocks; block_inst maps block -> instance. Without `finalize`
 // only the per-block partial sums are written (no solve, no state change).
 void gicp_pass(const IterInstance* insts, const uint32_t* block_inst, uint32_t total_blocks,
-               uint32_t count, uint32_t pts_per_thread, hipStream_t s, bool finalize = true);
+               uint32_t count, hipStream_t s, bool finalize = true);
 // Initialises instance states from poses (device array of 7 doubles per
 // instance: quaternion x y z w, translation).
 void init_states(InstState* states, const double* init7, uint32_t count, hipStream_t s);

@@ -88,6 +88,8 @@ SIGNATURES = {
     "sr_gicp_align": (_I32, [ctypes.POINTER(XyzView), ctypes.POINTER(XyzView),
                              ctypes.POINTER(GicpConfigC), PolicyC, ctypes.POINTER(ResultC),
                              ctypes.POINTER(ReceiptC)]),
+    "sr_batch_align_pairs": (_I32, [ctypes.POINTER(XyzView), ctypes.POINTER(XyzView), ctypes.c_uint32,
+                                    ctypes.POINTER(GicpConfigC), ctypes.POINTER(ResultC)]),
     "sr_gicp_normal_equations": (_I32, [ctypes.POINTER(XyzView), ctypes.POINTER(XyzView),
                                         ctypes.POINTER(GicpConfigC), _P, _P, _P, _P, _P, _P, _P]),
     "sr_voxel_downsample": (_I32, [ctypes.POINTER(XyzView), ctypes.c_float, _P, PolicyC, _P, _P, _P,

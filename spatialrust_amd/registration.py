@@ -409,6 +409,23 @@ def batch_align(maps, sources, config: GicpConfig):
     return [RegistrationResult._from_c(out[i]) for i in range(count)]
 
 
+def batch_align_pairs(sources, targets, config: GicpConfig):
+    """Registers every (source, target) pair: all clouds preprocessed on the
+    device's worker streams, then one set of Gauss-Newton launches."""
+    count = len(sources)
+    keep = []
+    sv = (_n.XyzView * max(count, 1))()
+    tv = (_n.XyzView * max(count, 1))()
+    for i, (a, b) in enumerate(zip(sources, targets)):
+        va, ka = _view(a)
+        vb, kb = _view(b)
+        sv[i], tv[i] = va, vb
+        keep += [ka, kb]
+    out = (_n.ResultC * max(count, 1))()
+    _check(_n.load().sr_batch_align_pairs(sv, tv, count, ctypes.byref(config._c()), out))
+    return [RegistrationResult._from_c(out[i]) for i in range(count)]
+
+
 def register_gicp(source, target, max_correspondence_distance: float = 1.0,
                   max_iterations: int = 50, k_neighbors: int = 20) -> RegistrationResult:
     """register_gicp (crates/spatialrust-py/src/lib.rs:2103-2121), same signature and defaults."""

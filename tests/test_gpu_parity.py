@@ -236,6 +236,23 @@ def test_batch_alone_equals_in_mix(hip):
         m.close()
 
 
+def test_batch_pairs_equal_plain_align(hip):
+    """sr_batch_align_pairs (clouds preprocessed on worker streams, one set of
+    Gauss-Newton launches) returns bit-identical poses to one-at-a-time
+    sr_gicp_align, including instances large enough to pack several points
+    per thread."""
+    pairs = synth.batch_pairs(count=12, lo=3000, hi=60000, seed_offset=5)
+    cfg = sr.GicpConfig(max_iterations=12, voxel_leaf=0.1, transformation_epsilon=0.0,
+                        fitness_epsilon=0.0)
+    batched = sr.batch_align_pairs([p.source for p in pairs], [p.target for p in pairs], cfg)
+    reg = sr.GicpRegistration(cfg)
+    for i, p in enumerate(pairs):
+        alone = reg.align(p.source, p.target)
+        assert np.array_equal(alone.rotation64, batched[i].rotation64), i
+        assert np.array_equal(alone.translation64, batched[i].translation64), i
+        assert alone.iterations == batched[i].iterations
+
+
 def test_receipt_records_stages_and_bytes(hip):
     w = synth.small_cube_pair()
     out = sr.GicpRegistration(sr.GicpConfig(max_iterations=3, voxel_leaf=0.1)).align_with_policy_and_receipt(
