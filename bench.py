@@ -79,6 +79,50 @@ def _cpu_baseline(w, budget_iters=(1, 3)):
             "host_cpu": _cpu_model(), "host_nproc": os.cpu_count()}
 
 
+def _batched_vs_plain(count):
+    """256 independent pairs of the small-cube generator (source sizes
+    log-uniform in [5k, 120k], target 1.5x, offsets up to 10 deg / 1 m; leaf 0.1,
+    k 20, d_max 1.0, 30 fixed iterations), all clouds resident in HBM. Plain:
+    one sr_gicp_align per pair, one after another. Batched: one
+    sr_batch_align_pairs call for all pairs. Both legs do the same work (both
+    clouds preprocessed, 31 passes) and must return bit-identical poses."""
+    import spatialrust_amd as sr
+    from spatialrust_amd import devmem, synth
+    pairs = synth.batch_pairs(count=count)
+    srcs = [devmem.DeviceCloud(p.source) for p in pairs]
+    tgts = [devmem.DeviceCloud(p.target) for p in pairs]
+    cfg = sr.GicpConfig(max_iterations=30, k_neighbors=20, max_correspondence_distance=1.0,
+                        transformation_epsilon=0.0, fitness_epsilon=0.0, voxel_leaf=0.1)
+    reg = sr.GicpRegistration(cfg)
+    sv, tv = [s.view for s in srcs], [t.view for t in tgts]
+    # warm-up of both legs (allocator, graph cache, worker streams)
+    for s_, t_ in zip(sv[:8], tv[:8]):
+        reg.align(s_, t_)
+    sr.batch_align_pairs(sv[:8], tv[:8], cfg)
+    devmem.synchronize()
+    t0 = time.perf_counter()
+    plain = [reg.align(s_, t_) for s_, t_ in zip(sv, tv)]
+    devmem.synchronize()
+    t_plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    batched = sr.batch_align_pairs(sv, tv, cfg)
+    devmem.synchronize()
+    t_batch = time.perf_counter() - t0
+    identical = all(np.array_equal(a.rotation64, b.rotation64) and
+                    np.array_equal(a.translation64, b.translation64) for a, b in zip(plain, batched))
+    for c in srcs + tgts:
+        c.close()
+    return {"workload": f"batch_{count}_small_cube_pairs", "pairs": count,
+            "source_points": int(sum(len(p.source) for p in pairs)),
+            "target_points": int(sum(len(p.target) for p in pairs)),
+            "plain_registrations_per_s": count / t_plain,
+            "batched_registrations_per_s": count / t_batch,
+            "batched_over_plain": t_plain / t_batch,
+            "identical_poses": bool(identical),
+            "max_rot_err_vs_ground_truth_rad": float(max(
+                synth.rotation_angle_between(b.rotation64, p.gt_q) for b, p in zip(batched, pairs)))}
+
+
 def _traffic(kernel, workload):
     """HBM bytes per launch of `kernel` on `workload` from the committed
     counter summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
@@ -114,6 +158,9 @@ def main():
                     help="count grid-search work (rings/cells/points per query) on one extra run")
     ap.add_argument("--pass-times", action="store_true",
                     help="print each Gauss-Newton pass duration of the last registration")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched-vs-plain leg (256 independent small-cube pairs)")
+    ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (a plain copy "
                          "kernel of known size for calibrating hardware byte counters)")
@@ -238,6 +285,9 @@ def main():
                      "avg_launch_us": avg_pass_s * 1e6, "launches": pass_n},
         "cpu_baseline": None,
     }
+    if not args.no_batch:
+        print("batched vs plain leg ...", file=sys.stderr, flush=True)
+        line["batched_vs_plain"] = _batched_vs_plain(args.batch_count)
     if not args.no_cpu_baseline:
         print("timing the CPU oracle sample ...", file=sys.stderr, flush=True)
         line["cpu_baseline"] = _cpu_baseline(w)

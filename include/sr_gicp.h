@@ -248,28 +248,43 @@ int32_t sr_batch_align_pairs(const sr_xyz_view* sources, const sr_xyz_view* targ
 
 /* ---- continuous-batching server ---------------------------------------- */
 
+/* A serving loop over resident maps. The reference has no counterpart (it
+ * registers one pair per call, gicp.rs:103-224); every request it serves is
+ * that call. Requests queue on the host; each sr_server_step admits queued
+ * requests (FIFO) while active slots and the HBM budget allow, runs one
+ * Gauss-Newton pass over every active request in one launch, preprocesses the
+ * newly admitted sources on worker streams meanwhile, and retires finished
+ * requests. A request's result is bit-identical to sr_map_align of its source
+ * against its map with the request config (gicp + max_iterations and
+ * delta_pose_epsilon as transformation_epsilon). Submit and poll may be called
+ * from other threads while a step runs; steps must not overlap. */
 typedef struct sr_server sr_server;
 
 typedef struct {
   uint64_t hbm_budget_bytes; /* admission limit for per-request device memory */
   uint32_t max_active;       /* slots: requests iterating together */
-  uint32_t max_iterations;   /* per request */
-  double delta_pose_epsilon; /* converged when ||dx|| < this */
+  uint32_t max_iterations;   /* per request (0: the gicp config's) */
+  double delta_pose_epsilon; /* converged when ||dx|| < this (<= 0: the gicp config's) */
 } sr_server_config;
 
 int32_t sr_server_create(const sr_server_config* config, const sr_gicp_config* gicp,
                          sr_server** out);
 void sr_server_destroy(sr_server* server);
 /* Queues a registration of `source` (host xyz, copied) against `map`; returns a
- * ticket in *ticket. */
+ * ticket in *ticket. A request whose device footprint alone exceeds the budget
+ * is refused (InvalidArgument). */
 int32_t sr_server_submit(sr_server* server, const sr_map* map, const sr_xyz_view* source,
                          uint64_t* ticket);
-/* Advances the server by one scheduling step: admits queued requests as slots and
- * HBM budget allow, runs one Gauss-Newton pass over every active request, retires
- * converged ones. Returns the number of requests still queued or active. */
+/* One scheduling step (see above). *pending receives the number of requests
+ * still queued or active. */
 int32_t sr_server_step(sr_server* server, uint32_t* pending);
-/* Fetches a finished result. *ready = 1 when `ticket` has finished. */
+/* *ready = 1 and *out filled when `ticket` has finished (its error, if any, is
+ * returned as the status); the ticket is then released. */
 int32_t sr_server_poll(sr_server* server, uint64_t ticket, sr_result* out, uint8_t* ready);
+/* Seconds from submit to the step that finished `ticket` (before polling it). */
+int32_t sr_server_latency(sr_server* server, uint64_t ticket, double* seconds);
+/* Device bytes currently charged against the budget. */
+uint64_t sr_server_device_bytes(const sr_server* server);
 
 #ifdef __cplusplus
 }

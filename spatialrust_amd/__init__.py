@@ -7,7 +7,7 @@ kernels behind the C ABI in include/sr_gicp.h.
 """
 from .registration import (DeviceKind, DeviceUnavailable, ExecutionOutput, ExecutionPolicy,
                            ExecutionReceipt, GicpConfig, GicpRegistration, InvalidArgument,
-                           Isometry3, RegistrationResult, SpatialError, TargetMap,
+                           Isometry3, RegistrationResult, Server, SpatialError, TargetMap,
                            VoxelGridDownsample, VoxelGridDownsampleConfig, VoxelOutput,
                            batch_align, batch_align_pairs, device_available, device_view, gicp_normal_equations, knn,
                            plane_covariances, register_gicp)
@@ -15,7 +15,7 @@ from .registration import (DeviceKind, DeviceUnavailable, ExecutionOutput, Execu
 __all__ = [
     "DeviceKind", "DeviceUnavailable", "ExecutionOutput", "ExecutionPolicy", "ExecutionReceipt",
     "GicpConfig", "GicpRegistration", "InvalidArgument", "Isometry3", "RegistrationResult",
-    "SpatialError", "TargetMap", "VoxelGridDownsample", "VoxelGridDownsampleConfig", "VoxelOutput",
+    "Server", "SpatialError", "TargetMap", "VoxelGridDownsample", "VoxelGridDownsampleConfig", "VoxelOutput",
     "batch_align", "batch_align_pairs", "device_available", "device_view", "gicp_normal_equations", "knn",
     "plane_covariances", "register_gicp",
 ]

@@ -10,6 +10,7 @@
 #include <functional>
 #include <thread>
 #include <cstdlib>
+#include <chrono>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -433,6 +434,36 @@ struct Batch {
   uint32_t total_blocks = 0, count = 0, max_passes = 0;
 };
 
+// Work split of one registration's source over pass blocks.
+static uint32_t instance_blocks(uint32_t n_src) {
+  const uint32_t ppt = pick_ppt(n_src);
+  return std::max<uint32_t>(1, (n_src + kBlock * ppt - 1) / (kBlock * ppt));
+}
+
+// Launch descriptor of one registration: its clouds, config and the device
+// state, partials (instance_blocks x 64 floats) and match (n_src words) it owns.
+static IterInstance describe_instance(const Prepared& src, const Prepared& tgt,
+                                      const sr_gicp_config& cfg, InstState* state, float* partials,
+                                      uint32_t* match, uint32_t block_begin) {
+  IterInstance I{};
+  I.src = src.grid.pts.p;
+  I.src_n = src.normals.p;
+  I.tgt = tgt.grid.pts.p;
+  I.tgt_n = tgt.normals.p;
+  I.cell_start = tgt.grid.cell_start.p;
+  I.empty_radius = tgt.grid.empty_radius.p;
+  I.match = match;
+  I.grid = tgt.grid.desc;
+  I.cfg = inst_config(cfg);
+  I.state = state;
+  I.partials = partials;
+  I.n_src = src.n;
+  I.block_begin = block_begin;
+  I.nblocks = instance_blocks(src.n);
+  I.ppt = pick_ppt(src.n);
+  return I;
+}
+
 static void batch_setup(const std::vector<const Prepared*>& src, const std::vector<const Prepared*>& tgt,
                         const std::vector<sr_gicp_config>& cfgs, Batch& B, hipStream_t s,
                         const double* pose7 = nullptr) {
@@ -447,8 +478,7 @@ static void batch_setup(const std::vector<const Prepared*>& src, const std::vect
   B.max_passes = 0;
   std::vector<uint32_t> nbl(count);
   for (uint32_t i = 0; i < count; ++i) {
-    const uint32_t ppt = pick_ppt(src[i]->n);
-    nbl[i] = std::max<uint32_t>(1, (src[i]->n + kBlock * ppt - 1) / (kBlock * ppt));
+    nbl[i] = instance_blocks(src[i]->n);
     blocks += nbl[i];
   }
   B.states.resize(count);
@@ -458,23 +488,9 @@ static void batch_setup(const std::vector<const Prepared*>& src, const std::vect
   uint32_t begin = 0;
   uint64_t src_begin = 0;
   for (uint32_t i = 0; i < count; ++i) {
-    IterInstance& I = hi[i];
-    I.src = src[i]->grid.pts.p;
-    I.src_n = src[i]->normals.p;
-    I.tgt = tgt[i]->grid.pts.p;
-    I.tgt_n = tgt[i]->normals.p;
-    I.cell_start = tgt[i]->grid.cell_start.p;
-    I.empty_radius = tgt[i]->grid.empty_radius.p;
-    I.match = B.match.p + src_begin;
+    hi[i] = describe_instance(*src[i], *tgt[i], cfgs[i], B.states.p + i,
+                              B.partials.p + size_t(begin) * 64, B.match.p + src_begin, begin);
     src_begin += src[i]->n;
-    I.grid = tgt[i]->grid.desc;
-    I.cfg = inst_config(cfgs[i]);
-    I.state = B.states.p + i;
-    I.partials = B.partials.p + size_t(begin) * 64;
-    I.n_src = src[i]->n;
-    I.block_begin = begin;
-    I.nblocks = nbl[i];
-    I.ppt = pick_ppt(src[i]->n);
     for (uint32_t b = 0; b < nbl[i]; ++b) hbi.push_back(i);
     begin += nbl[i];
     for (int a = 0; a < 4; ++a) init[7 * size_t(i) + a] = double(cfgs[i].init_q[a]);
@@ -627,6 +643,182 @@ struct sr_map {
   Prepared tgt;
   sr_gicp_config cfg;
 };
+
+// ---- continuous-batching server ------------------------------------------
+//
+// Requests queue on the host; each scheduling step (server_step) runs one
+// Gauss-Newton pass (+ finalize) over every active request in a single launch
+// pair, while the requests admitted for the next step are preprocessed on the
+// worker streams, so the two overlap on the device. A request owns its
+// prepared source, state, partials and match buffer from admission to
+// retirement, and its sums never depend on who else is active: its result is
+// the one sr_map_align returns for it alone.
+
+namespace sr {
+
+struct ServerRequest {
+  uint64_t ticket = 0;
+  const sr_map* map = nullptr;
+  std::vector<float> x, y, z;
+  uint64_t reserve = 0;  // device bytes charged against the budget
+  std::chrono::steady_clock::time_point submitted;
+  Prepared src;
+  uint32_t slot = 0;  // index of its state in the server's state array
+  DevBuf<float> partials;
+  DevBuf<uint32_t> match;
+};
+
+struct ServerDone {
+  int32_t code = 0;
+  std::string msg;
+  sr_result result{};
+  double latency_s = 0.0;
+};
+
+}  // namespace sr
+
+struct sr_server {
+  sr_server_config cfg{};
+  sr_gicp_config gicp{};
+  std::mutex mu;  // queue, done, next_ticket (submit/poll may run beside a step)
+  std::deque<std::unique_ptr<sr::ServerRequest>> queue;
+  std::vector<std::unique_ptr<sr::ServerRequest>> active;
+  std::unordered_map<uint64_t, sr::ServerDone> done;
+  uint64_t next_ticket = 1;
+  uint64_t bytes_active = 0;
+  sr::DevBuf<sr::InstState> states;  // one slot per possible active request
+  std::vector<uint32_t> free_slots;
+  sr::DevBuf<sr::IterInstance> insts;
+  sr::DevBuf<uint32_t> block_inst;
+  sr::DevBuf<double> init;
+};
+
+namespace sr {
+
+// Per-request GICP config: the server's iteration cap and Delta-pose threshold
+// on top of the shared GICP settings.
+static sr_gicp_config request_config(const sr_server_config& sc, const sr_gicp_config& g) {
+  sr_gicp_config c = g;
+  if (sc.max_iterations) c.max_iterations = sc.max_iterations;
+  if (sc.delta_pose_epsilon > 0.0) c.transformation_epsilon = sc.delta_pose_epsilon;
+  return c;
+}
+
+// Upper bound of the device bytes a request of n raw source points holds while
+// active (packed cloud, voxel and grid scratch at their peak, grid, normals,
+// match, partials).
+static uint64_t request_bytes(uint64_t n) { return 256 * n + (1u << 20); }
+
+static void server_step(sr_server& S) {
+  hipStream_t s = thread_stream();
+  // Admission: FIFO while slots and budget allow.
+  std::vector<std::unique_ptr<ServerRequest>> admit;
+  {
+    std::lock_guard<std::mutex> lk(S.mu);
+    uint64_t bytes = S.bytes_active;
+    while (!S.queue.empty() && S.active.size() + admit.size() < S.cfg.max_active &&
+           bytes + S.queue.front()->reserve <= S.cfg.hbm_budget_bytes) {
+      bytes += S.queue.front()->reserve;
+      admit.push_back(std::move(S.queue.front()));
+      S.queue.pop_front();
+    }
+    S.bytes_active = bytes;
+  }
+  // One pass over the active requests (asynchronous on this thread's stream).
+  const uint32_t count = uint32_t(S.active.size());
+  if (count) {
+    std::vector<IterInstance> hi(count);
+    std::vector<uint32_t> hbi;
+    uint32_t begin = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+      ServerRequest& r = *S.active[i];
+      hi[i] = describe_instance(r.src, r.map->tgt, S.gicp, S.states.p + r.slot, r.partials.p,
+                                r.match.p, begin);
+      for (uint32_t b = 0; b < hi[i].nblocks; ++b) hbi.push_back(i);
+      begin += hi[i].nblocks;
+    }
+    S.insts.resize(count);
+    S.block_inst.resize(begin);
+    SR_HIP(hipMemcpyAsync(S.insts.p, hi.data(), sizeof(IterInstance) * count, hipMemcpyHostToDevice, s));
+    SR_HIP(hipMemcpyAsync(S.block_inst.p, hbi.data(), sizeof(uint32_t) * begin, hipMemcpyHostToDevice, s));
+    gicp_pass(S.insts.p, S.block_inst.p, begin, count, s);
+  }
+  // Meanwhile: preprocess the admitted sources on the worker streams.
+  if (!admit.empty()) {
+    std::vector<sr_xyz_view> views(admit.size());
+    std::vector<const sr_xyz_view*> vp(admit.size());
+    std::vector<Prepared> prepared(admit.size());
+    std::vector<sr_gicp_config> cfgs(admit.size(), S.gicp);
+    for (size_t i = 0; i < admit.size(); ++i) {
+      ServerRequest& r = *admit[i];
+      views[i] = sr_xyz_view{r.x.data(), r.y.data(), r.z.data(), r.x.size(), SR_MEM_HOST};
+      vp[i] = &views[i];
+    }
+    Receipt rc;
+    try {
+      prepare_many(vp, cfgs, prepared, false, rc);
+    } catch (const Error& e) {
+      // The whole admission group fails alike; report it on each ticket.
+      std::lock_guard<std::mutex> lk(S.mu);
+      for (auto& r : admit) {
+        ServerDone d;
+        d.code = e.code;
+        d.msg = e.msg;
+        S.done[r->ticket] = d;
+        S.bytes_active -= r->reserve;
+      }
+      admit.clear();
+    }
+    std::vector<double> init(7 * admit.size());
+    for (size_t i = 0; i < admit.size(); ++i) {
+      ServerRequest& r = *admit[i];
+      r.src = std::move(prepared[i]);
+      std::vector<float>().swap(r.x);  // the host copy is no longer needed
+      std::vector<float>().swap(r.y);
+      std::vector<float>().swap(r.z);
+      r.slot = S.free_slots.back();
+      S.free_slots.pop_back();
+      r.partials.resize(size_t(instance_blocks(r.src.n)) * 64);
+      r.match.resize(std::max<uint32_t>(r.src.n, 1));
+      fill_u32(r.match.p, kNone, r.src.n, s);
+      for (int a = 0; a < 4; ++a) init[7 * i + a] = double(S.gicp.init_q[a]);
+      for (int a = 0; a < 3; ++a) init[7 * i + 4 + a] = double(S.gicp.init_t[a]);
+    }
+    if (!admit.empty()) {
+      S.init.resize(init.size());
+      SR_HIP(hipMemcpyAsync(S.init.p, init.data(), sizeof(double) * init.size(), hipMemcpyHostToDevice, s));
+      for (size_t i = 0; i < admit.size(); ++i)
+        init_states(S.states.p + admit[i]->slot, S.init.p + 7 * i, 1, s);
+    }
+  }
+  // Retire finished requests (after the pass), then activate the admitted ones.
+  std::vector<InstState> slots(count ? S.states.n : 0);
+  if (count)
+    SR_HIP(hipMemcpyAsync(slots.data(), S.states.p, sizeof(InstState) * slots.size(),
+                          hipMemcpyDeviceToHost, s));
+  SR_HIP(hipStreamSynchronize(s));
+  const auto now = std::chrono::steady_clock::now();
+  std::vector<std::unique_ptr<ServerRequest>> still;
+  std::lock_guard<std::mutex> lk(S.mu);
+  for (uint32_t i = 0; i < count; ++i) {
+    ServerRequest& r = *S.active[i];
+    const InstState& st = slots[r.slot];
+    if (st.phase != 2) {
+      still.push_back(std::move(S.active[i]));
+      continue;
+    }
+    ServerDone d;
+    d.code = state_to_result(st, S.gicp, r.src, r.map->tgt, &d.result, &d.msg);
+    d.latency_s = std::chrono::duration<double>(now - r.submitted).count();
+    S.done[r.ticket] = std::move(d);
+    S.bytes_active -= r.reserve;
+    S.free_slots.push_back(r.slot);
+  }
+  for (auto& r : admit) still.push_back(std::move(r));
+  S.active = std::move(still);
+}
+
+}  // namespace sr
 
 template <typename F>
 static int32_t guarded(F&& f) {
@@ -1079,23 +1271,97 @@ int32_t sr_batch_align_pairs(const sr_xyz_view* sources, const sr_xyz_view* targ
   });
 }
 
-int32_t sr_server_create(const sr_server_config*, const sr_gicp_config*, sr_server** out) {
-  if (out) *out = nullptr;
-  t_last_error = "the serving loop is not built yet";
-  return SR_ERR_INVALID_ARGUMENT;
+int32_t sr_server_create(const sr_server_config* config, const sr_gicp_config* gicp,
+                         sr_server** out) {
+  return guarded([&] {
+    if (!out) fail(SR_ERR_INVALID_ARGUMENT, "out must not be null");
+    *out = nullptr;
+    if (!config || !gicp) fail(SR_ERR_INVALID_ARGUMENT, "server config must not be null");
+    if (config->max_active == 0) fail(SR_ERR_INVALID_ARGUMENT, "max_active must be at least 1");
+    if (gicp->k_neighbors < 3)
+      fail(SR_ERR_INVALID_ARGUMENT, "GICP needs k_neighbors >= 3 for covariance estimation");
+    ensure_device();
+    auto sv = std::make_unique<sr_server>();
+    sv->cfg = *config;
+    sv->gicp = request_config(*config, *gicp);
+    sv->states.resize(config->max_active);
+    for (uint32_t i = config->max_active; i > 0; --i) sv->free_slots.push_back(i - 1);
+    *out = sv.release();
+  });
 }
-void sr_server_destroy(sr_server*) {}
-int32_t sr_server_submit(sr_server*, const sr_map*, const sr_xyz_view*, uint64_t*) {
-  t_last_error = "the serving loop is not built yet";
-  return SR_ERR_INVALID_ARGUMENT;
+
+void sr_server_destroy(sr_server* server) {
+  if (!server) return;
+  (void)hipDeviceSynchronize();
+  delete server;
 }
-int32_t sr_server_step(sr_server*, uint32_t*) {
-  t_last_error = "the serving loop is not built yet";
-  return SR_ERR_INVALID_ARGUMENT;
+
+int32_t sr_server_submit(sr_server* server, const sr_map* map, const sr_xyz_view* source,
+                         uint64_t* ticket) {
+  return guarded([&] {
+    if (!server || !map || !ticket) fail(SR_ERR_INVALID_ARGUMENT, "null server argument");
+    validate_view(source);
+    const sr_xyz_view dummy{nullptr, nullptr, nullptr, map->tgt.n, 0};
+    validate_cfg(source, &dummy, &server->gicp);
+    if (source->mem != SR_MEM_HOST)
+      fail(SR_ERR_INVALID_ARGUMENT, "server requests take host xyz columns");
+    const uint64_t need = request_bytes(source->n);
+    if (need > server->cfg.hbm_budget_bytes)
+      fail(SR_ERR_INVALID_ARGUMENT, "request needs " + std::to_string(need) +
+                                        " device bytes, above the server's HBM budget");
+    auto r = std::make_unique<ServerRequest>();
+    r->map = map;
+    r->x.assign(source->x, source->x + source->n);
+    r->y.assign(source->y, source->y + source->n);
+    r->z.assign(source->z, source->z + source->n);
+    r->reserve = need;
+    r->submitted = std::chrono::steady_clock::now();
+    std::lock_guard<std::mutex> lk(server->mu);
+    r->ticket = server->next_ticket++;
+    *ticket = r->ticket;
+    server->queue.push_back(std::move(r));
+  });
 }
-int32_t sr_server_poll(sr_server*, uint64_t, sr_result*, uint8_t*) {
-  t_last_error = "the serving loop is not built yet";
-  return SR_ERR_INVALID_ARGUMENT;
+
+int32_t sr_server_step(sr_server* server, uint32_t* pending) {
+  return guarded([&] {
+    if (!server) fail(SR_ERR_INVALID_ARGUMENT, "server must not be null");
+    server_step(*server);
+    if (pending) {
+      std::lock_guard<std::mutex> lk(server->mu);
+      *pending = uint32_t(server->queue.size() + server->active.size());
+    }
+  });
+}
+
+int32_t sr_server_poll(sr_server* server, uint64_t ticket, sr_result* out, uint8_t* ready) {
+  return guarded([&] {
+    if (!server || !ready) fail(SR_ERR_INVALID_ARGUMENT, "null server argument");
+    *ready = 0;
+    std::lock_guard<std::mutex> lk(server->mu);
+    auto it = server->done.find(ticket);
+    if (it == server->done.end()) return;
+    *ready = 1;
+    if (out) *out = it->second.result;
+    const int32_t code = it->second.code;
+    const std::string msg = it->second.msg;
+    server->done.erase(it);
+    if (code) fail(code, msg);
+  });
+}
+
+int32_t sr_server_latency(sr_server* server, uint64_t ticket, double* seconds) {
+  return guarded([&] {
+    if (!server || !seconds) fail(SR_ERR_INVALID_ARGUMENT, "null server argument");
+    std::lock_guard<std::mutex> lk(server->mu);
+    auto it = server->done.find(ticket);
+    if (it == server->done.end()) fail(SR_ERR_INVALID_ARGUMENT, "ticket has not finished");
+    *seconds = it->second.latency_s;
+  });
+}
+
+uint64_t sr_server_device_bytes(const sr_server* server) {
+  return server ? server->bytes_active : 0;
 }
 
 }  // extern "C"

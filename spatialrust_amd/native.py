@@ -114,6 +114,8 @@ SIGNATURES = {
     "sr_server_step": (_I32, [_P, ctypes.POINTER(ctypes.c_uint32)]),
     "sr_server_poll": (_I32, [_P, ctypes.c_uint64, ctypes.POINTER(ResultC),
                               ctypes.POINTER(ctypes.c_uint8)]),
+    "sr_server_latency": (_I32, [_P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]),
+    "sr_server_device_bytes": (ctypes.c_uint64, [_P]),
 }
 
 _lib = None

@@ -409,6 +409,60 @@ def batch_align(maps, sources, config: GicpConfig):
     return [RegistrationResult._from_c(out[i]) for i in range(count)]
 
 
+class Server:
+    """Continuous-batching registration server over resident maps (sr_server):
+    submit() queues a source, step() runs one scheduling step (admission by
+    slots and HBM budget, one Gauss-Newton pass over every active request,
+    retirement), poll() fetches a finished result."""
+
+    def __init__(self, config: GicpConfig, hbm_budget_bytes: int = 200 << 30,
+                 max_active: int = 256, max_iterations: int = 0,
+                 delta_pose_epsilon: float = 0.0):
+        sc = _n.ServerConfigC(int(hbm_budget_bytes), int(max_active), int(max_iterations),
+                              float(delta_pose_epsilon))
+        h = ctypes.c_void_p()
+        _check(_n.load().sr_server_create(ctypes.byref(sc), ctypes.byref(config._c()),
+                                          ctypes.byref(h)))
+        self._h = h
+
+    def submit(self, target_map: "TargetMap", source) -> int:
+        sv, _k = _view(source)
+        t = ctypes.c_uint64(0)
+        _check(_n.load().sr_server_submit(self._h, target_map.handle, ctypes.byref(sv),
+                                          ctypes.byref(t)))
+        return int(t.value)
+
+    def step(self) -> int:
+        pending = ctypes.c_uint32(0)
+        _check(_n.load().sr_server_step(self._h, ctypes.byref(pending)))
+        return int(pending.value)
+
+    def latency(self, ticket: int) -> float:
+        sec = ctypes.c_double(0.0)
+        _check(_n.load().sr_server_latency(self._h, int(ticket), ctypes.byref(sec)))
+        return float(sec.value)
+
+    def poll(self, ticket: int) -> Optional[RegistrationResult]:
+        out = _n.ResultC()
+        ready = ctypes.c_uint8(0)
+        _check(_n.load().sr_server_poll(self._h, int(ticket), ctypes.byref(out), ctypes.byref(ready)))
+        return RegistrationResult._from_c(out) if ready.value else None
+
+    def device_bytes(self) -> int:
+        return int(_n.load().sr_server_device_bytes(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            _n.load().sr_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def batch_align_pairs(sources, targets, config: GicpConfig):
     """Registers every (source, target) pair: all clouds preprocessed on the
     device's worker streams, then one set of Gauss-Newton launches."""

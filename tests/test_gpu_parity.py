@@ -302,3 +302,45 @@ def test_pass_matches_fp64_oracle(hip, orc, pose):
     assert np.linalg.norm(H - Ho) <= 1e-4 * np.linalg.norm(Ho)
     assert np.linalg.norm(g - go) <= 1e-3 * np.linalg.norm(go)
     assert np.allclose(H, H.T)
+
+
+def test_server_results_equal_alone(hip):
+    """Continuous batching: requests submitted while others iterate, admitted as
+    slots and budget free up, each finish with the pose sr_map_align gives the
+    same source alone (bit-identical)."""
+    pairs = synth.batch_pairs(count=10, lo=3000, hi=40000, seed_offset=9)
+    base = sr.GicpConfig(voxel_leaf=0.1, fitness_epsilon=0.0)
+    maps = [sr.TargetMap(p.target, base) for p in pairs[:3]]
+    server = sr.Server(base, hbm_budget_bytes=40 << 20, max_active=4, max_iterations=25,
+                       delta_pose_epsilon=1e-6)
+    req_cfg = sr.GicpConfig(voxel_leaf=0.1, fitness_epsilon=0.0, max_iterations=25,
+                            transformation_epsilon=1e-6)
+    tickets = {}
+    waves = [pairs[:4], pairs[4:7], pairs[7:]]
+    pending = 0
+    for wave in waves:
+        for j, p in enumerate(wave):
+            idx = pairs.index(p)
+            tickets[server.submit(maps[idx % 3], p.source)] = idx
+        for _ in range(3):
+            pending = server.step()
+    steps = 0
+    while pending:
+        pending = server.step()
+        steps += 1
+        assert steps < 500
+    assert server.device_bytes() == 0
+    for t, idx in tickets.items():
+        lat = server.latency(t)
+        got = server.poll(t)
+        assert got is not None and lat > 0.0
+        alone = maps[idx % 3].align(pairs[idx].source, req_cfg)
+        assert np.array_equal(got.rotation64, alone.rotation64), idx
+        assert np.array_equal(got.translation64, alone.translation64), idx
+        assert got.iterations == alone.iterations and got.converged == alone.converged
+        assert server.poll(t) is None  # released
+    with pytest.raises(sr.InvalidArgument, match="HBM budget"):
+        server.submit(maps[0], np.zeros((400000, 3), np.float32))
+    server.close()
+    for m in maps:
+        m.close()
