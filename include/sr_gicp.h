@@ -286,6 +286,37 @@ int32_t sr_server_latency(sr_server* server, uint64_t ticket, double* seconds);
 /* Device bytes currently charged against the budget. */
 uint64_t sr_server_device_bytes(const sr_server* server);
 
+/* ---- scan-to-map odometry (rolling device-resident voxel map) ------------ */
+
+/* A GICP scan matcher over a rolling map. The reference's odometry contract is
+ * the ScanMatcher trait (crates/spatialrust-mapping/src/scan.rs:12-20), which it
+ * implements only with ICP and scan-to-scan; this keeps the map on the device:
+ * each pushed scan is voxel-filtered (voxel.rs:550-636 semantics, origin = the
+ * scan's minimum), registered against the map from a constant-velocity guess
+ * (the last pose composed with the last motion), and its voxels absent from
+ * the map (keys floor(p / leaf) in the map frame) are appended; the lowest
+ * point index of a scan wins a new voxel. Beyond max_map_points the oldest
+ * voxels are dropped. The first scan defines the map frame. */
+typedef struct sr_odometry sr_odometry;
+
+typedef struct {
+  float voxel_leaf;        /* scan filter and map voxel edge */
+  uint32_t pad_;
+  uint64_t max_map_points; /* rolling map capacity (voxels) */
+} sr_odometry_config;
+
+int32_t sr_odometry_create(const sr_odometry_config* config, const sr_gicp_config* gicp,
+                           sr_odometry** out);
+void sr_odometry_destroy(sr_odometry* odometry);
+/* Registers and merges one scan (sensor-frame xyz). out->q64/t64 (and q/t) is
+ * map_T_scan; source_points = scan voxels, target_points = map voxels after
+ * the merge. */
+int32_t sr_odometry_push(sr_odometry* odometry, const sr_xyz_view* scan, sr_result* out);
+uint64_t sr_odometry_map_points(const sr_odometry* odometry);
+/* Copies up to `cap` map points (interleaved xyz, insertion order) to host
+ * memory; *n receives the map size. */
+int32_t sr_odometry_map_read(const sr_odometry* odometry, float* xyz, uint64_t cap, uint64_t* n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -6,16 +6,18 @@ nearest-neighbour correspondences and the Gauss-Newton iteration, as HIP
 kernels behind the C ABI in include/sr_gicp.h.
 """
 from .registration import (DeviceKind, DeviceUnavailable, ExecutionOutput, ExecutionPolicy,
-                           ExecutionReceipt, GicpConfig, GicpRegistration, InvalidArgument,
-                           Isometry3, RegistrationResult, Server, SpatialError, TargetMap,
+                           ExecutionReceipt, GicpConfig, GicpRegistration, GicpScanMatcher,
+                           InvalidArgument,
+                           Isometry3, RegistrationResult, RollingMapOdometry, Server, SpatialError,
+                           TargetMap,
                            VoxelGridDownsample, VoxelGridDownsampleConfig, VoxelOutput,
                            batch_align, batch_align_pairs, device_available, device_view, gicp_normal_equations, knn,
                            plane_covariances, register_gicp)
 
 __all__ = [
     "DeviceKind", "DeviceUnavailable", "ExecutionOutput", "ExecutionPolicy", "ExecutionReceipt",
-    "GicpConfig", "GicpRegistration", "InvalidArgument", "Isometry3", "RegistrationResult",
-    "Server", "SpatialError", "TargetMap", "VoxelGridDownsample", "VoxelGridDownsampleConfig", "VoxelOutput",
+    "GicpConfig", "GicpRegistration", "GicpScanMatcher", "InvalidArgument", "Isometry3", "RegistrationResult",
+    "RollingMapOdometry", "Server", "SpatialError", "TargetMap", "VoxelGridDownsample", "VoxelGridDownsampleConfig", "VoxelOutput",
     "batch_align", "batch_align_pairs", "device_available", "device_view", "gicp_normal_equations", "knn",
     "plane_covariances", "register_gicp",
 ]

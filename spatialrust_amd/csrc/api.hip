@@ -263,6 +263,9 @@ static float knn_occupancy(uint32_t k) {
 }
 
 // Voxel filter (optional) + grid + plane normals of one cloud.
+static void prepare_cloud(const Cloud& c, const sr_gicp_config& cfg, Prepared& out, Receipt& rc,
+                          hipStream_t s, bool is_target);
+
 static void prepare(const sr_xyz_view& v, const sr_gicp_config& cfg, Prepared& out, Receipt& rc,
                     hipStream_t s, bool is_target) {
   if (v.n > 0xFFFFFFF0ull) fail(SR_ERR_INVALID_ARGUMENT, "point clouds above 4G points are not supported");
@@ -275,6 +278,13 @@ static void prepare(const sr_xyz_view& v, const sr_gicp_config& cfg, Prepared& o
     cloud = &vox.centroids;
     rc.stages |= SR_STAGE_VOXEL;
   }
+  prepare_cloud(*cloud, cfg, out, rc, s, is_target);
+}
+
+// Grid and covariances of a packed device cloud (no voxel filter).
+static void prepare_cloud(const Cloud& c, const sr_gicp_config& cfg, Prepared& out, Receipt& rc,
+                          hipStream_t s, bool is_target) {
+  const Cloud* cloud = &c;
   out.n = cloud->n;
   uint32_t hb[6];
   host_bounds(*cloud, hb, s);
@@ -820,6 +830,69 @@ static void server_step(sr_server& S) {
 
 }  // namespace sr
 
+// ---- scan-to-map odometry ------------------------------------------------
+
+namespace sr {
+
+// FP64 rigid pose (unit quaternion x y z w, translation) on the host.
+struct Pose64 {
+  double q[4] = {0.0, 0.0, 0.0, 1.0};
+  double t[3] = {0.0, 0.0, 0.0};
+};
+
+static void quat_mul64(const double* a, const double* b, double* o) {
+  o[0] = a[3] * b[0] + a[0] * b[3] + a[1] * b[2] - a[2] * b[1];
+  o[1] = a[3] * b[1] - a[0] * b[2] + a[1] * b[3] + a[2] * b[0];
+  o[2] = a[3] * b[2] + a[0] * b[1] - a[1] * b[0] + a[2] * b[3];
+  o[3] = a[3] * b[3] - a[0] * b[0] - a[1] * b[1] - a[2] * b[2];
+}
+
+static void quat_mat64(const double* q, double* r) {
+  const double x = q[0], y = q[1], z = q[2], w = q[3];
+  r[0] = 1 - 2 * (y * y + z * z); r[1] = 2 * (x * y - z * w);     r[2] = 2 * (x * z + y * w);
+  r[3] = 2 * (x * y + z * w);     r[4] = 1 - 2 * (x * x + z * z); r[5] = 2 * (y * z - x * w);
+  r[6] = 2 * (x * z - y * w);     r[7] = 2 * (y * z + x * w);     r[8] = 1 - 2 * (x * x + y * y);
+}
+
+// a o b (apply b, then a), as Isometry3::compose (crates/spatialrust-math/src/transform.rs:104-108).
+static Pose64 compose64(const Pose64& a, const Pose64& b) {
+  Pose64 o;
+  quat_mul64(a.q, b.q, o.q);
+  const double n = std::sqrt(o.q[0] * o.q[0] + o.q[1] * o.q[1] + o.q[2] * o.q[2] + o.q[3] * o.q[3]);
+  for (double& v : o.q) v /= n;
+  double r[9];
+  quat_mat64(a.q, r);
+  for (int i = 0; i < 3; ++i)
+    o.t[i] = r[3 * i] * b.t[0] + r[3 * i + 1] * b.t[1] + r[3 * i + 2] * b.t[2] + a.t[i];
+  return o;
+}
+
+static Pose64 inverse64(const Pose64& a) {
+  Pose64 o;
+  o.q[0] = -a.q[0];
+  o.q[1] = -a.q[1];
+  o.q[2] = -a.q[2];
+  o.q[3] = a.q[3];
+  double r[9];
+  quat_mat64(o.q, r);
+  for (int i = 0; i < 3; ++i)
+    o.t[i] = -(r[3 * i] * a.t[0] + r[3 * i + 1] * a.t[1] + r[3 * i + 2] * a.t[2]);
+  return o;
+}
+
+}  // namespace sr
+
+// Scan-to-map odometry: each scan is voxel-filtered, registered against the
+// rolling map with a constant-velocity initial guess, and merged into the map.
+struct sr_odometry {
+  sr_odometry_config cfg{};
+  sr_gicp_config gicp{};
+  sr::VoxelMap map;
+  sr::Prepared map_prep;  // grid, empty field and normals of map.list
+  sr::Pose64 pose, prev;  // map_T_scan of the last two scans
+  uint64_t scans = 0;
+};
+
 template <typename F>
 static int32_t guarded(F&& f) {
   try {
@@ -1362,6 +1435,111 @@ int32_t sr_server_latency(sr_server* server, uint64_t ticket, double* seconds) {
 
 uint64_t sr_server_device_bytes(const sr_server* server) {
   return server ? server->bytes_active : 0;
+}
+
+int32_t sr_odometry_create(const sr_odometry_config* config, const sr_gicp_config* gicp,
+                           sr_odometry** out) {
+  return guarded([&] {
+    if (!out) fail(SR_ERR_INVALID_ARGUMENT, "out must not be null");
+    *out = nullptr;
+    if (!config || !gicp) fail(SR_ERR_INVALID_ARGUMENT, "odometry config must not be null");
+    if (!(config->voxel_leaf > 0.0f)) fail(SR_ERR_INVALID_ARGUMENT, "leaf_size must be greater than zero");
+    if (config->max_map_points == 0 || config->max_map_points > 0x7FFFFFFFull)
+      fail(SR_ERR_INVALID_ARGUMENT, "max_map_points must be in [1, 2^31)");
+    if (gicp->k_neighbors < 3)
+      fail(SR_ERR_INVALID_ARGUMENT, "GICP needs k_neighbors >= 3 for covariance estimation");
+    ensure_device();
+    auto o = std::make_unique<sr_odometry>();
+    o->cfg = *config;
+    o->gicp = *gicp;
+    o->gicp.voxel_leaf = 0.0f;  // scans are filtered here, the map holds voxels already
+    hipStream_t s = thread_stream();
+    o->map.init(config->voxel_leaf, uint32_t(config->max_map_points), s);
+    SR_HIP(hipStreamSynchronize(s));
+    *out = o.release();
+  });
+}
+
+void sr_odometry_destroy(sr_odometry* odo) {
+  if (!odo) return;
+  (void)hipDeviceSynchronize();
+  delete odo;
+}
+
+int32_t sr_odometry_push(sr_odometry* odo, const sr_xyz_view* scan, sr_result* out) {
+  return guarded([&] {
+    if (!odo) fail(SR_ERR_INVALID_ARGUMENT, "odometry must not be null");
+    validate_view(scan);
+    if (scan->n == 0) fail(SR_ERR_INVALID_ARGUMENT, "GICP requires non-empty source and target point clouds");
+    if (scan->n > 0xFFFFFFF0ull) fail(SR_ERR_INVALID_ARGUMENT, "point clouds above 4G points are not supported");
+    hipStream_t s = thread_stream();
+    Receipt rc;
+    Cloud raw;
+    pack_cloud(scan->x, scan->y, scan->z, uint32_t(scan->n), scan->mem == SR_MEM_DEVICE, raw, s, &rc.h2d);
+    VoxelOut vox;
+    voxel_downsample(raw, odo->cfg.voxel_leaf, nullptr, false, vox, s);
+    sr_result r;
+    std::memset(&r, 0, sizeof(r));
+    Pose64 pose;  // map_T_scan; the first scan defines the map frame
+    if (odo->scans > 0) {
+      // constant velocity: last pose composed with the last scan-to-scan motion
+      const Pose64 guess = compose64(odo->pose, compose64(inverse64(odo->prev), odo->pose));
+      Prepared src;
+      prepare_cloud(vox.centroids, odo->gicp, src, rc, s, false);
+      double pose7[7] = {guess.q[0], guess.q[1], guess.q[2], guess.q[3],
+                         guess.t[0], guess.t[1], guess.t[2]};
+      Batch B;
+      batch_setup({&src}, {&odo->map_prep}, {odo->gicp}, B, s, pose7);
+      batch_run(B, s);
+      InstState st;
+      SR_HIP(hipMemcpyAsync(&st, B.states.p, sizeof(st), hipMemcpyDeviceToHost, s));
+      SR_HIP(hipStreamSynchronize(s));
+      std::string msg;
+      const int32_t code = state_to_result(st, odo->gicp, src, odo->map_prep, &r, &msg);
+      if (code) fail(code, "scan " + std::to_string(odo->scans) + ": " + msg);
+      for (int a = 0; a < 4; ++a) pose.q[a] = st.q[a];
+      for (int a = 0; a < 3; ++a) pose.t[a] = st.t[a];
+    } else {
+      r.q[3] = 1.0f;
+      r.q64[3] = 1.0;
+      r.converged = 1;
+      r.source_points = vox.cells;
+    }
+    MapPose mp;
+    quat_mat64(pose.q, mp.r);
+    for (int a = 0; a < 3; ++a) mp.t[a] = pose.t[a];
+    odo->map.insert(vox.centroids.pts.p, vox.cells, mp, s);
+    Prepared fresh;
+    prepare_cloud(odo->map.list, odo->gicp, fresh, rc, s, true);
+    SR_HIP(hipStreamSynchronize(s));
+    odo->map_prep = std::move(fresh);
+    odo->prev = odo->scans > 0 ? odo->pose : pose;
+    odo->pose = pose;
+    ++odo->scans;
+    r.target_points = odo->map.list.n;
+    if (out) *out = r;
+  });
+}
+
+uint64_t sr_odometry_map_points(const sr_odometry* odo) { return odo ? odo->map.list.n : 0; }
+
+int32_t sr_odometry_map_read(const sr_odometry* odo, float* xyz, uint64_t cap, uint64_t* n) {
+  return guarded([&] {
+    if (!odo || !n) fail(SR_ERR_INVALID_ARGUMENT, "null odometry argument");
+    *n = odo->map.list.n;
+    const uint64_t m = std::min<uint64_t>(cap, odo->map.list.n);
+    if (!m) return;
+    if (!xyz) fail(SR_ERR_INVALID_ARGUMENT, "xyz must not be null");
+    std::vector<float4> host(m);
+    hipStream_t s = thread_stream();
+    SR_HIP(hipMemcpyAsync(host.data(), odo->map.list.pts.p, sizeof(float4) * m, hipMemcpyDeviceToHost, s));
+    SR_HIP(hipStreamSynchronize(s));
+    for (uint64_t i = 0; i < m; ++i) {
+      xyz[3 * i] = host[i].x;
+      xyz[3 * i + 1] = host[i].y;
+      xyz[3 * i + 2] = host[i].z;
+    }
+  });
 }
 
 }  // extern "C"

@@ -243,4 +243,29 @@ void gicp_pass(const IterInstance* insts, const uint32_t* block_inst, uint32_t t
 // instance: quaternion x y z w, translation).
 void init_states(InstState* states, const double* init7, uint32_t count, hipStream_t s);
 
+// ---- rolling voxel map (odometry) ----------------------------------------
+
+// Rigid transform for map insertion: R row-major and t, FP64.
+struct MapPose {
+  double r[9];
+  double t[3];
+};
+
+// Voxel representatives in insertion order plus a key -> list index hash.
+struct VoxelMap {
+  float leaf = 0.0f, inv_leaf = 0.0f;
+  uint64_t capacity = 0;  // list capacity (voxels)
+  uint64_t mask = 0;      // hash slots - 1
+  DevBuf<uint64_t> keys;
+  DevBuf<uint32_t> val, claim, flags, scal;  // scal: [new count, overflow flag]
+  Cloud list;             // map points, list.n of them
+
+  void init(float leaf, uint32_t capacity, hipStream_t s);
+  // Transforms n scan points (sensor frame) by `pose` and adds the voxels the
+  // map lacks; returns the number added. Drops the oldest voxels first when the
+  // result would exceed the capacity.
+  uint32_t insert(const float4* scan_local, uint32_t n, const MapPose& pose, hipStream_t s);
+  void drop_oldest(uint32_t count, hipStream_t s);
+};
+
 }  // namespace sr

@@ -65,6 +65,11 @@ class ReceiptC(ctypes.Structure):
                 ("stage_mask", ctypes.c_uint32), ("pad_", ctypes.c_uint32)]
 
 
+class OdometryConfigC(ctypes.Structure):
+    _fields_ = [("voxel_leaf", ctypes.c_float), ("pad_", ctypes.c_uint32),
+                ("max_map_points", ctypes.c_uint64)]
+
+
 class ServerConfigC(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("max_active", ctypes.c_uint32),
                 ("max_iterations", ctypes.c_uint32), ("delta_pose_epsilon", ctypes.c_double)]
@@ -116,6 +121,12 @@ SIGNATURES = {
                               ctypes.POINTER(ctypes.c_uint8)]),
     "sr_server_latency": (_I32, [_P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]),
     "sr_server_device_bytes": (ctypes.c_uint64, [_P]),
+    "sr_odometry_create": (_I32, [ctypes.POINTER(OdometryConfigC), ctypes.POINTER(GicpConfigC),
+                                  ctypes.POINTER(_P)]),
+    "sr_odometry_destroy": (None, [_P]),
+    "sr_odometry_push": (_I32, [_P, ctypes.POINTER(XyzView), ctypes.POINTER(ResultC)]),
+    "sr_odometry_map_points": (ctypes.c_uint64, [_P]),
+    "sr_odometry_map_read": (_I32, [_P, _P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),
 }
 
 _lib = None

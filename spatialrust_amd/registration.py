@@ -409,6 +409,63 @@ def batch_align(maps, sources, config: GicpConfig):
     return [RegistrationResult._from_c(out[i]) for i in range(count)]
 
 
+class GicpScanMatcher:
+    """ScanMatcher (crates/spatialrust-mapping/src/scan.rs:12-20) backed by GICP
+    on the device: match_scans(previous, current) estimates current_T_previous,
+    i.e. registers `previous` (source) onto `current` (target), as
+    IcpScanMatcher does with ICP (scan.rs:219-250)."""
+
+    def __init__(self, config: GicpConfig):
+        self._reg = GicpRegistration(config)
+
+    def config(self) -> GicpConfig:
+        return self._reg.config()
+
+    def match_scans(self, previous, current) -> Isometry3:
+        return self._reg.align(previous, current).transform
+
+
+class RollingMapOdometry:
+    """Scan-to-map odometry over a rolling device-resident voxel map
+    (sr_odometry): push() registers a sensor-frame scan against the map from a
+    constant-velocity guess, merges its new voxels and returns map_T_scan."""
+
+    def __init__(self, config: GicpConfig, voxel_leaf: float = 0.25,
+                 max_map_points: int = 4_000_000):
+        oc = _n.OdometryConfigC(float(voxel_leaf), 0, int(max_map_points))
+        h = ctypes.c_void_p()
+        _check(_n.load().sr_odometry_create(ctypes.byref(oc), ctypes.byref(config._c()),
+                                            ctypes.byref(h)))
+        self._h = h
+
+    def push(self, scan) -> RegistrationResult:
+        sv, _k = _view(scan)
+        out = _n.ResultC()
+        _check(_n.load().sr_odometry_push(self._h, ctypes.byref(sv), ctypes.byref(out)))
+        return RegistrationResult._from_c(out)
+
+    def map_points(self) -> int:
+        return int(_n.load().sr_odometry_map_points(self._h))
+
+    def map_cloud(self) -> np.ndarray:
+        n = self.map_points()
+        out = np.zeros((max(n, 1), 3), np.float32)
+        got = ctypes.c_uint64(0)
+        _check(_n.load().sr_odometry_map_read(self._h, out.ctypes.data, n, ctypes.byref(got)))
+        return out[:n]
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            _n.load().sr_odometry_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Server:
     """Continuous-batching registration server over resident maps (sr_server):
     submit() queues a source, step() runs one scheduling step (admission by

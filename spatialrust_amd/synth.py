@@ -58,6 +58,7 @@ class Rng:
 class Scene:
     """Axis-aligned rectangles, spheres and vertical cylinders (lateral + top)."""
     rects: list = field(default_factory=list)      # (origin[3], u[3], v[3])
+    boxes: list = field(default_factory=list)      # (lo[3], hi[3]) of box primitives (their faces are in rects)
     spheres: list = field(default_factory=list)    # (center[3], r)
     cylinders: list = field(default_factory=list)  # (cx, cy, z0, r, h)
     extent: float = 1.0
@@ -101,6 +102,7 @@ def small_cube_scene(seed: int) -> Scene:
     for i in range(12):
         lo = np.array([c[i, 0] - e[i, 0] / 2, c[i, 1] - e[i, 1] / 2, 0.0])
         s.rects.extend(_box_faces(lo, lo + e[i]))
+        s.boxes.append((lo, lo + e[i]))
     r = rng.uniform(6, 0.3, 1.5)
     cc = rng.uniform(12, -h + 2.0, h - 2.0).reshape(6, 2)
     cz = rng.uniform(6, 0.0, 4.0)
@@ -128,6 +130,7 @@ def large_block_scene(seed: int) -> Scene:
     for i in range(300):
         lo = np.array([c[i, 0] - fp[i, 0] / 2, c[i, 1] - fp[i, 1] / 2, 0.0])
         s.rects.extend(_box_faces(lo, lo + np.array([fp[i, 0], fp[i, 1], ht[i]])))
+        s.boxes.append((lo, lo + np.array([fp[i, 0], fp[i, 1], ht[i]])))
     r = rng.uniform(100, 0.2, 1.5)
     hc = rng.uniform(100, 1.0, H)
     cc = rng.uniform(200, -h + 3.0, h - 3.0).reshape(100, 2)
@@ -183,6 +186,146 @@ def sample_scene(scene: Scene, n: int, sigma: float, seed: int) -> np.ndarray:
         out[m] = np.where(on_top[:, None], np.stack([tx, ty, z0 + h], 1), np.stack([px, py, pz], 1))
     noise = np.stack([rng.normal(n), rng.normal(n), rng.normal(n)], 1) * sigma
     return (out + noise).astype(np.float32)
+
+
+# ---------------------------------------------------------- spinning LiDAR --
+
+ODOMETRY_RING = 30.0      # loop radius of the odometry trajectory (m)
+ODOMETRY_CORRIDOR = 4.0   # obstacles within this distance of the loop are removed
+
+
+def odometry_scene(seed: int) -> Scene:
+    """The large_block scene with the boxes and cylinders that reach within
+    ODOMETRY_CORRIDOR of the odometry loop (a circle of radius ODOMETRY_RING about
+    the origin) removed, so the sensor drives through free space."""
+    src = large_block_scene(seed)
+    s = Scene(extent=src.extent)
+    s.rects = list(src.rects[:5])  # ground and walls
+    lo_r, hi_r = ODOMETRY_RING - ODOMETRY_CORRIDOR, ODOMETRY_RING + ODOMETRY_CORRIDOR
+    for lo, hi in src.boxes:
+        nx = np.clip(0.0, lo[0], hi[0])
+        ny = np.clip(0.0, lo[1], hi[1])
+        dmin = np.hypot(nx, ny)
+        dmax = max(np.hypot(x, y) for x in (lo[0], hi[0]) for y in (lo[1], hi[1]))
+        if dmin <= hi_r and dmax >= lo_r:
+            continue
+        s.boxes.append((lo, hi))
+        s.rects.extend(_box_faces(lo, hi))
+    for (cx, cy, z0, r, h) in src.cylinders:
+        d = np.hypot(cx, cy)
+        if d - r <= hi_r and d + r >= lo_r:
+            continue
+        s.cylinders.append((cx, cy, z0, r, h))
+    return s
+
+
+def odometry_trajectory(n_scans: int, seed: int, speed: float = 10.0, hz: float = 10.0,
+                        height: float = 1.8):
+    """world_T_sensor poses (q (x,y,z,w), t) of a smooth loop at `speed` m/s
+    sampled at `hz`: radius ODOMETRY_RING + 1.5 sin(2 phi + seeded phase), heading
+    along the path (yaw rate about 20 deg/s, below 30 deg/s)."""
+    ph = float(Rng(seed).uniform(1, 0.0, 2.0 * np.pi)[0])
+    dphi = 1e-3
+    phis = [0.0]
+    # arc-length parametrisation of r(phi) by fine integration
+    def radius(phi):
+        return ODOMETRY_RING + 1.5 * np.sin(2.0 * phi + ph)
+    step = speed / hz
+    phi, acc = 0.0, 0.0
+    while len(phis) < n_scans:
+        r0, r1 = radius(phi), radius(phi + dphi)
+        p0 = np.array([r0 * np.cos(phi), r0 * np.sin(phi)])
+        p1 = np.array([r1 * np.cos(phi + dphi), r1 * np.sin(phi + dphi)])
+        acc += float(np.linalg.norm(p1 - p0))
+        phi += dphi
+        if acc >= step:
+            phis.append(phi)
+            acc -= step
+    poses = []
+    for phi in phis:
+        r = radius(phi)
+        p = np.array([r * np.cos(phi), r * np.sin(phi)])
+        r2 = radius(phi + dphi)
+        p2 = np.array([r2 * np.cos(phi + dphi), r2 * np.sin(phi + dphi)])
+        yaw = float(np.arctan2(p2[1] - p[1], p2[0] - p[0]))
+        q = np.array([0.0, 0.0, np.sin(yaw / 2.0), np.cos(yaw / 2.0)])
+        poses.append((q, np.array([p[0], p[1], height])))
+    return poses
+
+
+def lidar_scan(scene: Scene, q, t, seed: int, beams: int = 64, azimuth_bins: int = 2048,
+               elevation_deg=(-25.0, 15.0), sigma: float = 0.02, dropout: float = 0.05,
+               max_range: float = 120.0, device: str = "cpu") -> np.ndarray:
+    """One spinning-LiDAR scan from sensor pose (q, t) = world_T_sensor: beams
+    with elevations linearly spaced over elevation_deg, azimuth_bins per
+    revolution, ray-cast against the scene's ground and walls (rects[:5]), boxes
+    (slab test) and cylinders (lateral surface and top). Range noise N(0, sigma)
+    and a random `dropout` fraction removed. Returns sensor-frame xyz (N, 3) f32.
+    The ray casting runs in FP64 torch on `device` (the result does not depend
+    on it beyond FP64 rounding of the hit distance)."""
+    import torch
+    rng = Rng(seed)
+    el = np.deg2rad(np.linspace(elevation_deg[0], elevation_deg[1], beams))
+    az = 2.0 * np.pi * np.arange(azimuth_bins) / azimuth_bins
+    E, A = np.meshgrid(el, az, indexing="ij")
+    d_s = np.stack([np.cos(E) * np.cos(A), np.cos(E) * np.sin(A), np.sin(E)], -1).reshape(-1, 3)
+    R = quat_to_mat(q)
+    f64 = dict(dtype=torch.float64, device=device)
+    D = torch.as_tensor(d_s @ R.T, **f64)
+    O = torch.as_tensor(np.asarray(t, np.float64), **f64)
+    inf = torch.tensor(float("inf"), **f64)
+    best = torch.full((D.shape[0],), float("inf"), **f64)
+    inv = 1.0 / D
+    # ground and walls: axis-aligned rectangles
+    for (o, u, v) in scene.rects[:5]:
+        o, u, v = (np.asarray(a, np.float64) for a in (o, u, v))
+        ax = int(np.nonzero((u == 0) & (v == 0))[0][0])
+        tt = (float(o[ax]) - O[ax]) * inv[:, ax]
+        ok = tt > 1e-6
+        for b in range(3):
+            if b == ax:
+                continue
+            lo_b, hi_b = sorted((float(o[b]), float(o[b] + u[b] + v[b])))
+            hb = O[b] + tt * D[:, b]
+            ok &= (hb >= lo_b) & (hb <= hi_b)
+        best = torch.where(ok & (tt < best), tt, best)
+    # boxes: slab test
+    if scene.boxes:
+        B_lo = torch.as_tensor(np.array([b[0] for b in scene.boxes]), **f64)
+        B_hi = torch.as_tensor(np.array([b[1] for b in scene.boxes]), **f64)
+        for c0 in range(0, B_lo.shape[0], 16):
+            t1 = (B_lo[None, c0:c0 + 16] - O) * inv[:, None, :]
+            t2 = (B_hi[None, c0:c0 + 16] - O) * inv[:, None, :]
+            tmin = torch.minimum(t1, t2).nan_to_num(nan=-float("inf")).amax(dim=2)
+            tmax = torch.maximum(t1, t2).nan_to_num(nan=float("inf")).amin(dim=2)
+            ok = (tmax >= tmin) & (tmin > 1e-6)
+            best = torch.minimum(best, torch.where(ok, tmin, inf).amin(dim=1))
+    # cylinders: lateral surface (entry root) and top disc
+    if scene.cylinders:
+        C = torch.as_tensor(np.array(scene.cylinders, np.float64), **f64)  # (M, 5)
+        for c0 in range(0, C.shape[0], 16):
+            cx, cy, z0, r, h = (C[c0:c0 + 16, k][None] for k in range(5))
+            ox, oy = O[0] - cx, O[1] - cy
+            dx, dy, dz = D[:, 0:1], D[:, 1:2], D[:, 2:3]
+            a = dx * dx + dy * dy
+            bb = ox * dx + oy * dy
+            cc = ox * ox + oy * oy - r * r
+            disc = bb * bb - a * cc
+            tl = (-bb - torch.sqrt(torch.clamp(disc, min=0.0))) / a
+            zl = O[2] + tl * dz
+            okl = (disc >= 0) & (tl > 1e-6) & (zl >= z0) & (zl <= z0 + h)
+            tt = (z0 + h - O[2]) / dz
+            hx = ox + tt * dx
+            hy = oy + tt * dy
+            okt = (tt > 1e-6) & (hx * hx + hy * hy <= r * r)
+            cand = torch.minimum(torch.where(okl, tl, inf), torch.where(okt, tt, inf)).amin(dim=1)
+            best = torch.minimum(best, cand)
+    best = best.cpu().numpy()
+    keep = np.isfinite(best) & (best <= max_range)
+    rng_noisy = best + sigma * rng.normal(len(best))
+    keep &= rng.uniform(len(best)) >= dropout
+    pts = d_s[keep] * rng_noisy[keep][:, None]
+    return pts.astype(np.float32)
 
 
 # ------------------------------------------------------------ rigid motion --
