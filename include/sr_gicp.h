@@ -197,6 +197,19 @@ int32_t sr_voxel_downsample(const sr_xyz_view* input, float leaf, const float* o
 int32_t sr_plane_covariances(const sr_xyz_view* cloud, uint32_t k_neighbors, float radius,
                              double epsilon, float* out_cov6, float* out_normals);
 
+/* Replaces GpuNormalEstimator::estimate with NormalEstimationConfig::search_radius
+ * = Some(radius) (crates/spatialrust-features/src/normal_gpu.rs:33-82): per point
+ * the unit normal of the smallest eigenvalue of the radius-neighbourhood
+ * covariance and its curvature lambda_min / trace, with NORMALS_GRID_WGSL
+ * semantics (crates/spatialrust-gpu/src/kernels/normals_grid.rs:25-198: grid of
+ * edge = radius, fewer than 3 neighbours give (0,0,1) and curvature 0), then
+ * oriented toward `viewpoint` (3 floats, may be NULL) as
+ * orient_normal_towards_viewpoint does (crates/spatialrust-features/src/normal.rs:406-421).
+ * Host outputs: out_normals 3 f32 per point, out_curvature 1 f32 per point
+ * (either may be NULL). */
+int32_t sr_grid_normals(const sr_xyz_view* cloud, float radius, const float* viewpoint,
+                        float* out_normals, float* out_curvature);
+
 /* ---- nearest neighbour (NearestNeighborIndex::nearest_k / nearest_one) - */
 
 /* Exact k-NN of each query against `points` on a device uniform grid.

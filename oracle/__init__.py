@@ -75,6 +75,8 @@ def _declare(L):
     L.or_radius_covariances.restype = ctypes.c_int32
     L.or_radius_covariances.argtypes = [_F, _F, _F, ctypes.c_int64, ctypes.c_float,
                                         ctypes.c_float, _F, _F]
+    L.or_grid_normals.restype = ctypes.c_int32
+    L.or_grid_normals.argtypes = [_F, _F, _F, ctypes.c_int64, ctypes.c_float, _F, _F]
 
 
 def _f32(a):
@@ -171,6 +173,20 @@ def radius_covariances(points, radius, epsilon=1e-3):
     if st != 0:
         raise ValueError("grid would exceed the dense cell cap")
     return out, nrm
+
+
+def grid_normals(points, radius, viewpoint=None):
+    """GpuNormalEstimator with a search radius (normal_gpu.rs:33-82, normals_grid.rs):
+    unit normals (N,3) f32 and curvature (N,) f32, oriented to `viewpoint` if given."""
+    x, y, z = _cols(points)
+    n = len(x)
+    out = np.zeros((max(n, 1), 4), np.float32)
+    vp = None if viewpoint is None else np.ascontiguousarray(viewpoint, dtype=np.float32)
+    st = lib().or_grid_normals(_p(x, _F), _p(y, _F), _p(z, _F), n, float(radius),
+                               _p(vp, _F) if vp is not None else None, _p(out, _F))
+    if st != 0:
+        raise ValueError("grid would exceed the dense cell cap")
+    return out[:n, :3], out[:n, 3]
 
 
 def gicp_normal_equations(source, target, src_cov, tgt_cov, q, t, dmax):

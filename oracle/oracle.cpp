@@ -1084,8 +1084,30 @@ int32_t or_grid_bounds(const float* x, const float* y, const float* z, int64_t n
 // centred sums, 16 cyclic Jacobi sweeps in f32, eigenvalues (eps,1,1) with the
 // smallest mapped to eps; count < 3 gives eps*I. Output 6 f32 per point
 // [c00,c11,c22,c01,c02,c12]; normals (nullable) 3 f32 per point (zero when count < 3).
-int32_t or_radius_covariances(const float* x, const float* y, const float* z, int64_t n,
-                              float radius, float eps, float* out6, float* normals) {
+// orient_normal_towards_viewpoint (crates/spatialrust-features/src/normal.rs:406-421):
+// flip when n . (viewpoint - p) < 0, then Vec3::normalize (vec.rs:94-100).
+static void orient_towards(float* nv, float px, float py, float pz, const float* vp) {
+  float vx = vp[0] - px, vy = vp[1] - py, vz = vp[2] - pz;
+  if (nv[0] * vx + nv[1] * vy + nv[2] * vz < 0.0f) {
+    nv[0] = -nv[0];
+    nv[1] = -nv[1];
+    nv[2] = -nv[2];
+  }
+  float len = std::sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+  if (len == 0.0f) return;
+  nv[0] /= len;
+  nv[1] /= len;
+  nv[2] /= len;
+}
+
+// Shared radius-grid evaluation. out6/normals: COV_GRID_WGSL outputs
+// (covariances_grid.rs:81-206); grid4: NORMALS_GRID_WGSL outputs
+// (normals_grid.rs:25-198: unit normal of the smallest eigenvalue and its
+// curvature, (0,0,1,0) below 3 neighbours), oriented to `vp` when given as
+// GpuNormalEstimator::estimate does (features/src/normal_gpu.rs:62-71).
+static int32_t radius_grid_eval(const float* x, const float* y, const float* z, int64_t n,
+                                float radius, float eps, float* out6, float* normals,
+                                float* grid4, const float* vp) {
   if (n == 0) return 0;
   float org[3];
   uint32_t dims[3];
@@ -1209,20 +1231,55 @@ int32_t or_radius_covariances(const float* x, const float* y, const float* z, in
         reg[4] += lam * ax * az;
         reg[5] += lam * ay * az;
       }
-      std::memcpy(out6 + 6 * i, reg, sizeof(reg));
+      if (out6) std::memcpy(out6 + 6 * i, reg, sizeof(reg));
       if (normals) {
         normals[3 * i + 0] = v[0][mi];
         normals[3 * i + 1] = v[1][mi];
         normals[3 * i + 2] = v[2][mi];
       }
+      if (grid4) {
+        float nv[3] = {v[0][mi], v[1][mi], v[2][mi]};
+        float len = std::max(std::sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]), 1e-20f);
+        nv[0] /= len;
+        nv[1] /= len;
+        nv[2] /= len;
+        float tr = eig[0] + eig[1] + eig[2];
+        float curv = tr > 0.0f ? eig[mi] / tr : 0.0f;
+        if (vp) orient_towards(nv, px, py, pz, vp);
+        grid4[4 * i + 0] = nv[0];
+        grid4[4 * i + 1] = nv[1];
+        grid4[4 * i + 2] = nv[2];
+        grid4[4 * i + 3] = curv;
+      }
     }
     if (count < 3.0f) {
       float iso[6] = {eps, eps, eps, 0, 0, 0};
-      std::memcpy(out6 + 6 * i, iso, sizeof(iso));
+      if (out6) std::memcpy(out6 + 6 * i, iso, sizeof(iso));
       if (normals) normals[3 * i] = normals[3 * i + 1] = normals[3 * i + 2] = 0.0f;
+      if (grid4) {
+        float nv[3] = {0.0f, 0.0f, 1.0f};
+        if (vp) orient_towards(nv, px, py, pz, vp);
+        grid4[4 * i + 0] = nv[0];
+        grid4[4 * i + 1] = nv[1];
+        grid4[4 * i + 2] = nv[2];
+        grid4[4 * i + 3] = 0.0f;
+      }
     }
   }
   return 0;
+}
+
+int32_t or_radius_covariances(const float* x, const float* y, const float* z, int64_t n,
+                              float radius, float eps, float* out6, float* normals) {
+  return radius_grid_eval(x, y, z, n, radius, eps, out6, normals, nullptr, nullptr);
+}
+
+// GpuNormalEstimator::estimate with a search radius (crates/spatialrust-features/src/normal_gpu.rs:33-82)
+// over NORMALS_GRID_WGSL (crates/spatialrust-gpu/src/kernels/normals_grid.rs:25-198): out4
+// per point = (unit normal, curvature); vp (3 floats) may be null (no orientation).
+int32_t or_grid_normals(const float* x, const float* y, const float* z, int64_t n, float radius,
+                        const float* vp, float* out4) {
+  return radius_grid_eval(x, y, z, n, radius, 0.0f, nullptr, nullptr, out4, vp);
 }
 
 }  // extern "C"

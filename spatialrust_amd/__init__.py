@@ -7,7 +7,7 @@ kernels behind the C ABI in include/sr_gicp.h.
 """
 from .registration import (DeviceKind, DeviceUnavailable, ExecutionOutput, ExecutionPolicy,
                            ExecutionReceipt, GicpConfig, GicpRegistration, GicpScanMatcher,
-                           InvalidArgument,
+                           GpuNormalEstimator, InvalidArgument, NormalEstimationConfig,
                            Isometry3, RegistrationResult, RollingMapOdometry, Server, SpatialError,
                            TargetMap,
                            VoxelGridDownsample, VoxelGridDownsampleConfig, VoxelOutput,
@@ -16,7 +16,8 @@ from .registration import (DeviceKind, DeviceUnavailable, ExecutionOutput, Execu
 
 __all__ = [
     "DeviceKind", "DeviceUnavailable", "ExecutionOutput", "ExecutionPolicy", "ExecutionReceipt",
-    "GicpConfig", "GicpRegistration", "GicpScanMatcher", "InvalidArgument", "Isometry3", "RegistrationResult",
+    "GicpConfig", "GicpRegistration", "GicpScanMatcher", "GpuNormalEstimator", "InvalidArgument",
+    "NormalEstimationConfig", "Isometry3", "RegistrationResult",
     "RollingMapOdometry", "Server", "SpatialError", "TargetMap", "VoxelGridDownsample", "VoxelGridDownsampleConfig", "VoxelOutput",
     "batch_align", "batch_align_pairs", "device_available", "device_view", "gicp_normal_equations", "knn",
     "plane_covariances", "register_gicp",

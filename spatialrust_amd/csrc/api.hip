@@ -1170,6 +1170,38 @@ int32_t sr_plane_covariances(const sr_xyz_view* cloud, uint32_t k_neighbors, flo
   });
 }
 
+int32_t sr_grid_normals(const sr_xyz_view* cloud, float radius, const float* viewpoint,
+                        float* out_normals, float* out_curvature) {
+  return guarded([&] {
+    ensure_device();
+    validate_view(cloud);
+    if (cloud->n == 0) return;  // normal_gpu.rs:35-37
+    if (!(radius > 0.0f)) fail(SR_ERR_INVALID_ARGUMENT, "grid radius must be positive");
+    if (cloud->n > 0xFFFFFFF0ull) fail(SR_ERR_INVALID_ARGUMENT, "point clouds above 4G points are not supported");
+    hipStream_t s = thread_stream();
+    Cloud c;
+    pack_cloud(cloud->x, cloud->y, cloud->z, uint32_t(cloud->n), cloud->mem == SR_MEM_DEVICE, c, s, nullptr);
+    uint32_t hb[6];
+    host_bounds(c, hb, s);
+    Grid rg;
+    build_grid(c, hb, radius, rg, s);
+    const uint32_t n = c.n;
+    DevBuf<float4> out(n);
+    grid_normals(rg, radius, viewpoint, out.p, s);
+    std::vector<float4> host(n);
+    SR_HIP(hipMemcpyAsync(host.data(), out.p, sizeof(float4) * n, hipMemcpyDeviceToHost, s));
+    SR_HIP(hipStreamSynchronize(s));
+    for (uint32_t i = 0; i < n; ++i) {
+      if (out_normals) {
+        out_normals[3 * size_t(i)] = host[i].x;
+        out_normals[3 * size_t(i) + 1] = host[i].y;
+        out_normals[3 * size_t(i) + 2] = host[i].z;
+      }
+      if (out_curvature) out_curvature[i] = host[i].w;
+    }
+  });
+}
+
 int32_t sr_knn(const sr_xyz_view* points, const sr_xyz_view* queries, uint32_t k,
                int32_t* out_index, float* out_d2) {
   return guarded([&] {

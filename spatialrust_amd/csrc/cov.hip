@@ -330,18 +330,16 @@ __device__ __forceinline__ void wgsl_rotate(float a[3][3], float v[3][3], int p,
   }
 }
 
-// One point per thread over a grid whose edge is the radius. Neighbours are
-// read from the grid-ordered copy (ascending index inside each cell, the order
-// build_grid produces), so the f32 sums happen in the reference's order.
-__global__ void __launch_bounds__(kBlock) k_radius_cov(GridDesc g, const uint32_t* __restrict__ cs,
-                                                       const float4* __restrict__ gp, float r2,
-                                                       float eps, float4* __restrict__ nrm_orig,
-                                                       float* __restrict__ cov6,
-                                                       float* __restrict__ normal3) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= g.npoints) return;
-  const float4 p = gp[i];
-  const uint32_t orig = __float_as_uint(p.w);
+// Two-pass f32 covariance of the radius neighbourhood of p over a grid whose
+// edge is the radius, then 16 Jacobi sweeps, in the WGSL kernels' order
+// (covariances_grid.rs:81-206, normals_grid.rs:86-160 are identical up to this
+// point). Neighbours are read from the grid-ordered copy (ascending index
+// inside each cell, the order build_grid produces), so the f32 sums happen in
+// the reference's order. Returns the neighbour count; a and v hold the rotated
+// matrix and the eigenvectors (columns) when the count is at least 3.
+__device__ __forceinline__ float radius_eigen(const GridDesc& g, const uint32_t* __restrict__ cs,
+                                              const float4* __restrict__ gp, const float4& p,
+                                              float r2, float (&a)[3][3], float (&v)[3][3]) {
   const int cx = cell_coord(p.x, g.ox, g.inv_h, g.dx), cy = cell_coord(p.y, g.oy, g.inv_h, g.dy),
             cz = cell_coord(p.z, g.oz, g.inv_h, g.dz);
   float mx = 0.0f, my = 0.0f, mz = 0.0f, count = 0.0f;
@@ -369,48 +367,68 @@ __global__ void __launch_bounds__(kBlock) k_radius_cov(GridDesc g, const uint32_
       }
     }
   }
-  float reg[6] = {eps, eps, eps, 0.0f, 0.0f, 0.0f};
-  float nv[3] = {0.0f, 0.0f, 0.0f};
-  float a_iso = eps;  // C = a*I - (1-eps) n n^T; eps*I when count < 3
-  if (count >= 3.0f) {
-    mx = __fdiv_rn(mx, count);
-    my = __fdiv_rn(my, count);
-    mz = __fdiv_rn(mz, count);
-    float c00 = 0, c11 = 0, c22 = 0, c01 = 0, c02 = 0, c12 = 0;
-    for (int dz = -1; dz <= 1; ++dz) {
-      int nz = cz + dz;
-      if (nz < 0 || nz >= g.dz) continue;
-      for (int dy = -1; dy <= 1; ++dy) {
-        int ny = cy + dy;
-        if (ny < 0 || ny >= g.dy) continue;
-        for (int dx = -1; dx <= 1; ++dx) {
-          int nx = cx + dx;
-          if (nx < 0 || nx >= g.dx) continue;
-          uint32_t c = (uint32_t(nz) * uint32_t(g.dy) + uint32_t(ny)) * uint32_t(g.dx) + uint32_t(nx);
-          for (uint32_t s = cs[c]; s < cs[c + 1]; ++s) {
-            float4 q = gp[s];
-            float rx = __fsub_rn(q.x, p.x), ry = __fsub_rn(q.y, p.y), rz = __fsub_rn(q.z, p.z);
-            float d2 = __fadd_rn(__fadd_rn(__fmul_rn(rx, rx), __fmul_rn(ry, ry)), __fmul_rn(rz, rz));
-            if (d2 <= r2) {
-              float ex = __fsub_rn(q.x, mx), ey = __fsub_rn(q.y, my), ez = __fsub_rn(q.z, mz);
-              c00 = __fadd_rn(c00, __fmul_rn(ex, ex));
-              c11 = __fadd_rn(c11, __fmul_rn(ey, ey));
-              c22 = __fadd_rn(c22, __fmul_rn(ez, ez));
-              c01 = __fadd_rn(c01, __fmul_rn(ex, ey));
-              c02 = __fadd_rn(c02, __fmul_rn(ex, ez));
-              c12 = __fadd_rn(c12, __fmul_rn(ey, ez));
-            }
+  if (count < 3.0f) return count;
+  mx = __fdiv_rn(mx, count);
+  my = __fdiv_rn(my, count);
+  mz = __fdiv_rn(mz, count);
+  float c00 = 0, c11 = 0, c22 = 0, c01 = 0, c02 = 0, c12 = 0;
+  for (int dz = -1; dz <= 1; ++dz) {
+    int nz = cz + dz;
+    if (nz < 0 || nz >= g.dz) continue;
+    for (int dy = -1; dy <= 1; ++dy) {
+      int ny = cy + dy;
+      if (ny < 0 || ny >= g.dy) continue;
+      for (int dx = -1; dx <= 1; ++dx) {
+        int nx = cx + dx;
+        if (nx < 0 || nx >= g.dx) continue;
+        uint32_t c = (uint32_t(nz) * uint32_t(g.dy) + uint32_t(ny)) * uint32_t(g.dx) + uint32_t(nx);
+        for (uint32_t s = cs[c]; s < cs[c + 1]; ++s) {
+          float4 q = gp[s];
+          float rx = __fsub_rn(q.x, p.x), ry = __fsub_rn(q.y, p.y), rz = __fsub_rn(q.z, p.z);
+          float d2 = __fadd_rn(__fadd_rn(__fmul_rn(rx, rx), __fmul_rn(ry, ry)), __fmul_rn(rz, rz));
+          if (d2 <= r2) {
+            float ex = __fsub_rn(q.x, mx), ey = __fsub_rn(q.y, my), ez = __fsub_rn(q.z, mz);
+            c00 = __fadd_rn(c00, __fmul_rn(ex, ex));
+            c11 = __fadd_rn(c11, __fmul_rn(ey, ey));
+            c22 = __fadd_rn(c22, __fmul_rn(ez, ez));
+            c01 = __fadd_rn(c01, __fmul_rn(ex, ey));
+            c02 = __fadd_rn(c02, __fmul_rn(ex, ez));
+            c12 = __fadd_rn(c12, __fmul_rn(ey, ez));
           }
         }
       }
     }
-    float a[3][3] = {{c00, c01, c02}, {c01, c11, c12}, {c02, c12, c22}};
-    float v[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-    for (int sweep = 0; sweep < 16; ++sweep) {
-      wgsl_rotate(a, v, 0, 1);
-      wgsl_rotate(a, v, 0, 2);
-      wgsl_rotate(a, v, 1, 2);
-    }
+  }
+  a[0][0] = c00; a[0][1] = c01; a[0][2] = c02;
+  a[1][0] = c01; a[1][1] = c11; a[1][2] = c12;
+  a[2][0] = c02; a[2][1] = c12; a[2][2] = c22;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) v[r][c] = r == c ? 1.0f : 0.0f;
+  for (int sweep = 0; sweep < 16; ++sweep) {
+    wgsl_rotate(a, v, 0, 1);
+    wgsl_rotate(a, v, 0, 2);
+    wgsl_rotate(a, v, 1, 2);
+  }
+  return count;
+}
+
+// Radius covariances, COV_GRID_WGSL semantics: eigenvalues (eps, 1, 1) with the
+// smallest mapped to eps; eps*I below 3 neighbours.
+__global__ void __launch_bounds__(kBlock) k_radius_cov(GridDesc g, const uint32_t* __restrict__ cs,
+                                                       const float4* __restrict__ gp, float r2,
+                                                       float eps, float4* __restrict__ nrm_orig,
+                                                       float* __restrict__ cov6,
+                                                       float* __restrict__ normal3) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.npoints) return;
+  const float4 p = gp[i];
+  const uint32_t orig = __float_as_uint(p.w);
+  float a[3][3], v[3][3];
+  const float count = radius_eigen(g, cs, gp, p, r2, a, v);
+  float reg[6] = {eps, eps, eps, 0.0f, 0.0f, 0.0f};
+  float nv[3] = {0.0f, 0.0f, 0.0f};
+  float a_iso = eps;  // C = a*I - (1-eps) n n^T; eps*I when count < 3
+  if (count >= 3.0f) {
     int mi = 0;
     if (a[1][1] < a[mi][mi]) mi = 1;
     if (a[2][2] < a[mi][mi]) mi = 2;
@@ -449,6 +467,66 @@ void radius_normals(const Grid& rg, float radius, float eps, float4* nrm_orig, f
   ProfScope prof("radius_cov", s);
   hipLaunchKernelGGL(k_radius_cov, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, rg.desc,
                      rg.cell_start.p, rg.pts.p, radius * radius, eps, nrm_orig, cov6, normal3);
+}
+
+// GpuNormalEstimator with a search radius (features/src/normal_gpu.rs:33-82):
+// NORMALS_GRID_WGSL outputs (normals_grid.rs:160-176: unit normal of the
+// smallest eigenvalue, curvature = lambda_min / trace; (0,0,1,0) below 3
+// neighbours), then, when a viewpoint is given, orient_normal_towards_viewpoint
+// (features/src/normal.rs:406-421). out4 in original point order.
+__global__ void __launch_bounds__(kBlock) k_grid_normals(GridDesc g, const uint32_t* __restrict__ cs,
+                                                         const float4* __restrict__ gp, float r2,
+                                                         int has_vp, float vx, float vy, float vz,
+                                                         float4* __restrict__ out4) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.npoints) return;
+  const float4 p = gp[i];
+  const uint32_t orig = __float_as_uint(p.w);
+  float a[3][3], v[3][3];
+  const float count = radius_eigen(g, cs, gp, p, r2, a, v);
+  float n0 = 0.0f, n1 = 0.0f, n2 = 1.0f, curv = 0.0f;
+  if (count >= 3.0f) {
+    const float eig[3] = {a[0][0], a[1][1], a[2][2]};
+    int mi = 0;
+    if (eig[1] < eig[mi]) mi = 1;
+    if (eig[2] < eig[mi]) mi = 2;
+    n0 = v[0][mi];
+    n1 = v[1][mi];
+    n2 = v[2][mi];
+    const float len = fmaxf(
+        sqrtf(__fadd_rn(__fadd_rn(__fmul_rn(n0, n0), __fmul_rn(n1, n1)), __fmul_rn(n2, n2))), 1e-20f);
+    n0 = __fdiv_rn(n0, len);
+    n1 = __fdiv_rn(n1, len);
+    n2 = __fdiv_rn(n2, len);
+    const float tr = __fadd_rn(__fadd_rn(eig[0], eig[1]), eig[2]);
+    if (tr > 0.0f) curv = __fdiv_rn(eig[mi], tr);
+  }
+  if (has_vp) {
+    const float wx = __fsub_rn(vx, p.x), wy = __fsub_rn(vy, p.y), wz = __fsub_rn(vz, p.z);
+    if (__fadd_rn(__fadd_rn(__fmul_rn(n0, wx), __fmul_rn(n1, wy)), __fmul_rn(n2, wz)) < 0.0f) {
+      n0 = -n0;
+      n1 = -n1;
+      n2 = -n2;
+    }
+    const float len = sqrtf(__fadd_rn(__fadd_rn(__fmul_rn(n0, n0), __fmul_rn(n1, n1)), __fmul_rn(n2, n2)));
+    if (len != 0.0f) {
+      n0 = __fdiv_rn(n0, len);
+      n1 = __fdiv_rn(n1, len);
+      n2 = __fdiv_rn(n2, len);
+    }
+  }
+  out4[orig] = make_float4(n0, n1, n2, curv);
+}
+
+void grid_normals(const Grid& rg, float radius, const float* viewpoint, float4* out_orig,
+                  hipStream_t s) {
+  const uint32_t n = rg.desc.npoints;
+  if (n == 0) return;
+  ProfScope prof("grid_normals", s);
+  hipLaunchKernelGGL(k_grid_normals, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, rg.desc,
+                     rg.cell_start.p, rg.pts.p, radius * radius, viewpoint ? 1 : 0,
+                     viewpoint ? viewpoint[0] : 0.0f, viewpoint ? viewpoint[1] : 0.0f,
+                     viewpoint ? viewpoint[2] : 0.0f, out_orig);
 }
 
 }  // namespace sr

@@ -167,6 +167,11 @@ void knn_normals(const Grid& g, uint32_t k, double eps, float4* normals_grid_ord
 // radius. normals in grid order of `g_out_order` (the GICP grid); cov6 in original order.
 void radius_normals(const Grid& rg, float radius, float eps, float4* normals_orig_order,
                     float* cov6, float* normal3, hipStream_t s);
+// Unit normals and curvature (NORMALS_GRID_WGSL semantics) over a grid of
+// edge = radius, oriented to `viewpoint` (3 floats, host) when non-null; out in
+// original point order as float4(n, curvature).
+void grid_normals(const Grid& rg, float radius, const float* viewpoint, float4* out_orig,
+                  hipStream_t s);
 // Permutes per-point float4 values from original order to grid order.
 void gather_to_grid_order(const Grid& g, const float4* orig, float4* out, hipStream_t s);
 

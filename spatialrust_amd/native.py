@@ -93,6 +93,7 @@ SIGNATURES = {
     "sr_gicp_align": (_I32, [ctypes.POINTER(XyzView), ctypes.POINTER(XyzView),
                              ctypes.POINTER(GicpConfigC), PolicyC, ctypes.POINTER(ResultC),
                              ctypes.POINTER(ReceiptC)]),
+    "sr_grid_normals": (_I32, [ctypes.POINTER(XyzView), ctypes.c_float, _P, _P, _P]),
     "sr_batch_align_pairs": (_I32, [ctypes.POINTER(XyzView), ctypes.POINTER(XyzView), ctypes.c_uint32,
                                     ctypes.POINTER(GicpConfigC), ctypes.POINTER(ResultC)]),
     "sr_gicp_normal_equations": (_I32, [ctypes.POINTER(XyzView), ctypes.POINTER(XyzView),

@@ -321,6 +321,36 @@ def plane_covariances(points, k_neighbors: int = 20, radius: Optional[float] = N
     return cov[:n], nrm[:n]
 
 
+@dataclass
+class NormalEstimationConfig:
+    """The radius-mode fields of NormalEstimationConfig (features/src/normal.rs)."""
+    search_radius: float = 0.1
+    viewpoint: Optional[Tuple[float, float, float]] = None
+
+
+class GpuNormalEstimator:
+    """GpuNormalEstimator with a search radius (features/src/normal_gpu.rs:33-82) on
+    the HIP device: estimate() returns (normals (N,3) f32, curvature (N,) f32)."""
+
+    def __init__(self, config: NormalEstimationConfig):
+        self._config = config
+
+    def config(self) -> NormalEstimationConfig:
+        return self._config
+
+    def estimate(self, points):
+        cols = _Cols(points)
+        n = len(cols.x)
+        nrm = np.zeros((max(n, 1), 3), np.float32)
+        curv = np.zeros(max(n, 1), np.float32)
+        vp = self._config.viewpoint
+        vpa = None if vp is None else np.ascontiguousarray(vp, dtype=np.float32)
+        _check(_n.load().sr_grid_normals(ctypes.byref(cols.view), float(self._config.search_radius),
+                                         vpa.ctypes.data if vpa is not None else None,
+                                         nrm.ctypes.data, curv.ctypes.data))
+        return nrm[:n], curv[:n]
+
+
 def knn(points, queries, k: int):
     """Exact k-NN (ascending distance, ties to the lower index); -1 pads."""
     pc = _Cols(points)

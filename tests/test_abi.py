@@ -109,3 +109,24 @@ def test_device_kind_is_gpu():
     assert sr.DeviceKind.HIP.is_gpu() and not sr.DeviceKind.CPU.is_gpu()
     assert sr.ExecutionPolicy.auto().allows_fallback()
     assert not sr.ExecutionPolicy.gpu().allows_fallback()
+
+
+def test_pipeline_transform_point_cloud_matches_oracle(orc):
+    """The MVP stage's source synthesis (transform_point_cloud, f32) equals the
+    oracle's restatement of Isometry3::transform_point bit for bit."""
+    from spatialrust_amd.pipeline import transform_point_cloud
+    from spatialrust_amd.registration import Isometry3
+    rng = np.random.default_rng(3)
+    pts = (rng.standard_normal((5000, 3)) * 20).astype(np.float32)
+    for q, t in (((0.0, 0.0, np.sin(0.025), np.cos(0.025)), (0.01, -0.008, 0.012)),
+                 ((0.3, -0.2, 0.1, 0.9), (1.5, 2.0, -3.0)), ((0.0, 0.0, 0.0, 0.0), (0.0, 0.0, 1.0))):
+        got = transform_point_cloud(pts, Isometry3(q, t))
+        want = orc.transform_points(np.array(q, np.float32), np.array(t, np.float32), pts)
+        assert np.array_equal(got, want)
+
+
+def test_pipeline_rejects_non_gicp_methods():
+    from spatialrust_amd import pipeline as pl
+    from spatialrust_amd.registration import InvalidArgument
+    with pytest.raises(InvalidArgument, match="no HIP backend"):
+        pl.registration_stage(np.zeros((10, 3), np.float32), pl.MvpIcpConfig())

@@ -344,3 +344,56 @@ def test_server_results_equal_alone(hip):
     server.close()
     for m in maps:
         m.close()
+
+
+def test_mvp_registration_stage_gicp(hip):
+    """The MVP pipeline's GICP step (mvp.rs:428-473 runs_gicp_registration) with a
+    policy: corner cloud, voxel 0.05, d_max 0.3, 40 iterations; the receipt
+    reports the HIP device and the registration stage."""
+    from spatialrust_amd import pipeline as pl
+    pts = []
+    for i in range(12):
+        for j in range(12):
+            a, b = np.float32(i * 0.06), np.float32(j * 0.06)
+            pts += [[a, b, 0.0], [a, 0.0, b + np.float32(0.03)],
+                    [0.0, a + np.float32(0.03), b + np.float32(0.03)]]
+    cloud = np.array(pts, np.float32)
+    half = 0.025
+    icp = pl.MvpIcpConfig(icp=pl.IcpConfig(max_correspondence_distance=0.3, max_iterations=40,
+                                           min_correspondences=6),
+                          source_transform=sr.Isometry3((0.0, 0.0, np.sin(half), np.cos(half)),
+                                                        (0.01, -0.008, 0.012)),
+                          method=pl.MvpRegistrationMethod.GICP)
+    run = pl.MvpRegistrationPipeline(sr.VoxelGridDownsampleConfig(0.05), icp).run(cloud)
+    assert np.isfinite(run.registration.fitness) and run.registration.fitness < 1e-1
+    assert run.registration_receipt.resolved_policy == sr.ExecutionPolicy.gpu(sr.DeviceKind.HIP)
+    assert run.registration_receipt.stages[-1] == "registration"
+    assert run.voxel_receipt.resolved_policy == sr.ExecutionPolicy.gpu(sr.DeviceKind.HIP)
+
+
+@pytest.mark.parametrize("case", ["plane_up", "plane_none", "random_vp", "small_cube_vp"])
+def test_grid_normals_bit_exact(hip, orc, case):
+    """GpuNormalEstimator radius mode: normals and curvature bit-exact vs the
+    CPU evaluation of NORMALS_GRID_WGSL + orient_normal_towards_viewpoint; the
+    reference's planar-patch assertions hold (normals_grid.rs:358-377,
+    normal_gpu.rs:126-153)."""
+    rng = np.random.default_rng(4)
+    vp = None
+    if case.startswith("plane"):
+        x, y = np.meshgrid(np.arange(12) * np.float32(0.1), np.arange(12) * np.float32(0.1), indexing="ij")
+        pts = np.stack([x.ravel(), y.ravel(), np.zeros(144)], 1).astype(np.float32)
+        radius = 0.25
+        vp = (0.0, 0.0, 10.0) if case == "plane_up" else None
+    elif case == "random_vp":
+        pts = (rng.random((30000, 3)) * [8, 8, 2]).astype(np.float32)
+        radius, vp = 0.3, (4.0, 4.0, 20.0)
+    else:
+        pts, radius, vp = synth.small_cube_pair().target, 0.3, (0.0, 0.0, 1.8)
+    est = sr.GpuNormalEstimator(sr.NormalEstimationConfig(search_radius=radius, viewpoint=vp))
+    n, c = est.estimate(pts)
+    on, oc = orc.grid_normals(pts, radius, viewpoint=vp)
+    assert np.array_equal(n, on) and np.array_equal(c, oc)
+    if case.startswith("plane"):
+        assert np.all(np.abs(n[:, 2]) > 0.99) and np.all(c < 1e-3)
+        if case == "plane_up":
+            assert np.all(n[:, 2] > 0.99)

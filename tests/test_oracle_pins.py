@@ -402,3 +402,51 @@ def test_radius_covariances_match_bruteforce_neighbourhoods(orc):
         assert np.allclose(cov[i], want, atol=2e-4), (i, cov[i], want)
         checked += 1
     assert checked > 1000
+
+
+def _plane_grid():
+    x, y = np.meshgrid(np.arange(12) * np.float32(0.1), np.arange(12) * np.float32(0.1), indexing="ij")
+    return np.stack([x.ravel(), y.ravel(), np.zeros(144)], 1).astype(np.float32)
+
+
+def test_grid_normals_planar_patch(orc):
+    # normals_grid.rs:358-377 (vertical, curvature < 1e-3) and normal_gpu.rs:126-153
+    # (viewpoint above: nz > 0.99)
+    n, curv = orc.grid_normals(_plane_grid(), 0.25)
+    assert np.all(np.abs(n[:, 2]) > 0.99) and np.all(curv < 1e-3)
+    n, _ = orc.grid_normals(_plane_grid(), 0.25, viewpoint=(0.0, 0.0, 10.0))
+    assert np.all(n[:, 2] > 0.99)
+    n, _ = orc.grid_normals(_plane_grid(), 0.25, viewpoint=(0.0, 0.0, -10.0))
+    assert np.all(n[:, 2] < -0.99)
+
+
+def test_grid_normals_match_bruteforce(orc):
+    # normal = smallest-eigenvalue eigenvector of the r-neighbourhood covariance
+    # (up to sign, fixed by the viewpoint), curvature = lambda_min / trace;
+    # fewer than 3 neighbours -> (0, 0, 1) then oriented, curvature 0.
+    rng = np.random.default_rng(21)
+    plane = np.c_[rng.uniform(0, 3, (1200, 2)), rng.normal(0, 0.01, 1200)]
+    blob = rng.normal([1.5, 1.5, 1.0], 0.3, (300, 3))
+    pts = np.r_[plane, blob, [[9.0, 9.0, 9.0]]].astype(np.float32)
+    vp = np.array([1.0, -2.0, 5.0], np.float32)
+    nrm, curv = orc.grid_normals(pts, 0.25, viewpoint=vp)
+    d = pts[:, None, :] - pts[None, :, :]
+    d2 = (d * d).sum(-1, dtype=np.float32)
+    checked = 0
+    for i in range(len(pts)):
+        view = vp.astype(np.float64) - pts[i]
+        assert np.dot(nrm[i], view) >= 0.0
+        assert abs(np.linalg.norm(nrm[i]) - 1.0) < 1e-6
+        nb = pts[d2[i] <= np.float32(0.25 * 0.25)].astype(np.float64)
+        if len(nb) < 3:
+            want = np.array([0.0, 0.0, 1.0]) * (1.0 if view[2] >= 0 else -1.0)
+            assert np.allclose(nrm[i], want) and curv[i] == 0.0
+            continue
+        c = np.cov(nb.T, bias=True)
+        w, v = np.linalg.eigh(c)
+        if w[1] - w[0] < 1e-2 * w[2]:
+            continue
+        assert abs(abs(np.dot(nrm[i], v[:, 0])) - 1.0) < 1e-4, i
+        assert abs(curv[i] - w[0] / w.sum()) < 1e-4 * max(1.0, w[0] / w.sum()) + 1e-6, i
+        checked += 1
+    assert checked > 1000
