@@ -235,12 +235,13 @@ def main():
         reg.align(src.view, tgt.view)
         lib.sr_profile_enable(0)
         for name in ("knn_normals", "gicp_pass"):
-            st = (ctypes.c_uint64 * 8)()
+            st = (ctypes.c_uint64 * 10)()
             lib.sr_profile_read_stats(name.encode(), st)
             q = max(st[0], 1)
             print(f"search stats {name}: queries {st[0]} rings/q {st[1] / q:.2f} "
                   f"cells/q {st[2] / q:.2f} points/q {st[3] / q:.2f} outcomes "
-                  f"rejected/block/ring-found/ring-none {list(st[4:8])}", file=sys.stderr, flush=True)
+                  f"rejected/block/ring-found/ring-none {list(st[4:8])} max cells {st[8]} "
+                  f"max points {st[9]}", file=sys.stderr, flush=True)
 
     err = None
     golden = os.path.join(ROOT, "tests", "golden", "fp64_poses.json")

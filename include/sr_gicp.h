@@ -143,10 +143,11 @@ int32_t sr_profile_read(const char* name, double* total_ms, uint64_t* launches);
  * returns the number recorded (or -1 on a HIP error). */
 int64_t sr_profile_read_each(const char* name, double* out_ms, uint64_t cap);
 /* With sr_profile_enable(2), the grid searches also count their work; reads and
- * clears {queries, rings, cells visited, points tested, then queries rejected
- * without a scan, finished in the 3x3x3 block, finished in the ring search with
- * a neighbour, finished there without one} of "knn_normals" or "gicp_pass". */
-int32_t sr_profile_read_stats(const char* name, uint64_t* out8);
+ * clears {queries, rings, cells (rows) visited, points tested, then queries
+ * rejected without a scan, finished in the 3x3x3 block, finished in the ring
+ * search with a neighbour, finished there without one, then the largest cell
+ * and point counts of a single query} of "knn_normals" or "gicp_pass". */
+int32_t sr_profile_read_stats(const char* name, uint64_t* out10);
 
 /* ---- GICP (drop-in for GicpRegistration::align_with_diagnostics) ------- */
 

@@ -917,7 +917,7 @@ const char* sr_last_error(void) { return t_last_error.c_str(); }
 
 void sr_profile_enable(int32_t on) { profiler().on.store(on == 2 ? 2 : (on ? 1 : 0)); }
 
-int32_t sr_profile_read_stats(const char* name, uint64_t* out8) {
+int32_t sr_profile_read_stats(const char* name, uint64_t* out10) {
   return guarded([&] {
     Profiler& P = profiler();
     SR_HIP(hipDeviceSynchronize());
@@ -928,11 +928,13 @@ int32_t sr_profile_read_stats(const char* name, uint64_t* out8) {
       SR_HIP(hipMemcpy(&h, it->second, sizeof(h), hipMemcpyDeviceToHost));
       SR_HIP(hipMemset(it->second, 0, sizeof(h)));
     }
-    out8[0] = h.queries;
-    out8[1] = h.rings;
-    out8[2] = h.cells;
-    out8[3] = h.points;
-    for (int o = 0; o < 4; ++o) out8[4 + o] = h.outcome[o];
+    out10[0] = h.queries;
+    out10[1] = h.rings;
+    out10[2] = h.cells;
+    out10[3] = h.points;
+    for (int o = 0; o < 4; ++o) out10[4 + o] = h.outcome[o];
+    out10[8] = h.max_cells;
+    out10[9] = h.max_points;
   });
 }
 

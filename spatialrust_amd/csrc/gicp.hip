@@ -219,7 +219,7 @@ __device__ __forceinline__ uint32_t nearest_within(const GridDesc& g, const uint
                                                    float pz, float dmax, float dmax2, uint32_t prev,
                                                    float& best_d, SearchStats* stats) {
   best_d = INFINITY;
-  uint32_t best_pos = kNone, best_id = kNone;
+  uint32_t best_pos = kNone;
   uint32_t n_rings = 0, n_cells = 0, n_pts = 0;
   // Quick reject: query farther than dmax from the grid's box.
   {
@@ -247,12 +247,15 @@ __device__ __forceinline__ uint32_t nearest_within(const GridDesc& g, const uint
   }
   // thr = min(best so far, dmax^2): cells and rows beyond it are never touched.
   float thr = dmax2;
+  // (d2, original index) packed as one u64 key: d2 >= 0, so its bits order like
+  // the float, and the index breaks ties. The initial key (dmax2, ~0) admits
+  // exactly d2 <= dmax2 (NaN distances compare above it).
+  uint64_t best_key = (uint64_t(__float_as_uint(dmax2)) << 32) | 0xFFFFFFFFull;
   auto consider = [&](uint32_t s, const float4& q) {
     const float d2 = dist2(px, py, pz, q.x, q.y, q.z);
-    const uint32_t qi = __float_as_uint(q.w);
-    if (d2 <= dmax2 && (d2 < best_d || (d2 == best_d && qi < best_id))) {
-      best_d = d2;
-      best_id = qi;
+    const uint64_t key = (uint64_t(__float_as_uint(d2)) << 32) | __float_as_uint(q.w);
+    if (key < best_key) {
+      best_key = key;
       best_pos = s;
       thr = d2;
     }
@@ -269,6 +272,7 @@ __device__ __forceinline__ uint32_t nearest_within(const GridDesc& g, const uint
     const float cl = ring_clearance(g, px, py, pz, hx, hy, hz, 1);
     if (cl > 0.0f && thr < cl * cl) {
       stats_add(stats, n_rings, n_cells, n_pts, kBlockDone);
+      if (best_pos != kNone) best_d = __uint_as_float(uint32_t(best_key >> 32));
       return best_pos;
     }
   }
@@ -281,6 +285,7 @@ __device__ __forceinline__ uint32_t nearest_within(const GridDesc& g, const uint
               consider(s, q);
             });
   stats_add(stats, n_rings, n_cells, n_pts, best_pos == kNone ? kRingNone : kRingFound);
+  if (best_pos != kNone) best_d = __uint_as_float(uint32_t(best_key >> 32));
   return best_pos;
 }
 

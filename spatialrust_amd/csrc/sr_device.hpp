@@ -18,6 +18,7 @@ enum { kRejected = 0, kBlockDone = 1, kRingFound = 2, kRingNone = 3 };
 struct SearchStats {
   unsigned long long queries, rings, cells, points;
   unsigned long long outcome[4];
+  unsigned long long max_cells, max_points;  // largest single-query counts
 };
 
 __device__ __forceinline__ void stats_add(SearchStats* st, uint32_t rings, uint32_t cells,
@@ -28,6 +29,8 @@ __device__ __forceinline__ void stats_add(SearchStats* st, uint32_t rings, uint3
   atomicAdd(&st->cells, (unsigned long long)cells);
   atomicAdd(&st->points, (unsigned long long)points);
   atomicAdd(&st->outcome[outcome], 1ull);
+  atomicMax(&st->max_cells, (unsigned long long)cells);
+  atomicMax(&st->max_points, (unsigned long long)points);
 }
 
 // Uniform grid description. Cell (cx,cy,cz) has linear id (cz*dy + cy)*dx + cx,
