@@ -76,10 +76,14 @@ __device__ int knn_search(const GridDesc& g, const uint32_t* __restrict__ cs,
                      max(max(-hz, hz - (g.dz - 1)), 0));
   float worst_d = INFINITY;
   uint32_t worst_i = kNone;
+  // admission test on the packed (d2, index) key: one u64 compare (the list
+  // itself keeps d and id separately, which needs fewer registers)
+  uint64_t worst_key = ~0ull;
   auto consider = [&](uint32_t s, const float4& q) {
     const float d2 = dist2(q.x, q.y, q.z, px, py, pz);
     const uint32_t qi = __float_as_uint(q.w);
-    if (d2 < worst_d || (d2 == worst_d && qi < worst_i)) {
+    const uint64_t key = (uint64_t(__float_as_uint(d2)) << 32) | qi;
+    if (key < worst_key && d2 < INFINITY) {  // non-finite distances never enter
       L.insert(d2, qi, s);
 #pragma unroll
       for (int j = 0; j < K; ++j) {
@@ -88,6 +92,7 @@ __device__ int knn_search(const GridDesc& g, const uint32_t* __restrict__ cs,
           worst_i = L.id[j];
         }
       }
+      worst_key = worst_i == kNone ? ~0ull : (uint64_t(__float_as_uint(worst_d)) << 32) | worst_i;
     }
   };
   int rfirst = r0;
