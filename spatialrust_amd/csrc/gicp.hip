@@ -213,6 +213,7 @@ __device__ __forceinline__ void finalize(InstState* st, const InstConfig& cfg, c
 // Returns the grid position or kNone. `prev` (grid position or kNone) is the
 // previous pass's correspondence of this source point: a real candidate, so it
 // seeds the best and the search radius without changing the answer.
+template <bool kStats>
 __device__ __forceinline__ uint32_t nearest_within(const GridDesc& g, const uint32_t* __restrict__ cs,
                                                    const uint8_t* __restrict__ empty,
                                                    const float4* __restrict__ tp, float px, float py,
@@ -228,7 +229,7 @@ __device__ __forceinline__ uint32_t nearest_within(const GridDesc& g, const uint
     float bz = fmaxf(fmaxf(g.oz - pz, pz - (g.oz + g.h * g.dz)), 0.0f);
     float lb = bx * bx + by * by + bz * bz;
     if (lb > dmax2 + 2.0f * dmax * g.margin + g.margin * g.margin) {
-      stats_add(stats, 0, 0, 0, kRejected);
+      if constexpr (kStats) stats_add(stats, 0, 0, 0, kRejected);
       return kNone;
     }
   }
@@ -241,7 +242,7 @@ __device__ __forceinline__ uint32_t nearest_within(const GridDesc& g, const uint
     // outside the box of rings 0..e-1: its clearance bounds the distance.
     e = empty[(uint32_t(hz) * uint32_t(g.dy) + uint32_t(hy)) * uint32_t(g.dx) + uint32_t(hx)];
     if (e > 0 && ring_clearance(g, px, py, pz, hx, hy, hz, e - 1) > dmax) {
-      stats_add(stats, 0, 0, 0, kRejected);
+      if constexpr (kStats) stats_add(stats, 0, 0, 0, kRejected);
       return kNone;
     }
   }
@@ -264,27 +265,28 @@ __device__ __forceinline__ uint32_t nearest_within(const GridDesc& g, const uint
   if (inside && e <= 1) {
     // Rings 0 and 1 as nine contiguous point ranges, cut to the sphere of the
     // seeded radius; most queries of a converging registration end here.
-    ++n_rings;
+    if constexpr (kStats) ++n_rings;
     scan_block27_within(g, cs, tp, px, py, pz, hx, hy, hz, thr, [&](uint32_t s, const float4& q) {
-      ++n_pts;
+      if constexpr (kStats) ++n_pts;
       consider(s, q);
     });
     const float cl = ring_clearance(g, px, py, pz, hx, hy, hz, 1);
     if (cl > 0.0f && thr < cl * cl) {
-      stats_add(stats, n_rings, n_cells, n_pts, kBlockDone);
+      if constexpr (kStats) stats_add(stats, n_rings, n_cells, n_pts, kBlockDone);
       if (best_pos != kNone) best_d = __uint_as_float(uint32_t(best_key >> 32));
       return best_pos;
     }
   }
   // Not certified by the block: every row of the sphere of the current radius,
   // each once (the block's cells excluded when it was scanned).
-  ++n_rings;
+  if constexpr (kStats) ++n_rings;
   scan_ball(g, cs, tp, px, py, pz, hx, hy, hz, thr, inside && e <= 1, n_cells,
             [&](uint32_t s, const float4& q) {
-              ++n_pts;
+              if constexpr (kStats) ++n_pts;
               consider(s, q);
             });
-  stats_add(stats, n_rings, n_cells, n_pts, best_pos == kNone ? kRingNone : kRingFound);
+  if constexpr (kStats)
+    stats_add(stats, n_rings, n_cells, n_pts, best_pos == kNone ? kRingNone : kRingFound);
   if (best_pos != kNone) best_d = __uint_as_float(uint32_t(best_key >> 32));
   return best_pos;
 }
@@ -340,6 +342,7 @@ __device__ __forceinline__ void combined_inverse(float ux, float uy, float uz, f
   M[5] = ia + k * wz * wz;
 }
 
+template <bool kStats>
 __global__ void __launch_bounds__(kBlock) k_gicp_pass(const IterInstance* __restrict__ insts,
                                                       const uint32_t* __restrict__ block_inst,
                                                       SearchStats* __restrict__ stats) {
@@ -393,7 +396,7 @@ __global__ void __launch_bounds__(kBlock) k_gicp_pass(const IterInstance* __rest
     const float py = fmaf(R[3], p.x, fmaf(R[4], p.y, fmaf(R[5], p.z, T[1] + ly)));
     const float pz = fmaf(R[6], p.x, fmaf(R[7], p.y, fmaf(R[8], p.z, T[2] + lz)));
     float d2;
-    const uint32_t j = nearest_within(g, I.cell_start, I.empty_radius, I.tgt, px, py, pz, cfg.dmax,
+    const uint32_t j = nearest_within<kStats>(g, I.cell_start, I.empty_radius, I.tgt, px, py, pz, cfg.dmax,
                                       cfg.dmax2, I.match[i], d2, stats);
     I.match[i] = j;
     if (j == kNone) continue;
@@ -566,8 +569,13 @@ void gicp_pass(const IterInstance* insts, const uint32_t* block_inst, uint32_t t
   if (total_blocks == 0) return;
   {
     ProfScope prof("gicp_pass", s);
-    hipLaunchKernelGGL(k_gicp_pass, dim3(total_blocks), dim3(kBlock), 0, s, insts, block_inst,
-                       prof_stats("gicp_pass"));
+    SearchStats* st = prof_stats("gicp_pass");
+    if (st)
+      hipLaunchKernelGGL(k_gicp_pass<true>, dim3(total_blocks), dim3(kBlock), 0, s, insts,
+                         block_inst, st);
+    else
+      hipLaunchKernelGGL(k_gicp_pass<false>, dim3(total_blocks), dim3(kBlock), 0, s, insts,
+                         block_inst, st);
   }
   if (!finalize) return;
   ProfScope prof("gicp_finalize", s);
