@@ -264,7 +264,7 @@ static float knn_occupancy(uint32_t k) {
 
 // Voxel filter (optional) + grid + plane normals of one cloud.
 static void prepare_cloud(const Cloud& c, const sr_gicp_config& cfg, Prepared& out, Receipt& rc,
-                          hipStream_t s, bool is_target);
+                          hipStream_t s, bool is_target, float leaf_hint = 0.0f);
 
 static void prepare(const sr_xyz_view& v, const sr_gicp_config& cfg, Prepared& out, Receipt& rc,
                     hipStream_t s, bool is_target) {
@@ -278,17 +278,17 @@ static void prepare(const sr_xyz_view& v, const sr_gicp_config& cfg, Prepared& o
     cloud = &vox.centroids;
     rc.stages |= SR_STAGE_VOXEL;
   }
-  prepare_cloud(*cloud, cfg, out, rc, s, is_target);
+  prepare_cloud(*cloud, cfg, out, rc, s, is_target, cfg.voxel_leaf > 0.0f ? cfg.voxel_leaf : 0.0f);
 }
 
 // Grid and covariances of a packed device cloud (no voxel filter).
 static void prepare_cloud(const Cloud& c, const sr_gicp_config& cfg, Prepared& out, Receipt& rc,
-                          hipStream_t s, bool is_target) {
+                          hipStream_t s, bool is_target, float leaf_hint) {
   const Cloud* cloud = &c;
   out.n = cloud->n;
   uint32_t hb[6];
   host_bounds(*cloud, hb, s);
-  const float h = choose_cell_edge(*cloud, hb, knn_occupancy(cfg.k_neighbors));
+  const float h = choose_cell_edge(*cloud, hb, knn_occupancy(cfg.k_neighbors), leaf_hint);
   build_grid(*cloud, hb, h, out.grid, s);
   if (is_target) build_empty_radius(out.grid, s);
   rc.stages |= SR_STAGE_GRID;
@@ -1519,7 +1519,7 @@ int32_t sr_odometry_push(sr_odometry* odo, const sr_xyz_view* scan, sr_result* o
       // constant velocity: last pose composed with the last scan-to-scan motion
       const Pose64 guess = compose64(odo->pose, compose64(inverse64(odo->prev), odo->pose));
       Prepared src;
-      prepare_cloud(vox.centroids, odo->gicp, src, rc, s, false);
+      prepare_cloud(vox.centroids, odo->gicp, src, rc, s, false, odo->cfg.voxel_leaf);
       double pose7[7] = {guess.q[0], guess.q[1], guess.q[2], guess.q[3],
                          guess.t[0], guess.t[1], guess.t[2]};
       Batch B;
@@ -1544,7 +1544,7 @@ int32_t sr_odometry_push(sr_odometry* odo, const sr_xyz_view* scan, sr_result* o
     for (int a = 0; a < 3; ++a) mp.t[a] = pose.t[a];
     odo->map.insert(vox.centroids.pts.p, vox.cells, mp, s);
     Prepared fresh;
-    prepare_cloud(odo->map.list, odo->gicp, fresh, rc, s, true);
+    prepare_cloud(odo->map.list, odo->gicp, fresh, rc, s, true, odo->cfg.voxel_leaf);
     SR_HIP(hipStreamSynchronize(s));
     odo->map_prep = std::move(fresh);
     odo->prev = odo->scans > 0 ? odo->pose : pose;

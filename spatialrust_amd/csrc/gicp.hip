@@ -517,6 +517,14 @@ __global__ void __launch_bounds__(kBlock) k_gicp_finalize(const IterInstance* __
   InstState* st = I.state;
   if (st->phase == 2) return;
   const InstConfig cfg = I.cfg;
+  // The state is staged in LDS: the serial bookkeeping below reads and writes
+  // it field by field, which from global memory would chain many round trips.
+  static_assert(sizeof(InstState) % 8 == 0 && sizeof(InstState) / 8 <= kBlock, "state staging");
+  constexpr int kStateWords = int(sizeof(InstState) / 8);
+  __shared__ InstState sst;
+  if (threadIdx.x < kStateWords)
+    reinterpret_cast<unsigned long long*>(&sst)[threadIdx.x] =
+        reinterpret_cast<const unsigned long long*>(st)[threadIdx.x];
   __shared__ double wsum[kBlock / kWave][kTerms];
   __shared__ unsigned long long wcnt[kBlock / kWave];
   __shared__ double dtot[kTerms];
@@ -560,8 +568,12 @@ __global__ void __launch_bounds__(kBlock) k_gicp_finalize(const IterInstance* __
   if (threadIdx.x == 0) {
     unsigned long long cnt = 0;
     for (int w = 0; w < kBlock / kWave; ++w) cnt += wcnt[w];
-    finalize(st, cfg, dtot, dtot + 21, dtot[27], cnt, work);
+    finalize(&sst, cfg, dtot, dtot + 21, dtot[27], cnt, work);
   }
+  __syncthreads();
+  if (threadIdx.x < kStateWords)
+    reinterpret_cast<unsigned long long*>(st)[threadIdx.x] =
+        reinterpret_cast<const unsigned long long*>(&sst)[threadIdx.x];
 }
 
 void gicp_pass(const IterInstance* insts, const uint32_t* block_inst, uint32_t total_blocks,

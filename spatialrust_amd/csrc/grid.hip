@@ -226,7 +226,7 @@ void gather_to_grid_order(const Grid& g, const float4* orig, float4* out, hipStr
 // Picks the cell edge from the measured mean occupancy of non-empty cells:
 // surface clouds fill cells in proportion to h^2, so one correction from a
 // first guess lands close to the requested occupancy.
-float choose_cell_edge(const Cloud& c, const uint32_t* hb, float occupancy) {
+float choose_cell_edge(const Cloud& c, const uint32_t* hb, float occupancy, float leaf_hint) {
   hipStream_t s = thread_stream();
   float span[3];
   for (int a = 0; a < 3; ++a) span[a] = std::max(ord2f(hb[3 + a]) - ord2f(hb[a]), 0.0f);
@@ -240,6 +240,11 @@ float choose_cell_edge(const Cloud& c, const uint32_t* hb, float occupancy) {
     s2 = std::max(a[1] * a[2], smax * smax * 1e-6f);
   }
   float h = std::sqrt(s2 * occupancy / float(c.n));
+  if (leaf_hint > 0.0f) {
+    // One point per voxel of edge leaf_hint on surfaces: a cell of edge h
+    // holds about (h / leaf)^2 of them.
+    h = leaf_hint * std::sqrt(occupancy);
+  }
   DevBuf<uint32_t> nonempty(1);
   for (int pass = 0; pass < 2; ++pass) {
     // keep the dense grid under the cell cap

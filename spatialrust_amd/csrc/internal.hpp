@@ -148,8 +148,11 @@ constexpr int kEmptyCap = 14;
 // Fills g.empty_radius (Chebyshev distance transform of cell occupancy, capped).
 void build_empty_radius(Grid& g, hipStream_t s);
 
-// Chooses a cell edge so that occupied cells hold about `occupancy` points.
-float choose_cell_edge(const Cloud& c, const uint32_t* host_bounds, float occupancy);
+// Chooses a cell edge so that occupied cells hold about `occupancy` points
+// (measured, starting from a guess; leaf_hint > 0 says the cloud holds one
+// point per voxel of that edge, which gives a closer first guess).
+float choose_cell_edge(const Cloud& c, const uint32_t* host_bounds, float occupancy,
+                       float leaf_hint = 0.0f);
 // Builds a dense counting-sort grid with edge h over a device cloud (points in
 // each cell sorted by ascending original index, like build_grid).
 void build_grid(const Cloud& c, const uint32_t* host_bounds, float h, Grid& g, hipStream_t s);
