@@ -209,19 +209,53 @@ __device__ __forceinline__ void finalize(InstState* st, const InstConfig& cfg, c
   }
 }
 
-// Nearest target point of p within dmax (exact; ties to the lower index).
-// Returns the grid position or kNone. `prev` (grid position or kNone) is the
-// previous pass's correspondence of this source point: a real candidate, so it
-// seeds the best and the search radius without changing the answer.
+// ---- nearest target point within dmax (exact; ties to the lower index) ----
+//
+// Phase 1 runs per lane: the box and empty-space rejects, the previous pass's
+// correspondence as a seed (a real candidate, so it tightens the radius without
+// changing the answer), and the 3x3x3 block cut to the seeded sphere. A query
+// the block's clearance certifies is done. The others (in early passes about
+// half) would each walk dozens of grid rows while the rest of the wave idles,
+// so phase 2 shares them out: each wave's uncertified queries are taken by
+// groups of kNNGroup lanes that split the rows of each ring of the query's
+// ball between them and agree on the radius after every ring.
+
+struct NNQuery {
+  uint64_t best_key;  // (d2 bits << 32 | original index); starts at (dmax2, ~0)
+  uint32_t best_pos;  // grid position of the best, kNone if none
+  float thr;          // search radius^2 = min(dmax2, best d2)
+  int hx, hy, hz;     // query cell
+  bool hard;          // phase 2 needed
+  bool skip_block;    // the 3x3x3 block was scanned in phase 1
+  uint32_t rows, pts; // work counters (search-statistics mode)
+};
+
+// (d2, original index) packed as one u64 key: d2 >= 0, so its bits order like
+// the float, and the index breaks ties; the initial key (dmax2, ~0) admits
+// exactly d2 <= dmax2 (NaN distances compare above it).
+__device__ __forceinline__ void nn_consider(NNQuery& Q, float px, float py, float pz, uint32_t s,
+                                            const float4& q) {
+  const float d2 = dist2(px, py, pz, q.x, q.y, q.z);
+  const uint64_t key = (uint64_t(__float_as_uint(d2)) << 32) | __float_as_uint(q.w);
+  if (key < Q.best_key) {
+    Q.best_key = key;
+    Q.best_pos = s;
+    Q.thr = d2;
+  }
+}
+
 template <bool kStats>
-__device__ __forceinline__ uint32_t nearest_within(const GridDesc& g, const uint32_t* __restrict__ cs,
-                                                   const uint8_t* __restrict__ empty,
-                                                   const float4* __restrict__ tp, float px, float py,
-                                                   float pz, float dmax, float dmax2, uint32_t prev,
-                                                   float& best_d, SearchStats* stats) {
-  best_d = INFINITY;
-  uint32_t best_pos = kNone;
-  uint32_t n_rings = 0, n_cells = 0, n_pts = 0;
+__device__ __forceinline__ void nn_phase1(const GridDesc& g, const uint32_t* __restrict__ cs,
+                                          const uint8_t* __restrict__ empty,
+                                          const float4* __restrict__ tp, float px, float py,
+                                          float pz, float dmax, float dmax2, uint32_t prev,
+                                          NNQuery& Q, SearchStats* stats) {
+  Q.best_key = (uint64_t(__float_as_uint(dmax2)) << 32) | 0xFFFFFFFFull;
+  Q.best_pos = kNone;
+  Q.thr = dmax2;
+  Q.hard = false;
+  Q.skip_block = false;
+  Q.rows = Q.pts = 0;
   // Quick reject: query farther than dmax from the grid's box.
   {
     float bx = fmaxf(fmaxf(g.ox - px, px - (g.ox + g.h * g.dx)), 0.0f);
@@ -230,11 +264,13 @@ __device__ __forceinline__ uint32_t nearest_within(const GridDesc& g, const uint
     float lb = bx * bx + by * by + bz * bz;
     if (lb > dmax2 + 2.0f * dmax * g.margin + g.margin * g.margin) {
       if constexpr (kStats) stats_add(stats, 0, 0, 0, kRejected);
-      return kNone;
+      return;
     }
   }
-  const int hx = cell_coord_free(px, g.ox, g.inv_h), hy = cell_coord_free(py, g.oy, g.inv_h),
-            hz = cell_coord_free(pz, g.oz, g.inv_h);
+  Q.hx = cell_coord_free(px, g.ox, g.inv_h);
+  Q.hy = cell_coord_free(py, g.oy, g.inv_h);
+  Q.hz = cell_coord_free(pz, g.oz, g.inv_h);
+  const int hx = Q.hx, hy = Q.hy, hz = Q.hz;
   const bool inside = hx >= 0 && hx < g.dx && hy >= 0 && hy < g.dy && hz >= 0 && hz < g.dz;
   int e = 0;
   if (empty && inside) {
@@ -243,52 +279,188 @@ __device__ __forceinline__ uint32_t nearest_within(const GridDesc& g, const uint
     e = empty[(uint32_t(hz) * uint32_t(g.dy) + uint32_t(hy)) * uint32_t(g.dx) + uint32_t(hx)];
     if (e > 0 && ring_clearance(g, px, py, pz, hx, hy, hz, e - 1) > dmax) {
       if constexpr (kStats) stats_add(stats, 0, 0, 0, kRejected);
-      return kNone;
+      return;
     }
   }
-  // thr = min(best so far, dmax^2): cells and rows beyond it are never touched.
-  float thr = dmax2;
-  // (d2, original index) packed as one u64 key: d2 >= 0, so its bits order like
-  // the float, and the index breaks ties. The initial key (dmax2, ~0) admits
-  // exactly d2 <= dmax2 (NaN distances compare above it).
-  uint64_t best_key = (uint64_t(__float_as_uint(dmax2)) << 32) | 0xFFFFFFFFull;
-  auto consider = [&](uint32_t s, const float4& q) {
-    const float d2 = dist2(px, py, pz, q.x, q.y, q.z);
-    const uint64_t key = (uint64_t(__float_as_uint(d2)) << 32) | __float_as_uint(q.w);
-    if (key < best_key) {
-      best_key = key;
-      best_pos = s;
-      thr = d2;
-    }
-  };
-  if (prev != kNone) consider(prev, tp[prev]);
+  if (prev != kNone) nn_consider(Q, px, py, pz, prev, tp[prev]);
   if (inside && e <= 1) {
     // Rings 0 and 1 as nine contiguous point ranges, cut to the sphere of the
     // seeded radius; most queries of a converging registration end here.
-    if constexpr (kStats) ++n_rings;
-    scan_block27_within(g, cs, tp, px, py, pz, hx, hy, hz, thr, [&](uint32_t s, const float4& q) {
-      if constexpr (kStats) ++n_pts;
-      consider(s, q);
+    scan_block27_within(g, cs, tp, px, py, pz, hx, hy, hz, Q.thr, [&](uint32_t s, const float4& q) {
+      if constexpr (kStats) ++Q.pts;
+      nn_consider(Q, px, py, pz, s, q);
     });
+    Q.skip_block = true;
     const float cl = ring_clearance(g, px, py, pz, hx, hy, hz, 1);
-    if (cl > 0.0f && thr < cl * cl) {
-      if constexpr (kStats) stats_add(stats, n_rings, n_cells, n_pts, kBlockDone);
-      if (best_pos != kNone) best_d = __uint_as_float(uint32_t(best_key >> 32));
-      return best_pos;
+    if (cl > 0.0f && Q.thr < cl * cl) {
+      if constexpr (kStats) stats_add(stats, 1, 0, Q.pts, kBlockDone);
+      return;
     }
   }
-  // Not certified by the block: every row of the sphere of the current radius,
-  // each once (the block's cells excluded when it was scanned).
-  if constexpr (kStats) ++n_rings;
-  scan_ball(g, cs, tp, px, py, pz, hx, hy, hz, thr, inside && e <= 1, n_cells,
-            [&](uint32_t s, const float4& q) {
-              if constexpr (kStats) ++n_pts;
-              consider(s, q);
-            });
-  if constexpr (kStats)
-    stats_add(stats, n_rings, n_cells, n_pts, best_pos == kNone ? kRingNone : kRingFound);
-  if (best_pos != kNone) best_d = __uint_as_float(uint32_t(best_key >> 32));
-  return best_pos;
+  Q.hard = true;
+}
+
+#ifndef SR_NN_GROUP
+#define SR_NN_GROUP 4
+#endif
+constexpr int kNNGroup = SR_NN_GROUP;  // lanes per query in phase 2 (a power of two <= 64)
+static_assert(kNNGroup >= 1 && kNNGroup <= 64 && (kNNGroup & (kNNGroup - 1)) == 0, "group size");
+#ifndef SR_ROW_BATCH
+#define SR_ROW_BATCH 2
+#endif
+constexpr int kRowBatch = SR_ROW_BATCH;  // rows per bound-loading batch in phase 2
+
+// Row t of Chebyshev ring r (r >= 1) of (y,z) rows around the query's row:
+// the perimeter of the (2r+1)^2 square, each row once.
+__device__ __forceinline__ void ring_row(int r, int t, int& dy, int& dz) {
+  if (r == 0) {
+    dy = dz = 0;
+    return;
+  }
+  const int side = t / (2 * r), o = t - side * (2 * r);
+  if (side == 0) {
+    dz = -r;
+    dy = -r + o;
+  } else if (side == 1) {
+    dy = r;
+    dz = -r + o;
+  } else if (side == 2) {
+    dz = r;
+    dy = r - o;
+  } else {
+    dy = -r;
+    dz = r - o;
+  }
+}
+
+// Phase 2 for the calling wave; every lane of the wave must call it.
+template <bool kStats>
+__device__ __forceinline__ void nn_phase2(const GridDesc& g, const uint32_t* __restrict__ cs,
+                                          const float4* __restrict__ tp, float px, float py,
+                                          float pz, NNQuery& Q, SearchStats* stats) {
+  const int lane = int(threadIdx.x & (kWave - 1));
+  const unsigned long long hard = __ballot(Q.hard);
+  if (hard == 0) return;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const int H = __popcll(hard);
+  // Source lane of the k-th hard query: hard lanes send their lane id to slot
+  // (rank), the others to slots after H, so the permutation has no collisions.
+  const int slot = Q.hard ? __popcll(hard & below) : H + __popcll(~hard & below);
+  const int src_of = __builtin_amdgcn_ds_permute(slot << 2, lane);
+  constexpr int G = kWave / kNNGroup;
+  const int sub = lane % kNNGroup;
+  for (int base = 0; base < H; base += G) {
+    const int k = base + lane / kNNGroup;
+    const bool active = k < H;
+    const int src = __shfl(src_of, active ? k : 0);
+    const float qx = __shfl(px, src), qy = __shfl(py, src), qz = __shfl(pz, src);
+    // rings are centred on the query's row clamped to one row outside the grid,
+    // so a query beside the grid still reaches every grid row within
+    // max(dy, dz) rings (the gap and clearance tests use the true position)
+    const int hx = __shfl(Q.hx, src);
+    const int hy = min(max(__shfl(Q.hy, src), -1), g.dy), hz = min(max(__shfl(Q.hz, src), -1), g.dz);
+    const bool skip = __shfl(int(Q.skip_block), src) != 0;
+    NNQuery W;
+    W.best_key = (uint64_t(__shfl(uint32_t(Q.best_key >> 32), src)) << 32) |
+                 __shfl(uint32_t(Q.best_key), src);
+    W.best_pos = __shfl(Q.best_pos, src);
+    W.thr = __shfl(Q.thr, src);
+    W.rows = W.pts = 0;
+    if (active) {
+      const int rmax = max(max(hy, g.dy - 1 - hy), max(hz, g.dz - 1 - hz));
+      for (int r = 0; r <= rmax; ++r) {
+        if (r > 0) {  // W.thr is the same on the group's lanes here
+          const float c = clearance_yz(g, qy, qz, hy, hz, r - 1);
+          if (c > 0.0f && c * c > W.thr) break;
+        }
+        const int nrows = r == 0 ? 1 : 8 * r;
+        // This lane's rows of the ring, kRowBatch at a time: the bounds of a
+        // batch's ranges are loaded together and then its points streamed, so
+        // a batch costs one bound round trip instead of one per row. A row
+        // whose (y,z) gap exceeds the radius contributes an empty range.
+        for (int t0 = sub; t0 < nrows; t0 += kNNGroup * kRowBatch) {
+          uint32_t ia[2 * kRowBatch], ib[2 * kRowBatch];
+#pragma unroll
+          for (int m = 0; m < kRowBatch; ++m) {
+            ia[2 * m] = ib[2 * m] = ia[2 * m + 1] = ib[2 * m + 1] = 0;
+            const int t = t0 + m * kNNGroup;
+            if (t >= nrows) continue;
+            int dy, dz;
+            ring_row(r, t, dy, dz);
+            const int cz = hz + dz, cy = hy + dy;
+            if (cz < 0 || cz >= g.dz || cy < 0 || cy >= g.dy) continue;
+            const float gz = axis_gap(qz, g.oz, g.h, g.margin, cz);
+            const float gy = axis_gap(qy, g.oy, g.h, g.margin, cy);
+            const float gyz = gz * gz + gy * gy;
+            if (gyz > W.thr) continue;
+            const float rx = __builtin_amdgcn_sqrtf(W.thr - gyz) + g.margin;
+            const int x0 = max(cell_coord_free(qx - rx, g.ox, g.inv_h) - 1, 0);
+            const int x1 = min(cell_coord_free(qx + rx, g.ox, g.inv_h) + 1, g.dx - 1);
+            if (x0 > x1) continue;
+            const uint32_t cbase = (uint32_t(cz) * uint32_t(g.dy) + uint32_t(cy)) * uint32_t(g.dx);
+            if constexpr (kStats) ++W.rows;
+            if (skip && r <= 1) {
+              // left and right of the block's three cells
+              if (x0 <= hx - 2) {
+                ia[2 * m] = cbase + uint32_t(x0);
+                ib[2 * m] = cbase + uint32_t(min(x1, hx - 2)) + 1;
+              }
+              if (x1 >= hx + 2) {
+                ia[2 * m + 1] = cbase + uint32_t(max(x0, hx + 2));
+                ib[2 * m + 1] = cbase + uint32_t(x1) + 1;
+              }
+            } else {
+              ia[2 * m] = cbase + uint32_t(x0);
+              ib[2 * m] = cbase + uint32_t(x1) + 1;
+            }
+          }
+          uint32_t lo[2 * kRowBatch], hi[2 * kRowBatch];
+#pragma unroll
+          for (int m = 0; m < 2 * kRowBatch; ++m) {
+            lo[m] = cs[ia[m]];
+            hi[m] = cs[ib[m]];
+          }
+          stream_ranges(lo, hi, tp, [&](uint32_t s, const float4& q) {
+            if constexpr (kStats) ++W.pts;
+            nn_consider(W, qx, qy, qz, s, q);
+          });
+        }
+        // the group agrees on the smallest radius before the next ring
+#pragma unroll
+        for (int off = 1; off < kNNGroup; off <<= 1) W.thr = fminf(W.thr, __shfl_xor(W.thr, off));
+      }
+    }
+    // group minimum of (key, position); the group's lanes are all active or all idle
+#pragma unroll
+    for (int off = 1; off < kNNGroup; off <<= 1) {
+      const uint32_t ohi = __shfl_xor(uint32_t(W.best_key >> 32), off);
+      const uint32_t olo = __shfl_xor(uint32_t(W.best_key), off);
+      const uint32_t opos = __shfl_xor(W.best_pos, off);
+      const uint64_t ok = (uint64_t(ohi) << 32) | olo;
+      if (ok < W.best_key) {
+        W.best_key = ok;
+        W.best_pos = opos;
+      }
+      if constexpr (kStats) {
+        W.rows += __shfl_xor(W.rows, off);
+        W.pts += __shfl_xor(W.pts, off);
+      }
+    }
+    // hand each result back to its query's lane (the group's first lane holds it)
+    const int my = Q.hard ? __popcll(hard & below) : -1;
+    const bool mine = my >= base && my < base + G;
+    const int leader = mine ? (my - base) * kNNGroup : lane;
+    const uint32_t rhi = __shfl(uint32_t(W.best_key >> 32), leader);
+    const uint32_t rlo = __shfl(uint32_t(W.best_key), leader);
+    const uint32_t rpos = __shfl(W.best_pos, leader);
+    const uint32_t rrows = __shfl(W.rows, leader), rpts = __shfl(W.pts, leader);
+    if (mine) {
+      Q.best_key = (uint64_t(rhi) << 32) | rlo;
+      Q.best_pos = rpos;
+      if constexpr (kStats)
+        stats_add(stats, 2, rrows, Q.pts + rpts, rpos == kNone ? kRingNone : kRingFound);
+    }
+  }
 }
 
 struct PointTerms {
@@ -382,9 +554,11 @@ __global__ void __launch_bounds__(kBlock) k_gicp_pass(const IterInstance* __rest
 
   const uint32_t ppt = I.ppt;
   for (uint32_t k = 0; k < ppt; ++k) {
+    // Every lane stays in the loop (the search's phase 2 is wave-collective);
+    // lanes past the end compute on a clamped point and skip the results.
     const uint32_t i = (lb * ppt + k) * kBlock + threadIdx.x;
-    if (i >= I.n_src) break;
-    const float4 p = I.src[i];
+    const bool valid = i < I.n_src;
+    const float4 p = I.src[valid ? i : 0u];
     // p' = R p + t (Isometry3::transform_point, transform.rs:126-128) with the
     // FP64 pose split into f32 high and low parts: the low part removes the
     // systematic error of an f32-rounded pose, which would otherwise floor the
@@ -395,11 +569,16 @@ __global__ void __launch_bounds__(kBlock) k_gicp_pass(const IterInstance* __rest
     const float px = fmaf(R[0], p.x, fmaf(R[1], p.y, fmaf(R[2], p.z, T[0] + lx)));
     const float py = fmaf(R[3], p.x, fmaf(R[4], p.y, fmaf(R[5], p.z, T[1] + ly)));
     const float pz = fmaf(R[6], p.x, fmaf(R[7], p.y, fmaf(R[8], p.z, T[2] + lz)));
-    float d2;
-    const uint32_t j = nearest_within<kStats>(g, I.cell_start, I.empty_radius, I.tgt, px, py, pz, cfg.dmax,
-                                      cfg.dmax2, I.match[i], d2, stats);
+    NNQuery Q;
+    nn_phase1<kStats>(g, I.cell_start, I.empty_radius, I.tgt, px, py, pz, cfg.dmax, cfg.dmax2,
+                      valid ? I.match[i] : kNone, Q, valid ? stats : nullptr);
+    if (!valid) Q.hard = false;
+    nn_phase2<kStats>(g, I.cell_start, I.tgt, px, py, pz, Q, stats);
+    if (!valid) continue;
+    const uint32_t j = Q.best_pos;
     I.match[i] = j;
     if (j == kNone) continue;
+    const float d2 = __uint_as_float(uint32_t(Q.best_key >> 32));
     const float4 q = I.tgt[j];
     const float4 nt = I.tgt_n[j];
     const float4 ns = I.src_n[i];

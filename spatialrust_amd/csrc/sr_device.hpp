@@ -244,68 +244,6 @@ __device__ __forceinline__ float clearance_yz(const GridDesc& g, float py, float
   return fminf(fminf(py - ylo, yhi - py), fminf(pz - zlo, zhi - pz)) - g.margin;
 }
 
-// Visits every point within sqrt(thr) of p (thr is re-read as `visit` tightens
-// it), plus some beyond. Grid rows (x lines) are taken in Chebyshev rings of
-// (y,z) around the query's row, nearest first, and each row once: a row whose
-// (y,z) gap exceeds thr or whose slab bitmap says empty is skipped, and a kept
-// row is one contiguous point range, its x span cut to the sphere (+1 cell of
-// slack for the approximate sqrt). The walk stops at the first ring whose
-// (y,z) clearance exceeds sqrt(thr). With skip_block the 3x3x3 block around
-// (hx,hy,hz), already scanned, is cut out of the nine central rows.
-// `rows` counts the rows scanned; W is the streaming width (stream_ranges).
-template <int W = 4, typename VisitPoint>
-__device__ __forceinline__ void scan_ball(const GridDesc& g, const uint32_t* __restrict__ cs,
-                                          const float4* __restrict__ pts, float px, float py,
-                                          float pz, int hx, int hy, int hz, const float& thr,
-                                          bool skip_block, uint32_t& rows, VisitPoint&& visit) {
-  const int rmax = max(max(hy, g.dy - 1 - hy), max(hz, g.dz - 1 - hz));
-  for (int r = 0; r <= rmax; ++r) {
-    if (r > 0) {
-      const float c = clearance_yz(g, py, pz, hy, hz, r - 1);
-      if (c > 0.0f && c * c > thr) break;
-    }
-    for (int dz = -r; dz <= r; ++dz) {
-      const int cz = hz + dz;
-      if (cz < 0 || cz >= g.dz) continue;
-      const float gz = axis_gap(pz, g.oz, g.h, g.margin, cz);
-      const float gz2 = gz * gz;
-      if (gz2 > thr) continue;
-      const uint32_t* sb = g.rows + size_t(cz) * g.wps;
-      const int step = (dz == -r || dz == r) ? 1 : 2 * r;
-      for (int dy = -r; dy <= r; dy += max(step, 1)) {
-        const int cy = hy + dy;
-        if (cy < 0 || cy >= g.dy) continue;
-        if (!((sb[cy >> 5] >> (cy & 31)) & 1u)) continue;
-        const float gy = axis_gap(py, g.oy, g.h, g.margin, cy);
-        const float gyz = gz2 + gy * gy;
-        if (gyz > thr) continue;
-        const float rx = __builtin_amdgcn_sqrtf(thr - gyz) + g.margin;
-        const int x0 = max(cell_coord_free(px - rx, g.ox, g.inv_h) - 1, 0);
-        const int x1 = min(cell_coord_free(px + rx, g.ox, g.inv_h) + 1, g.dx - 1);
-        if (x0 > x1) continue;
-        const uint32_t cbase = (uint32_t(cz) * uint32_t(g.dy) + uint32_t(cy)) * uint32_t(g.dx);
-        uint32_t lo[2] = {0, 0}, hi[2] = {0, 0};
-        if (skip_block && r <= 1) {
-          // left and right of the block's three cells
-          if (x0 <= hx - 2) {
-            lo[0] = cs[cbase + uint32_t(x0)];
-            hi[0] = cs[cbase + uint32_t(min(x1, hx - 2)) + 1];
-          }
-          if (x1 >= hx + 2) {
-            lo[1] = cs[cbase + uint32_t(max(x0, hx + 2))];
-            hi[1] = cs[cbase + uint32_t(x1) + 1];
-          }
-        } else {
-          lo[0] = cs[cbase + uint32_t(x0)];
-          hi[0] = cs[cbase + uint32_t(x1) + 1];
-        }
-        ++rows;
-        stream_ranges<W>(lo, hi, pts, visit);
-      }
-    }
-  }
-}
-
 // Visits every point filed in the 3x3x3 block of cells around (hx,hy,hz)
 // (rings 0 and 1). In grid order the three cells of a block row are adjacent,
 // so the block is nine contiguous point ranges; all eighteen range bounds are

@@ -8,7 +8,8 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libsr_gicp.so")
+# SR_GICP_LIBRARY selects another build of the library (e.g. a tuning variant).
+LIB_PATH = os.environ.get("SR_GICP_LIBRARY") or os.path.join(_HERE, "lib", "libsr_gicp.so")
 
 SR_OK = 0
 SR_ERR_INVALID_ARGUMENT = 3
