@@ -177,7 +177,20 @@ __device__ __forceinline__ void finalize(InstState* st, const InstConfig& cfg, c
     ay /= al;
     az /= al;
     double sh, ch;
-    sincos(0.5 * angle, &sh, &ch);
+    const double h = 0.5 * angle;
+    if (h < 0.0625) {
+      // Taylor series to x^17 / x^16: below 1/16 the first omitted terms are
+      // under 2^-100 relative, so this is sin / cos to FP64 rounding without
+      // the general argument reduction.
+      const double h2 = h * h;
+      sh = h * (1.0 + h2 * (-1.0 / 6 + h2 * (1.0 / 120 + h2 * (-1.0 / 5040 + h2 * (1.0 / 362880 +
+           h2 * (-1.0 / 39916800 + h2 * (1.0 / 6227020800.0 + h2 * (-1.0 / 1307674368000.0 +
+           h2 * (1.0 / 355687428096000.0)))))))));
+      ch = 1.0 + h2 * (-0.5 + h2 * (1.0 / 24 + h2 * (-1.0 / 720 + h2 * (1.0 / 40320 + h2 * (-1.0 / 3628800 +
+           h2 * (1.0 / 479001600 + h2 * (-1.0 / 87178291200.0 + h2 * (1.0 / 20922789888000.0))))))));
+    } else {
+      sincos(h, &sh, &ch);
+    }
     dq[0] = ax * sh;
     dq[1] = ay * sh;
     dq[2] = az * sh;
@@ -684,12 +697,14 @@ __global__ void __launch_bounds__(kBlock) k_gicp_pass(const IterInstance* __rest
 }
 
 // One workgroup per registration: FP64 sum of its block partials, then the
-// Gauss-Newton bookkeeping, solve and pose update on one thread. Every thread
-// takes blocks tid, tid+256, ... (many loads in flight), then a fixed
-// xor-butterfly per wave and a fixed wave order: the same order on every run,
-// whatever the launch mix.
-__global__ void __launch_bounds__(kBlock) k_gicp_finalize(const IterInstance* __restrict__ insts,
-                                                          uint32_t count) {
+// Gauss-Newton bookkeeping, solve and pose update on one thread. The partials
+// are summed in a fixed order (below), the same on every run whatever the
+// launch mix.
+constexpr int kFinBlock = 1024;
+constexpr int kFinGroups = kFinBlock / 32;
+
+__global__ void __launch_bounds__(kFinBlock) k_gicp_finalize(const IterInstance* __restrict__ insts,
+                                                             uint32_t count) {
   const uint32_t inst_id = blockIdx.x;
   if (inst_id >= count) return;
   const IterInstance& I = insts[inst_id];
@@ -698,56 +713,56 @@ __global__ void __launch_bounds__(kBlock) k_gicp_finalize(const IterInstance* __
   const InstConfig cfg = I.cfg;
   // The state is staged in LDS: the serial bookkeeping below reads and writes
   // it field by field, which from global memory would chain many round trips.
-  static_assert(sizeof(InstState) % 8 == 0 && sizeof(InstState) / 8 <= kBlock, "state staging");
+  static_assert(sizeof(InstState) % 8 == 0 && sizeof(InstState) / 8 <= kFinBlock, "state staging");
   constexpr int kStateWords = int(sizeof(InstState) / 8);
   __shared__ InstState sst;
   if (threadIdx.x < kStateWords)
     reinterpret_cast<unsigned long long*>(&sst)[threadIdx.x] =
         reinterpret_cast<const unsigned long long*>(st)[threadIdx.x];
-  __shared__ double wsum[kBlock / kWave][kTerms];
-  __shared__ unsigned long long wcnt[kBlock / kWave];
+  // A block partial is 32 u64 slots: slot t < kTerms holds term t as an f32
+  // (hi, lo) pair, slot 30 the correspondence count. Thread (group, slot)
+  // sums one slot over partials group, group + kFinGroups, ... in FP64, so a
+  // wave reads whole partials (coalesced) and no cross-lane reduction is needed;
+  // the kFinGroups sums of a slot are then added in group order.
+  __shared__ double gsum[kFinGroups][32];
+  __shared__ unsigned long long gcnt[kFinGroups];
   __shared__ double dtot[kTerms];
   __shared__ double work[64];
   const uint32_t nb = I.nblocks;
-  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-  // All 28 terms of a block partial are loaded together (14 float4 + the
-  // count), so each thread has its whole share of loads in flight at once.
-  double v[kTerms];
+  const int slot = threadIdx.x & 31, grp = threadIdx.x >> 5;
+  const unsigned long long* pp = reinterpret_cast<const unsigned long long*>(I.partials);
+  double acc = 0.0;
+  unsigned long long cnt = 0;
+  // kFinUnroll loads in flight per thread, then their adds in partial order
+  constexpr int kFinUnroll = 8;
+  for (uint32_t b0 = grp; b0 < nb; b0 += kFinGroups * kFinUnroll) {
+    unsigned long long w[kFinUnroll];
 #pragma unroll
-  for (int t = 0; t < kTerms; ++t) v[t] = 0.0;
-  unsigned long long cs = 0;
-  for (uint32_t bb = threadIdx.x; bb < nb; bb += kBlock) {
-    const float4* pp = reinterpret_cast<const float4*>(I.partials + size_t(bb) * 64);
-    float4 q[kTerms / 2];
+    for (int u = 0; u < kFinUnroll; ++u) {
+      const uint32_t bb = b0 + uint32_t(u) * kFinGroups;
+      w[u] = bb < nb ? pp[size_t(bb) * 32 + slot] : 0ull;
+    }
 #pragma unroll
-    for (int j = 0; j < kTerms / 2; ++j) q[j] = pp[j];
-    cs += reinterpret_cast<const unsigned long long*>(pp)[30];
-#pragma unroll
-    for (int j = 0; j < kTerms / 2; ++j) {
-      v[2 * j] += double(q[j].x) + double(q[j].y);
-      v[2 * j + 1] += double(q[j].z) + double(q[j].w);
+    for (int u = 0; u < kFinUnroll; ++u) {
+      if (slot < kTerms)
+        acc += double(__uint_as_float(uint32_t(w[u]))) + double(__uint_as_float(uint32_t(w[u] >> 32)));
+      else if (slot == 30)
+        cnt += w[u];
     }
   }
-#pragma unroll
-  for (int t = 0; t < kTerms; ++t)
-    for (int off = kWave / 2; off > 0; off >>= 1) v[t] += __shfl_xor(v[t], off);
-  for (int off = kWave / 2; off > 0; off >>= 1) cs += __shfl_xor(cs, off);
-  if (lane == 0) {
-#pragma unroll
-    for (int t = 0; t < kTerms; ++t) wsum[wid][t] = v[t];
-    wcnt[wid] = cs;
-  }
+  if (slot < kTerms) gsum[grp][slot] = acc;
+  if (slot == 30) gcnt[grp] = cnt;
   __syncthreads();
   if (threadIdx.x < kTerms) {
     double sacc = 0.0;
-    for (int w = 0; w < kBlock / kWave; ++w) sacc += wsum[w][threadIdx.x];
+    for (int w = 0; w < kFinGroups; ++w) sacc += gsum[w][threadIdx.x];
     dtot[threadIdx.x] = sacc;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned long long cnt = 0;
-    for (int w = 0; w < kBlock / kWave; ++w) cnt += wcnt[w];
-    finalize(&sst, cfg, dtot, dtot + 21, dtot[27], cnt, work);
+    unsigned long long c = 0;
+    for (int w = 0; w < kFinGroups; ++w) c += gcnt[w];
+    finalize(&sst, cfg, dtot, dtot + 21, dtot[27], c, work);
   }
   __syncthreads();
   if (threadIdx.x < kStateWords)
@@ -770,7 +785,7 @@ void gicp_pass(const IterInstance* insts, const uint32_t* block_inst, uint32_t t
   }
   if (!finalize) return;
   ProfScope prof("gicp_finalize", s);
-  hipLaunchKernelGGL(k_gicp_finalize, dim3(count), dim3(kBlock), 0, s, insts, count);
+  hipLaunchKernelGGL(k_gicp_finalize, dim3(count), dim3(kFinBlock), 0, s, insts, count);
 }
 
 }  // namespace sr
