@@ -176,6 +176,79 @@ __device__ __forceinline__ void smallest_eigvec(float a00, float a11, float a22,
   n[2] = z * inv;
 }
 
+// Per-lane k-th distance by a sorting network: d[0..K] holds the K+1
+// smallest squared distances seen so far, ascending; a candidate is merged by
+// min/max exchanges (two VALU per slot, no branch), so a wave pays the same
+// for every candidate whichever lanes would have inserted it.
+template <int K>
+struct KDist {
+  float d[K + 1];
+  __device__ __forceinline__ void init() {
+#pragma unroll
+    for (int j = 0; j <= K; ++j) d[j] = INFINITY;
+  }
+  __device__ __forceinline__ void add(float x) {
+    x = x < INFINITY ? x : INFINITY;  // NaN sorts as +inf
+#pragma unroll
+    for (int j = 0; j <= K; ++j) {
+      const float lo = fminf(d[j], x);
+      x = fmaxf(d[j], x);
+      d[j] = lo;
+    }
+  }
+  __device__ __forceinline__ float at(int j) const {  // d[j] for a runtime j <= K
+    float v = d[0];
+#pragma unroll
+    for (int m = 1; m <= K; ++m)
+      if (m == j) v = d[m];
+    return v;
+  }
+};
+
+// The same network over u32 indices (the tie-break among equal distances).
+template <int K>
+struct KIds {
+  uint32_t d[K + 1];
+  __device__ __forceinline__ void init() {
+#pragma unroll
+    for (int j = 0; j <= K; ++j) d[j] = 0xFFFFFFFFu;
+  }
+  __device__ __forceinline__ void add(uint32_t x) {
+#pragma unroll
+    for (int j = 0; j <= K; ++j) {
+      const uint32_t lo = min(d[j], x);
+      x = max(d[j], x);
+      d[j] = lo;
+    }
+  }
+  __device__ __forceinline__ uint32_t at(int j) const {
+    uint32_t v = d[0];
+#pragma unroll
+    for (int m = 1; m <= K; ++m)
+      if (m == j) v = d[m];
+    return v;
+  }
+};
+
+// The candidates of an inside query: the 3x3x3 block, then rings 2..rend cut
+// to radius^2 thr (a superset of the points within sqrt(thr)).
+template <typename Visit>
+__device__ __forceinline__ void knn_candidates(const GridDesc& g, const uint32_t* __restrict__ cs,
+                                               const float4* __restrict__ gp, float px, float py,
+                                               float pz, int hx, int hy, int hz, int rend,
+                                               const float& thr, Visit&& visit) {
+  scan_block27(g, cs, gp, hx, hy, hz, visit);
+  uint32_t rows = 0;
+  for (int r = 2; r <= rend; ++r) scan_ring<1>(g, cs, gp, px, py, pz, hx, hy, hz, r, thr, rows, visit);
+}
+
+// k-NN normals. The k nearest (ties to the lower index) are found in three
+// passes over the same candidates instead of keeping a sorted neighbour list:
+// (1) the k-th and (k+1)-th smallest distances by the sorting network, the
+// rings extended until the k-th (tau) is certified; (2) the mean of the points
+// closer than tau plus those at tau with the lowest indices (exactly k); (3)
+// the centred sums. Only when the k-th and (k+1)-th distances are equal does a
+// pass over the points at distance tau pick the index cut.
 template <int K>
 __global__ void __launch_bounds__(kBlock) k_knn_normals(GridDesc g, const uint32_t* __restrict__ cs,
                                                         const float4* __restrict__ gp, int k,
@@ -187,31 +260,72 @@ __global__ void __launch_bounds__(kBlock) k_knn_normals(GridDesc g, const uint32
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= g.npoints) return;
   const float4 p = gp[i];
-  KList<K> L;
-  const int found = knn_search<K>(g, cs, gp, p.x, p.y, p.z, k, L, stats);
-  float n[3] = {1.0f, 0.0f, 0.0f};  // count < 2: identity -> eigenvector e_x (gicp.rs:268-270)
-  if (found >= 2) {
-    // Mean and centred sums relative to the query point (f32, two passes).
-    float mx = 0.0f, my = 0.0f, mz = 0.0f;
-#pragma unroll
-    for (int j = 0; j < K; ++j) {
-      if (j < found) {
-        float4 q = gp[L.pos[j]];
-        mx += q.x - p.x;
-        my += q.y - p.y;
-        mz += q.z - p.z;
+  const int hx = cell_coord_free(p.x, g.ox, g.inv_h), hy = cell_coord_free(p.y, g.oy, g.inv_h),
+            hz = cell_coord_free(p.z, g.oz, g.inv_h);
+  const int rmax = max(max(max(hx, g.dx - 1 - hx), max(hy, g.dy - 1 - hy)),
+                       max(hz, g.dz - 1 - hz));
+  // pass 1
+  float tau, next;
+  int c_lt = 0, rend = 1;
+  {
+    KDist<K> D;
+    D.init();
+    auto net = [&](uint32_t, const float4& q) { D.add(dist2(q.x, q.y, q.z, p.x, p.y, p.z)); };
+    scan_block27(g, cs, gp, hx, hy, hz, net);
+    float worst = D.at(k - 1);
+    const float cl = ring_clearance(g, p.x, p.y, p.z, hx, hy, hz, 1);
+    if (!(worst < INFINITY && cl > 0.0f && worst < cl * cl)) {
+      uint32_t rows = 0;
+      for (int r = 2; r <= rmax; ++r) {
+        scan_ring<1>(g, cs, gp, p.x, p.y, p.z, hx, hy, hz, r, worst, rows, net);
+        rend = r;
+        worst = D.at(k - 1);
+        const float c = ring_clearance(g, p.x, p.y, p.z, hx, hy, hz, r);
+        if (worst < INFINITY && c > 0.0f && worst < c * c) break;
       }
     }
+    tau = worst;
+    next = D.at(k);
+#pragma unroll
+    for (int j = 0; j < K; ++j) c_lt += (j < k && D.d[j] < tau) ? 1 : 0;
+  }
+  if (stats) stats_add(stats, uint32_t(rend), 0, 0, rend == 1 ? kBlockDone : kRingFound);
+  uint32_t id_cut = 0xFFFFFFFFu;
+  if (tau < INFINITY && next == tau) {
+    // k-th and (k+1)-th tie: of the points at distance tau, take the
+    // k - c_lt lowest indices
+    KIds<K> T;
+    T.init();
+    knn_candidates(g, cs, gp, p.x, p.y, p.z, hx, hy, hz, rend, tau, [&](uint32_t, const float4& q) {
+      if (dist2(q.x, q.y, q.z, p.x, p.y, p.z) == tau) T.add(__float_as_uint(q.w));
+    });
+    id_cut = T.at(k - c_lt - 1);
+  }
+  auto selected = [&](const float4& q) {
+    const float d2 = dist2(q.x, q.y, q.z, p.x, p.y, p.z);
+    return d2 < tau || (d2 == tau && __float_as_uint(q.w) <= id_cut);
+  };
+  float n[3] = {1.0f, 0.0f, 0.0f};  // count < 2: identity -> eigenvector e_x (gicp.rs:268-270)
+  float mx = 0.0f, my = 0.0f, mz = 0.0f, c00 = 0, c11 = 0, c22 = 0, c01 = 0, c02 = 0, c12 = 0;
+  int found = 0;
+  // pass 2: mean of the selected points, relative to the query
+  knn_candidates(g, cs, gp, p.x, p.y, p.z, hx, hy, hz, rend, tau, [&](uint32_t, const float4& q) {
+    if (selected(q)) {
+      mx += q.x - p.x;
+      my += q.y - p.y;
+      mz += q.z - p.z;
+      ++found;
+    }
+  });
+  if (found >= 2) {
     const float invn = 1.0f / float(found);
     mx *= invn;
     my *= invn;
     mz *= invn;
-    float c00 = 0, c11 = 0, c22 = 0, c01 = 0, c02 = 0, c12 = 0;
-#pragma unroll
-    for (int j = 0; j < K; ++j) {
-      if (j < found) {
-        float4 q = gp[L.pos[j]];
-        float ex = (q.x - p.x) - mx, ey = (q.y - p.y) - my, ez = (q.z - p.z) - mz;
+    // pass 3: centred sums
+    knn_candidates(g, cs, gp, p.x, p.y, p.z, hx, hy, hz, rend, tau, [&](uint32_t, const float4& q) {
+      if (selected(q)) {
+        const float ex = (q.x - p.x) - mx, ey = (q.y - p.y) - my, ez = (q.z - p.z) - mz;
         c00 += ex * ex;
         c11 += ey * ey;
         c22 += ez * ez;
@@ -219,9 +333,9 @@ __global__ void __launch_bounds__(kBlock) k_knn_normals(GridDesc g, const uint32
         c02 += ex * ez;
         c12 += ey * ez;
       }
-    }
-    smallest_eigvec(c00, c11, c22, c01, c02, c12, n);
+    });
   }
+  if (found >= 2) smallest_eigvec(c00, c11, c22, c01, c02, c12, n);
   nrm_grid[i] = make_float4(n[0], n[1], n[2], 1.0f);
   const uint32_t orig = __float_as_uint(p.w);
   if (cov6) {
