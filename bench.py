@@ -13,8 +13,10 @@ gives the per-kernel times and the roofline figure.
 Default workload (single large instance): 2M-point target, 200k-point source,
 100 x 100 x 10 m block, voxel leaf 0.2 m, k = 15, d_max = 2.0 m, 50 iterations.
 
-One JSON line on stdout; `roofline` is the dominant kernel (the Gauss-Newton
-pass), timed with HIP events on the stream it runs on; `cpu_baseline` is the
+One JSON line on stdout; `roofline` is the dominant kernel work, the
+Gauss-Newton pass (its three stage kernels k_gicp_search, k_gicp_hard and
+k_gicp_terms, bracketed together), timed with HIP events on the stream it runs
+on; `cpu_baseline` is the
 CPU oracle (f32-faithful restatement of the reference, one thread) on a bounded
 sample of the same workload.
 """
@@ -124,9 +126,9 @@ def _batched_vs_plain(count):
 
 
 def _traffic(kernel, workload):
-    """HBM bytes per launch of `kernel` on `workload` from the committed
-    counter summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
-    calibration), or None when no summary exists."""
+    """HBM bytes per launch of `kernel` (per pass for the three pass stages) on
+    `workload` from the committed counter summary (FETCH_SIZE + WRITE_SIZE,
+    scaled by the copy-kernel calibration), or None when no summary exists."""
     path = os.path.join(ROOT, "profiles", f"{kernel}.{workload}.pmc.md")
     try:
         for line in open(path):
@@ -166,8 +168,9 @@ def main():
                          "kernel of known size for calibrating hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (registrations "
-              "shard across GPUs with no communication, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (independent "
+              "registrations shard across GPUs with no communication, so one GPU's rate is "
+              "the figure)", file=sys.stderr)
         sys.exit(2)
 
     import spatialrust_amd as sr
@@ -279,9 +282,10 @@ def main():
         "error_vs_ground_truth": gt,
         "kernels_ms_per_step": {k: v[0] / args.steps for k, v in kern.items()},
         "ms_per_step_eager_timed_launches": total_eager / args.steps * 1e3,
-        "roofline": {"kernel": "k_gicp_pass", "bound": "hbm", "achieved": achieved,
+        "roofline": {"kernel": "gicp pass (k_gicp_search + k_gicp_hard + k_gicp_terms)",
+                     "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": _traffic("k_gicp_pass", w.name),
+                     "traffic": _traffic("gicp_pass", w.name),
                      "algorithmic_bytes_per_launch": algo_bytes,
                      "avg_launch_us": avg_pass_s * 1e6, "launches": pass_n},
         "cpu_baseline": None,

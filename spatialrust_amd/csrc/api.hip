@@ -428,7 +428,21 @@ static void prepare_many(const std::vector<const sr_xyz_view*>& views,
 struct LoopArgs {
   DevBuf<IterInstance> insts;
   DevBuf<uint32_t> block_inst;
+  DevBuf<HardQuery> queue;  // pass scratch (PassScratch)
+  DevBuf<uint32_t> ctl, done;
 };
+
+// Grows a pass scratch to `points` queue slots and `blocks` flags; the queue
+// length starts at zero (the finalize kernel keeps it so between passes).
+static PassScratch grow_scratch(DevBuf<HardQuery>& queue, DevBuf<uint32_t>& ctl,
+                                DevBuf<uint32_t>& done, uint64_t points, uint32_t blocks,
+                                hipStream_t s) {
+  queue.resize(std::max<uint64_t>(points, 1));
+  done.resize(std::max<uint32_t>(blocks, 1));
+  ctl.resize(1);
+  fill_u32(ctl.p, 0u, 1, s);
+  return PassScratch{queue.p, ctl.p, done.p};
+}
 static LoopArgs& thread_loop_args() {
   static thread_local LoopArgs* a = new LoopArgs();  // lives until the thread ends
   return *a;
@@ -442,6 +456,8 @@ struct Batch {
   DevBuf<double> init;
   DevBuf<uint32_t> match;
   uint32_t total_blocks = 0, count = 0, max_passes = 0;
+  PassScratch scratch{};
+  uint32_t hard_blocks = 1;
 };
 
 // Work split of one registration's source over pass blocks.
@@ -510,6 +526,11 @@ static void batch_setup(const std::vector<const Prepared*>& src, const std::vect
     B.max_passes = std::max(B.max_passes, cfgs[i].max_iterations + 1);
   }
   B.total_blocks = blocks;
+  {
+    LoopArgs& la = thread_loop_args();
+    B.scratch = grow_scratch(la.queue, la.ctl, la.done, total_pts, blocks, s);
+    B.hard_blocks = hard_stage_blocks(total_pts);
+  }
   B.insts.resize(count);
   B.block_inst.resize(blocks);
   B.init.resize(init.size());
@@ -530,28 +551,48 @@ static void batch_setup(const std::vector<const Prepared*>& src, const std::vect
 struct GraphEntry {
   const void* insts;
   const void* block_inst;
-  uint32_t blocks, count, passes;
+  const void* queue;
+  const void* done;
+  uint32_t blocks, count, passes, hard_blocks;
   hipGraphExec_t exec;
 };
 
 static void batch_run(Batch& B, hipStream_t s) {
   if (profiler().on.load() != 0) {  // eager, so each launch can be timed
     for (uint32_t k = 0; k < B.max_passes; ++k)
-      gicp_pass(B.insts.p, B.block_inst.p, B.total_blocks, B.count, s);
+      gicp_pass(B.insts.p, B.block_inst.p, B.total_blocks, B.count, B.scratch, B.hard_blocks, s);
     return;
   }
   static thread_local std::vector<GraphEntry>* cache = new std::vector<GraphEntry>();
+  // Shapes seen once: capturing and instantiating a graph costs more than the
+  // eager launches of one loop, so a shape is captured on its second run (a
+  // stream of one-off shapes, such as odometry scans, stays eager).
+  static thread_local std::vector<GraphEntry>* seen = new std::vector<GraphEntry>();
+  const GraphEntry key{B.insts.p, B.block_inst.p, B.scratch.queue, B.scratch.done,
+                       B.total_blocks, B.count, B.max_passes, B.hard_blocks, nullptr};
+  auto same = [&](const GraphEntry& e) {
+    return e.insts == key.insts && e.block_inst == key.block_inst && e.queue == key.queue &&
+           e.done == key.done && e.blocks == key.blocks && e.count == key.count &&
+           e.passes == key.passes && e.hard_blocks == key.hard_blocks;
+  };
   hipGraphExec_t exec = nullptr;
   for (const GraphEntry& e : *cache)
-    if (e.insts == B.insts.p && e.block_inst == B.block_inst.p && e.blocks == B.total_blocks &&
-        e.count == B.count && e.passes == B.max_passes)
-      exec = e.exec;
+    if (same(e)) exec = e.exec;
   if (!exec) {
+    bool again = false;
+    for (const GraphEntry& e : *seen) again = again || same(e);
+    if (!again) {
+      if (seen->size() >= 16) seen->erase(seen->begin());
+      seen->push_back(key);
+      for (uint32_t k = 0; k < B.max_passes; ++k)
+        gicp_pass(B.insts.p, B.block_inst.p, B.total_blocks, B.count, B.scratch, B.hard_blocks, s);
+      return;
+    }
     SR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     hipGraph_t graph = nullptr;
     try {
       for (uint32_t k = 0; k < B.max_passes; ++k)
-        gicp_pass(B.insts.p, B.block_inst.p, B.total_blocks, B.count, s);
+        gicp_pass(B.insts.p, B.block_inst.p, B.total_blocks, B.count, B.scratch, B.hard_blocks, s);
     } catch (...) {
       (void)hipStreamEndCapture(s, &graph);
       if (graph) (void)hipGraphDestroy(graph);
@@ -566,8 +607,8 @@ static void batch_run(Batch& B, hipStream_t s) {
       (void)hipGraphExecDestroy(cache->front().exec);
       cache->erase(cache->begin());
     }
-    cache->push_back(GraphEntry{B.insts.p, B.block_inst.p, B.total_blocks, B.count,
-                                B.max_passes, exec});
+    cache->push_back(GraphEntry{B.insts.p, B.block_inst.p, B.scratch.queue, B.scratch.done,
+                                B.total_blocks, B.count, B.max_passes, B.hard_blocks, exec});
   }
   SR_HIP(hipGraphLaunch(exec, s));
 }
@@ -701,6 +742,8 @@ struct sr_server {
   sr::DevBuf<sr::IterInstance> insts;
   sr::DevBuf<uint32_t> block_inst;
   sr::DevBuf<double> init;
+  sr::DevBuf<sr::HardQuery> pass_queue;  // pass scratch
+  sr::DevBuf<uint32_t> pass_ctl, pass_done;
 };
 
 namespace sr {
@@ -740,18 +783,21 @@ static void server_step(sr_server& S) {
     std::vector<IterInstance> hi(count);
     std::vector<uint32_t> hbi;
     uint32_t begin = 0;
+    uint64_t points = 0;
     for (uint32_t i = 0; i < count; ++i) {
       ServerRequest& r = *S.active[i];
       hi[i] = describe_instance(r.src, r.map->tgt, S.gicp, S.states.p + r.slot, r.partials.p,
                                 r.match.p, begin);
       for (uint32_t b = 0; b < hi[i].nblocks; ++b) hbi.push_back(i);
       begin += hi[i].nblocks;
+      points += r.src.n;
     }
     S.insts.resize(count);
     S.block_inst.resize(begin);
     SR_HIP(hipMemcpyAsync(S.insts.p, hi.data(), sizeof(IterInstance) * count, hipMemcpyHostToDevice, s));
     SR_HIP(hipMemcpyAsync(S.block_inst.p, hbi.data(), sizeof(uint32_t) * begin, hipMemcpyHostToDevice, s));
-    gicp_pass(S.insts.p, S.block_inst.p, begin, count, s);
+    const PassScratch ps = grow_scratch(S.pass_queue, S.pass_ctl, S.pass_done, points, begin, s);
+    gicp_pass(S.insts.p, S.block_inst.p, begin, count, ps, hard_stage_blocks(points), s);
   }
   // Meanwhile: preprocess the admitted sources on the worker streams.
   if (!admit.empty()) {
@@ -1063,7 +1109,7 @@ int32_t sr_gicp_normal_equations(const sr_xyz_view* source, const sr_xyz_view* t
     double pose[7] = {q[0], q[1], q[2], q[3], t[0], t[1], t[2]};
     Batch B;
     batch_setup({&src}, {&tgt}, {cfg}, B, s, pose);
-    gicp_pass(B.insts.p, B.block_inst.p, B.total_blocks, 1, s, false);
+    gicp_pass(B.insts.p, B.block_inst.p, B.total_blocks, 1, B.scratch, B.hard_blocks, s, false);
     std::vector<float> part(size_t(B.total_blocks) * 64);
     std::vector<uint32_t> match(src.n), sorder(src.n), torder(tgt.n);
     SR_HIP(hipMemcpyAsync(part.data(), B.partials.p, part.size() * sizeof(float), hipMemcpyDeviceToHost, s));

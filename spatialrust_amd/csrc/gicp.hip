@@ -224,14 +224,14 @@ __device__ __forceinline__ void finalize(InstState* st, const InstConfig& cfg, c
 
 // ---- nearest target point within dmax (exact; ties to the lower index) ----
 //
-// Phase 1 runs per lane: the box and empty-space rejects, the previous pass's
-// correspondence as a seed (a real candidate, so it tightens the radius without
-// changing the answer), and the 3x3x3 block cut to the seeded sphere. A query
-// the block's clearance certifies is done. The others (in early passes about
-// half) would each walk dozens of grid rows while the rest of the wave idles,
-// so phase 2 shares them out: each wave's uncertified queries are taken by
-// groups of kNNGroup lanes that split the rows of each ring of the query's
-// ball between them and agree on the radius after every ring.
+// Phase 1 runs per lane in the pass's first stage: the box and empty-space
+// rejects, the previous pass's correspondence as a seed (a real candidate, so it
+// tightens the radius without changing the answer), and the 3x3x3 block cut to
+// the seeded sphere. A query the block's clearance certifies is done. The others
+// (in early passes about a third, clustered in space) go to a queue that the
+// second stage drains with dynamic work distribution: groups of kNNGroup lanes
+// each take one queued query and split the rows of each ring of its ball
+// between them, agreeing on the radius after every ring.
 
 struct NNQuery {
   uint64_t best_key;  // (d2 bits << 32 | original index); starts at (dmax2, ~0)
@@ -299,7 +299,9 @@ __device__ __forceinline__ void nn_phase1(const GridDesc& g, const uint32_t* __r
   if (inside && e <= 1) {
     // Rings 0 and 1 as nine contiguous point ranges, cut to the sphere of the
     // seeded radius; most queries of a converging registration end here.
-    scan_block27_within(g, cs, tp, px, py, pz, hx, hy, hz, Q.thr, [&](uint32_t s, const float4& q) {
+    // the block's nine ranges streamed as one sequence, eight loads in flight
+    scan_block27_within<8, true>(g, cs, tp, px, py, pz, hx, hy, hz, Q.thr,
+                                 [&](uint32_t s, const float4& q) {
       if constexpr (kStats) ++Q.pts;
       nn_consider(Q, px, py, pz, s, q);
     });
@@ -314,7 +316,7 @@ __device__ __forceinline__ void nn_phase1(const GridDesc& g, const uint32_t* __r
 }
 
 #ifndef SR_NN_GROUP
-#define SR_NN_GROUP 4
+#define SR_NN_GROUP 16
 #endif
 constexpr int kNNGroup = SR_NN_GROUP;  // lanes per query in phase 2 (a power of two <= 64)
 static_assert(kNNGroup >= 1 && kNNGroup <= 64 && (kNNGroup & (kNNGroup - 1)) == 0, "group size");
@@ -322,6 +324,10 @@ static_assert(kNNGroup >= 1 && kNNGroup <= 64 && (kNNGroup & (kNNGroup - 1)) == 
 #define SR_ROW_BATCH 2
 #endif
 constexpr int kRowBatch = SR_ROW_BATCH;  // rows per bound-loading batch in phase 2
+#ifndef SR_STREAM_W
+#define SR_STREAM_W 4
+#endif
+constexpr int kStreamW = SR_STREAM_W;  // point loads in flight per lane in phase 2
 
 // Row t of Chebyshev ring r (r >= 1) of (y,z) rows around the query's row:
 // the perimeter of the (2r+1)^2 square, each row once.
@@ -346,139 +352,99 @@ __device__ __forceinline__ void ring_row(int r, int t, int& dy, int& dz) {
   }
 }
 
-// Phase 2 for the calling wave; every lane of the wave must call it.
+// Ring search of one query by the kNNGroup lanes of a group, which must all
+// call it together. W holds the best so far (phase 1) and its radius; rings are
+// centred on the query's row clamped to one row outside the grid, so a query
+// beside the grid still reaches every grid row within max(dy, dz) rings (the
+// gap and clearance tests use the true position). On return every lane of the
+// group holds the group's best key and position.
 template <bool kStats>
-__device__ __forceinline__ void nn_phase2(const GridDesc& g, const uint32_t* __restrict__ cs,
-                                          const float4* __restrict__ tp, float px, float py,
-                                          float pz, NNQuery& Q, SearchStats* stats) {
-  const int lane = int(threadIdx.x & (kWave - 1));
-  const unsigned long long hard = __ballot(Q.hard);
-  if (hard == 0) return;
-  const unsigned long long below = (1ull << lane) - 1ull;
-  const int H = __popcll(hard);
-  // Source lane of the k-th hard query: hard lanes send their lane id to slot
-  // (rank), the others to slots after H, so the permutation has no collisions.
-  const int slot = Q.hard ? __popcll(hard & below) : H + __popcll(~hard & below);
-  const int src_of = __builtin_amdgcn_ds_permute(slot << 2, lane);
-  constexpr int G = kWave / kNNGroup;
-  const int sub = lane % kNNGroup;
-  for (int base = 0; base < H; base += G) {
-    const int k = base + lane / kNNGroup;
-    const bool active = k < H;
-    const int src = __shfl(src_of, active ? k : 0);
-    const float qx = __shfl(px, src), qy = __shfl(py, src), qz = __shfl(pz, src);
-    // rings are centred on the query's row clamped to one row outside the grid,
-    // so a query beside the grid still reaches every grid row within
-    // max(dy, dz) rings (the gap and clearance tests use the true position)
-    const int hx = __shfl(Q.hx, src);
-    const int hy = min(max(__shfl(Q.hy, src), -1), g.dy), hz = min(max(__shfl(Q.hz, src), -1), g.dz);
-    const bool skip = __shfl(int(Q.skip_block), src) != 0;
-    NNQuery W;
-    W.best_key = (uint64_t(__shfl(uint32_t(Q.best_key >> 32), src)) << 32) |
-                 __shfl(uint32_t(Q.best_key), src);
-    W.best_pos = __shfl(Q.best_pos, src);
-    W.thr = __shfl(Q.thr, src);
-    W.rows = W.pts = 0;
-    if (active) {
-      const int rmax = max(max(hy, g.dy - 1 - hy), max(hz, g.dz - 1 - hz));
-      for (int r = 0; r <= rmax; ++r) {
-        if (r > 0) {  // W.thr is the same on the group's lanes here
-          const float c = clearance_yz(g, qy, qz, hy, hz, r - 1);
-          if (c > 0.0f && c * c > W.thr) break;
-        }
-        const int nrows = r == 0 ? 1 : 8 * r;
-        // This lane's rows of the ring, kRowBatch at a time: the bounds of a
-        // batch's ranges are loaded together and then its points streamed, so
-        // a batch costs one bound round trip instead of one per row. A row
-        // whose (y,z) gap exceeds the radius contributes an empty range.
-        for (int t0 = sub; t0 < nrows; t0 += kNNGroup * kRowBatch) {
-          uint32_t ia[2 * kRowBatch], ib[2 * kRowBatch];
-#pragma unroll
-          for (int m = 0; m < kRowBatch; ++m) {
-            ia[2 * m] = ib[2 * m] = ia[2 * m + 1] = ib[2 * m + 1] = 0;
-            const int t = t0 + m * kNNGroup;
-            if (t >= nrows) continue;
-            int dy, dz;
-            ring_row(r, t, dy, dz);
-            const int cz = hz + dz, cy = hy + dy;
-            if (cz < 0 || cz >= g.dz || cy < 0 || cy >= g.dy) continue;
-            const float gz = axis_gap(qz, g.oz, g.h, g.margin, cz);
-            const float gy = axis_gap(qy, g.oy, g.h, g.margin, cy);
-            const float gyz = gz * gz + gy * gy;
-            if (gyz > W.thr) continue;
-            const float rx = __builtin_amdgcn_sqrtf(W.thr - gyz) + g.margin;
-            const int x0 = max(cell_coord_free(qx - rx, g.ox, g.inv_h) - 1, 0);
-            const int x1 = min(cell_coord_free(qx + rx, g.ox, g.inv_h) + 1, g.dx - 1);
-            if (x0 > x1) continue;
-            const uint32_t cbase = (uint32_t(cz) * uint32_t(g.dy) + uint32_t(cy)) * uint32_t(g.dx);
-            if constexpr (kStats) ++W.rows;
-            if (skip && r <= 1) {
-              // left and right of the block's three cells
-              if (x0 <= hx - 2) {
-                ia[2 * m] = cbase + uint32_t(x0);
-                ib[2 * m] = cbase + uint32_t(min(x1, hx - 2)) + 1;
-              }
-              if (x1 >= hx + 2) {
-                ia[2 * m + 1] = cbase + uint32_t(max(x0, hx + 2));
-                ib[2 * m + 1] = cbase + uint32_t(x1) + 1;
-              }
-            } else {
-              ia[2 * m] = cbase + uint32_t(x0);
-              ib[2 * m] = cbase + uint32_t(x1) + 1;
-            }
-          }
-          uint32_t lo[2 * kRowBatch], hi[2 * kRowBatch];
-#pragma unroll
-          for (int m = 0; m < 2 * kRowBatch; ++m) {
-            lo[m] = cs[ia[m]];
-            hi[m] = cs[ib[m]];
-          }
-          stream_ranges(lo, hi, tp, [&](uint32_t s, const float4& q) {
-            if constexpr (kStats) ++W.pts;
-            nn_consider(W, qx, qy, qz, s, q);
-          });
-        }
-        // the group agrees on the smallest radius before the next ring
-#pragma unroll
-        for (int off = 1; off < kNNGroup; off <<= 1) W.thr = fminf(W.thr, __shfl_xor(W.thr, off));
-      }
+__device__ __forceinline__ void group_search(const GridDesc& g, const uint32_t* __restrict__ cs,
+                                             const float4* __restrict__ tp, float qx, float qy,
+                                             float qz, bool skip, NNQuery& W) {
+  const int sub = int(threadIdx.x & (kWave - 1)) % kNNGroup;
+  const int hx = cell_coord_free(qx, g.ox, g.inv_h);
+  const int hy = min(max(cell_coord_free(qy, g.oy, g.inv_h), -1), g.dy);
+  const int hz = min(max(cell_coord_free(qz, g.oz, g.inv_h), -1), g.dz);
+  const int rmax = max(max(hy, g.dy - 1 - hy), max(hz, g.dz - 1 - hz));
+  for (int r = 0; r <= rmax; ++r) {
+    if (r > 0) {  // W.thr is the same on the group's lanes here
+      const float c = clearance_yz(g, qy, qz, hy, hz, r - 1);
+      if (c > 0.0f && c * c > W.thr) break;
     }
-    // group minimum of (key, position); the group's lanes are all active or all idle
+    const int nrows = r == 0 ? 1 : 8 * r;
+    // This lane's rows of the ring, kRowBatch at a time: the bounds of a
+    // batch's ranges are loaded together and then its points streamed, so a
+    // batch costs one bound round trip instead of one per row. A row whose
+    // (y,z) gap exceeds the radius contributes an empty range.
+    for (int t0 = sub; t0 < nrows; t0 += kNNGroup * kRowBatch) {
+      uint32_t ia[2 * kRowBatch], ib[2 * kRowBatch];
 #pragma unroll
-    for (int off = 1; off < kNNGroup; off <<= 1) {
-      const uint32_t ohi = __shfl_xor(uint32_t(W.best_key >> 32), off);
-      const uint32_t olo = __shfl_xor(uint32_t(W.best_key), off);
-      const uint32_t opos = __shfl_xor(W.best_pos, off);
-      const uint64_t ok = (uint64_t(ohi) << 32) | olo;
-      if (ok < W.best_key) {
-        W.best_key = ok;
-        W.best_pos = opos;
+      for (int m = 0; m < kRowBatch; ++m) {
+        ia[2 * m] = ib[2 * m] = ia[2 * m + 1] = ib[2 * m + 1] = 0;
+        const int t = t0 + m * kNNGroup;
+        if (t >= nrows) continue;
+        int dy, dz;
+        ring_row(r, t, dy, dz);
+        const int cz = hz + dz, cy = hy + dy;
+        if (cz < 0 || cz >= g.dz || cy < 0 || cy >= g.dy) continue;
+        const float gz = axis_gap(qz, g.oz, g.h, g.margin, cz);
+        const float gy = axis_gap(qy, g.oy, g.h, g.margin, cy);
+        const float gyz = gz * gz + gy * gy;
+        if (gyz > W.thr) continue;
+        const float rx = __builtin_amdgcn_sqrtf(W.thr - gyz) + g.margin;
+        const int x0 = max(cell_coord_free(qx - rx, g.ox, g.inv_h) - 1, 0);
+        const int x1 = min(cell_coord_free(qx + rx, g.ox, g.inv_h) + 1, g.dx - 1);
+        if (x0 > x1) continue;
+        const uint32_t cbase = (uint32_t(cz) * uint32_t(g.dy) + uint32_t(cy)) * uint32_t(g.dx);
+        if constexpr (kStats) ++W.rows;
+        if (skip && r <= 1) {
+          // left and right of the block's three cells
+          if (x0 <= hx - 2) {
+            ia[2 * m] = cbase + uint32_t(x0);
+            ib[2 * m] = cbase + uint32_t(min(x1, hx - 2)) + 1;
+          }
+          if (x1 >= hx + 2) {
+            ia[2 * m + 1] = cbase + uint32_t(max(x0, hx + 2));
+            ib[2 * m + 1] = cbase + uint32_t(x1) + 1;
+          }
+        } else {
+          ia[2 * m] = cbase + uint32_t(x0);
+          ib[2 * m] = cbase + uint32_t(x1) + 1;
+        }
       }
-      if constexpr (kStats) {
-        W.rows += __shfl_xor(W.rows, off);
-        W.pts += __shfl_xor(W.pts, off);
+      uint32_t lo[2 * kRowBatch], hi[2 * kRowBatch];
+#pragma unroll
+      for (int m = 0; m < 2 * kRowBatch; ++m) {
+        lo[m] = cs[ia[m]];
+        hi[m] = cs[ib[m]];
       }
+      stream_ranges<kStreamW>(lo, hi, tp, [&](uint32_t s, const float4& q) {
+        if constexpr (kStats) ++W.pts;
+        nn_consider(W, qx, qy, qz, s, q);
+      });
     }
-    // hand each result back to its query's lane (the group's first lane holds it)
-    const int my = Q.hard ? __popcll(hard & below) : -1;
-    const bool mine = my >= base && my < base + G;
-    const int leader = mine ? (my - base) * kNNGroup : lane;
-    const uint32_t rhi = __shfl(uint32_t(W.best_key >> 32), leader);
-    const uint32_t rlo = __shfl(uint32_t(W.best_key), leader);
-    const uint32_t rpos = __shfl(W.best_pos, leader);
-    const uint32_t rrows = __shfl(W.rows, leader), rpts = __shfl(W.pts, leader);
-    if (mine) {
-      Q.best_key = (uint64_t(rhi) << 32) | rlo;
-      Q.best_pos = rpos;
-      if constexpr (kStats)
-        stats_add(stats, 2, rrows, Q.pts + rpts, rpos == kNone ? kRingNone : kRingFound);
+    // the group agrees on the smallest radius before the next ring
+#pragma unroll
+    for (int off = 1; off < kNNGroup; off <<= 1) W.thr = fminf(W.thr, __shfl_xor(W.thr, off));
+  }
+  // group minimum of (key, position)
+#pragma unroll
+  for (int off = 1; off < kNNGroup; off <<= 1) {
+    const uint32_t ohi = __shfl_xor(uint32_t(W.best_key >> 32), off);
+    const uint32_t olo = __shfl_xor(uint32_t(W.best_key), off);
+    const uint32_t opos = __shfl_xor(W.best_pos, off);
+    const uint64_t ok = (uint64_t(ohi) << 32) | olo;
+    if (ok < W.best_key) {
+      W.best_key = ok;
+      W.best_pos = opos;
+    }
+    if constexpr (kStats) {
+      W.rows += __shfl_xor(W.rows, off);
+      W.pts += __shfl_xor(W.pts, off);
     }
   }
 }
-
-struct PointTerms {
-  float v[kTerms];
-};
 
 // Closed-form M = (C_t + R C_s R^T)^-1 with C = a I - beta n n^T.
 __device__ __forceinline__ void combined_inverse(float ux, float uy, float uz, float at, float vx,
@@ -527,162 +493,158 @@ __device__ __forceinline__ void combined_inverse(float ux, float uy, float uz, f
   M[5] = ia + k * wz * wz;
 }
 
-template <bool kStats>
-__global__ void __launch_bounds__(kBlock) k_gicp_pass(const IterInstance* __restrict__ insts,
-                                                      const uint32_t* __restrict__ block_inst,
-                                                      SearchStats* __restrict__ stats) {
-  // XCD-aware block order: the hardware deals consecutive workgroup ids round
-  // robin over the 8 XCDs; give each XCD a contiguous range of logical blocks so
-  // neighbouring source tiles (which touch the same target cells) share an L2.
+// ---- the pass in three stages ---------------------------------------------
+//
+// k_gicp_search: per source point, p' = R p + t and phase 1. Certified points
+//   record their match; uncertified ones are queued (wave-aggregated append).
+//   A block none of whose points was queued also adds its terms and writes its
+//   partial, and flags itself done.
+// k_gicp_hard: drains the queue, kWave/kNNGroup queries per wave, so the
+//   clustered hard queries spread over the whole chip instead of lengthening
+//   the waves that hold them.
+// k_gicp_terms: the blocks not flagged done add their terms from the recorded
+//   matches and write their partials.
+// A block's terms are added in the same order whichever stage does it, so the
+// sums do not depend on how the points were split between the stages.
+
+// Logical block of this workgroup, XCD-aware: the hardware deals consecutive
+// workgroup ids round robin over the 8 XCDs; give each XCD a contiguous range
+// of logical blocks so neighbouring source tiles (which touch the same target
+// cells) share an L2.
+__device__ __forceinline__ uint32_t xcd_block() {
   const uint32_t G = gridDim.x, hw = blockIdx.x;
   const uint32_t xcd = hw & 7u, q8 = G >> 3, rem = G & 7u;
-  const uint32_t b = xcd * q8 + min(xcd, rem) + (hw >> 3);
+  return xcd * q8 + min(xcd, rem) + (hw >> 3);
+}
 
-  const uint32_t inst_id = block_inst[b];
-  const IterInstance& I = insts[inst_id];
-  InstState* st = I.state;
-  if (st->phase == 2) return;
-  const uint32_t lb = b - I.block_begin;
-  const GridDesc g = I.grid;
-  const InstConfig cfg = I.cfg;
-  const float beta = cfg.one_minus_eps, eps = 1.0f - cfg.one_minus_eps;
+// The f32 pose of a pass: high parts R, T and low parts Rl, Tl.
+struct PassPose {
   float R[9], T[3], Rl[9], Tl[3];
+};
+
+__device__ __forceinline__ void load_pose(const InstState* st, PassPose& P) {
 #pragma unroll
   for (int a = 0; a < 9; ++a) {
-    R[a] = st->rf[a];
-    Rl[a] = st->rlo[a];
+    P.R[a] = st->rf[a];
+    P.Rl[a] = st->rlo[a];
   }
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    T[a] = st->tf[a];
-    Tl[a] = st->tlo[a];
+    P.T[a] = st->tf[a];
+    P.Tl[a] = st->tlo[a];
   }
+}
 
-  // Plain f32 per-thread sums (a thread holds only ppt points); compensation
-  // happens in the block reduction below.
-  float acc[kTerms];
-#pragma unroll
-  for (int t = 0; t < kTerms; ++t) acc[t] = 0.0f;
-  uint32_t count = 0;
+// p' = R p + t (Isometry3::transform_point, transform.rs:126-128) with the FP64
+// pose split into f32 high and low parts: the low part removes the systematic
+// error of an f32-rounded pose, which would otherwise floor the Gauss-Newton
+// update at ~1e-7 (the reference's own f32 pose has that floor).
+__device__ __forceinline__ void transform_point(const PassPose& P, const float4& p, float& px,
+                                                float& py, float& pz) {
+  const float lx = fmaf(P.Rl[0], p.x, fmaf(P.Rl[1], p.y, fmaf(P.Rl[2], p.z, P.Tl[0])));
+  const float ly = fmaf(P.Rl[3], p.x, fmaf(P.Rl[4], p.y, fmaf(P.Rl[5], p.z, P.Tl[1])));
+  const float lz = fmaf(P.Rl[6], p.x, fmaf(P.Rl[7], p.y, fmaf(P.Rl[8], p.z, P.Tl[2])));
+  px = fmaf(P.R[0], p.x, fmaf(P.R[1], p.y, fmaf(P.R[2], p.z, P.T[0] + lx)));
+  py = fmaf(P.R[3], p.x, fmaf(P.R[4], p.y, fmaf(P.R[5], p.z, P.T[1] + ly)));
+  pz = fmaf(P.R[6], p.x, fmaf(P.R[7], p.y, fmaf(P.R[8], p.z, P.T[2] + lz)));
+}
 
-  const uint32_t ppt = I.ppt;
-  for (uint32_t k = 0; k < ppt; ++k) {
-    // Every lane stays in the loop (the search's phase 2 is wave-collective);
-    // lanes past the end compute on a clamped point and skip the results.
-    const uint32_t i = (lb * ppt + k) * kBlock + threadIdx.x;
-    const bool valid = i < I.n_src;
-    const float4 p = I.src[valid ? i : 0u];
-    // p' = R p + t (Isometry3::transform_point, transform.rs:126-128) with the
-    // FP64 pose split into f32 high and low parts: the low part removes the
-    // systematic error of an f32-rounded pose, which would otherwise floor the
-    // Gauss-Newton update at ~1e-7 (the reference's own f32 pose has that floor).
-    const float lx = fmaf(Rl[0], p.x, fmaf(Rl[1], p.y, fmaf(Rl[2], p.z, Tl[0])));
-    const float ly = fmaf(Rl[3], p.x, fmaf(Rl[4], p.y, fmaf(Rl[5], p.z, Tl[1])));
-    const float lz = fmaf(Rl[6], p.x, fmaf(Rl[7], p.y, fmaf(Rl[8], p.z, Tl[2])));
-    const float px = fmaf(R[0], p.x, fmaf(R[1], p.y, fmaf(R[2], p.z, T[0] + lx)));
-    const float py = fmaf(R[3], p.x, fmaf(R[4], p.y, fmaf(R[5], p.z, T[1] + ly)));
-    const float pz = fmaf(R[6], p.x, fmaf(R[7], p.y, fmaf(R[8], p.z, T[2] + lz)));
-    NNQuery Q;
-    nn_phase1<kStats>(g, I.cell_start, I.empty_radius, I.tgt, px, py, pz, cfg.dmax, cfg.dmax2,
-                      valid ? I.match[i] : kNone, Q, valid ? stats : nullptr);
-    if (!valid) Q.hard = false;
-    nn_phase2<kStats>(g, I.cell_start, I.tgt, px, py, pz, Q, stats);
-    if (!valid) continue;
-    const uint32_t j = Q.best_pos;
-    I.match[i] = j;
-    if (j == kNone) continue;
-    const float d2 = __uint_as_float(uint32_t(Q.best_key >> 32));
-    const float4 q = I.tgt[j];
-    const float4 nt = I.tgt_n[j];
-    const float4 ns = I.src_n[i];
-    const float vx = R[0] * ns.x + R[1] * ns.y + R[2] * ns.z;
-    const float vy = R[3] * ns.x + R[4] * ns.y + R[5] * ns.z;
-    const float vz = R[6] * ns.x + R[7] * ns.y + R[8] * ns.z;
-    float M[6];
-    combined_inverse(nt.x, nt.y, nt.z, nt.w, vx, vy, vz, ns.w, beta, eps, M);
-    const float ex = px - q.x, ey = py - q.y, ez = pz - q.z;
-    // w = M e
-    const float wx = M[0] * ex + M[1] * ey + M[2] * ez;
-    const float wy = M[1] * ex + M[3] * ey + M[4] * ez;
-    const float wz = M[2] * ex + M[4] * ey + M[5] * ez;
-    // A = skew(p') M ; H_rt = A ; H_rr = -A skew(p') ; H_tt = M
-    const float m[3][3] = {{M[0], M[1], M[2]}, {M[1], M[3], M[4]}, {M[2], M[4], M[5]}};
-    float A[3][3];
+// Adds one correspondence's H (upper triangle, row-major over the 6x6), g and
+// squared error to the per-thread sums.
+__device__ __forceinline__ void add_terms(float (&acc)[kTerms], const PassPose& P, float beta,
+                                          float eps, float px, float py, float pz, const float4& q,
+                                          const float4& nt, const float4& ns, float d2) {
+  const float* R = P.R;
+  const float vx = R[0] * ns.x + R[1] * ns.y + R[2] * ns.z;
+  const float vy = R[3] * ns.x + R[4] * ns.y + R[5] * ns.z;
+  const float vz = R[6] * ns.x + R[7] * ns.y + R[8] * ns.z;
+  float M[6];
+  combined_inverse(nt.x, nt.y, nt.z, nt.w, vx, vy, vz, ns.w, beta, eps, M);
+  const float ex = px - q.x, ey = py - q.y, ez = pz - q.z;
+  // w = M e
+  const float wx = M[0] * ex + M[1] * ey + M[2] * ez;
+  const float wy = M[1] * ex + M[3] * ey + M[4] * ez;
+  const float wz = M[2] * ex + M[4] * ey + M[5] * ez;
+  // A = skew(p') M ; H_rt = A ; H_rr = -A skew(p') ; H_tt = M
+  const float m[3][3] = {{M[0], M[1], M[2]}, {M[1], M[3], M[4]}, {M[2], M[4], M[5]}};
+  float A[3][3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      A[0][c] = -pz * m[1][c] + py * m[2][c];
-      A[1][c] = pz * m[0][c] - px * m[2][c];
-      A[2][c] = -py * m[0][c] + px * m[1][c];
-    }
-    float Hrr[3][3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      Hrr[a][0] = -(A[a][1] * pz - A[a][2] * py);
-      Hrr[a][1] = -(-A[a][0] * pz + A[a][2] * px);
-      Hrr[a][2] = -(A[a][0] * py - A[a][1] * px);
-    }
-    // H upper triangle, row-major over the 6x6.
-    acc[ut(0, 0)] += (Hrr[0][0]);
-    acc[ut(0, 1)] += (Hrr[0][1]);
-    acc[ut(0, 2)] += (Hrr[0][2]);
-    acc[ut(1, 1)] += (Hrr[1][1]);
-    acc[ut(1, 2)] += (Hrr[1][2]);
-    acc[ut(2, 2)] += (Hrr[2][2]);
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) acc[ut(a, 3 + c)] += (A[a][c]);
-    acc[ut(3, 3)] += (M[0]);
-    acc[ut(3, 4)] += (M[1]);
-    acc[ut(3, 5)] += (M[2]);
-    acc[ut(4, 4)] += (M[3]);
-    acc[ut(4, 5)] += (M[4]);
-    acc[ut(5, 5)] += (M[5]);
-    // g = [p' x w ; w]
-    acc[21] += (py * wz - pz * wy);
-    acc[22] += (pz * wx - px * wz);
-    acc[23] += (px * wy - py * wx);
-    acc[24] += (wx);
-    acc[25] += (wy);
-    acc[26] += (wz);
-    acc[27] += (d2);
-    ++count;
+  for (int c = 0; c < 3; ++c) {
+    A[0][c] = -pz * m[1][c] + py * m[2][c];
+    A[1][c] = pz * m[0][c] - px * m[2][c];
+    A[2][c] = -py * m[0][c] + px * m[1][c];
   }
-
-  // ---- block reduction (fixed order, compensated) -------------------------
-  // Rows padded by 8 floats and chunks read interleaved (element e*8 + ch), so
-  // the 64 lanes of a wave spread over the LDS banks.
-  constexpr int kRow = kBlock + 8;
-  __shared__ float red[kTerms][kRow];
-  __shared__ float part_hi[kTerms][8], part_lo[kTerms][8];
-  __shared__ unsigned long long wcount[kBlock / kWave];
+  float Hrr[3][3];
 #pragma unroll
-  for (int t = 0; t < kTerms; ++t) red[t][threadIdx.x] = acc[t];
+  for (int a = 0; a < 3; ++a) {
+    Hrr[a][0] = -(A[a][1] * pz - A[a][2] * py);
+    Hrr[a][1] = -(-A[a][0] * pz + A[a][2] * px);
+    Hrr[a][2] = -(A[a][0] * py - A[a][1] * px);
+  }
+  acc[ut(0, 0)] += (Hrr[0][0]);
+  acc[ut(0, 1)] += (Hrr[0][1]);
+  acc[ut(0, 2)] += (Hrr[0][2]);
+  acc[ut(1, 1)] += (Hrr[1][1]);
+  acc[ut(1, 2)] += (Hrr[1][2]);
+  acc[ut(2, 2)] += (Hrr[2][2]);
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) acc[ut(a, 3 + c)] += (A[a][c]);
+  acc[ut(3, 3)] += (M[0]);
+  acc[ut(3, 4)] += (M[1]);
+  acc[ut(3, 5)] += (M[2]);
+  acc[ut(4, 4)] += (M[3]);
+  acc[ut(4, 5)] += (M[4]);
+  acc[ut(5, 5)] += (M[5]);
+  // g = [p' x w ; w]
+  acc[21] += (py * wz - pz * wy);
+  acc[22] += (pz * wx - px * wz);
+  acc[23] += (px * wy - py * wx);
+  acc[24] += (wx);
+  acc[25] += (wy);
+  acc[26] += (wz);
+  acc[27] += (d2);
+}
+
+// Block reduction (fixed order, compensated). Rows padded by 8 floats and
+// chunks read interleaved (element e*8 + ch), so the 64 lanes of a wave spread
+// over the LDS banks.
+constexpr int kRedRow = kBlock + 8;
+struct RedLds {
+  float red[kTerms][kRedRow];
+  float part_hi[kTerms][8], part_lo[kTerms][8];
+  unsigned long long wcount[kBlock / kWave];
+};
+
+// Writes the block partial: 28 (hi, lo) pairs + u64 count. The finalize kernel
+// of this pass reads it after the kernel boundary, so no device-scope fence (an
+// L2 write-back + invalidate on every XCD) is needed here.
+__device__ __forceinline__ void write_partial(RedLds& L, const float (&acc)[kTerms],
+                                              uint32_t count, float* out) {
+#pragma unroll
+  for (int t = 0; t < kTerms; ++t) L.red[t][threadIdx.x] = acc[t];
   unsigned long long c64 = count;
   for (int off = kWave / 2; off > 0; off >>= 1) c64 += __shfl_xor(c64, off);
-  if ((threadIdx.x & (kWave - 1)) == 0) wcount[threadIdx.x / kWave] = c64;
+  if ((threadIdx.x & (kWave - 1)) == 0) L.wcount[threadIdx.x / kWave] = c64;
   __syncthreads();
   if (threadIdx.x < kTerms * 8) {
     const int t = threadIdx.x >> 3, ch = threadIdx.x & 7;
     Kahan k;
     k.init();
-    for (int e = 0; e < kBlock / 8; ++e) k.add(red[t][e * 8 + ch]);
-    part_hi[t][ch] = k.s;
-    part_lo[t][ch] = -k.c;
+    for (int e = 0; e < kBlock / 8; ++e) k.add(L.red[t][e * 8 + ch]);
+    L.part_hi[t][ch] = k.s;
+    L.part_lo[t][ch] = -k.c;
   }
   __syncthreads();
-  // Block partial: 28 (hi, lo) pairs + u64 count. The finalize kernel of this
-  // pass reads them after the kernel boundary, so no device-scope fence (an L2
-  // write-back + invalidate on every XCD) is needed here.
-  float* out = I.partials + size_t(lb) * 64;
   if (threadIdx.x < kTerms) {
     const int t = threadIdx.x;
     float hi = 0.0f, lo = 0.0f;
     for (int ch = 0; ch < 8; ++ch) {
       float s, e;
-      two_sum(hi, part_hi[t][ch], s, e);
+      two_sum(hi, L.part_hi[t][ch], s, e);
       hi = s;
-      lo += e + part_lo[t][ch];
+      lo += e + L.part_lo[t][ch];
     }
     float s, e;
     two_sum(hi, lo, s, e);
@@ -691,9 +653,161 @@ __global__ void __launch_bounds__(kBlock) k_gicp_pass(const IterInstance* __rest
   }
   if (threadIdx.x == kTerms) {
     unsigned long long tot = 0;
-    for (int w = 0; w < kBlock / kWave; ++w) tot += wcount[w];
+    for (int w = 0; w < kBlock / kWave; ++w) tot += L.wcount[w];
     reinterpret_cast<unsigned long long*>(out)[30] = tot;  // floats 60..61
   }
+}
+
+constexpr uint32_t kHardBlockFlag = 0x80000000u;  // HardQuery.inst: block scanned in phase 1
+
+// The queue stage is bound by the latency of its chains of dependent loads, so
+// it is compiled for as many resident waves as the SIMD takes.
+#ifndef SR_HARD_WAVES
+#define SR_HARD_WAVES 8
+#endif
+
+template <bool kStats>
+__global__ void __launch_bounds__(kBlock) k_gicp_search(const IterInstance* __restrict__ insts,
+                                                        const uint32_t* __restrict__ block_inst,
+                                                        PassScratch ps,
+                                                        SearchStats* __restrict__ stats) {
+  const uint32_t b = xcd_block();
+  const uint32_t inst_id = block_inst[b];
+  const IterInstance& I = insts[inst_id];
+  const InstState* st = I.state;
+  if (st->phase == 2) return;
+  const uint32_t lb = b - I.block_begin;
+  const GridDesc g = I.grid;
+  const InstConfig cfg = I.cfg;
+  const float beta = cfg.one_minus_eps, eps = 1.0f - cfg.one_minus_eps;
+  PassPose P;
+  load_pose(st, P);
+  const int lane = int(threadIdx.x & (kWave - 1));
+  const unsigned long long below = (1ull << lane) - 1ull;
+
+  // Plain f32 per-thread sums (a thread holds only ppt points); compensation
+  // happens in the block reduction.
+  float acc[kTerms];
+#pragma unroll
+  for (int t = 0; t < kTerms; ++t) acc[t] = 0.0f;
+  uint32_t count = 0;
+  bool queued = false;
+  const uint32_t ppt = I.ppt;
+  for (uint32_t k = 0; k < ppt; ++k) {
+    const uint32_t i = (lb * ppt + k) * kBlock + threadIdx.x;
+    const bool valid = i < I.n_src;
+    const float4 p = I.src[valid ? i : 0u];
+    float px, py, pz;
+    transform_point(P, p, px, py, pz);
+    NNQuery Q;
+    nn_phase1<kStats>(g, I.cell_start, I.empty_radius, I.tgt, px, py, pz, cfg.dmax, cfg.dmax2,
+                      valid ? I.match[i] : kNone, Q, valid ? stats : nullptr);
+    const bool hard = valid && Q.hard;
+    const unsigned long long hm = __ballot(hard);
+    if (hm) {
+      queued = true;
+      const int leader = __ffsll((long long)hm) - 1;
+      uint32_t base = 0;
+      if (lane == leader) base = atomicAdd(&ps.ctl[0], uint32_t(__popcll(hm)));
+      base = __shfl(base, leader);
+      if (hard) {
+        HardQuery h;
+        h.px = px;
+        h.py = py;
+        h.pz = pz;
+        h.inst = inst_id | (Q.skip_block ? kHardBlockFlag : 0u);
+        h.i = i;
+        h.pos = Q.best_pos;
+        h.key = Q.best_key;
+        ps.queue[base + uint32_t(__popcll(hm & below))] = h;
+      }
+    }
+    if (!valid || hard) continue;
+    const uint32_t j = Q.best_pos;
+    I.match[i] = j;
+    if (j == kNone) continue;
+    const float d2 = __uint_as_float(uint32_t(Q.best_key >> 32));
+    add_terms(acc, P, beta, eps, px, py, pz, I.tgt[j], I.tgt_n[j], I.src_n[i], d2);
+    ++count;
+  }
+  // A block with queued points gets its partial from k_gicp_terms.
+  const int any = __syncthreads_or(queued ? 1 : 0);
+  if (threadIdx.x == 0) ps.done[b] = any ? 0u : 1u;
+  if (any) return;
+  __shared__ RedLds L;
+  write_partial(L, acc, count, I.partials + size_t(lb) * 64);
+}
+
+template <bool kStats>
+__global__ void __launch_bounds__(kBlock)
+    __attribute__((amdgpu_waves_per_eu(SR_HARD_WAVES))) k_gicp_hard(const IterInstance* __restrict__ insts,
+                                                      PassScratch ps,
+                                                      SearchStats* __restrict__ stats) {
+  const uint32_t qlen = ps.ctl[0];
+  const int lane = int(threadIdx.x & (kWave - 1));
+  constexpr uint32_t G = kWave / kNNGroup;
+  constexpr uint32_t kWavesPerBlock = kBlock / kWave;
+  // Static split, kWave/kNNGroup queries per wave and round: the grid holds a
+  // wave per such slice of a fully queued launch, so every wave takes at most
+  // one slice (no shared cursor to contend on).
+  const uint32_t waves = gridDim.x * kWavesPerBlock;
+  const uint32_t wid = xcd_block() * kWavesPerBlock + threadIdx.x / kWave;
+  for (uint32_t base = wid * G; base < qlen; base += waves * G) {
+    const uint32_t e = base + uint32_t(lane / kNNGroup);
+    const bool active = e < qlen;  // the same on a group's lanes
+    const HardQuery h = ps.queue[active ? e : base];
+    const IterInstance& I = insts[h.inst & ~kHardBlockFlag];
+    NNQuery W;
+    W.best_key = h.key;
+    W.best_pos = h.pos;
+    W.thr = __uint_as_float(uint32_t(h.key >> 32));
+    W.rows = W.pts = 0;
+    if (active) {
+      const GridDesc g = I.grid;
+      group_search<kStats>(g, I.cell_start, I.tgt, h.px, h.py, h.pz,
+                           (h.inst & kHardBlockFlag) != 0, W);
+      if (lane % kNNGroup == 0) {
+        I.match[h.i] = W.best_pos;
+        if constexpr (kStats)
+          stats_add(stats, 2, W.rows, W.pts, W.best_pos == kNone ? kRingNone : kRingFound);
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_gicp_terms(const IterInstance* __restrict__ insts,
+                                                       const uint32_t* __restrict__ block_inst,
+                                                       PassScratch ps) {
+  const uint32_t b = xcd_block();
+  const uint32_t inst_id = block_inst[b];
+  const IterInstance& I = insts[inst_id];
+  const InstState* st = I.state;
+  if (st->phase == 2 || ps.done[b]) return;
+  const uint32_t lb = b - I.block_begin;
+  const InstConfig cfg = I.cfg;
+  const float beta = cfg.one_minus_eps, eps = 1.0f - cfg.one_minus_eps;
+  PassPose P;
+  load_pose(st, P);
+  float acc[kTerms];
+#pragma unroll
+  for (int t = 0; t < kTerms; ++t) acc[t] = 0.0f;
+  uint32_t count = 0;
+  const uint32_t ppt = I.ppt;
+  for (uint32_t k = 0; k < ppt; ++k) {
+    const uint32_t i = (lb * ppt + k) * kBlock + threadIdx.x;
+    if (i >= I.n_src) continue;
+    const uint32_t j = I.match[i];
+    if (j == kNone) continue;
+    float px, py, pz;
+    transform_point(P, I.src[i], px, py, pz);
+    const float4 q = I.tgt[j];
+    // the squared distance the search compared (same operations, same operands)
+    const float d2 = dist2(px, py, pz, q.x, q.y, q.z);
+    add_terms(acc, P, beta, eps, px, py, pz, q, I.tgt_n[j], I.src_n[i], d2);
+    ++count;
+  }
+  __shared__ RedLds L;
+  write_partial(L, acc, count, I.partials + size_t(lb) * 64);
 }
 
 // One workgroup per registration: FP64 sum of its block partials, then the
@@ -704,7 +818,9 @@ constexpr int kFinBlock = 1024;
 constexpr int kFinGroups = kFinBlock / 32;
 
 __global__ void __launch_bounds__(kFinBlock) k_gicp_finalize(const IterInstance* __restrict__ insts,
-                                                             uint32_t count) {
+                                                             uint32_t count, uint32_t* ctl) {
+  // the pass's queue is drained: empty it for the next pass
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl[0] = 0u;
   const uint32_t inst_id = blockIdx.x;
   if (inst_id >= count) return;
   const IterInstance& I = insts[inst_id];
@@ -770,22 +886,36 @@ __global__ void __launch_bounds__(kFinBlock) k_gicp_finalize(const IterInstance*
         reinterpret_cast<const unsigned long long*>(&sst)[threadIdx.x];
 }
 
+uint32_t hard_stage_blocks(uint64_t points) {
+  // a wave per kWave/kNNGroup queries of a fully queued launch, capped at what
+  // the chip holds at once (256 CUs x 8 waves per SIMD; larger queues take
+  // several rounds per wave), so an empty queue costs a small launch
+  const uint64_t per_block = uint64_t(kBlock / kWave) * (kWave / kNNGroup);
+  const uint64_t want = (points + per_block - 1) / per_block;
+  return uint32_t(std::min<uint64_t>(std::max<uint64_t>(want, 1), 2048));
+}
+
 void gicp_pass(const IterInstance* insts, const uint32_t* block_inst, uint32_t total_blocks,
-               uint32_t count, hipStream_t s, bool finalize) {
+               uint32_t count, const PassScratch& ps, uint32_t hard_blocks, hipStream_t s,
+               bool finalize) {
   if (total_blocks == 0) return;
   {
     ProfScope prof("gicp_pass", s);
     SearchStats* st = prof_stats("gicp_pass");
-    if (st)
-      hipLaunchKernelGGL(k_gicp_pass<true>, dim3(total_blocks), dim3(kBlock), 0, s, insts,
-                         block_inst, st);
-    else
-      hipLaunchKernelGGL(k_gicp_pass<false>, dim3(total_blocks), dim3(kBlock), 0, s, insts,
-                         block_inst, st);
+    if (st) {
+      hipLaunchKernelGGL(k_gicp_search<true>, dim3(total_blocks), dim3(kBlock), 0, s, insts,
+                         block_inst, ps, st);
+      hipLaunchKernelGGL(k_gicp_hard<true>, dim3(hard_blocks), dim3(kBlock), 0, s, insts, ps, st);
+    } else {
+      hipLaunchKernelGGL(k_gicp_search<false>, dim3(total_blocks), dim3(kBlock), 0, s, insts,
+                         block_inst, ps, st);
+      hipLaunchKernelGGL(k_gicp_hard<false>, dim3(hard_blocks), dim3(kBlock), 0, s, insts, ps, st);
+    }
+    hipLaunchKernelGGL(k_gicp_terms, dim3(total_blocks), dim3(kBlock), 0, s, insts, block_inst, ps);
   }
   if (!finalize) return;
   ProfScope prof("gicp_finalize", s);
-  hipLaunchKernelGGL(k_gicp_finalize, dim3(count), dim3(kFinBlock), 0, s, insts, count);
+  hipLaunchKernelGGL(k_gicp_finalize, dim3(count), dim3(kFinBlock), 0, s, insts, count, ps.ctl);
 }
 
 }  // namespace sr

@@ -242,11 +242,35 @@ constexpr int kTerms = 28;  // 21 H (upper triangle) + 6 g + fitness sum
 // of the 21 H terms in a block partial.
 __host__ __device__ constexpr int ut(int a, int b) { return a * 6 - a * (a - 1) / 2 + (b - a); }
 
+// A source point whose nearest target point phase 1 of the search could not
+// certify, queued for the ring search.
+struct HardQuery {
+  float px, py, pz;  // transformed source point
+  uint32_t inst;     // instance; bit 31 set: phase 1 scanned the 3x3x3 block
+  uint32_t i;        // source point within the instance
+  uint32_t pos;      // best grid position so far (kNone: none)
+  uint64_t key;      // best packed (d2 bits << 32 | original index) so far
+};
+
+// Scratch of a pass launch: the queue of uncertified points (room for every
+// source point of the launch), its length ctl[0] (zero when a pass starts; the
+// finalize kernel zeroes it for the next) and one flag per block (the first
+// stage wrote the block's partial).
+struct PassScratch {
+  HardQuery* queue;
+  uint32_t* ctl;
+  uint32_t* done;
+};
+
+// Workgroups of the queue-draining stage for a launch over `points` source points.
+uint32_t hard_stage_blocks(uint64_t points);
+
 // Launches one Gauss-Newton pass over `count` instances (device array `insts`),
 // total_blocks blocks; block_inst maps block -> instance. Without `finalize`
 // only the per-block partial sums are written (no solve, no state change).
 void gicp_pass(const IterInstance* insts, const uint32_t* block_inst, uint32_t total_blocks,
-               uint32_t count, hipStream_t s, bool finalize = true);
+               uint32_t count, const PassScratch& ps, uint32_t hard_blocks, hipStream_t s,
+               bool finalize = true);
 // Initialises instance states from poses (device array of 7 doubles per
 // instance: quaternion x y z w, translation).
 void init_states(InstState* states, const double* init7, uint32_t count, hipStream_t s);

@@ -136,31 +136,62 @@ __device__ __forceinline__ float axis_gap(float v, float o, float h, float m, in
 }
 
 // Visits the points of N contiguous ranges [lo[k], hi[k]) of a grid-ordered
-// array. With W = 4, four independent loads per step (indices clamped into the
-// range, the extra lanes of a step skipped) keep several requests in flight per
-// lane; W = 1 visits one point per step, for visitors too large to inline four
-// times.
+// array. W independent loads per step (indices clamped into the range, the
+// extra lanes of a step skipped) keep several requests in flight per lane;
+// W = 1 visits one point per step, for visitors too large to inline W times.
 template <int W = 4, int N, typename VisitPoint>
 __device__ __forceinline__ void stream_ranges(const uint32_t (&lo)[N], const uint32_t (&hi)[N],
                                               const float4* __restrict__ pts, VisitPoint&& visit) {
-  static_assert(W == 1 || W == 4, "stream width");
+  static_assert(W >= 1 && W <= 16 && (W & (W - 1)) == 0, "stream width");
 #pragma unroll
   for (int k = 0; k < N; ++k) {
     if (W == 1) {
       for (uint32_t s = lo[k]; s < hi[k]; ++s) visit(s, pts[s]);
       continue;
     }
-    for (uint32_t s = lo[k]; s < hi[k]; s += 4) {
+    for (uint32_t s = lo[k]; s < hi[k]; s += W) {
       const uint32_t last = hi[k] - 1;
-      const float4 q0 = pts[s];
-      const float4 q1 = pts[min(s + 1, last)];
-      const float4 q2 = pts[min(s + 2, last)];
-      const float4 q3 = pts[min(s + 3, last)];
-      visit(s, q0);
-      if (s + 1 <= last) visit(s + 1, q1);
-      if (s + 2 <= last) visit(s + 2, q2);
-      if (s + 3 <= last) visit(s + 3, q3);
+      float4 q[W];
+#pragma unroll
+      for (int w = 0; w < W; ++w) q[w] = pts[min(s + uint32_t(w), last)];
+      visit(s, q[0]);
+#pragma unroll
+      for (int w = 1; w < W; ++w)
+        if (s + uint32_t(w) <= last) visit(s + uint32_t(w), q[w]);
     }
+  }
+}
+
+// Visits the points of N contiguous ranges [lo[k], hi[k]) as one sequence:
+// each step loads W consecutive entries of the concatenation (range picked by
+// a compare chain on the running starts), so short ranges share steps and the
+// loads of several ranges are in flight together.
+template <int W, int N, typename VisitPoint>
+__device__ __forceinline__ void stream_concat(const uint32_t (&lo)[N], const uint32_t (&hi)[N],
+                                              const float4* __restrict__ pts, VisitPoint&& visit) {
+  uint32_t start[N], delta[N];
+  uint32_t total = 0;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    start[k] = total;
+    delta[k] = lo[k] - total;  // position = offset + delta (mod 2^32)
+    total += hi[k] - lo[k];
+  }
+  for (uint32_t c = 0; c < total; c += W) {
+    uint32_t pos[W];
+    float4 q[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      const uint32_t o = min(c + uint32_t(w), total - 1);
+      uint32_t d = delta[0];
+#pragma unroll
+      for (int k = 1; k < N; ++k) d = o >= start[k] ? delta[k] : d;
+      pos[w] = o + d;
+      q[w] = pts[pos[w]];
+    }
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+      if (c + uint32_t(w) < total) visit(pos[w], q[w]);
   }
 }
 
@@ -274,7 +305,7 @@ __device__ __forceinline__ void scan_block27(const GridDesc& g, const uint32_t* 
 // (y,z) gap exceeds thr is skipped, and each kept row is cut to the cells whose
 // x gap still fits (the query's own column always stays). Gaps use the filing
 // margin, so no point within sqrt(thr) is ever cut.
-template <typename VisitPoint>
+template <int W = 4, bool kConcat = false, typename VisitPoint>
 __device__ __forceinline__ void scan_block27_within(const GridDesc& g, const uint32_t* __restrict__ cs,
                                                     const float4* __restrict__ pts, float px,
                                                     float py, float pz, int hx, int hy, int hz,
@@ -303,7 +334,10 @@ __device__ __forceinline__ void scan_block27_within(const GridDesc& g, const uin
       hi[k] = cs[base + uint32_t(xb) + 1];
     }
   }
-  stream_ranges(lo, hi, pts, visit);
+  if constexpr (kConcat)
+    stream_concat<W>(lo, hi, pts, visit);
+  else
+    stream_ranges<W>(lo, hi, pts, visit);
 }
 
 // Kahan-compensated f32 accumulator.
