@@ -249,6 +249,35 @@ __device__ __forceinline__ void knn_candidates(const GridDesc& g, const uint32_t
 // closer than tau plus those at tau with the lowest indices (exactly k); (3)
 // the centred sums. Only when the k-th and (k+1)-th distances are equal does a
 // pass over the points at distance tau pick the index cut.
+//
+// A point still uncertified after kKnnLaneRings rings (an isolated point of a
+// sparse cloud, whose neighbours may lie dozens of rings away) is queued for
+// k_knn_normals_far instead, so one such point does not hold its whole wave.
+constexpr int kKnnLaneRings = 4;
+
+// Output of one point: normal in grid order; optionally the regularized
+// covariance and the normal in original order.
+__device__ __forceinline__ void write_knn_normal(uint32_t i, uint32_t orig, const float n[3],
+                                                 float b, float4* __restrict__ nrm_grid,
+                                                 float* __restrict__ cov6,
+                                                 float* __restrict__ normal3) {
+  nrm_grid[i] = make_float4(n[0], n[1], n[2], 1.0f);
+  if (cov6) {
+    float* o = cov6 + 6 * size_t(orig);
+    o[0] = 1.0f - b * n[0] * n[0];
+    o[1] = 1.0f - b * n[1] * n[1];
+    o[2] = 1.0f - b * n[2] * n[2];
+    o[3] = -b * n[0] * n[1];
+    o[4] = -b * n[0] * n[2];
+    o[5] = -b * n[1] * n[2];
+  }
+  if (normal3) {
+    normal3[3 * size_t(orig) + 0] = n[0];
+    normal3[3 * size_t(orig) + 1] = n[1];
+    normal3[3 * size_t(orig) + 2] = n[2];
+  }
+}
+
 template <int K>
 __global__ void __launch_bounds__(kBlock) k_knn_normals(GridDesc g, const uint32_t* __restrict__ cs,
                                                         const float4* __restrict__ gp, int k,
@@ -256,6 +285,8 @@ __global__ void __launch_bounds__(kBlock) k_knn_normals(GridDesc g, const uint32
                                                         float4* __restrict__ nrm_grid,
                                                         float* __restrict__ cov6,
                                                         float* __restrict__ normal3,
+                                                        uint32_t* __restrict__ far_queue,
+                                                        uint32_t* __restrict__ far_count,
                                                         SearchStats* __restrict__ stats) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= g.npoints) return;
@@ -276,12 +307,20 @@ __global__ void __launch_bounds__(kBlock) k_knn_normals(GridDesc g, const uint32
     const float cl = ring_clearance(g, p.x, p.y, p.z, hx, hy, hz, 1);
     if (!(worst < INFINITY && cl > 0.0f && worst < cl * cl)) {
       uint32_t rows = 0;
-      for (int r = 2; r <= rmax; ++r) {
+      bool certified = false;
+      for (int r = 2; r <= min(rmax, kKnnLaneRings); ++r) {
         scan_ring<1>(g, cs, gp, p.x, p.y, p.z, hx, hy, hz, r, worst, rows, net);
         rend = r;
         worst = D.at(k - 1);
         const float c = ring_clearance(g, p.x, p.y, p.z, hx, hy, hz, r);
-        if (worst < INFINITY && c > 0.0f && worst < c * c) break;
+        if (worst < INFINITY && c > 0.0f && worst < c * c) {
+          certified = true;
+          break;
+        }
+      }
+      if (!certified && rmax > kKnnLaneRings) {
+        far_queue[atomicAdd(far_count, 1u)] = i;
+        return;
       }
     }
     tau = worst;
@@ -289,7 +328,8 @@ __global__ void __launch_bounds__(kBlock) k_knn_normals(GridDesc g, const uint32
 #pragma unroll
     for (int j = 0; j < K; ++j) c_lt += (j < k && D.d[j] < tau) ? 1 : 0;
   }
-  if (stats) stats_add(stats, uint32_t(rend), 0, 0, rend == 1 ? kBlockDone : kRingFound);
+  // search statistics: rings per query (also as "cells", so the maximum is kept)
+  if (stats) stats_add(stats, uint32_t(rend), uint32_t(rend), 0, rend == 1 ? kBlockDone : kRingFound);
   uint32_t id_cut = 0xFFFFFFFFu;
   if (tau < INFINITY && next == tau) {
     // k-th and (k+1)-th tie: of the points at distance tau, take the
@@ -336,22 +376,189 @@ __global__ void __launch_bounds__(kBlock) k_knn_normals(GridDesc g, const uint32
     });
   }
   if (found >= 2) smallest_eigvec(c00, c11, c22, c01, c02, c12, n);
-  nrm_grid[i] = make_float4(n[0], n[1], n[2], 1.0f);
-  const uint32_t orig = __float_as_uint(p.w);
-  if (cov6) {
-    const float b = one_minus_eps;
-    float* o = cov6 + 6 * size_t(orig);
-    o[0] = 1.0f - b * n[0] * n[0];
-    o[1] = 1.0f - b * n[1] * n[1];
-    o[2] = 1.0f - b * n[2] * n[2];
-    o[3] = -b * n[0] * n[1];
-    o[4] = -b * n[0] * n[2];
-    o[5] = -b * n[1] * n[2];
+  write_knn_normal(i, __float_as_uint(p.w), n, one_minus_eps, nrm_grid, cov6, normal3);
+}
+
+// ---- far points: one wave per queued point --------------------------------
+//
+// The same three passes with the point's rings shared by the 64 lanes: ring
+// r's (y,z) rows (face rows whole, interior rows their two end cells) are dealt
+// round robin, each lane keeps its own sorting network, and the wave merges
+// them only where needed. After each ring the wave counts its candidates
+// closer than the ring's clearance (each lane's count is exact up to K+1): k of
+// them certify the k-th distance. The rings are cut to the smallest of the
+// lanes' k-th distances, which bounds the wave's k-th from above.
+
+// Visits this lane's share of the points filed in Chebyshev ring r (r >= 0)
+// whose cells can hold a point within sqrt(thr) of p.
+template <typename Visit>
+__device__ __forceinline__ void coop_scan_ring(const GridDesc& g, const uint32_t* __restrict__ cs,
+                                               const float4* __restrict__ gp, float px, float py,
+                                               float pz, int hx, int hy, int hz, int r, float thr,
+                                               Visit&& visit) {
+  const int lane = int(threadIdx.x & (kWave - 1));
+  const int side = 2 * r + 1;
+  for (int t = lane; t < side * side; t += kWave) {
+    const int zi = t / side, yi = t - zi * side;
+    const int cz = hz - r + zi, cy = hy - r + yi;
+    if (cz < 0 || cz >= g.dz || cy < 0 || cy >= g.dy) continue;
+    if (!((g.rows[size_t(cz) * g.wps + uint32_t(cy >> 5)] >> (cy & 31)) & 1u)) continue;
+    const float gz = axis_gap(pz, g.oz, g.h, g.margin, cz);
+    const float gy = axis_gap(py, g.oy, g.h, g.margin, cy);
+    const float gyz = gz * gz + gy * gy;
+    if (gyz > thr) continue;
+    const uint32_t cbase = (uint32_t(cz) * uint32_t(g.dy) + uint32_t(cy)) * uint32_t(g.dx);
+    uint32_t lo[2] = {0, 0}, hi[2] = {0, 0};
+    if (zi == 0 || zi == side - 1 || yi == 0 || yi == side - 1) {
+      int x0 = max(hx - r, 0), x1 = min(hx + r, g.dx - 1);
+      if (thr < INFINITY) {
+        const float rx = __builtin_amdgcn_sqrtf(thr - gyz) + g.margin;
+        x0 = max(x0, cell_coord_free(px - rx, g.ox, g.inv_h) - 1);
+        x1 = min(x1, cell_coord_free(px + rx, g.ox, g.inv_h) + 1);
+      }
+      if (x0 <= x1) {
+        lo[0] = cs[cbase + uint32_t(x0)];
+        hi[0] = cs[cbase + uint32_t(x1) + 1];
+      }
+    } else {
+      // interior row of the ring: its two end cells
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const int cx = e == 0 ? hx - r : hx + r;
+        if (cx < 0 || cx >= g.dx) continue;
+        const float gx = axis_gap(px, g.ox, g.h, g.margin, cx);
+        if (gyz + gx * gx > thr) continue;
+        lo[e] = cs[cbase + uint32_t(cx)];
+        hi[e] = cs[cbase + uint32_t(cx) + 1];
+      }
+    }
+    stream_ranges<1>(lo, hi, gp, visit);
   }
-  if (normal3) {
-    normal3[3 * size_t(orig) + 0] = n[0];
-    normal3[3 * size_t(orig) + 1] = n[1];
-    normal3[3 * size_t(orig) + 2] = n[2];
+}
+
+__device__ __forceinline__ float wave_sum_f(float v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+template <int K>
+__global__ void __launch_bounds__(kBlock) k_knn_normals_far(GridDesc g, const uint32_t* __restrict__ cs,
+                                                            const float4* __restrict__ gp, int k,
+                                                            float one_minus_eps,
+                                                            float4* __restrict__ nrm_grid,
+                                                            float* __restrict__ cov6,
+                                                            float* __restrict__ normal3,
+                                                            const uint32_t* __restrict__ far_queue,
+                                                            const uint32_t* __restrict__ far_count) {
+  const uint32_t n_far = *far_count;
+  const uint32_t waves = gridDim.x * (kBlock / kWave);
+  const int lane = int(threadIdx.x & (kWave - 1));
+  for (uint32_t w = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; w < n_far; w += waves) {
+    const uint32_t i = far_queue[w];
+    const float4 p = gp[i];
+    const int hx = cell_coord_free(p.x, g.ox, g.inv_h), hy = cell_coord_free(p.y, g.oy, g.inv_h),
+              hz = cell_coord_free(p.z, g.oz, g.inv_h);
+    const int rmax = max(max(max(hx, g.dx - 1 - hx), max(hy, g.dy - 1 - hy)),
+                         max(hz, g.dz - 1 - hz));
+    // pass 1: lane networks, rings until k candidates lie inside the clearance
+    KDist<K> D;
+    D.init();
+    auto net = [&](uint32_t, const float4& q) { D.add(dist2(q.x, q.y, q.z, p.x, p.y, p.z)); };
+    float thr = INFINITY;
+    int rend = 0;
+    for (int r = 0; r <= rmax; ++r) {
+      coop_scan_ring(g, cs, gp, p.x, p.y, p.z, hx, hy, hz, r, thr, net);
+      rend = r;
+      const float c = ring_clearance(g, p.x, p.y, p.z, hx, hy, hz, r);
+      float inside = 0.0f;
+#pragma unroll
+      for (int j = 0; j <= K; ++j) inside += (c > 0.0f && D.d[j] < c * c) ? 1.0f : 0.0f;
+      if (wave_sum_f(inside) >= float(k)) break;
+      float lane_kth = D.at(k - 1);
+#pragma unroll
+      for (int off = kWave / 2; off > 0; off >>= 1) lane_kth = fminf(lane_kth, __shfl_xor(lane_kth, off));
+      thr = lane_kth;
+    }
+    // merge the lane networks: every lane ends with the wave's K+1 smallest
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+      float other[K + 1];
+#pragma unroll
+      for (int j = 0; j <= K; ++j) other[j] = __shfl_xor(D.d[j], off);
+#pragma unroll
+      for (int j = 0; j <= K; ++j) D.add(other[j]);
+    }
+    const float tau = D.at(k - 1), next = D.at(k);
+    int c_lt = 0;
+#pragma unroll
+    for (int j = 0; j < K; ++j) c_lt += (j < k && D.d[j] < tau) ? 1 : 0;
+    uint32_t id_cut = 0xFFFFFFFFu;
+    if (tau < INFINITY && next == tau) {
+      KIds<K> T;
+      T.init();
+      for (int r = 0; r <= rend; ++r)
+        coop_scan_ring(g, cs, gp, p.x, p.y, p.z, hx, hy, hz, r, tau, [&](uint32_t, const float4& q) {
+          if (dist2(q.x, q.y, q.z, p.x, p.y, p.z) == tau) T.add(__float_as_uint(q.w));
+        });
+#pragma unroll
+      for (int off = kWave / 2; off > 0; off >>= 1) {
+        uint32_t other[K + 1];
+#pragma unroll
+        for (int j = 0; j <= K; ++j) other[j] = __shfl_xor(T.d[j], off);
+#pragma unroll
+        for (int j = 0; j <= K; ++j) T.add(other[j]);
+      }
+      id_cut = T.at(k - c_lt - 1);
+    }
+    auto selected = [&](const float4& q) {
+      const float d2 = dist2(q.x, q.y, q.z, p.x, p.y, p.z);
+      return d2 < tau || (d2 == tau && __float_as_uint(q.w) <= id_cut);
+    };
+    // pass 2: mean of the selected points, relative to the query
+    float mx = 0.0f, my = 0.0f, mz = 0.0f, cnt = 0.0f;
+    for (int r = 0; r <= rend; ++r)
+      coop_scan_ring(g, cs, gp, p.x, p.y, p.z, hx, hy, hz, r, tau, [&](uint32_t, const float4& q) {
+        if (selected(q)) {
+          mx += q.x - p.x;
+          my += q.y - p.y;
+          mz += q.z - p.z;
+          cnt += 1.0f;
+        }
+      });
+    mx = wave_sum_f(mx);
+    my = wave_sum_f(my);
+    mz = wave_sum_f(mz);
+    const int found = int(wave_sum_f(cnt));
+    float nv[3] = {1.0f, 0.0f, 0.0f};  // count < 2: identity -> eigenvector e_x (gicp.rs:268-270)
+    if (found >= 2) {
+      const float invn = 1.0f / float(found);
+      mx *= invn;
+      my *= invn;
+      mz *= invn;
+      // pass 3: centred sums
+      float c00 = 0, c11 = 0, c22 = 0, c01 = 0, c02 = 0, c12 = 0;
+      for (int r = 0; r <= rend; ++r)
+        coop_scan_ring(g, cs, gp, p.x, p.y, p.z, hx, hy, hz, r, tau, [&](uint32_t, const float4& q) {
+          if (selected(q)) {
+            const float ex = (q.x - p.x) - mx, ey = (q.y - p.y) - my, ez = (q.z - p.z) - mz;
+            c00 += ex * ex;
+            c11 += ey * ey;
+            c22 += ez * ez;
+            c01 += ex * ey;
+            c02 += ex * ez;
+            c12 += ey * ez;
+          }
+        });
+      c00 = wave_sum_f(c00);
+      c11 = wave_sum_f(c11);
+      c22 = wave_sum_f(c22);
+      c01 = wave_sum_f(c01);
+      c02 = wave_sum_f(c02);
+      c12 = wave_sum_f(c12);
+      smallest_eigvec(c00, c11, c22, c01, c02, c12, nv);
+    }
+    if (lane == 0) write_knn_normal(i, __float_as_uint(p.w), nv, one_minus_eps, nrm_grid, cov6, normal3);
   }
 }
 
@@ -359,9 +566,16 @@ template <int K>
 static void launch_knn_normals(const Grid& g, int k, float b, float4* nrm, float* cov6,
                                float* normal3, hipStream_t s) {
   const uint32_t n = g.desc.npoints;
+  DevBuf<uint32_t> far_queue(n), far_count(1);
+  fill_u32(far_count.p, 0u, 1, s);
   ProfScope prof("knn_normals", s);
   hipLaunchKernelGGL(k_knn_normals<K>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, g.desc,
-                     g.cell_start.p, g.pts.p, k, b, nrm, cov6, normal3, prof_stats("knn_normals"));
+                     g.cell_start.p, g.pts.p, k, b, nrm, cov6, normal3, far_queue.p, far_count.p,
+                     prof_stats("knn_normals"));
+  // one wave per far point, a grid the chip holds at once (few points qualify)
+  const uint32_t blocks = std::min<uint32_t>((n + kBlock / kWave - 1) / (kBlock / kWave), 1024);
+  hipLaunchKernelGGL(k_knn_normals_far<K>, dim3(blocks), dim3(kBlock), 0, s, g.desc,
+                     g.cell_start.p, g.pts.p, k, b, nrm, cov6, normal3, far_queue.p, far_count.p);
 }
 
 void knn_normals(const Grid& g, uint32_t k, double eps, float4* nrm, float* cov6, float* normal3,

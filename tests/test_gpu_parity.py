@@ -108,6 +108,29 @@ def test_knn_covariances_match_fp64_oracle(hip, orc):
     assert np.median(err) < 1e-6
 
 
+def test_knn_covariances_far_points(hip, orc):
+    """Points whose k neighbours lie many grid rings away (a sparse patch far
+    from a dense one, so the grid edge suits the dense one) take the
+    one-wave-per-point path; their covariances, and the dense patch's, match
+    the FP64 oracle."""
+    rng = np.random.default_rng(7)
+    g = np.stack(np.meshgrid(np.arange(70), np.arange(70), indexing="ij"), -1).reshape(-1, 2) * 0.14
+    patch = np.column_stack([g, 0.01 * rng.standard_normal(len(g))])
+    h = np.stack(np.meshgrid(np.arange(8), np.arange(8), indexing="ij"), -1).reshape(-1, 2) * 1.0
+    sparse = np.column_stack([30.0 + h[:, 0], 25.0 + h[:, 1], 5.0 + 0.01 * rng.standard_normal(len(h))])
+    pts = np.concatenate([patch, sparse]).astype(np.float32)
+    cov6, _ = sr.plane_covariances(pts, 20)
+    ref9, _ = orc.covariances(pts, 20, 1e-3, mode="fp64")
+    g9 = np.zeros_like(ref9)
+    g9[:, 0, 0], g9[:, 1, 1], g9[:, 2, 2] = cov6[:, 0], cov6[:, 1], cov6[:, 2]
+    g9[:, 0, 1] = g9[:, 1, 0] = cov6[:, 3]
+    g9[:, 0, 2] = g9[:, 2, 0] = cov6[:, 4]
+    g9[:, 1, 2] = g9[:, 2, 1] = cov6[:, 5]
+    err = np.abs(g9 - ref9).reshape(len(pts), 9).max(1)
+    assert err[len(patch):].max() <= 1e-5, float(err[len(patch):].max())
+    assert np.median(err) < 1e-6
+
+
 def test_radius_covariances_bit_exact(hip, orc):
     w = synth.small_cube_pair()
     pts, _, _ = orc.voxel_centroid(w.target, 0.1)
