@@ -337,7 +337,10 @@ static uint32_t pick_ppt(uint64_t points) {
 class Workers {
  public:
   static Workers& get() {
-    static Workers* w = new Workers(4);  // one per default hardware queue
+    // Eight threads: their host waits overlap each other's device work (measured
+    // against 4 and 16 on 256 small pairs); SR_PREP_WORKERS overrides.
+    static Workers* w =
+        new Workers(unsigned(std::min(64.0f, std::max(1.0f, env_float("SR_PREP_WORKERS", 8.0f)))));
     return *w;
   }
   // Runs fn(i, stream) for i in [0, n) on the workers, each worker's stream
@@ -411,6 +414,28 @@ static void prepare_many(const std::vector<const sr_xyz_view*>& views,
   Workers::get().run(views.size(), [&](size_t i, hipStream_t s) {
     Receipt local;
     prepare(*views[i], cfgs[i], out[i], local, s, is_target);
+    std::lock_guard<std::mutex> lk(mu);
+    rc.h2d += local.h2d;
+    rc.d2h += local.d2h;
+    rc.d2d += local.d2d;
+    rc.stages |= local.stages;
+  });
+}
+
+// Prepares the targets and sources of many pairs in one job, so the workers
+// never idle between the two lists.
+static void prepare_pairs(const std::vector<const sr_xyz_view*>& tviews,
+                          const std::vector<const sr_xyz_view*>& sviews,
+                          const std::vector<sr_gicp_config>& cfgs, std::vector<Prepared>& tout,
+                          std::vector<Prepared>& sout, Receipt& rc) {
+  std::mutex mu;
+  Workers::get().run(2 * tviews.size(), [&](size_t j, hipStream_t s) {
+    const size_t i = j / 2;
+    Receipt local;
+    if (j % 2 == 0)
+      prepare(*tviews[i], cfgs[i], tout[i], local, s, true);
+    else
+      prepare(*sviews[i], cfgs[i], sout[i], local, s, false);
     std::lock_guard<std::mutex> lk(mu);
     rc.h2d += local.h2d;
     rc.d2h += local.d2h;
@@ -1402,8 +1427,7 @@ int32_t sr_batch_align_pairs(const sr_xyz_view* sources, const sr_xyz_view* targ
       tp[i] = &tgts[i];
     }
     Receipt rc;
-    prepare_many(tv, cfgs, tgts, true, rc);
-    prepare_many(sv, cfgs, srcs, false, rc);
+    prepare_pairs(tv, sv, cfgs, tgts, srcs, rc);
     Batch B;
     batch_setup(sp, tp, cfgs, B, s);
     batch_run(B, s);

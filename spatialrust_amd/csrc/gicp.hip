@@ -336,7 +336,8 @@ __device__ __forceinline__ void ring_row(int r, int t, int& dy, int& dz) {
     dy = dz = 0;
     return;
   }
-  const int side = t / (2 * r), o = t - side * (2 * r);
+  const int e = 2 * r;  // rows per side; t < 4e
+  const int side = int(t >= e) + int(t >= 2 * e) + int(t >= 3 * e), o = t - side * e;
   if (side == 0) {
     dz = -r;
     dy = -r + o;
@@ -419,10 +420,11 @@ __device__ __forceinline__ void group_search(const GridDesc& g, const uint32_t* 
         lo[m] = cs[ia[m]];
         hi[m] = cs[ib[m]];
       }
-      stream_ranges<kStreamW>(lo, hi, tp, [&](uint32_t s, const float4& q) {
+      auto visit = [&](uint32_t s, const float4& q) {
         if constexpr (kStats) ++W.pts;
         nn_consider(W, qx, qy, qz, s, q);
-      });
+      };
+      stream_ranges<kStreamW>(lo, hi, tp, visit);
     }
     // the group agrees on the smallest radius before the next ring
 #pragma unroll
