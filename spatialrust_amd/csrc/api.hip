@@ -262,12 +262,15 @@ static float knn_occupancy(uint32_t k) {
   return std::max(4.0f, float(k) * 0.35f);
 }
 
-// Voxel filter (optional) + grid + plane normals of one cloud.
+// Voxel filter (optional) + grid + plane normals of one cloud. With defer_knn
+// the k-NN normals are left to the caller (knn_normals_many over many clouds);
+// out.normals is allocated either way.
 static void prepare_cloud(const Cloud& c, const sr_gicp_config& cfg, Prepared& out, Receipt& rc,
-                          hipStream_t s, bool is_target, float leaf_hint = 0.0f);
+                          hipStream_t s, bool is_target, float leaf_hint = 0.0f,
+                          bool defer_knn = false);
 
 static void prepare(const sr_xyz_view& v, const sr_gicp_config& cfg, Prepared& out, Receipt& rc,
-                    hipStream_t s, bool is_target) {
+                    hipStream_t s, bool is_target, bool defer_knn = false) {
   if (v.n > 0xFFFFFFF0ull) fail(SR_ERR_INVALID_ARGUMENT, "point clouds above 4G points are not supported");
   Cloud raw;
   pack_cloud(v.x, v.y, v.z, uint32_t(v.n), v.mem == SR_MEM_DEVICE, raw, s, &rc.h2d);
@@ -278,12 +281,13 @@ static void prepare(const sr_xyz_view& v, const sr_gicp_config& cfg, Prepared& o
     cloud = &vox.centroids;
     rc.stages |= SR_STAGE_VOXEL;
   }
-  prepare_cloud(*cloud, cfg, out, rc, s, is_target, cfg.voxel_leaf > 0.0f ? cfg.voxel_leaf : 0.0f);
+  prepare_cloud(*cloud, cfg, out, rc, s, is_target, cfg.voxel_leaf > 0.0f ? cfg.voxel_leaf : 0.0f,
+                defer_knn);
 }
 
 // Grid and covariances of a packed device cloud (no voxel filter).
 static void prepare_cloud(const Cloud& c, const sr_gicp_config& cfg, Prepared& out, Receipt& rc,
-                          hipStream_t s, bool is_target, float leaf_hint) {
+                          hipStream_t s, bool is_target, float leaf_hint, bool defer_knn) {
   const Cloud* cloud = &c;
   out.n = cloud->n;
   uint32_t hb[6];
@@ -300,7 +304,7 @@ static void prepare_cloud(const Cloud& c, const sr_gicp_config& cfg, Prepared& o
     radius_normals(rg, cfg.covariance_radius, float(cfg.epsilon), orig.p, nullptr, nullptr, s);
     gather_to_grid_order(out.grid, orig.p, out.normals.p, s);
     SR_HIP(hipStreamSynchronize(s));
-  } else {
+  } else if (!defer_knn) {
     knn_normals(out.grid, cfg.k_neighbors, cfg.epsilon, out.normals.p, nullptr, nullptr, s);
   }
   rc.stages |= SR_STAGE_COVARIANCE;
@@ -422,8 +426,11 @@ static void prepare_many(const std::vector<const sr_xyz_view*>& views,
   });
 }
 
-// Prepares the targets and sources of many pairs in one job, so the workers
-// never idle between the two lists.
+// Prepares the targets and sources of many pairs: voxel filters and grids in
+// one job on the workers (so they never idle between the two lists), then the
+// k-NN normals of all clouds in one launch per k on the caller's stream. Small
+// clouds each leave most of the chip idle while their slowest points finish;
+// in one launch those tails overlap.
 static void prepare_pairs(const std::vector<const sr_xyz_view*>& tviews,
                           const std::vector<const sr_xyz_view*>& sviews,
                           const std::vector<sr_gicp_config>& cfgs, std::vector<Prepared>& tout,
@@ -432,16 +439,29 @@ static void prepare_pairs(const std::vector<const sr_xyz_view*>& tviews,
   Workers::get().run(2 * tviews.size(), [&](size_t j, hipStream_t s) {
     const size_t i = j / 2;
     Receipt local;
+    const bool defer = !(cfgs[i].covariance_radius > 0.0f);
     if (j % 2 == 0)
-      prepare(*tviews[i], cfgs[i], tout[i], local, s, true);
+      prepare(*tviews[i], cfgs[i], tout[i], local, s, true, defer);
     else
-      prepare(*sviews[i], cfgs[i], sout[i], local, s, false);
+      prepare(*sviews[i], cfgs[i], sout[i], local, s, false, defer);
     std::lock_guard<std::mutex> lk(mu);
     rc.h2d += local.h2d;
     rc.d2h += local.d2h;
     rc.d2d += local.d2d;
     rc.stages |= local.stages;
   });
+  std::map<uint32_t, std::pair<double, std::vector<KnnCloud>>> by_k;
+  for (size_t i = 0; i < tviews.size(); ++i) {
+    if (cfgs[i].covariance_radius > 0.0f) continue;
+    auto& e = by_k[cfgs[i].k_neighbors];
+    if (e.second.empty()) e.first = cfgs[i].epsilon;
+    if (e.first != cfgs[i].epsilon) fail(SR_ERR_INVALID_ARGUMENT, "pairs of one batch must share epsilon");
+    e.second.push_back(knn_cloud(tout[i].grid, tout[i].normals.p, nullptr, nullptr));
+    e.second.push_back(knn_cloud(sout[i].grid, sout[i].normals.p, nullptr, nullptr));
+  }
+  hipStream_t s = thread_stream();
+  for (auto& kv : by_k) knn_normals_many(kv.second.second, kv.first, kv.second.first, s);
+  if (!by_k.empty()) rc.stages |= SR_STAGE_COVARIANCE;
 }
 
 // Runs the Gauss-Newton loop for `count` (source, target) pairs in one set of

@@ -230,6 +230,14 @@ struct KIds {
   }
 };
 
+// The 3x3x3 block of a k-NN pass is streamed as one sequence of its nine
+// ranges, kKnnBlockW loads in flight: one lane's chain of dependent loads is
+// then the block's points / kKnnBlockW steps instead of a few steps per range.
+#ifndef SR_KNN_BLOCK_W
+#define SR_KNN_BLOCK_W 8
+#endif
+constexpr int kKnnBlockW = SR_KNN_BLOCK_W;
+
 // The candidates of an inside query: the 3x3x3 block, then rings 2..rend cut
 // to radius^2 thr (a superset of the points within sqrt(thr)).
 template <typename Visit>
@@ -237,7 +245,7 @@ __device__ __forceinline__ void knn_candidates(const GridDesc& g, const uint32_t
                                                const float4* __restrict__ gp, float px, float py,
                                                float pz, int hx, int hy, int hz, int rend,
                                                const float& thr, Visit&& visit) {
-  scan_block27(g, cs, gp, hx, hy, hz, visit);
+  scan_block27<kKnnBlockW, true>(g, cs, gp, hx, hy, hz, visit);
   uint32_t rows = 0;
   for (int r = 2; r <= rend; ++r) scan_ring<1>(g, cs, gp, px, py, pz, hx, hy, hz, r, thr, rows, visit);
 }
@@ -278,17 +286,34 @@ __device__ __forceinline__ void write_knn_normal(uint32_t i, uint32_t orig, cons
   }
 }
 
+// The cloud of launch block b: the last cloud whose first block is <= b.
+__device__ __forceinline__ uint32_t knn_cloud_of_block(const KnnCloud* __restrict__ clouds,
+                                                       uint32_t nclouds, uint32_t b) {
+  uint32_t lo = 0, hi = nclouds;  // clouds[lo].block_begin <= b < clouds[hi].block_begin
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (clouds[mid].block_begin <= b) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 template <int K>
-__global__ void __launch_bounds__(kBlock) k_knn_normals(GridDesc g, const uint32_t* __restrict__ cs,
-                                                        const float4* __restrict__ gp, int k,
+__global__ void __launch_bounds__(kBlock) k_knn_normals(const KnnCloud* __restrict__ clouds,
+                                                        uint32_t nclouds, int k,
                                                         float one_minus_eps,
-                                                        float4* __restrict__ nrm_grid,
-                                                        float* __restrict__ cov6,
-                                                        float* __restrict__ normal3,
-                                                        uint32_t* __restrict__ far_queue,
+                                                        unsigned long long* __restrict__ far_queue,
                                                         uint32_t* __restrict__ far_count,
                                                         SearchStats* __restrict__ stats) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t c = knn_cloud_of_block(clouds, nclouds, blockIdx.x);
+  const KnnCloud& C = clouds[c];
+  const GridDesc g = C.g;
+  const uint32_t* __restrict__ cs = C.cs;
+  const float4* __restrict__ gp = C.gp;
+  float4* __restrict__ nrm_grid = C.nrm_grid;
+  float* __restrict__ cov6 = C.cov6;
+  float* __restrict__ normal3 = C.normal3;
+  const uint32_t i = (blockIdx.x - C.block_begin) * blockDim.x + threadIdx.x;
   if (i >= g.npoints) return;
   const float4 p = gp[i];
   const int hx = cell_coord_free(p.x, g.ox, g.inv_h), hy = cell_coord_free(p.y, g.oy, g.inv_h),
@@ -302,7 +327,7 @@ __global__ void __launch_bounds__(kBlock) k_knn_normals(GridDesc g, const uint32
     KDist<K> D;
     D.init();
     auto net = [&](uint32_t, const float4& q) { D.add(dist2(q.x, q.y, q.z, p.x, p.y, p.z)); };
-    scan_block27(g, cs, gp, hx, hy, hz, net);
+    scan_block27<kKnnBlockW, true>(g, cs, gp, hx, hy, hz, net);
     float worst = D.at(k - 1);
     const float cl = ring_clearance(g, p.x, p.y, p.z, hx, hy, hz, 1);
     if (!(worst < INFINITY && cl > 0.0f && worst < cl * cl)) {
@@ -319,7 +344,7 @@ __global__ void __launch_bounds__(kBlock) k_knn_normals(GridDesc g, const uint32
         }
       }
       if (!certified && rmax > kKnnLaneRings) {
-        far_queue[atomicAdd(far_count, 1u)] = i;
+        far_queue[atomicAdd(far_count, 1u)] = (uint64_t(c) << 32) | i;
         return;
       }
     }
@@ -443,19 +468,20 @@ __device__ __forceinline__ float wave_sum_f(float v) {
 }
 
 template <int K>
-__global__ void __launch_bounds__(kBlock) k_knn_normals_far(GridDesc g, const uint32_t* __restrict__ cs,
-                                                            const float4* __restrict__ gp, int k,
-                                                            float one_minus_eps,
-                                                            float4* __restrict__ nrm_grid,
-                                                            float* __restrict__ cov6,
-                                                            float* __restrict__ normal3,
-                                                            const uint32_t* __restrict__ far_queue,
+__global__ void __launch_bounds__(kBlock) k_knn_normals_far(const KnnCloud* __restrict__ clouds,
+                                                            int k, float one_minus_eps,
+                                                            const unsigned long long* __restrict__ far_queue,
                                                             const uint32_t* __restrict__ far_count) {
   const uint32_t n_far = *far_count;
   const uint32_t waves = gridDim.x * (kBlock / kWave);
   const int lane = int(threadIdx.x & (kWave - 1));
   for (uint32_t w = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; w < n_far; w += waves) {
-    const uint32_t i = far_queue[w];
+    const unsigned long long e = far_queue[w];
+    const KnnCloud& C = clouds[uint32_t(e >> 32)];
+    const GridDesc g = C.g;
+    const uint32_t* __restrict__ cs = C.cs;
+    const float4* __restrict__ gp = C.gp;
+    const uint32_t i = uint32_t(e);
     const float4 p = gp[i];
     const int hx = cell_coord_free(p.x, g.ox, g.inv_h), hy = cell_coord_free(p.y, g.oy, g.inv_h),
               hz = cell_coord_free(p.z, g.oz, g.inv_h);
@@ -558,37 +584,72 @@ __global__ void __launch_bounds__(kBlock) k_knn_normals_far(GridDesc g, const ui
       c12 = wave_sum_f(c12);
       smallest_eigvec(c00, c11, c22, c01, c02, c12, nv);
     }
-    if (lane == 0) write_knn_normal(i, __float_as_uint(p.w), nv, one_minus_eps, nrm_grid, cov6, normal3);
+    if (lane == 0)
+      write_knn_normal(i, __float_as_uint(p.w), nv, one_minus_eps, C.nrm_grid, C.cov6, C.normal3);
   }
 }
 
 template <int K>
-static void launch_knn_normals(const Grid& g, int k, float b, float4* nrm, float* cov6,
-                               float* normal3, hipStream_t s) {
-  const uint32_t n = g.desc.npoints;
-  DevBuf<uint32_t> far_queue(n), far_count(1);
+static void launch_knn_normals(const std::vector<KnnCloud>& host, int k, float b, hipStream_t s) {
+  uint64_t n = 0;
+  for (const KnnCloud& c : host) n += c.g.npoints;
+  const uint32_t blocks = host.back().block_begin +
+                          (host.back().g.npoints + kBlock - 1) / kBlock;
+  DevBuf<KnnCloud> clouds(host.size());
+  SR_HIP(hipMemcpyAsync(clouds.p, host.data(), sizeof(KnnCloud) * host.size(),
+                        hipMemcpyHostToDevice, s));
+  DevBuf<unsigned long long> far_queue(std::max<uint64_t>(n, 1));
+  DevBuf<uint32_t> far_count(1);
   fill_u32(far_count.p, 0u, 1, s);
   ProfScope prof("knn_normals", s);
-  hipLaunchKernelGGL(k_knn_normals<K>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, g.desc,
-                     g.cell_start.p, g.pts.p, k, b, nrm, cov6, normal3, far_queue.p, far_count.p,
+  hipLaunchKernelGGL(k_knn_normals<K>, dim3(blocks), dim3(kBlock), 0, s, clouds.p,
+                     uint32_t(host.size()), k, b, far_queue.p, far_count.p,
                      prof_stats("knn_normals"));
   // one wave per far point, a grid the chip holds at once (few points qualify)
-  const uint32_t blocks = std::min<uint32_t>((n + kBlock / kWave - 1) / (kBlock / kWave), 1024);
-  hipLaunchKernelGGL(k_knn_normals_far<K>, dim3(blocks), dim3(kBlock), 0, s, g.desc,
-                     g.cell_start.p, g.pts.p, k, b, nrm, cov6, normal3, far_queue.p, far_count.p);
+  const uint32_t far_blocks =
+      uint32_t(std::min<uint64_t>((n + kBlock / kWave - 1) / (kBlock / kWave), 1024));
+  hipLaunchKernelGGL(k_knn_normals_far<K>, dim3(far_blocks), dim3(kBlock), 0, s, clouds.p, k, b,
+                     far_queue.p, far_count.p);
+  // the descriptor table and the queue are stream-ordered scratch: the
+  // allocator hands them out again only to later work on this stream
+}
+
+void knn_normals_many(const std::vector<KnnCloud>& clouds_in, uint32_t k, double eps,
+                      hipStream_t s) {
+  std::vector<KnnCloud> clouds;
+  uint32_t blocks = 0;
+  for (KnnCloud c : clouds_in) {
+    if (c.g.npoints == 0) continue;
+    c.block_begin = blocks;
+    blocks += (c.g.npoints + kBlock - 1) / kBlock;
+    clouds.push_back(c);
+  }
+  if (clouds.empty()) return;
+  const float b = float(1.0 - eps);
+  if (k == 12) launch_knn_normals<12>(clouds, int(k), b, s);
+  else if (k == 15) launch_knn_normals<15>(clouds, int(k), b, s);
+  else if (k == 20) launch_knn_normals<20>(clouds, int(k), b, s);
+  else if (k <= 8) launch_knn_normals<8>(clouds, int(k), b, s);
+  else if (k <= 32) launch_knn_normals<32>(clouds, int(k), b, s);
+  else if (k <= 64) launch_knn_normals<64>(clouds, int(k), b, s);
+  else fail(3, "k_neighbors above 64 is not supported by the HIP covariance kernel");
+}
+
+KnnCloud knn_cloud(const Grid& g, float4* nrm, float* cov6, float* normal3) {
+  KnnCloud c{};
+  c.g = g.desc;
+  c.cs = g.cell_start.p;
+  c.gp = g.pts.p;
+  c.nrm_grid = nrm;
+  c.cov6 = cov6;
+  c.normal3 = normal3;
+  return c;
 }
 
 void knn_normals(const Grid& g, uint32_t k, double eps, float4* nrm, float* cov6, float* normal3,
                  hipStream_t s) {
   if (g.desc.npoints == 0) return;
-  const float b = float(1.0 - eps);
-  if (k == 12) launch_knn_normals<12>(g, int(k), b, nrm, cov6, normal3, s);
-  else if (k == 15) launch_knn_normals<15>(g, int(k), b, nrm, cov6, normal3, s);
-  else if (k == 20) launch_knn_normals<20>(g, int(k), b, nrm, cov6, normal3, s);
-  else if (k <= 8) launch_knn_normals<8>(g, int(k), b, nrm, cov6, normal3, s);
-  else if (k <= 32) launch_knn_normals<32>(g, int(k), b, nrm, cov6, normal3, s);
-  else if (k <= 64) launch_knn_normals<64>(g, int(k), b, nrm, cov6, normal3, s);
-  else fail(3, "k_neighbors above 64 is not supported by the HIP covariance kernel");
+  knn_normals_many({knn_cloud(g, nrm, cov6, normal3)}, k, eps, s);
 }
 
 // ------------------------------------------------------------ k-NN query ----

@@ -166,6 +166,21 @@ void host_bounds(const Cloud& c, uint32_t* host6, hipStream_t s);
 // regularized covariance entries per point in ORIGINAL index order.
 void knn_normals(const Grid& g, uint32_t k, double eps, float4* normals_grid_order, float* cov6,
                  float* normal3, hipStream_t s);
+// One cloud of a k-NN normals launch over many clouds (outputs as knn_normals).
+struct KnnCloud {
+  GridDesc g;
+  const uint32_t* cs;
+  const float4* gp;
+  float4* nrm_grid;
+  float* cov6;
+  float* normal3;
+  uint32_t block_begin;  // first workgroup of this cloud in the launch (set by the launcher)
+  uint32_t pad_;
+};
+KnnCloud knn_cloud(const Grid& g, float4* normals_grid_order, float* cov6, float* normal3);
+// knn_normals over many clouds in one launch (one K for all): a cloud's
+// results do not depend on the others.
+void knn_normals_many(const std::vector<KnnCloud>& clouds, uint32_t k, double eps, hipStream_t s);
 // Radius-grid covariances with COV_GRID_WGSL semantics over a grid of edge =
 // radius. normals in grid order of `g_out_order` (the GICP grid); cov6 in original order.
 void radius_normals(const Grid& rg, float radius, float eps, float4* normals_orig_order,

@@ -280,7 +280,7 @@ __device__ __forceinline__ float clearance_yz(const GridDesc& g, float py, float
 // so the block is nine contiguous point ranges; all eighteen range bounds are
 // loaded up front (independent loads) and the points are then streamed, which
 // keeps the chain of dependent memory round trips at two.
-template <typename VisitPoint>
+template <int W = 4, bool kConcat = false, typename VisitPoint>
 __device__ __forceinline__ void scan_block27(const GridDesc& g, const uint32_t* __restrict__ cs,
                                              const float4* __restrict__ pts, int hx, int hy, int hz,
                                              VisitPoint&& visit) {
@@ -297,7 +297,10 @@ __device__ __forceinline__ void scan_block27(const GridDesc& g, const uint32_t* 
       hi[k] = cs[base + uint32_t(x1) + 1];
     }
   }
-  stream_ranges(lo, hi, pts, visit);
+  if constexpr (kConcat)
+    stream_concat<W>(lo, hi, pts, visit);
+  else
+    stream_ranges<W>(lo, hi, pts, visit);
 }
 
 // scan_block27 restricted to the cells that can hold a point within sqrt(thr)
