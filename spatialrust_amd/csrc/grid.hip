@@ -167,10 +167,8 @@ __global__ void k_sort_big_cells(const uint32_t* __restrict__ start, uint32_t* _
 }
 
 // Occupancy bitmap: one thread per 32-cell word of an x-row.
-__global__ void k_grid_bits(const uint32_t* __restrict__ cs, GridDesc g, uint32_t nwords,
-                            uint32_t* __restrict__ bits) {
-  const uint32_t wi = blockIdx.x * blockDim.x + threadIdx.x;
-  if (wi >= nwords) return;
+__device__ __forceinline__ void grid_bits_word(const uint32_t* __restrict__ cs, const GridDesc& g,
+                                               uint32_t wi, uint32_t* __restrict__ bits) {
   const uint32_t row = wi / g.wpr, w = wi % g.wpr;
   const uint32_t base = row * uint32_t(g.dx);
   uint32_t m = 0;
@@ -182,12 +180,17 @@ __global__ void k_grid_bits(const uint32_t* __restrict__ cs, GridDesc g, uint32_
   bits[wi] = m;
 }
 
+__global__ void k_grid_bits(const uint32_t* __restrict__ cs, GridDesc g, uint32_t nwords,
+                            uint32_t* __restrict__ bits) {
+  const uint32_t wi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (wi < nwords) grid_bits_word(cs, g, wi, bits);
+}
+
 // Row bitmap: one thread per 32-row word of a z-slab; bit set when any cell of
 // the x-row is occupied.
-__global__ void k_grid_rowbits(const uint32_t* __restrict__ bits, GridDesc g, uint32_t nwords,
-                               uint32_t* __restrict__ rows) {
-  const uint32_t wi = blockIdx.x * blockDim.x + threadIdx.x;
-  if (wi >= nwords) return;
+__device__ __forceinline__ void grid_rowbits_word(const uint32_t* __restrict__ bits,
+                                                  const GridDesc& g, uint32_t wi,
+                                                  uint32_t* __restrict__ rows) {
   const uint32_t cz = wi / g.wps, wy = wi % g.wps;
   uint32_t m = 0;
   for (uint32_t b = 0; b < 32; ++b) {
@@ -199,6 +202,12 @@ __global__ void k_grid_rowbits(const uint32_t* __restrict__ bits, GridDesc g, ui
     if (any) m |= 1u << b;
   }
   rows[wi] = m;
+}
+
+__global__ void k_grid_rowbits(const uint32_t* __restrict__ bits, GridDesc g, uint32_t nwords,
+                               uint32_t* __restrict__ rows) {
+  const uint32_t wi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (wi < nwords) grid_rowbits_word(bits, g, wi, rows);
 }
 
 __global__ void k_grid_gather(const float4* __restrict__ pts, const uint32_t* __restrict__ order,
@@ -226,12 +235,18 @@ void gather_to_grid_order(const Grid& g, const float4* orig, float4* out, hipStr
 // Picks the cell edge from the measured mean occupancy of non-empty cells:
 // surface clouds fill cells in proportion to h^2, so one correction from a
 // first guess lands close to the requested occupancy.
-float choose_cell_edge(const Cloud& c, const uint32_t* hb, float occupancy, float leaf_hint) {
-  hipStream_t s = thread_stream();
+// The cell-edge choice in three host steps, shared by the one-cloud and the
+// many-cloud builds so both pick the same edge for the same cloud.
+// First guess; returns false when the cloud has no extent (h is then final).
+static bool edge_first_guess(const uint32_t* hb, uint32_t n, float occupancy, float leaf_hint,
+                             float& h) {
   float span[3];
   for (int a = 0; a < 3; ++a) span[a] = std::max(ord2f(hb[3 + a]) - ord2f(hb[a]), 0.0f);
   float smax = std::max(span[0], std::max(span[1], span[2]));
-  if (c.n <= 1 || !(smax > 0.0f)) return smax > 0.0f ? smax : 1.0f;
+  if (n <= 1 || !(smax > 0.0f)) {
+    h = smax > 0.0f ? smax : 1.0f;
+    return false;
+  }
   // First guess: points spread over the bounding box's largest face.
   float s2 = 1.0f;
   {
@@ -239,22 +254,43 @@ float choose_cell_edge(const Cloud& c, const uint32_t* hb, float occupancy, floa
     std::sort(a, a + 3);
     s2 = std::max(a[1] * a[2], smax * smax * 1e-6f);
   }
-  float h = std::sqrt(s2 * occupancy / float(c.n));
+  h = std::sqrt(s2 * occupancy / float(n));
   if (leaf_hint > 0.0f) {
     // One point per voxel of edge leaf_hint on surfaces: a cell of edge h
     // holds about (h / leaf)^2 of them.
     h = leaf_hint * std::sqrt(occupancy);
   }
+  return true;
+}
+
+// Grows h until the dense grid fits the cell cap.
+static float edge_fit_cap(const uint32_t* hb, float h, uint32_t n) {
+  for (int guard = 0; guard < 64; ++guard) {
+    GridDesc d;
+    grid_desc(hb, h, n, d);
+    if (uint64_t(d.dx) * uint64_t(d.dy) * uint64_t(d.dz) <= kMaxCells) break;
+    h *= 1.25f;
+  }
+  return h;
+}
+
+// One measurement: ne occupied cells at edge h. Returns true when h is kept.
+static bool edge_correct(uint32_t n, uint32_t ne, float occupancy, float& h) {
+  float occ = float(n) / float(std::max(ne, 1u));
+  float scale = std::sqrt(occupancy / occ);
+  scale = std::min(std::max(scale, 0.25f), 4.0f);
+  if (std::fabs(scale - 1.0f) < 0.15f) return true;
+  h *= scale;
+  return false;
+}
+
+float choose_cell_edge(const Cloud& c, const uint32_t* hb, float occupancy, float leaf_hint) {
+  hipStream_t s = thread_stream();
+  float h = 1.0f;
+  if (!edge_first_guess(hb, c.n, occupancy, leaf_hint, h)) return h;
   DevBuf<uint32_t> nonempty(1);
   for (int pass = 0; pass < 2; ++pass) {
-    // keep the dense grid under the cell cap
-    for (int guard = 0; guard < 64; ++guard) {
-      GridDesc d;
-      grid_desc(hb, h, c.n, d);
-      uint64_t cells = uint64_t(d.dx) * uint64_t(d.dy) * uint64_t(d.dz);
-      if (cells <= kMaxCells) break;
-      h *= 1.25f;
-    }
+    h = edge_fit_cap(hb, h, c.n);  // keep the dense grid under the cell cap
     GridDesc d;
     grid_desc(hb, h, c.n, d);
     DevBuf<uint32_t> counts(d.ncells);
@@ -265,19 +301,9 @@ float choose_cell_edge(const Cloud& c, const uint32_t* hb, float occupancy, floa
     uint32_t ne = 0;
     SR_HIP(hipMemcpyAsync(&ne, nonempty.p, 4, hipMemcpyDeviceToHost, s));
     SR_HIP(hipStreamSynchronize(s));
-    float occ = float(c.n) / float(std::max(ne, 1u));
-    float scale = std::sqrt(occupancy / occ);
-    scale = std::min(std::max(scale, 0.25f), 4.0f);
-    if (std::fabs(scale - 1.0f) < 0.15f) break;
-    h *= scale;
+    if (edge_correct(c.n, ne, occupancy, h)) break;
   }
-  for (int guard = 0; guard < 64; ++guard) {
-    GridDesc d;
-    grid_desc(hb, h, c.n, d);
-    if (uint64_t(d.dx) * uint64_t(d.dy) * uint64_t(d.dz) <= kMaxCells) break;
-    h *= 1.25f;
-  }
-  return h;
+  return edge_fit_cap(hb, h, c.n);
 }
 
 void build_grid(const Cloud& c, const uint32_t* hb, float h, Grid& g, hipStream_t s) {
@@ -336,10 +362,9 @@ __global__ void k_empty_init(const uint32_t* __restrict__ cs, uint32_t ncells,
   if (c < ncells) out[c] = cs[c + 1] > cs[c] ? 0 : uint8_t(kEmptyCap + 1);
 }
 
-__global__ void k_empty_pass(const uint8_t* __restrict__ in, uint8_t* __restrict__ out, GridDesc g,
-                             int axis) {
-  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= g.ncells) return;
+__device__ __forceinline__ void empty_pass_cell(const uint8_t* __restrict__ in,
+                                                uint8_t* __restrict__ out, const GridDesc& g,
+                                                int axis, uint32_t c) {
   const uint32_t dx = uint32_t(g.dx), dy = uint32_t(g.dy);
   const int cx = int(c % dx), cy = int((c / dx) % dy), cz = int(c / (dx * dy));
   const int coord = axis == 0 ? cx : (axis == 1 ? cy : cz);
@@ -353,6 +378,12 @@ __global__ void k_empty_pass(const uint8_t* __restrict__ in, uint8_t* __restrict
   out[c] = uint8_t(best);
 }
 
+__global__ void k_empty_pass(const uint8_t* __restrict__ in, uint8_t* __restrict__ out, GridDesc g,
+                             int axis) {
+  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < g.ncells) empty_pass_cell(in, out, g, axis, c);
+}
+
 void build_empty_radius(Grid& g, hipStream_t s) {
   const uint32_t nc = g.desc.ncells;
   g.empty_radius.resize(std::max<uint32_t>(nc, 1));
@@ -364,6 +395,279 @@ void build_empty_radius(Grid& g, hipStream_t s) {
   hipLaunchKernelGGL(k_empty_pass, grid, dim3(kBlock), 0, s, tmp.p, g.empty_radius.p, g.desc, 1);
   hipLaunchKernelGGL(k_empty_pass, grid, dim3(kBlock), 0, s, g.empty_radius.p, tmp.p, g.desc, 2);
   SR_HIP(hipMemcpyAsync(g.empty_radius.p, tmp.p, nc, hipMemcpyDeviceToDevice, s));
+}
+
+// ---- many clouds in one set of launches -----------------------------------
+//
+// Segment s is points [off[s], off[s+1]) of one cloud; each gets its own grid
+// (edge, origin, dims) and the grids share flat arrays: a segment's cells are
+// ids [cell_off[s], cell_off[s+1]) of one counting sort, whose positions equal
+// the points' (so a segment's points stay in [off[s], off[s+1]) in grid order),
+// and its cell_start (ncells + 1 entries at cs_off[s]) counts from 0 within
+// the segment, as the one-cloud build does.
+
+struct SegGrid {
+  GridDesc d;         // bits/rows already point at the segment's words
+  uint32_t pt_off;    // first point
+  uint32_t n;         // points
+  uint32_t cell_off;  // first cell id
+  uint32_t cs_off;    // first cell_start entry
+  uint32_t word_off;  // first occupancy word
+  uint32_t row_off;   // first row-bitmap word
+  uint32_t pad_[2];
+};
+
+__global__ void k_seg_grid_count(const float4* __restrict__ pts, const SegGrid* __restrict__ sg,
+                                 uint32_t nseg, const uint32_t* __restrict__ blk_begin,
+                                 uint32_t* __restrict__ counts, uint32_t* __restrict__ cid,
+                                 uint32_t* __restrict__ nonempty) {
+  const uint32_t s = seg_of(blk_begin, nseg, blockIdx.x);
+  const SegGrid& G = sg[s];
+  const uint32_t il = (blockIdx.x - blk_begin[s]) * blockDim.x + threadIdx.x;
+  bool first = false;
+  if (il < G.n) {
+    const GridDesc& g = G.d;
+    const float4 p = pts[G.pt_off + il];
+    int cx = cell_coord(p.x, g.ox, g.inv_h, g.dx);
+    int cy = cell_coord(p.y, g.oy, g.inv_h, g.dy);
+    int cz = cell_coord(p.z, g.oz, g.inv_h, g.dz);
+    uint32_t c = G.cell_off + (uint32_t(cz) * uint32_t(g.dy) + uint32_t(cy)) * uint32_t(g.dx) + uint32_t(cx);
+    uint32_t old = atomicAdd(&counts[c], 1u);
+    if (cid) cid[G.pt_off + il] = c;
+    first = old == 0;
+  }
+  if (nonempty) {
+    const unsigned long long m = __ballot(first);
+    if ((threadIdx.x & (kWave - 1)) == 0 && m) atomicAdd(&nonempty[s], uint32_t(__popcll(m)));
+  }
+}
+
+// Per segment: cell_start entries from the flat exclusive scan, minus the
+// segment's first point.
+__global__ void k_seg_local_cs(const uint32_t* __restrict__ csg, const SegGrid* __restrict__ sg,
+                               uint32_t nseg, const uint32_t* __restrict__ cs_begin, uint32_t total,
+                               uint32_t* __restrict__ cs_local) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const uint32_t s = seg_of(cs_begin, nseg, t);
+  const SegGrid& G = sg[s];
+  cs_local[t] = csg[G.cell_off + (t - G.cs_off)] - G.pt_off;
+}
+
+__global__ void k_seg_grid_bits(const uint32_t* __restrict__ cs_local, const SegGrid* __restrict__ sg,
+                                uint32_t nseg, const uint32_t* __restrict__ word_begin,
+                                uint32_t total, uint32_t* __restrict__ occ) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const uint32_t s = seg_of(word_begin, nseg, t);
+  const SegGrid& G = sg[s];
+  grid_bits_word(cs_local + G.cs_off, G.d, t - G.word_off, occ + G.word_off);
+}
+
+__global__ void k_seg_grid_rowbits(const uint32_t* __restrict__ occ, const SegGrid* __restrict__ sg,
+                                   uint32_t nseg, const uint32_t* __restrict__ row_begin,
+                                   uint32_t total, uint32_t* __restrict__ rows) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const uint32_t s = seg_of(row_begin, nseg, t);
+  const SegGrid& G = sg[s];
+  grid_rowbits_word(occ + G.word_off, G.d, t - G.row_off, rows + G.row_off);
+}
+
+__global__ void k_seg_grid_gather(const float4* __restrict__ pts, const uint32_t* __restrict__ order,
+                                  const uint32_t* __restrict__ pt_begin, uint32_t nseg, uint32_t n,
+                                  float4* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t s = seg_of(pt_begin, nseg, i);
+  const uint32_t o = order[i];
+  const float4 p = pts[o];
+  out[i] = make_float4(p.x, p.y, p.z, __uint_as_float(o - pt_begin[s]));
+}
+
+__global__ void k_seg_empty_init(const uint32_t* __restrict__ cs_local, const SegGrid* __restrict__ sg,
+                                 uint32_t nseg, const uint32_t* __restrict__ cell_begin,
+                                 uint32_t total, uint8_t* __restrict__ out) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const uint32_t s = seg_of(cell_begin, nseg, t);
+  const SegGrid& G = sg[s];
+  const uint32_t* cs = cs_local + G.cs_off;
+  const uint32_t c = t - G.cell_off;
+  out[t] = cs[c + 1] > cs[c] ? 0 : uint8_t(kEmptyCap + 1);
+}
+
+__global__ void k_seg_empty_pass(const uint8_t* __restrict__ in, uint8_t* __restrict__ out,
+                                 const SegGrid* __restrict__ sg, uint32_t nseg,
+                                 const uint32_t* __restrict__ cell_begin, uint32_t total, int axis) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const uint32_t s = seg_of(cell_begin, nseg, t);
+  const SegGrid& G = sg[s];
+  empty_pass_cell(in + G.cell_off, out + G.cell_off, G.d, axis, t - G.cell_off);
+}
+
+template <typename T>
+static void upload_vec(DevBuf<T>& d, const std::vector<T>& h, hipStream_t s) {
+  d.resize(std::max<size_t>(h.size(), 1));
+  if (!h.empty())
+    SR_HIP(hipMemcpyAsync(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s));
+}
+
+static std::vector<uint32_t> blocks_of(const std::vector<uint32_t>& off) {
+  std::vector<uint32_t> b(off.size());
+  b[0] = 0;
+  for (size_t s = 0; s + 1 < off.size(); ++s)
+    b[s + 1] = b[s] + std::max<uint32_t>(1, (off[s + 1] - off[s] + kBlock - 1) / kBlock);
+  return b;
+}
+
+void choose_cell_edges_many(const Cloud& c, const std::vector<uint32_t>& off,
+                            const std::vector<uint32_t>& hb, float occupancy, float leaf_hint,
+                            std::vector<float>& h, hipStream_t s) {
+  const uint32_t nseg = uint32_t(off.size() - 1);
+  h.assign(nseg, 1.0f);
+  std::vector<char> open(nseg, 0);
+  for (uint32_t g = 0; g < nseg; ++g)
+    open[g] = edge_first_guess(&hb[6 * g], off[g + 1] - off[g], occupancy, leaf_hint, h[g]) ? 1 : 0;
+  for (int pass = 0; pass < 2; ++pass) {
+    // one trial count over the segments still open, as choose_cell_edge's
+    std::vector<uint32_t> seg_ids, sub_off(1, 0);
+    std::vector<SegGrid> sg;
+    uint64_t cells = 0;
+    for (uint32_t g = 0; g < nseg; ++g) {
+      if (!open[g]) continue;
+      h[g] = edge_fit_cap(&hb[6 * g], h[g], off[g + 1] - off[g]);
+      SegGrid G{};
+      grid_desc(&hb[6 * g], h[g], off[g + 1] - off[g], G.d);
+      G.pt_off = off[g];
+      G.n = off[g + 1] - off[g];
+      G.cell_off = uint32_t(cells);
+      cells += G.d.ncells;
+      if (cells > 0xFFFFFFFFull) fail(3, "too many grid cells for one batch");
+      sg.push_back(G);
+      seg_ids.push_back(g);
+      sub_off.push_back(sub_off.back() + G.n);
+    }
+    if (sg.empty()) break;
+    const uint32_t m = uint32_t(sg.size());
+    std::vector<uint32_t> blk = blocks_of(sub_off);
+    DevBuf<SegGrid> d_sg;
+    DevBuf<uint32_t> d_blk, counts(std::max<uint64_t>(cells, 1)), nonempty(m);
+    upload_vec(d_sg, sg, s);
+    upload_vec(d_blk, blk, s);
+    fill_u32(counts.p, 0u, cells, s);
+    fill_u32(nonempty.p, 0u, m, s);
+    hipLaunchKernelGGL(k_seg_grid_count, dim3(blk.back()), dim3(kBlock), 0, s, c.pts.p, d_sg.p, m,
+                       d_blk.p, counts.p, (uint32_t*)nullptr, nonempty.p);
+    std::vector<uint32_t> ne(m);
+    SR_HIP(hipMemcpyAsync(ne.data(), nonempty.p, sizeof(uint32_t) * m, hipMemcpyDeviceToHost, s));
+    SR_HIP(hipStreamSynchronize(s));
+    for (uint32_t j = 0; j < m; ++j) {
+      const uint32_t g = seg_ids[j];
+      if (edge_correct(off[g + 1] - off[g], ne[j], occupancy, h[g])) open[g] = 0;
+    }
+  }
+  for (uint32_t g = 0; g < nseg; ++g) h[g] = edge_fit_cap(&hb[6 * g], h[g], off[g + 1] - off[g]);
+}
+
+void build_grids_many(const Cloud& c, const std::vector<uint32_t>& off,
+                      const std::vector<uint32_t>& hb, const std::vector<float>& h, bool empty,
+                      GridSet& out, hipStream_t s) {
+  const uint32_t nseg = uint32_t(off.size() - 1), n = off.back();
+  out.desc.assign(nseg, GridDesc{});
+  out.pt_off = off;
+  out.cell_off.assign(nseg + 1, 0);
+  out.cs_off.assign(nseg, 0);
+  std::vector<SegGrid> sg(nseg);
+  std::vector<uint32_t> word_begin(nseg), row_begin(nseg), cs_begin(nseg);
+  uint64_t cells = 0, words = 0, rws = 0;
+  for (uint32_t g = 0; g < nseg; ++g) {
+    SegGrid& G = sg[g];
+    grid_desc(&hb[6 * g], h[g], off[g + 1] - off[g], G.d);
+    const uint64_t nc = uint64_t(G.d.dx) * uint64_t(G.d.dy) * uint64_t(G.d.dz);
+    if (nc > kMaxCells)
+      fail(3, "grid would need " + std::to_string(nc) + " cells (cap " + std::to_string(kMaxCells) +
+                  "); use a larger radius or the CPU path");
+    G.pt_off = off[g];
+    G.n = off[g + 1] - off[g];
+    G.cell_off = uint32_t(cells);
+    G.cs_off = uint32_t(cells + g);
+    G.word_off = uint32_t(words);
+    G.row_off = uint32_t(rws);
+    out.cell_off[g] = G.cell_off;
+    out.cs_off[g] = G.cs_off;
+    word_begin[g] = G.word_off;
+    row_begin[g] = G.row_off;
+    cs_begin[g] = G.cs_off;
+    cells += G.d.ncells;
+    words += uint64_t(G.d.wpr) * uint64_t(G.d.dy) * uint64_t(G.d.dz);
+    rws += uint64_t(G.d.wps) * uint64_t(G.d.dz);
+    if (cells + nseg > 0xFFFFFFFFull || words > 0xFFFFFFFFull)
+      fail(3, "too many grid cells for one batch");
+  }
+  out.cell_off[nseg] = uint32_t(cells);
+  const uint32_t ncells = uint32_t(cells), ncs = uint32_t(cells + nseg);
+  out.cell_start.resize(std::max<uint32_t>(ncs, 1));
+  out.pts.resize(std::max<uint32_t>(n, 1));
+  out.occ.resize(std::max<uint64_t>(words, 1));
+  out.rowocc.resize(std::max<uint64_t>(rws, 1));
+  for (uint32_t g = 0; g < nseg; ++g) {
+    sg[g].d.bits = out.occ.p + sg[g].word_off;
+    sg[g].d.rows = out.rowocc.p + sg[g].row_off;
+    out.desc[g] = sg[g].d;
+  }
+  std::vector<uint32_t> blk = blocks_of(off);
+  DevBuf<SegGrid> d_sg;
+  DevBuf<uint32_t> d_blk, d_word, d_row, d_cs, d_cell, d_pt;
+  upload_vec(d_sg, sg, s);
+  upload_vec(d_blk, blk, s);
+  upload_vec(d_word, word_begin, s);
+  upload_vec(d_row, row_begin, s);
+  upload_vec(d_cs, cs_begin, s);
+  std::vector<uint32_t> cell_begin(out.cell_off.begin(), out.cell_off.end() - 1);
+  upload_vec(d_cell, cell_begin, s);
+  upload_vec(d_pt, off, s);
+  DevBuf<uint32_t> counts(size_t(ncells) + 1), csg(size_t(ncells) + 1), cid(std::max<uint32_t>(n, 1)),
+      cursor(std::max<uint32_t>(ncells, 1)), order(std::max<uint32_t>(n, 1)), scal(2);
+  fill_u32(counts.p, 0u, size_t(ncells) + 1, s);
+  fill_u32(cursor.p, 0u, ncells, s);
+  hipLaunchKernelGGL(k_seg_grid_count, dim3(blk.back()), dim3(kBlock), 0, s, c.pts.p, d_sg.p, nseg,
+                     d_blk.p, counts.p, cid.p, (uint32_t*)nullptr);
+  exclusive_scan_u32(counts.p, csg.p, ncells + 1, nullptr, s);
+  auto grid1 = [](uint64_t t) { return dim3(uint32_t(std::max<uint64_t>((t + kBlock - 1) / kBlock, 1))); };
+  hipLaunchKernelGGL(k_seg_local_cs, grid1(ncs), dim3(kBlock), 0, s, csg.p, d_sg.p, nseg, d_cs.p, ncs,
+                     out.cell_start.p);
+  if (words)
+    hipLaunchKernelGGL(k_seg_grid_bits, grid1(words), dim3(kBlock), 0, s, out.cell_start.p, d_sg.p,
+                       nseg, d_word.p, uint32_t(words), out.occ.p);
+  if (rws)
+    hipLaunchKernelGGL(k_seg_grid_rowbits, grid1(rws), dim3(kBlock), 0, s, out.occ.p, d_sg.p, nseg,
+                       d_row.p, uint32_t(rws), out.rowocc.p);
+  if (n) {
+    hipLaunchKernelGGL(k_grid_scatter, grid1(n), dim3(kBlock), 0, s, cid.p, n, csg.p, cursor.p, order.p);
+    fill_u32(scal.p, 0u, 1, s);
+    DevBuf<uint32_t> big(std::max<uint32_t>(ncells, 1));
+    hipLaunchKernelGGL(k_sort_cells, grid1(ncells), dim3(kBlock), 0, s, csg.p, ncells, order.p,
+                       big.p, scal.p);
+    hipLaunchKernelGGL(k_sort_big_cells, dim3(128), dim3(kBlock), 0, s, csg.p, order.p, big.p, scal.p);
+    hipLaunchKernelGGL(k_seg_grid_gather, grid1(n), dim3(kBlock), 0, s, c.pts.p, order.p, d_pt.p, nseg,
+                       n, out.pts.p);
+  }
+  if (empty && ncells) {
+    out.empty_radius.resize(ncells);
+    DevBuf<uint8_t> tmp(ncells);
+    hipLaunchKernelGGL(k_seg_empty_init, grid1(ncells), dim3(kBlock), 0, s, out.cell_start.p, d_sg.p,
+                       nseg, d_cell.p, ncells, out.empty_radius.p);
+    for (int axis = 0; axis < 3; ++axis) {
+      const uint8_t* in = axis == 1 ? tmp.p : out.empty_radius.p;
+      uint8_t* o = axis == 1 ? out.empty_radius.p : tmp.p;
+      hipLaunchKernelGGL(k_seg_empty_pass, grid1(ncells), dim3(kBlock), 0, s, in, o, d_sg.p, nseg,
+                         d_cell.p, ncells, axis);
+    }
+    SR_HIP(hipMemcpyAsync(out.empty_radius.p, tmp.p, ncells, hipMemcpyDeviceToDevice, s));
+  }
+  SR_HIP(hipStreamSynchronize(s));  // the tables above are released on return
 }
 
 }  // namespace sr

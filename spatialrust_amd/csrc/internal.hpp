@@ -40,22 +40,37 @@ template <typename T>
 struct DevBuf {
   T* p = nullptr;
   size_t n = 0;
+  bool owned = true;  // false: a view into memory another buffer owns
   DevBuf() = default;
   explicit DevBuf(size_t count) { resize(count); }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), owned(o.owned) {
+    o.p = nullptr;
+    o.n = 0;
+    o.owned = true;
+  }
   DevBuf& operator=(DevBuf&& o) noexcept {
     if (this != &o) {
       release();
       p = o.p;
       n = o.n;
+      owned = o.owned;
       o.p = nullptr;
       o.n = 0;
+      o.owned = true;
     }
     return *this;
   }
   ~DevBuf() { release(); }
+  // A non-owning view of count elements at ptr (the owner must outlive it).
+  static DevBuf view(T* ptr, size_t count) {
+    DevBuf b;
+    b.p = ptr;
+    b.n = count;
+    b.owned = false;
+    return b;
+  }
   void resize(size_t count) {
     if (count <= n && p) return;
     release();
@@ -63,9 +78,10 @@ struct DevBuf {
     n = count;
   }
   void release() {
-    if (p) dev_free(p);
+    if (p && owned) dev_free(p);
     p = nullptr;
     n = 0;
+    owned = true;
   }
   size_t bytes() const { return n * sizeof(T); }
 };
@@ -126,6 +142,27 @@ struct VoxelOut {
   uint32_t cells = 0;
 };
 
+// Source of one segment of pack_many: f32 columns in device memory.
+struct SegSource {
+  const float* x;
+  const float* y;
+  const float* z;
+  uint32_t n;
+  uint32_t offset;  // first point in the packed cloud
+};
+// Packs many device xyz columns into one float4 cloud, segment after segment.
+void pack_many(const std::vector<SegSource>& srcs, Cloud& out, hipStream_t s);
+// Bounds (6 ordered words per segment) of segments [off[s], off[s+1]) of pts;
+// synchronises the stream.
+void seg_bounds(const float4* pts, const std::vector<uint32_t>& off, uint32_t* dev_bounds,
+                hipStream_t s);
+// voxel_downsample of every segment [off[s], off[s+1]) of `in` (origin = the
+// segment's minimum), in one set of launches. The segments' centroids are
+// out.centroids[cell_off[s], cell_off[s+1]), each exactly what voxel_downsample
+// returns for the segment alone. Synchronises the stream.
+void voxel_downsample_many(const Cloud& in, const std::vector<uint32_t>& off, float leaf,
+                           VoxelOut& out, std::vector<uint32_t>& cell_off, hipStream_t s);
+
 // Centroid voxel filter of a device cloud (voxel.rs:550-636 semantics).
 void voxel_downsample(const Cloud& in, float leaf, const float* origin, bool want_keys,
                       VoxelOut& out, hipStream_t s);
@@ -147,6 +184,31 @@ constexpr int kEmptyCap = 14;
 
 // Fills g.empty_radius (Chebyshev distance transform of cell occupancy, capped).
 void build_empty_radius(Grid& g, hipStream_t s);
+
+// Grids of many segments of one cloud (segment s = points [pt_off[s],
+// pt_off[s+1])), each the grid build_grid makes for the segment alone, in
+// shared flat arrays: segment s's cell_start is cell_start[cs_off[s] ..] with
+// values counting from 0 within it, its points are pts[pt_off[s] ..] (.w = the
+// index within the segment), its empty radius is empty_radius[cell_off[s] ..],
+// and desc[s].bits / .rows point at its bitmap words.
+struct GridSet {
+  std::vector<GridDesc> desc;
+  std::vector<uint32_t> pt_off, cell_off, cs_off;
+  DevBuf<uint32_t> cell_start;
+  DevBuf<float4> pts;
+  DevBuf<uint32_t> occ, rowocc;
+  DevBuf<uint8_t> empty_radius;
+};
+// choose_cell_edge for every segment (hb: 6 bound words per segment), with the
+// trial counts of all segments in one launch per round; same edges.
+void choose_cell_edges_many(const Cloud& c, const std::vector<uint32_t>& off,
+                            const std::vector<uint32_t>& hb, float occupancy, float leaf_hint,
+                            std::vector<float>& h, hipStream_t s);
+// build_grid (and build_empty_radius when `empty`) for every segment, edge h[s].
+// Synchronises the stream.
+void build_grids_many(const Cloud& c, const std::vector<uint32_t>& off,
+                      const std::vector<uint32_t>& hb, const std::vector<float>& h, bool empty,
+                      GridSet& out, hipStream_t s);
 
 // Chooses a cell edge so that occupied cells hold about `occupancy` points
 // (measured, starting from a guess; leaf_hint > 0 says the cloud holds one

@@ -50,6 +50,19 @@ struct GridDesc {
   uint32_t pad2_;
 };
 
+// Segment of x in a launch over several segments: the last s < nseg with
+// begins[s] <= x (begins ascending, begins[0] = 0).
+__device__ __forceinline__ uint32_t seg_of(const uint32_t* __restrict__ begins, uint32_t nseg,
+                                           uint32_t x) {
+  uint32_t lo = 0, hi = nseg;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (begins[mid] <= x) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 // Order-preserving map of f32 onto u32 so atomicMin/atomicMax work on floats.
 __device__ __forceinline__ uint32_t f2ord(float f) {
   uint32_t u = __float_as_uint(f);

@@ -263,10 +263,10 @@ __device__ __forceinline__ void voxel_key(const VoxelParams& P, float x, float y
   }
 }
 
-// One thread: origin, key mode and key ranges from the bounds.
-__global__ void k_voxel_params(const uint32_t* __restrict__ b, float leaf_inv, int explicit_origin,
-                               float o0, float o1, float o2, VoxelParams* P) {
-  if (threadIdx.x != 0) return;
+// Origin, key mode and key ranges from the bounds.
+__device__ void voxel_params_from_bounds(const uint32_t* __restrict__ b, float leaf_inv,
+                                         int explicit_origin, float o0, float o1, float o2,
+                                         VoxelParams* P) {
   float mn[3] = {ord2f(b[0]), ord2f(b[1]), ord2f(b[2])};
   float mx[3] = {ord2f(b[3]), ord2f(b[4]), ord2f(b[5])};
   VoxelParams p;
@@ -299,6 +299,11 @@ __global__ void k_voxel_params(const uint32_t* __restrict__ b, float leaf_inv, i
   *P = p;
 }
 
+__global__ void k_voxel_params(const uint32_t* __restrict__ b, float leaf_inv, int explicit_origin,
+                               float o0, float o1, float o2, VoxelParams* P) {
+  if (threadIdx.x == 0) voxel_params_from_bounds(b, leaf_inv, explicit_origin, o0, o1, o2, P);
+}
+
 __device__ __forceinline__ uint32_t hash64(unsigned long long k, uint32_t mask) {
   k ^= k >> 33;
   k *= 0xff51afd7ed558ccdull;
@@ -310,6 +315,14 @@ __device__ __forceinline__ uint32_t hash64(unsigned long long k, uint32_t mask) 
 
 constexpr unsigned long long kEmptyKey = ~0ull;
 
+__device__ __forceinline__ unsigned long long linear_voxel_key(const VoxelParams& P, const float4& p) {
+  int64_t k[3];
+  voxel_key(P, p.x, p.y, p.z, k);
+  return ((unsigned long long)(k[2] - P.kmin[2]) * P.dims[1] +
+          (unsigned long long)(k[1] - P.kmin[1])) * P.dims[0] +
+         (unsigned long long)(k[0] - P.kmin[0]);
+}
+
 __global__ void k_voxel_insert(const float4* __restrict__ pts, uint32_t n,
                                const VoxelParams* __restrict__ Pp,
                                unsigned long long* __restrict__ tkeys, uint32_t* __restrict__ tfirst,
@@ -320,12 +333,7 @@ __global__ void k_voxel_insert(const float4* __restrict__ pts, uint32_t n,
   const VoxelParams P = *Pp;
   if (P.error) return;
   float4 p = pts[i];
-  int64_t k[3];
-  voxel_key(P, p.x, p.y, p.z, k);
-  unsigned long long lin =
-      ((unsigned long long)(k[2] - P.kmin[2]) * P.dims[1] + (unsigned long long)(k[1] - P.kmin[1])) *
-          P.dims[0] +
-      (unsigned long long)(k[0] - P.kmin[0]);
+  const unsigned long long lin = linear_voxel_key(P, p);
   uint32_t h = hash64(lin, mask);
   while (true) {
     unsigned long long prev = atomicCAS(&tkeys[h], kEmptyKey, lin);
@@ -524,6 +532,243 @@ void voxel_downsample(const Cloud& in, float leaf, const float* origin, bool wan
   if (host[2]) fail(3, "voxel key range does not fit a 64-bit linear key; use a larger leaf_size");
   out.cells = host[0];
   out.centroids.n = host[0];
+  out.counts = std::move(ocount);
+}
+
+// ---- many clouds in one set of launches -----------------------------------
+//
+// The clouds are concatenated (segment s = points [off[s], off[s+1])). Every
+// per-point kernel finds its segment from its workgroup (a segment's points
+// start a fresh workgroup), so a segment's results are those of a launch over
+// it alone: each has its own bounds, key parameters and hash-table region, and
+// the global first-occurrence order keeps the segments apart and each in its
+// own first-occurrence order.
+
+constexpr uint32_t kSegBoundsChunk = kBlock * 8;  // points per workgroup of k_seg_bounds
+
+__global__ void k_seg_pack(const SegSource* __restrict__ src, uint32_t nseg,
+                           const uint32_t* __restrict__ blk_begin, float4* __restrict__ out) {
+  const uint32_t s = seg_of(blk_begin, nseg, blockIdx.x);
+  const SegSource S = src[s];
+  const uint32_t i = (blockIdx.x - blk_begin[s]) * blockDim.x + threadIdx.x;
+  if (i < S.n) out[S.offset + i] = make_float4(S.x[i], S.y[i], S.z[i], 0.0f);
+}
+
+__global__ void k_seg_bounds_init(uint32_t* __restrict__ b, uint32_t nseg) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < 6 * nseg) b[t] = (t % 6) < 3 ? 0xFF800000u : 0x007FFFFFu;  // f2ord(+inf) / f2ord(-inf)
+}
+
+// k_bounds per segment: workgroup w of segment s reduces points
+// [off[s] + w*kSegBoundsChunk, ...) of it.
+__global__ void k_seg_bounds(const float4* __restrict__ pts, const uint32_t* __restrict__ off,
+                             uint32_t nseg, const uint32_t* __restrict__ blk_begin,
+                             uint32_t* __restrict__ b) {
+  const uint32_t s = seg_of(blk_begin, nseg, blockIdx.x);
+  const uint32_t lo = off[s] + (blockIdx.x - blk_begin[s]) * kSegBoundsChunk;
+  const uint32_t hi = min(off[s + 1], lo + kSegBoundsChunk);
+  float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    float4 p = pts[i];
+    mn[0] = fminf(mn[0], p.x);
+    mn[1] = fminf(mn[1], p.y);
+    mn[2] = fminf(mn[2], p.z);
+    mx[0] = fmaxf(mx[0], p.x);
+    mx[1] = fmaxf(mx[1], p.y);
+    mx[2] = fmaxf(mx[2], p.z);
+  }
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    for (int a = 0; a < 3; ++a) {
+      mn[a] = fminf(mn[a], __shfl_xor(mn[a], o));
+      mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], o));
+    }
+  }
+  __shared__ float smn[kBlock / kWave][3], smx[kBlock / kWave][3];
+  const int wid = threadIdx.x / kWave;
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    for (int a = 0; a < 3; ++a) {
+      smn[wid][a] = mn[a];
+      smx[wid][a] = mx[a];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int a = threadIdx.x;
+    float l = smn[0][a], h = smx[0][a];
+    for (int w = 1; w < kBlock / kWave; ++w) {
+      l = fminf(l, smn[w][a]);
+      h = fmaxf(h, smx[w][a]);
+    }
+    if (l <= h) {
+      atomicMin(&b[6 * s + a], f2ord(l));
+      atomicMax(&b[6 * s + 3 + a], f2ord(h));
+    }
+  }
+}
+
+__global__ void k_seg_voxel_params(const uint32_t* __restrict__ b, uint32_t nseg, float leaf_inv,
+                                   VoxelParams* __restrict__ P) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nseg) voxel_params_from_bounds(b + 6 * s, leaf_inv, 0, 0.0f, 0.0f, 0.0f, P + s);
+}
+
+struct SegTable {
+  uint32_t base;  // first slot of the segment's hash-table region
+  uint32_t mask;  // region size - 1 (a power of two)
+};
+
+__global__ void k_seg_voxel_insert(const float4* __restrict__ pts, const uint32_t* __restrict__ off,
+                                   uint32_t nseg, const uint32_t* __restrict__ blk_begin,
+                                   const VoxelParams* __restrict__ Pp,
+                                   const SegTable* __restrict__ tab,
+                                   unsigned long long* __restrict__ tkeys,
+                                   uint32_t* __restrict__ tfirst, uint32_t* __restrict__ tcount,
+                                   uint32_t* __restrict__ slot_of) {
+  const uint32_t s = seg_of(blk_begin, nseg, blockIdx.x);
+  const uint32_t i = off[s] + (blockIdx.x - blk_begin[s]) * blockDim.x + threadIdx.x;
+  if (i >= off[s + 1]) return;
+  const VoxelParams P = Pp[s];
+  if (P.error) return;
+  const SegTable T = tab[s];
+  const unsigned long long lin = linear_voxel_key(P, pts[i]);
+  uint32_t h = hash64(lin, T.mask);
+  while (true) {
+    unsigned long long prev = atomicCAS(&tkeys[T.base + h], kEmptyKey, lin);
+    if (prev == kEmptyKey || prev == lin) break;
+    h = (h + 1) & T.mask;
+  }
+  atomicMin(&tfirst[T.base + h], i);
+  atomicAdd(&tcount[T.base + h], 1u);
+  slot_of[i] = T.base + h;
+}
+
+// Per segment: its first output cell (the exclusive rank of its first point),
+// and a key-range error of any segment.
+__global__ void k_seg_voxel_tail(const uint32_t* __restrict__ rank, const uint32_t* __restrict__ off,
+                                 uint32_t nseg, const VoxelParams* __restrict__ P,
+                                 const uint32_t* __restrict__ total, uint32_t* __restrict__ out) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nseg) {
+    out[s] = off[s] < off[nseg] ? rank[off[s]] : *total;
+    if (P[s].error) atomicOr(&out[nseg + 1], 1u);
+  }
+  if (s == 0) out[nseg] = *total;
+}
+
+// Workgroups per segment, as a launch's first-workgroup table.
+static std::vector<uint32_t> seg_blocks(const std::vector<uint32_t>& off, uint32_t per_block) {
+  std::vector<uint32_t> b(off.size());
+  b[0] = 0;
+  for (size_t s = 0; s + 1 < off.size(); ++s)
+    b[s + 1] = b[s] + std::max<uint32_t>(1, (off[s + 1] - off[s] + per_block - 1) / per_block);
+  return b;
+}
+
+template <typename T>
+static void upload(DevBuf<T>& d, const std::vector<T>& h, hipStream_t s) {
+  d.resize(std::max<size_t>(h.size(), 1));
+  if (!h.empty())
+    SR_HIP(hipMemcpyAsync(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s));
+}
+
+void pack_many(const std::vector<SegSource>& srcs, Cloud& out, hipStream_t s) {
+  std::vector<uint32_t> off(srcs.size() + 1, 0);
+  for (size_t i = 0; i < srcs.size(); ++i) off[i + 1] = off[i] + srcs[i].n;
+  out.n = off.back();
+  out.pts.resize(std::max<uint32_t>(out.n, 1));
+  if (srcs.empty() || out.n == 0) return;
+  std::vector<uint32_t> blk = seg_blocks(off, kBlock);
+  DevBuf<SegSource> d_src;
+  DevBuf<uint32_t> d_blk;
+  upload(d_src, srcs, s);
+  upload(d_blk, blk, s);
+  hipLaunchKernelGGL(k_seg_pack, dim3(blk.back()), dim3(kBlock), 0, s, d_src.p,
+                     uint32_t(srcs.size()), d_blk.p, out.pts.p);
+  SR_HIP(hipStreamSynchronize(s));  // the tables above are released on return
+}
+
+void seg_bounds(const float4* pts, const std::vector<uint32_t>& off, uint32_t* dev_bounds,
+                hipStream_t s) {
+  const uint32_t nseg = uint32_t(off.size() - 1);
+  std::vector<uint32_t> blk = seg_blocks(off, kSegBoundsChunk);
+  DevBuf<uint32_t> d_off, d_blk;
+  upload(d_off, off, s);
+  upload(d_blk, blk, s);
+  hipLaunchKernelGGL(k_seg_bounds_init, dim3((6 * nseg + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                     dev_bounds, nseg);
+  hipLaunchKernelGGL(k_seg_bounds, dim3(blk.back()), dim3(kBlock), 0, s, pts, d_off.p, nseg, d_blk.p,
+                     dev_bounds);
+  SR_HIP(hipStreamSynchronize(s));
+}
+
+void voxel_downsample_many(const Cloud& in, const std::vector<uint32_t>& off, float leaf,
+                           VoxelOut& out, std::vector<uint32_t>& cell_off, hipStream_t s) {
+  const uint32_t n = in.n, nseg = uint32_t(off.size() - 1);
+  out.cells = 0;
+  cell_off.assign(nseg + 1, 0);
+  if (n == 0 || nseg == 0) return;
+  const float inv_leaf = 1.0f / leaf;  // voxel.rs:267, f32 on the host
+  DevBuf<uint32_t> bounds(6 * size_t(nseg));
+  seg_bounds(in.pts.p, off, bounds.p, s);
+  DevBuf<VoxelParams> params(nseg);
+  hipLaunchKernelGGL(k_seg_voxel_params, dim3((nseg + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                     bounds.p, nseg, inv_leaf, params.p);
+  // one hash-table region per segment, sized as voxel_downsample sizes its table
+  std::vector<SegTable> tab(nseg);
+  uint64_t slots = 0;
+  for (uint32_t g = 0; g < nseg; ++g) {
+    uint32_t cap = 1024;
+    while (cap < 2u * (off[g + 1] - off[g])) cap <<= 1;
+    tab[g] = SegTable{uint32_t(slots), cap - 1};
+    slots += cap;
+  }
+  if (slots > 0xFFFFFFFFull) fail(3, "too many points for one batch");
+  const uint32_t cap = uint32_t(slots);
+  std::vector<uint32_t> blk = seg_blocks(off, kBlock);
+  DevBuf<SegTable> d_tab;
+  DevBuf<uint32_t> d_off, d_blk;
+  upload(d_tab, tab, s);
+  upload(d_off, off, s);
+  upload(d_blk, blk, s);
+  DevBuf<unsigned long long> tkeys(cap);
+  DevBuf<uint32_t> tfirst(cap), tcount(cap), cell_of_slot(cap);
+  hipLaunchKernelGGL(k_fill_u64, dim3(fill_blocks(cap)), dim3(kBlock), 0, s, tkeys.p, kEmptyKey,
+                     size_t(cap));
+  fill_u32(tfirst.p, 0xFFFFFFFFu, cap, s);
+  fill_u32(tcount.p, 0u, cap, s);
+  DevBuf<uint32_t> slot_of(n), flag(n), rank(n), ocount(n), cstart(n), cursor(n), members(n),
+      scal(4), tailv(nseg + 2);
+  const unsigned blocks = (n + kBlock - 1) / kBlock;
+  ProfScope prof_insert("voxel_insert", s);
+  hipLaunchKernelGGL(k_seg_voxel_insert, dim3(blk.back()), dim3(kBlock), 0, s, in.pts.p, d_off.p,
+                     nseg, d_blk.p, params.p, d_tab.p, tkeys.p, tfirst.p, tcount.p, slot_of.p);
+  hipLaunchKernelGGL(k_voxel_flags, dim3(blocks), dim3(kBlock), 0, s, slot_of.p, tfirst.p, n, flag.p);
+  exclusive_scan_u32(flag.p, rank.p, n, scal.p, s);  // scal[0] = cell count
+  fill_u32(ocount.p, 0u, n, s);
+  fill_u32(cursor.p, 0u, n, s);
+  hipLaunchKernelGGL(k_voxel_cellinfo, dim3(blocks), dim3(kBlock), 0, s, in.pts.p, slot_of.p,
+                     flag.p, rank.p, tcount.p, n, params.p, cell_of_slot.p, ocount.p,
+                     (int64_t*)nullptr);
+  exclusive_scan_u32(ocount.p, cstart.p, n, nullptr, s);
+  hipLaunchKernelGGL(k_voxel_scatter, dim3(blocks), dim3(kBlock), 0, s, slot_of.p, cell_of_slot.p,
+                     cstart.p, cursor.p, n, members.p);
+  out.centroids.pts.resize(n);
+  fill_u32(scal.p + 1, 0u, 1, s);
+  DevBuf<uint32_t> big_list(n);
+  hipLaunchKernelGGL(k_voxel_centroids, dim3(blocks), dim3(kBlock), 0, s, in.pts.p, scal.p,
+                     cstart.p, ocount.p, members.p, out.centroids.pts.p, big_list.p, scal.p + 1);
+  hipLaunchKernelGGL(k_voxel_big_cells, dim3(256), dim3(kBlock), 0, s, in.pts.p, cstart.p,
+                     ocount.p, members.p, big_list.p, scal.p + 1, out.centroids.pts.p);
+  fill_u32(tailv.p + nseg + 1, 0u, 1, s);
+  hipLaunchKernelGGL(k_seg_voxel_tail, dim3((nseg + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                     rank.p, d_off.p, nseg, params.p, scal.p, tailv.p);
+  std::vector<uint32_t> host(nseg + 2);
+  SR_HIP(hipMemcpyAsync(host.data(), tailv.p, sizeof(uint32_t) * host.size(), hipMemcpyDeviceToHost, s));
+  SR_HIP(hipStreamSynchronize(s));
+  if (host[nseg + 1]) fail(3, "voxel key range does not fit a 64-bit linear key; use a larger leaf_size");
+  for (uint32_t g = 0; g <= nseg; ++g) cell_off[g] = host[g];
+  out.cells = host[nseg];
+  out.centroids.n = host[nseg];
   out.counts = std::move(ocount);
 }
 

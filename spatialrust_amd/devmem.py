@@ -12,7 +12,10 @@ _hip = None
 def _rt():
     global _hip
     if _hip is None:
-        _hip = ctypes.CDLL("libamdhip64.so")
+        # By soname: the HIP runtime already in the process (the library's, or
+        # one a framework loaded first) is reused. A second runtime instance,
+        # which loading another file would start, sees no device.
+        _hip = ctypes.CDLL("libamdhip64.so.7")
         _hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
         _hip.hipFree.argtypes = [ctypes.c_void_p]
         _hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]

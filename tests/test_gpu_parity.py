@@ -276,6 +276,32 @@ def test_batch_pairs_equal_plain_align(hip):
         assert alone.iterations == batched[i].iterations
 
 
+def test_batch_pairs_device_clouds_equal_plain_align(hip):
+    """With every cloud in device memory sr_batch_align_pairs prepares all of
+    them as segments of flat buffers (one voxel filter, grid build and k-NN
+    launch set for the whole batch); poses stay bit-identical to one-at-a-time
+    sr_gicp_align, for clouds of very different sizes."""
+    from spatialrust_amd import devmem
+    pairs = synth.batch_pairs(count=10, lo=2000, hi=80000, seed_offset=9)
+    cfg = sr.GicpConfig(max_iterations=10, voxel_leaf=0.1, transformation_epsilon=0.0,
+                        fitness_epsilon=0.0)
+    srcs = [devmem.DeviceCloud(p.source) for p in pairs]
+    tgts = [devmem.DeviceCloud(p.target) for p in pairs]
+    try:
+        batched = sr.batch_align_pairs([c.view for c in srcs], [c.view for c in tgts], cfg)
+        reg = sr.GicpRegistration(cfg)
+        for i, p in enumerate(pairs):
+            alone = reg.align(p.source, p.target)
+            assert np.array_equal(alone.rotation64, batched[i].rotation64), i
+            assert np.array_equal(alone.translation64, batched[i].translation64), i
+            assert alone.iterations == batched[i].iterations
+            assert alone.source_points == batched[i].source_points
+            assert alone.target_points == batched[i].target_points
+    finally:
+        for c in srcs + tgts:
+            c.close()
+
+
 def test_receipt_records_stages_and_bytes(hip):
     w = synth.small_cube_pair()
     out = sr.GicpRegistration(sr.GicpConfig(max_iterations=3, voxel_leaf=0.1)).align_with_policy_and_receipt(
