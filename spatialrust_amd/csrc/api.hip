@@ -593,6 +593,7 @@ struct Batch {
   DevBuf<float> partials;
   DevBuf<double> init;
   DevBuf<uint32_t> match;
+  DevBuf<float> budget;
   uint32_t total_blocks = 0, count = 0, max_passes = 0;
   PassScratch scratch{};
   uint32_t hard_blocks = 1;
@@ -608,7 +609,7 @@ static uint32_t instance_blocks(uint32_t n_src) {
 // state, partials (instance_blocks x 64 floats) and match (n_src words) it owns.
 static IterInstance describe_instance(const Prepared& src, const Prepared& tgt,
                                       const sr_gicp_config& cfg, InstState* state, float* partials,
-                                      uint32_t* match, uint32_t block_begin) {
+                                      uint32_t* match, float* budget, uint32_t block_begin) {
   IterInstance I{};
   I.src = src.grid.pts.p;
   I.src_n = src.normals.p;
@@ -617,6 +618,19 @@ static IterInstance describe_instance(const Prepared& src, const Prepared& tgt,
   I.cell_start = tgt.grid.cell_start.p;
   I.empty_radius = tgt.grid.empty_radius.p;
   I.match = match;
+  I.budget = budget;
+  {
+    // the source grid's box holds every source point
+    const GridDesc& d = src.grid.desc;
+    const float lo[3] = {d.ox, d.oy, d.oz};
+    const float ext[3] = {d.h * float(d.dx), d.h * float(d.dy), d.h * float(d.dz)};
+    double r2 = 0.0;
+    for (int a = 0; a < 3; ++a) {
+      const double m = std::max(std::fabs(double(lo[a])), std::fabs(double(lo[a]) + ext[a]));
+      r2 += m * m;
+    }
+    I.src_radius = float(std::sqrt(r2) * (1.0 + 1e-6) + 1e-6);
+  }
   I.grid = tgt.grid.desc;
   I.cfg = inst_config(cfg);
   I.state = state;
@@ -649,11 +663,14 @@ static void batch_setup(const std::vector<const Prepared*>& src, const std::vect
   B.partials.resize(size_t(blocks) * 64);
   B.match.resize(std::max<uint64_t>(total_pts, 1));
   fill_u32(B.match.p, kNone, total_pts, s);
+  B.budget.resize(std::max<uint64_t>(total_pts, 1));
+  fill_u32(reinterpret_cast<uint32_t*>(B.budget.p), 0u, total_pts, s);
   uint32_t begin = 0;
   uint64_t src_begin = 0;
   for (uint32_t i = 0; i < count; ++i) {
     hi[i] = describe_instance(*src[i], *tgt[i], cfgs[i], B.states.p + i,
-                              B.partials.p + size_t(begin) * 64, B.match.p + src_begin, begin);
+                              B.partials.p + size_t(begin) * 64, B.match.p + src_begin,
+                              B.budget.p + src_begin, begin);
     src_begin += src[i]->n;
     for (uint32_t b = 0; b < nbl[i]; ++b) hbi.push_back(i);
     begin += nbl[i];
@@ -855,6 +872,7 @@ struct ServerRequest {
   uint32_t slot = 0;  // index of its state in the server's state array
   DevBuf<float> partials;
   DevBuf<uint32_t> match;
+  DevBuf<float> budget;
 };
 
 struct ServerDone {
@@ -925,7 +943,7 @@ static void server_step(sr_server& S) {
     for (uint32_t i = 0; i < count; ++i) {
       ServerRequest& r = *S.active[i];
       hi[i] = describe_instance(r.src, r.map->tgt, S.gicp, S.states.p + r.slot, r.partials.p,
-                                r.match.p, begin);
+                                r.match.p, r.budget.p, begin);
       for (uint32_t b = 0; b < hi[i].nblocks; ++b) hbi.push_back(i);
       begin += hi[i].nblocks;
       points += r.src.n;
@@ -975,6 +993,8 @@ static void server_step(sr_server& S) {
       r.partials.resize(size_t(instance_blocks(r.src.n)) * 64);
       r.match.resize(std::max<uint32_t>(r.src.n, 1));
       fill_u32(r.match.p, kNone, r.src.n, s);
+      r.budget.resize(std::max<uint32_t>(r.src.n, 1));
+      fill_u32(reinterpret_cast<uint32_t*>(r.budget.p), 0u, r.src.n, s);
       for (int a = 0; a < 4; ++a) init[7 * i + a] = double(S.gicp.init_q[a]);
       for (int a = 0; a < 3; ++a) init[7 * i + 4 + a] = double(S.gicp.init_t[a]);
     }

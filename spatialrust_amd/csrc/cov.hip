@@ -234,7 +234,7 @@ struct KIds {
 // ranges, kKnnBlockW loads in flight: one lane's chain of dependent loads is
 // then the block's points / kKnnBlockW steps instead of a few steps per range.
 #ifndef SR_KNN_BLOCK_W
-#define SR_KNN_BLOCK_W 8
+#define SR_KNN_BLOCK_W 4
 #endif
 constexpr int kKnnBlockW = SR_KNN_BLOCK_W;
 

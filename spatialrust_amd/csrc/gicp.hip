@@ -89,6 +89,8 @@ __global__ void k_init_states(InstState* states, const double* __restrict__ init
   s.converged = 0;
   s.counter = 0;
   s.pad_ = 0;
+  s.move_rot = INFINITY;
+  s.move_const = INFINITY;
   refresh_f32_pose(&s);
   states[i] = s;
 }
@@ -139,6 +141,8 @@ __device__ __forceinline__ void finalize(InstState* st, const InstConfig& cfg, c
                                       double* w) {
   const double F = cnt > 0 ? fit_sum / double(cnt) : DBL_MAX;
   st->count = cnt;
+  st->move_rot = 0.0f;  // unchanged pose unless updated below
+  st->move_const = 0.0f;
   if (st->phase == 1) {  // final fitness pass after transformation_epsilon convergence
     st->fitness = F;
     st->phase = 2;
@@ -209,6 +213,15 @@ __device__ __forceinline__ void finalize(InstState* st, const InstConfig& cfg, c
   double nt[3];
   for (int a = 0; a < 3; ++a)
     nt[a] = Rd[a][0] * st->t[0] + Rd[a][1] * st->t[1] + Rd[a][2] * st->t[2] + x[3 + a];
+  // Movement bound of the next pass: p'_new - p'_old = (dR - I)(R s + t) + dt,
+  // and |dR - I| = 2 sin(angle/2) <= angle, so |dp'| <= angle (|s| + |t|) + |dt|
+  // (rounded up; the f32 pose's own error is covered by the search's slack).
+  {
+    const double tn = sqrt(st->t[0] * st->t[0] + st->t[1] * st->t[1] + st->t[2] * st->t[2]);
+    const double dn = sqrt(x[3] * x[3] + x[4] * x[4] + x[5] * x[5]);
+    st->move_rot = float(angle * (1.0 + 1e-6) + 1e-30);
+    st->move_const = float((angle * tn + dn) * (1.0 + 1e-6) + 1e-30);
+  }
   for (int a = 0; a < 4; ++a) st->q[a] = nq[a] / l;
   for (int a = 0; a < 3; ++a) st->t[a] = nt[a];
   refresh_f32_pose(st);
@@ -237,6 +250,8 @@ struct NNQuery {
   uint64_t best_key;  // (d2 bits << 32 | original index); starts at (dmax2, ~0)
   uint32_t best_pos;  // grid position of the best, kNone if none
   float thr;          // search radius^2 = min(dmax2, best d2)
+  float second;       // phase 1: smallest d2 of the other points scanned
+  float budget;       // phase 1: movement before the match can change (0: none)
   int hx, hy, hz;     // query cell
   bool hard;          // phase 2 needed
   bool skip_block;    // the 3x3x3 block was scanned in phase 1
@@ -245,11 +260,19 @@ struct NNQuery {
 
 // (d2, original index) packed as one u64 key: d2 >= 0, so its bits order like
 // the float, and the index breaks ties; the initial key (dmax2, ~0) admits
-// exactly d2 <= dmax2 (NaN distances compare above it).
+// exactly d2 <= dmax2 (NaN distances compare above it). With kSecond the
+// smallest d2 of the other points seen is kept too (the same point seen again,
+// as the seed is, does not count).
+template <bool kSecond = false>
 __device__ __forceinline__ void nn_consider(NNQuery& Q, float px, float py, float pz, uint32_t s,
                                             const float4& q) {
   const float d2 = dist2(px, py, pz, q.x, q.y, q.z);
   const uint64_t key = (uint64_t(__float_as_uint(d2)) << 32) | __float_as_uint(q.w);
+  if constexpr (kSecond) {
+    const float old_best = Q.best_pos != kNone ? Q.thr : INFINITY;
+    Q.second = key < Q.best_key ? fminf(Q.second, old_best)
+                                : (key == Q.best_key ? Q.second : fminf(Q.second, d2));
+  }
   if (key < Q.best_key) {
     Q.best_key = key;
     Q.best_pos = s;
@@ -257,15 +280,29 @@ __device__ __forceinline__ void nn_consider(NNQuery& Q, float px, float py, floa
   }
 }
 
-template <bool kStats>
+#ifndef SR_BLOCK_W
+#define SR_BLOCK_W 8
+#endif
+constexpr int kBlockW = SR_BLOCK_W;  // point loads in flight per lane in the phase-1 block
+
+// Widening of the phase-1 block scan, in cells (see nn_phase1), used once the
+// pose moves every point of the source less than kGapMove cells per pass:
+// earlier, budgets would be spent at once and the widening and the
+// second-nearest tracking would only cost.
+constexpr float kGapMargin = 0.05f;
+constexpr float kGapMove = 0.01f;
+
+template <bool kStats, bool kGap>
 __device__ __forceinline__ void nn_phase1(const GridDesc& g, const uint32_t* __restrict__ cs,
                                           const uint8_t* __restrict__ empty,
                                           const float4* __restrict__ tp, float px, float py,
                                           float pz, float dmax, float dmax2, uint32_t prev,
-                                          NNQuery& Q, SearchStats* stats) {
+                                          float margin, NNQuery& Q, SearchStats* stats) {
   Q.best_key = (uint64_t(__float_as_uint(dmax2)) << 32) | 0xFFFFFFFFull;
   Q.best_pos = kNone;
   Q.thr = dmax2;
+  Q.second = INFINITY;
+  Q.budget = 0.0f;
   Q.hard = false;
   Q.skip_block = false;
   Q.rows = Q.pts = 0;
@@ -295,20 +332,33 @@ __device__ __forceinline__ void nn_phase1(const GridDesc& g, const uint32_t* __r
       return;
     }
   }
-  if (prev != kNone) nn_consider(Q, px, py, pz, prev, tp[prev]);
+  if (prev != kNone) nn_consider<kGap>(Q, px, py, pz, prev, tp[prev]);
   if (inside && e <= 1) {
-    // Rings 0 and 1 as nine contiguous point ranges, cut to the sphere of the
-    // seeded radius; most queries of a converging registration end here.
-    // the block's nine ranges streamed as one sequence, eight loads in flight
-    scan_block27_within<8, true>(g, cs, tp, px, py, pz, hx, hy, hz, Q.thr,
-                                 [&](uint32_t s, const float4& q) {
+    // Rings 0 and 1 as nine contiguous point ranges (streamed as one sequence,
+    // kBlockW loads in flight), cut to the sphere of the seeded radius widened
+    // by `margin`, so that a certified match also comes with a gap to every
+    // other point; most queries of a converging registration end here.
+    const float rclip = kGap ? __builtin_amdgcn_sqrtf(Q.thr) + margin : 0.0f;
+    scan_block27_within<kBlockW, true>(g, cs, tp, px, py, pz, hx, hy, hz,
+                                       kGap ? rclip * rclip : Q.thr,
+                                       [&](uint32_t s, const float4& q) {
       if constexpr (kStats) ++Q.pts;
-      nn_consider(Q, px, py, pz, s, q);
+      nn_consider<kGap>(Q, px, py, pz, s, q);
     });
     Q.skip_block = true;
     const float cl = ring_clearance(g, px, py, pz, hx, hy, hz, 1);
     if (cl > 0.0f && Q.thr < cl * cl) {
       if constexpr (kStats) stats_add(stats, 1, 0, Q.pts, kBlockDone);
+      if (kGap && Q.best_pos != kNone) {
+        // Every other point lies at least `other` away: scanned ones at
+        // sqrt(second), unscanned cells of the block beyond rclip, cells
+        // outside it beyond cl. While the point moves less than half the gap
+        // (less a rounding slack), its nearest point stays the same.
+        const float best = sqrtf(Q.thr);
+        const float other = fminf(fminf(sqrtf(Q.second), rclip), cl);
+        const float slack = 1e-6f * (fabsf(px) + fabsf(py) + fabsf(pz) + best + 1.0f);
+        Q.budget = fmaxf(0.5f * (other - best) - slack, 0.0f);
+      }
       return;
     }
   }
@@ -668,8 +718,13 @@ constexpr uint32_t kHardBlockFlag = 0x80000000u;  // HardQuery.inst: block scann
 #define SR_HARD_WAVES 8
 #endif
 
+#ifndef SR_SEARCH_WAVES
+#define SR_SEARCH_WAVES 4
+#endif
+
 template <bool kStats>
-__global__ void __launch_bounds__(kBlock) k_gicp_search(const IterInstance* __restrict__ insts,
+__global__ void __launch_bounds__(kBlock)
+    __attribute__((amdgpu_waves_per_eu(SR_SEARCH_WAVES))) k_gicp_search(const IterInstance* __restrict__ insts,
                                                         const uint32_t* __restrict__ block_inst,
                                                         PassScratch ps,
                                                         SearchStats* __restrict__ stats) {
@@ -681,29 +736,41 @@ __global__ void __launch_bounds__(kBlock) k_gicp_search(const IterInstance* __re
   const uint32_t lb = b - I.block_begin;
   const GridDesc g = I.grid;
   const InstConfig cfg = I.cfg;
-  const float beta = cfg.one_minus_eps, eps = 1.0f - cfg.one_minus_eps;
   PassPose P;
   load_pose(st, P);
   const int lane = int(threadIdx.x & (kWave - 1));
   const unsigned long long below = (1ull << lane) - 1ull;
-
-  // Plain f32 per-thread sums (a thread holds only ppt points); compensation
-  // happens in the block reduction.
-  float acc[kTerms];
-#pragma unroll
-  for (int t = 0; t < kTerms; ++t) acc[t] = 0.0f;
-  uint32_t count = 0;
-  bool queued = false;
+  const float move_rot = st->move_rot, move_const = st->move_const;
+  // budgets are worth computing once no source point moves kGapMove cells
+  const bool gap = move_rot * I.src_radius + move_const < kGapMove * g.h;
   const uint32_t ppt = I.ppt;
+
+  // Loop 1, the search: every point's match (and movement budget) is recorded,
+  // and the uncertified ones are queued.
+  bool queued = false;
   for (uint32_t k = 0; k < ppt; ++k) {
     const uint32_t i = (lb * ppt + k) * kBlock + threadIdx.x;
     const bool valid = i < I.n_src;
     const float4 p = I.src[valid ? i : 0u];
     float px, py, pz;
     transform_point(P, p, px, py, pz);
+    const uint32_t prev = valid ? I.match[i] : kNone;
+    // The last search left this point a movement budget (half its gap to the
+    // second-nearest point); while the pose moves it less, the match stands.
+    const float bud = valid ? I.budget[i] : 0.0f;
+    const float move =
+        move_rot * (sqrtf(p.x * p.x + p.y * p.y + p.z * p.z) * 1.000001f) + move_const;
+    const bool keep = prev != kNone && bud > move;
     NNQuery Q;
-    nn_phase1<kStats>(g, I.cell_start, I.empty_radius, I.tgt, px, py, pz, cfg.dmax, cfg.dmax2,
-                      valid ? I.match[i] : kNone, Q, valid ? stats : nullptr);
+    Q.hard = false;
+    if (!keep) {
+      if (gap)
+        nn_phase1<kStats, true>(g, I.cell_start, I.empty_radius, I.tgt, px, py, pz, cfg.dmax,
+                                cfg.dmax2, prev, kGapMargin * g.h, Q, valid ? stats : nullptr);
+      else
+        nn_phase1<kStats, false>(g, I.cell_start, I.empty_radius, I.tgt, px, py, pz, cfg.dmax,
+                                 cfg.dmax2, prev, 0.0f, Q, valid ? stats : nullptr);
+    }
     const bool hard = valid && Q.hard;
     const unsigned long long hm = __ballot(hard);
     if (hm) {
@@ -722,20 +789,47 @@ __global__ void __launch_bounds__(kBlock) k_gicp_search(const IterInstance* __re
         h.pos = Q.best_pos;
         h.key = Q.best_key;
         ps.queue[base + uint32_t(__popcll(hm & below))] = h;
+        I.budget[i] = 0.0f;
       }
     }
     if (!valid || hard) continue;
-    const uint32_t j = Q.best_pos;
-    I.match[i] = j;
-    if (j == kNone) continue;
-    const float d2 = __uint_as_float(uint32_t(Q.best_key >> 32));
-    add_terms(acc, P, beta, eps, px, py, pz, I.tgt[j], I.tgt_n[j], I.src_n[i], d2);
-    ++count;
+    if (keep) {
+      if constexpr (kStats) stats_add(stats, 0, 0, 0, kBlockDone);
+      const float4 q = I.tgt[prev];
+      if (dist2(px, py, pz, q.x, q.y, q.z) <= cfg.dmax2) {
+        I.budget[i] = bud - move;
+      } else {  // the nearest point is beyond d_max: no correspondence, search next time
+        I.match[i] = kNone;
+        I.budget[i] = 0.0f;
+      }
+      continue;
+    }
+    I.budget[i] = Q.budget;
+    I.match[i] = Q.best_pos;
   }
   // A block with queued points gets its partial from k_gicp_terms.
   const int any = __syncthreads_or(queued ? 1 : 0);
   if (threadIdx.x == 0) ps.done[b] = any ? 0u : 1u;
   if (any) return;
+  // Loop 2, the terms, from the matches this thread just wrote (a separate
+  // loop, so the search's registers and the sums' are not live together).
+  const float beta = cfg.one_minus_eps, eps = 1.0f - cfg.one_minus_eps;
+  float acc[kTerms];
+#pragma unroll
+  for (int t = 0; t < kTerms; ++t) acc[t] = 0.0f;
+  uint32_t count = 0;
+  for (uint32_t k = 0; k < ppt; ++k) {
+    const uint32_t i = (lb * ppt + k) * kBlock + threadIdx.x;
+    if (i >= I.n_src) continue;
+    const uint32_t j = I.match[i];
+    if (j == kNone) continue;
+    float px, py, pz;
+    transform_point(P, I.src[i], px, py, pz);
+    const float4 q = I.tgt[j];
+    const float d2 = dist2(px, py, pz, q.x, q.y, q.z);
+    add_terms(acc, P, beta, eps, px, py, pz, q, I.tgt_n[j], I.src_n[i], d2);
+    ++count;
+  }
   __shared__ RedLds L;
   write_partial(L, acc, count, I.partials + size_t(lb) * 64);
 }

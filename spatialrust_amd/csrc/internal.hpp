@@ -279,6 +279,11 @@ struct InstState {
   uint32_t converged;
   uint32_t counter;  // blocks of the current pass that have finished
   uint32_t pad_;
+  // Bound on how far the next pass's pose moves a source point s from where
+  // the last pass put it: |dp'| <= move_rot * |s| + move_const (+inf before
+  // the first pass, 0 when the pose did not change).
+  float move_rot;
+  float move_const;
 };
 
 struct InstConfig {
@@ -302,6 +307,9 @@ struct IterInstance {
   const uint8_t* empty_radius;  // target grid's empty-radius field (may be null)
   uint32_t* match;        // per source point: target grid position matched by the
                           // previous pass (kNone: none); seeds the next search
+  float* budget;          // per source point: how far it may still move before its
+                          // match can change (0: search again)
+  float src_radius;       // bound on |s| over the source points
   GridDesc grid;
   InstConfig cfg;
   InstState* state;
