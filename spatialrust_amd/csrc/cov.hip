@@ -238,6 +238,7 @@ struct KIds {
 #endif
 constexpr int kKnnBlockW = SR_KNN_BLOCK_W;
 
+
 // The candidates of an inside query: the 3x3x3 block, then rings 2..rend cut
 // to radius^2 thr (a superset of the points within sqrt(thr)).
 template <typename Visit>
@@ -250,13 +251,14 @@ __device__ __forceinline__ void knn_candidates(const GridDesc& g, const uint32_t
   for (int r = 2; r <= rend; ++r) scan_ring<1>(g, cs, gp, px, py, pz, hx, hy, hz, r, thr, rows, visit);
 }
 
-// k-NN normals. The k nearest (ties to the lower index) are found in three
+// k-NN normals. The k nearest (ties to the lower index) are found in two
 // passes over the same candidates instead of keeping a sorted neighbour list:
 // (1) the k-th and (k+1)-th smallest distances by the sorting network, the
-// rings extended until the k-th (tau) is certified; (2) the mean of the points
-// closer than tau plus those at tau with the lowest indices (exactly k); (3)
-// the centred sums. Only when the k-th and (k+1)-th distances are equal does a
-// pass over the points at distance tau pick the index cut.
+// rings extended until the k-th (tau) is certified; (2) the first and second
+// moments, relative to the query, of the points closer than tau plus those at
+// tau with the lowest indices (exactly k). Only when the k-th and (k+1)-th
+// distances are equal does a pass over the points at distance tau pick the
+// index cut.
 //
 // A point still uncertified after kKnnLaneRings rings (an isolated point of a
 // sparse cloud, whose neighbours may lie dozens of rings away) is queued for
@@ -373,33 +375,34 @@ __global__ void __launch_bounds__(kBlock) k_knn_normals(const KnnCloud* __restri
   float n[3] = {1.0f, 0.0f, 0.0f};  // count < 2: identity -> eigenvector e_x (gicp.rs:268-270)
   float mx = 0.0f, my = 0.0f, mz = 0.0f, c00 = 0, c11 = 0, c22 = 0, c01 = 0, c02 = 0, c12 = 0;
   int found = 0;
-  // pass 2: mean of the selected points, relative to the query
+  // one pass: first and second moments relative to the query, then
+  // C = S2 - S1 S1^T / n (the offsets are within the k-NN radius, so the
+  // subtraction loses little)
   knn_candidates(g, cs, gp, p.x, p.y, p.z, hx, hy, hz, rend, tau, [&](uint32_t, const float4& q) {
     if (selected(q)) {
-      mx += q.x - p.x;
-      my += q.y - p.y;
-      mz += q.z - p.z;
+      const float ex = q.x - p.x, ey = q.y - p.y, ez = q.z - p.z;
+      mx += ex;
+      my += ey;
+      mz += ez;
+      c00 += ex * ex;
+      c11 += ey * ey;
+      c22 += ez * ez;
+      c01 += ex * ey;
+      c02 += ex * ez;
+      c12 += ey * ez;
       ++found;
     }
   });
   if (found >= 2) {
     const float invn = 1.0f / float(found);
-    mx *= invn;
-    my *= invn;
-    mz *= invn;
-    // pass 3: centred sums
-    knn_candidates(g, cs, gp, p.x, p.y, p.z, hx, hy, hz, rend, tau, [&](uint32_t, const float4& q) {
-      if (selected(q)) {
-        const float ex = (q.x - p.x) - mx, ey = (q.y - p.y) - my, ez = (q.z - p.z) - mz;
-        c00 += ex * ex;
-        c11 += ey * ey;
-        c22 += ez * ez;
-        c01 += ex * ey;
-        c02 += ex * ez;
-        c12 += ey * ez;
-      }
-    });
+    c00 -= mx * mx * invn;
+    c11 -= my * my * invn;
+    c22 -= mz * mz * invn;
+    c01 -= mx * my * invn;
+    c02 -= mx * mz * invn;
+    c12 -= my * mz * invn;
   }
+
   if (found >= 2) smallest_eigvec(c00, c11, c22, c01, c02, c12, n);
   write_knn_normal(i, __float_as_uint(p.w), n, one_minus_eps, nrm_grid, cov6, normal3);
 }
