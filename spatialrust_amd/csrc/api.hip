@@ -479,7 +479,13 @@ static std::shared_ptr<SegHolder> prepare_segmented(const std::vector<const sr_x
   choose_cell_edges_many(vox.centroids, voff, hb, knn_occupancy(cfg.k_neighbors), cfg.voxel_leaf,
                          h, s);
   auto hold = std::make_shared<SegHolder>();
-  build_grids_many(vox.centroids, voff, hb, h, true, hold->set, s);
+  // the empty radius serves the targets' searches: computed for the leading
+  // run of target segments (all of them when the targets come first)
+  uint32_t empty_segs = 0;
+  while (empty_segs < nseg && is_target[empty_segs]) ++empty_segs;
+  for (uint32_t c = empty_segs; c < nseg; ++c)
+    if (is_target[c]) empty_segs = nseg;
+  build_grids_many(vox.centroids, voff, hb, h, empty_segs, hold->set, s);
   rc.stages |= SR_STAGE_GRID;
   const GridSet& G = hold->set;
   hold->normals.resize(std::max<uint32_t>(voff.back(), 1));

@@ -744,6 +744,49 @@ __global__ void __launch_bounds__(kBlock)
   // budgets are worth computing once no source point moves kGapMove cells
   const bool gap = move_rot * I.src_radius + move_const < kGapMove * g.h;
   const uint32_t ppt = I.ppt;
+  const float beta = cfg.one_minus_eps, eps = 1.0f - cfg.one_minus_eps;
+  __shared__ RedLds L;
+
+  // Settled blocks: when every point of a one-point-per-thread block keeps its
+  // match, the search is skipped and the terms are added in the same sweep, so
+  // the block costs two rounds of loads (its own points, then the matched
+  // target points) instead of four. The terms are those loop 2 would add.
+  if (ppt == 1) {
+    const uint32_t i = lb * kBlock + threadIdx.x;
+    const bool valid = i < I.n_src;
+    const float4 p = I.src[valid ? i : 0u];
+    const float4 ns = I.src_n[valid ? i : 0u];
+    const uint32_t prev = valid ? I.match[i] : kNone;
+    const float bud = valid ? I.budget[i] : 0.0f;
+    const float move =
+        move_rot * (sqrtf(p.x * p.x + p.y * p.y + p.z * p.z) * 1.000001f) + move_const;
+    const bool keep = prev != kNone && bud > move;
+    if (__syncthreads_and(!valid || keep)) {
+      if (threadIdx.x == 0) ps.done[b] = 1u;
+      float acc[kTerms];
+#pragma unroll
+      for (int t = 0; t < kTerms; ++t) acc[t] = 0.0f;
+      uint32_t count = 0;
+      if (valid) {
+        if constexpr (kStats) stats_add(stats, 0, 0, 0, kBlockDone);
+        float px, py, pz;
+        transform_point(P, p, px, py, pz);
+        const float4 q = I.tgt[prev];
+        const float4 nt = I.tgt_n[prev];
+        const float d2 = dist2(px, py, pz, q.x, q.y, q.z);
+        if (d2 <= cfg.dmax2) {
+          I.budget[i] = bud - move;
+          add_terms(acc, P, beta, eps, px, py, pz, q, nt, ns, d2);
+          ++count;
+        } else {  // the nearest point is beyond d_max: no correspondence
+          I.match[i] = kNone;
+          I.budget[i] = 0.0f;
+        }
+      }
+      write_partial(L, acc, count, I.partials + size_t(lb) * 64);
+      return;
+    }
+  }
 
   // Loop 1, the search: every point's match (and movement budget) is recorded,
   // and the uncertified ones are queued.
@@ -813,7 +856,6 @@ __global__ void __launch_bounds__(kBlock)
   if (any) return;
   // Loop 2, the terms, from the matches this thread just wrote (a separate
   // loop, so the search's registers and the sums' are not live together).
-  const float beta = cfg.one_minus_eps, eps = 1.0f - cfg.one_minus_eps;
   float acc[kTerms];
 #pragma unroll
   for (int t = 0; t < kTerms; ++t) acc[t] = 0.0f;
@@ -830,7 +872,6 @@ __global__ void __launch_bounds__(kBlock)
     add_terms(acc, P, beta, eps, px, py, pz, q, I.tgt_n[j], I.src_n[i], d2);
     ++count;
   }
-  __shared__ RedLds L;
   write_partial(L, acc, count, I.partials + size_t(lb) * 64);
 }
 
@@ -871,39 +912,47 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// A workgroup per run of consecutive logical blocks (runs kept on one XCD), so
+// a pass whose blocks were all finished by k_gicp_search costs a small launch.
 __global__ void __launch_bounds__(kBlock) k_gicp_terms(const IterInstance* __restrict__ insts,
                                                        const uint32_t* __restrict__ block_inst,
-                                                       PassScratch ps) {
-  const uint32_t b = xcd_block();
-  const uint32_t inst_id = block_inst[b];
-  const IterInstance& I = insts[inst_id];
-  const InstState* st = I.state;
-  if (st->phase == 2 || ps.done[b]) return;
-  const uint32_t lb = b - I.block_begin;
-  const InstConfig cfg = I.cfg;
-  const float beta = cfg.one_minus_eps, eps = 1.0f - cfg.one_minus_eps;
-  PassPose P;
-  load_pose(st, P);
-  float acc[kTerms];
-#pragma unroll
-  for (int t = 0; t < kTerms; ++t) acc[t] = 0.0f;
-  uint32_t count = 0;
-  const uint32_t ppt = I.ppt;
-  for (uint32_t k = 0; k < ppt; ++k) {
-    const uint32_t i = (lb * ppt + k) * kBlock + threadIdx.x;
-    if (i >= I.n_src) continue;
-    const uint32_t j = I.match[i];
-    if (j == kNone) continue;
-    float px, py, pz;
-    transform_point(P, I.src[i], px, py, pz);
-    const float4 q = I.tgt[j];
-    // the squared distance the search compared (same operations, same operands)
-    const float d2 = dist2(px, py, pz, q.x, q.y, q.z);
-    add_terms(acc, P, beta, eps, px, py, pz, q, I.tgt_n[j], I.src_n[i], d2);
-    ++count;
-  }
+                                                       uint32_t total_blocks, PassScratch ps) {
+  const uint32_t w = xcd_block(), G = gridDim.x;
+  const uint32_t per = total_blocks / G, rem = total_blocks % G;
+  const uint32_t b0 = w * per + min(w, rem), b1 = b0 + per + (w < rem ? 1u : 0u);
   __shared__ RedLds L;
-  write_partial(L, acc, count, I.partials + size_t(lb) * 64);
+  for (uint32_t b = b0; b < b1; ++b) {
+    if (ps.done[b]) continue;
+    const uint32_t inst_id = block_inst[b];
+    const IterInstance& I = insts[inst_id];
+    const InstState* st = I.state;
+    if (st->phase == 2) continue;
+    const uint32_t lb = b - I.block_begin;
+    const InstConfig cfg = I.cfg;
+    const float beta = cfg.one_minus_eps, eps = 1.0f - cfg.one_minus_eps;
+    PassPose P;
+    load_pose(st, P);
+    float acc[kTerms];
+#pragma unroll
+    for (int t = 0; t < kTerms; ++t) acc[t] = 0.0f;
+    uint32_t count = 0;
+    const uint32_t ppt = I.ppt;
+    for (uint32_t k = 0; k < ppt; ++k) {
+      const uint32_t i = (lb * ppt + k) * kBlock + threadIdx.x;
+      if (i >= I.n_src) continue;
+      const uint32_t j = I.match[i];
+      if (j == kNone) continue;
+      float px, py, pz;
+      transform_point(P, I.src[i], px, py, pz);
+      const float4 q = I.tgt[j];
+      // the squared distance the search compared (same operations, same operands)
+      const float d2 = dist2(px, py, pz, q.x, q.y, q.z);
+      add_terms(acc, P, beta, eps, px, py, pz, q, I.tgt_n[j], I.src_n[i], d2);
+      ++count;
+    }
+    write_partial(L, acc, count, I.partials + size_t(lb) * 64);
+    __syncthreads();  // the LDS is reused by the next block
+  }
 }
 
 // One workgroup per registration: FP64 sum of its block partials, then the
@@ -991,6 +1040,10 @@ uint32_t hard_stage_blocks(uint64_t points) {
   return uint32_t(std::min<uint64_t>(std::max<uint64_t>(want, 1), 2048));
 }
 
+// Workgroups of the terms stage: about two per CU at a time on 256 CUs, each
+// taking a run of blocks.
+constexpr uint32_t kTermsGrid = 4096;
+
 void gicp_pass(const IterInstance* insts, const uint32_t* block_inst, uint32_t total_blocks,
                uint32_t count, const PassScratch& ps, uint32_t hard_blocks, hipStream_t s,
                bool finalize) {
@@ -1007,7 +1060,8 @@ void gicp_pass(const IterInstance* insts, const uint32_t* block_inst, uint32_t t
                          block_inst, ps, st);
       hipLaunchKernelGGL(k_gicp_hard<false>, dim3(hard_blocks), dim3(kBlock), 0, s, insts, ps, st);
     }
-    hipLaunchKernelGGL(k_gicp_terms, dim3(total_blocks), dim3(kBlock), 0, s, insts, block_inst, ps);
+    hipLaunchKernelGGL(k_gicp_terms, dim3(std::min<uint32_t>(total_blocks, kTermsGrid)), dim3(kBlock),
+                       0, s, insts, block_inst, total_blocks, ps);
   }
   if (!finalize) return;
   ProfScope prof("gicp_finalize", s);

@@ -572,8 +572,8 @@ void choose_cell_edges_many(const Cloud& c, const std::vector<uint32_t>& off,
 }
 
 void build_grids_many(const Cloud& c, const std::vector<uint32_t>& off,
-                      const std::vector<uint32_t>& hb, const std::vector<float>& h, bool empty,
-                      GridSet& out, hipStream_t s) {
+                      const std::vector<uint32_t>& hb, const std::vector<float>& h,
+                      uint32_t empty_segs, GridSet& out, hipStream_t s) {
   const uint32_t nseg = uint32_t(off.size() - 1), n = off.back();
   out.desc.assign(nseg, GridDesc{});
   out.pt_off = off;
@@ -654,18 +654,21 @@ void build_grids_many(const Cloud& c, const std::vector<uint32_t>& off,
     hipLaunchKernelGGL(k_seg_grid_gather, grid1(n), dim3(kBlock), 0, s, c.pts.p, order.p, d_pt.p, nseg,
                        n, out.pts.p);
   }
-  if (empty && ncells) {
-    out.empty_radius.resize(ncells);
-    DevBuf<uint8_t> tmp(ncells);
-    hipLaunchKernelGGL(k_seg_empty_init, grid1(ncells), dim3(kBlock), 0, s, out.cell_start.p, d_sg.p,
-                       nseg, d_cell.p, ncells, out.empty_radius.p);
+  empty_segs = std::min(empty_segs, nseg);
+  const uint32_t ecells = empty_segs ? out.cell_off[empty_segs] : 0u;
+  if (ecells) {
+    // ping-pong between two buffers; the third pass lands in `tmp`, which
+    // then becomes the result
+    DevBuf<uint8_t> er(ecells), tmp(ecells);
+    hipLaunchKernelGGL(k_seg_empty_init, grid1(ecells), dim3(kBlock), 0, s, out.cell_start.p, d_sg.p,
+                       empty_segs, d_cell.p, ecells, er.p);
     for (int axis = 0; axis < 3; ++axis) {
-      const uint8_t* in = axis == 1 ? tmp.p : out.empty_radius.p;
-      uint8_t* o = axis == 1 ? out.empty_radius.p : tmp.p;
-      hipLaunchKernelGGL(k_seg_empty_pass, grid1(ncells), dim3(kBlock), 0, s, in, o, d_sg.p, nseg,
-                         d_cell.p, ncells, axis);
+      const uint8_t* in = axis == 1 ? tmp.p : er.p;
+      uint8_t* o = axis == 1 ? er.p : tmp.p;
+      hipLaunchKernelGGL(k_seg_empty_pass, grid1(ecells), dim3(kBlock), 0, s, in, o, d_sg.p,
+                         empty_segs, d_cell.p, ecells, axis);
     }
-    SR_HIP(hipMemcpyAsync(out.empty_radius.p, tmp.p, ncells, hipMemcpyDeviceToDevice, s));
+    out.empty_radius = std::move(tmp);
   }
   SR_HIP(hipStreamSynchronize(s));  // the tables above are released on return
 }

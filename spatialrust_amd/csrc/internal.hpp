@@ -204,11 +204,12 @@ struct GridSet {
 void choose_cell_edges_many(const Cloud& c, const std::vector<uint32_t>& off,
                             const std::vector<uint32_t>& hb, float occupancy, float leaf_hint,
                             std::vector<float>& h, hipStream_t s);
-// build_grid (and build_empty_radius when `empty`) for every segment, edge h[s].
+// build_grid for every segment, edge h[s], and build_empty_radius for the
+// first `empty_segs` segments (the empty radius covers their cells only).
 // Synchronises the stream.
 void build_grids_many(const Cloud& c, const std::vector<uint32_t>& off,
-                      const std::vector<uint32_t>& hb, const std::vector<float>& h, bool empty,
-                      GridSet& out, hipStream_t s);
+                      const std::vector<uint32_t>& hb, const std::vector<float>& h,
+                      uint32_t empty_segs, GridSet& out, hipStream_t s);
 
 // Chooses a cell edge so that occupied cells hold about `occupancy` points
 // (measured, starting from a guess; leaf_hint > 0 says the cloud holds one
