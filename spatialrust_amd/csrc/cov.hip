@@ -230,6 +230,16 @@ struct KIds {
   }
 };
 
+// Slot j of a network: a fixed register when k equals the network size K (j is
+// then the compile-time jfix), else a runtime pick. A runtime index into the
+// register array would otherwise be served through per-lane scratch memory
+// (the whole network stored at each read).
+template <bool kExact, int K>
+__device__ __forceinline__ float kdist_slot(const KDist<K>& D, int j, int jfix) {
+  if constexpr (kExact) return D.d[jfix];
+  else return D.at(j);
+}
+
 // The 3x3x3 block of a k-NN pass is streamed as one sequence of its nine
 // ranges, kKnnBlockW loads in flight: one lane's chain of dependent loads is
 // then the block's points / kKnnBlockW steps instead of a few steps per range.
@@ -300,8 +310,13 @@ __device__ __forceinline__ uint32_t knn_cloud_of_block(const KnnCloud* __restric
   return lo;
 }
 
-template <int K>
-__global__ void __launch_bounds__(kBlock) k_knn_normals(const KnnCloud* __restrict__ clouds,
+#ifndef SR_KNN_WAVES
+#define SR_KNN_WAVES 5
+#endif
+
+template <int K, bool kExact>
+__global__ void __launch_bounds__(kBlock)
+    __attribute__((amdgpu_waves_per_eu(SR_KNN_WAVES))) k_knn_normals(const KnnCloud* __restrict__ clouds,
                                                         uint32_t nclouds, int k,
                                                         float one_minus_eps,
                                                         unsigned long long* __restrict__ far_queue,
@@ -330,7 +345,7 @@ __global__ void __launch_bounds__(kBlock) k_knn_normals(const KnnCloud* __restri
     D.init();
     auto net = [&](uint32_t, const float4& q) { D.add(dist2(q.x, q.y, q.z, p.x, p.y, p.z)); };
     scan_block27<kKnnBlockW, true>(g, cs, gp, hx, hy, hz, net);
-    float worst = D.at(k - 1);
+    float worst = kdist_slot<kExact>(D, k - 1, K - 1);
     const float cl = ring_clearance(g, p.x, p.y, p.z, hx, hy, hz, 1);
     if (!(worst < INFINITY && cl > 0.0f && worst < cl * cl)) {
       uint32_t rows = 0;
@@ -338,7 +353,7 @@ __global__ void __launch_bounds__(kBlock) k_knn_normals(const KnnCloud* __restri
       for (int r = 2; r <= min(rmax, kKnnLaneRings); ++r) {
         scan_ring<1>(g, cs, gp, p.x, p.y, p.z, hx, hy, hz, r, worst, rows, net);
         rend = r;
-        worst = D.at(k - 1);
+        worst = kdist_slot<kExact>(D, k - 1, K - 1);
         const float c = ring_clearance(g, p.x, p.y, p.z, hx, hy, hz, r);
         if (worst < INFINITY && c > 0.0f && worst < c * c) {
           certified = true;
@@ -351,7 +366,7 @@ __global__ void __launch_bounds__(kBlock) k_knn_normals(const KnnCloud* __restri
       }
     }
     tau = worst;
-    next = D.at(k);
+    next = kdist_slot<kExact>(D, k, K);
 #pragma unroll
     for (int j = 0; j < K; ++j) c_lt += (j < k && D.d[j] < tau) ? 1 : 0;
   }
@@ -470,7 +485,7 @@ __device__ __forceinline__ float wave_sum_f(float v) {
   return v;
 }
 
-template <int K>
+template <int K, bool kExact>
 __global__ void __launch_bounds__(kBlock) k_knn_normals_far(const KnnCloud* __restrict__ clouds,
                                                             int k, float one_minus_eps,
                                                             const unsigned long long* __restrict__ far_queue,
@@ -504,7 +519,7 @@ __global__ void __launch_bounds__(kBlock) k_knn_normals_far(const KnnCloud* __re
 #pragma unroll
       for (int j = 0; j <= K; ++j) inside += (c > 0.0f && D.d[j] < c * c) ? 1.0f : 0.0f;
       if (wave_sum_f(inside) >= float(k)) break;
-      float lane_kth = D.at(k - 1);
+      float lane_kth = kdist_slot<kExact>(D, k - 1, K - 1);
 #pragma unroll
       for (int off = kWave / 2; off > 0; off >>= 1) lane_kth = fminf(lane_kth, __shfl_xor(lane_kth, off));
       thr = lane_kth;
@@ -518,7 +533,7 @@ __global__ void __launch_bounds__(kBlock) k_knn_normals_far(const KnnCloud* __re
 #pragma unroll
       for (int j = 0; j <= K; ++j) D.add(other[j]);
     }
-    const float tau = D.at(k - 1), next = D.at(k);
+    const float tau = kdist_slot<kExact>(D, k - 1, K - 1), next = kdist_slot<kExact>(D, k, K);
     int c_lt = 0;
 #pragma unroll
     for (int j = 0; j < K; ++j) c_lt += (j < k && D.d[j] < tau) ? 1 : 0;
@@ -592,7 +607,7 @@ __global__ void __launch_bounds__(kBlock) k_knn_normals_far(const KnnCloud* __re
   }
 }
 
-template <int K>
+template <int K, bool kExact>
 static void launch_knn_normals(const std::vector<KnnCloud>& host, int k, float b, hipStream_t s) {
   uint64_t n = 0;
   for (const KnnCloud& c : host) n += c.g.npoints;
@@ -605,13 +620,13 @@ static void launch_knn_normals(const std::vector<KnnCloud>& host, int k, float b
   DevBuf<uint32_t> far_count(1);
   fill_u32(far_count.p, 0u, 1, s);
   ProfScope prof("knn_normals", s);
-  hipLaunchKernelGGL(k_knn_normals<K>, dim3(blocks), dim3(kBlock), 0, s, clouds.p,
+  hipLaunchKernelGGL((k_knn_normals<K, kExact>), dim3(blocks), dim3(kBlock), 0, s, clouds.p,
                      uint32_t(host.size()), k, b, far_queue.p, far_count.p,
                      prof_stats("knn_normals"));
   // one wave per far point, a grid the chip holds at once (few points qualify)
   const uint32_t far_blocks =
       uint32_t(std::min<uint64_t>((n + kBlock / kWave - 1) / (kBlock / kWave), 1024));
-  hipLaunchKernelGGL(k_knn_normals_far<K>, dim3(far_blocks), dim3(kBlock), 0, s, clouds.p, k, b,
+  hipLaunchKernelGGL((k_knn_normals_far<K, kExact>), dim3(far_blocks), dim3(kBlock), 0, s, clouds.p, k, b,
                      far_queue.p, far_count.p);
   // the descriptor table and the queue are stream-ordered scratch: the
   // allocator hands them out again only to later work on this stream
@@ -629,12 +644,12 @@ void knn_normals_many(const std::vector<KnnCloud>& clouds_in, uint32_t k, double
   }
   if (clouds.empty()) return;
   const float b = float(1.0 - eps);
-  if (k == 12) launch_knn_normals<12>(clouds, int(k), b, s);
-  else if (k == 15) launch_knn_normals<15>(clouds, int(k), b, s);
-  else if (k == 20) launch_knn_normals<20>(clouds, int(k), b, s);
-  else if (k <= 8) launch_knn_normals<8>(clouds, int(k), b, s);
-  else if (k <= 32) launch_knn_normals<32>(clouds, int(k), b, s);
-  else if (k <= 64) launch_knn_normals<64>(clouds, int(k), b, s);
+  if (k == 12) launch_knn_normals<12, true>(clouds, int(k), b, s);
+  else if (k == 15) launch_knn_normals<15, true>(clouds, int(k), b, s);
+  else if (k == 20) launch_knn_normals<20, true>(clouds, int(k), b, s);
+  else if (k <= 8) launch_knn_normals<8, false>(clouds, int(k), b, s);
+  else if (k <= 32) launch_knn_normals<32, false>(clouds, int(k), b, s);
+  else if (k <= 64) launch_knn_normals<64, false>(clouds, int(k), b, s);
   else fail(3, "k_neighbors above 64 is not supported by the HIP covariance kernel");
 }
 

@@ -661,32 +661,43 @@ __device__ __forceinline__ void add_terms(float (&acc)[kTerms], const PassPose& 
 
 // Block reduction (fixed order, compensated). Rows padded by 8 floats and
 // chunks read interleaved (element e*8 + ch), so the 64 lanes of a wave spread
-// over the LDS banks.
+// over the LDS banks. kRows terms go through LDS at a time (all 28, or half of
+// them in two rounds, for kernels whose occupancy the LDS would limit); the
+// arithmetic is the same either way.
 constexpr int kRedRow = kBlock + 8;
-struct RedLds {
-  float red[kTerms][kRedRow];
+template <int kRows>
+struct RedLdsT {
+  float red[kRows][kRedRow];
   float part_hi[kTerms][8], part_lo[kTerms][8];
   unsigned long long wcount[kBlock / kWave];
 };
+using RedLds = RedLdsT<kTerms>;
+using RedLdsHalf = RedLdsT<kTerms / 2>;
+static_assert(kTerms % 2 == 0, "terms split in halves");
 
 // Writes the block partial: 28 (hi, lo) pairs + u64 count. The finalize kernel
 // of this pass reads it after the kernel boundary, so no device-scope fence (an
 // L2 write-back + invalidate on every XCD) is needed here.
-__device__ __forceinline__ void write_partial(RedLds& L, const float (&acc)[kTerms],
+template <int kRows>
+__device__ __forceinline__ void write_partial(RedLdsT<kRows>& L, const float (&acc)[kTerms],
                                               uint32_t count, float* out) {
-#pragma unroll
-  for (int t = 0; t < kTerms; ++t) L.red[t][threadIdx.x] = acc[t];
   unsigned long long c64 = count;
   for (int off = kWave / 2; off > 0; off >>= 1) c64 += __shfl_xor(c64, off);
   if ((threadIdx.x & (kWave - 1)) == 0) L.wcount[threadIdx.x / kWave] = c64;
-  __syncthreads();
-  if (threadIdx.x < kTerms * 8) {
-    const int t = threadIdx.x >> 3, ch = threadIdx.x & 7;
-    Kahan k;
-    k.init();
-    for (int e = 0; e < kBlock / 8; ++e) k.add(L.red[t][e * 8 + ch]);
-    L.part_hi[t][ch] = k.s;
-    L.part_lo[t][ch] = -k.c;
+#pragma unroll
+  for (int r0 = 0; r0 < kTerms; r0 += kRows) {
+    if (r0 > 0) __syncthreads();  // the previous round's rows have been read
+#pragma unroll
+    for (int t = 0; t < kRows; ++t) L.red[t][threadIdx.x] = acc[r0 + t];
+    __syncthreads();
+    if (threadIdx.x < kRows * 8) {
+      const int t = threadIdx.x >> 3, ch = threadIdx.x & 7;
+      Kahan k;
+      k.init();
+      for (int e = 0; e < kBlock / 8; ++e) k.add(L.red[t][e * 8 + ch]);
+      L.part_hi[r0 + t][ch] = k.s;
+      L.part_lo[r0 + t][ch] = -k.c;
+    }
   }
   __syncthreads();
   if (threadIdx.x < kTerms) {
@@ -722,13 +733,74 @@ constexpr uint32_t kHardBlockFlag = 0x80000000u;  // HardQuery.inst: block scann
 #define SR_SEARCH_WAVES 4
 #endif
 
+// Settled blocks: when every point of a one-point-per-thread block keeps its
+// match, its terms are added right away from two rounds of loads (its own
+// points, then the matched target points); the block is flagged done. Other
+// blocks are left to k_gicp_search. Compiled without the search, so more
+// workgroups are resident to hide the two rounds (the block reduction goes
+// through LDS in two halves for the same reason).
 template <bool kStats>
-__global__ void __launch_bounds__(kBlock)
-    __attribute__((amdgpu_waves_per_eu(SR_SEARCH_WAVES))) k_gicp_search(const IterInstance* __restrict__ insts,
+__global__ void __launch_bounds__(kBlock) k_gicp_settle(const IterInstance* __restrict__ insts,
                                                         const uint32_t* __restrict__ block_inst,
                                                         PassScratch ps,
                                                         SearchStats* __restrict__ stats) {
   const uint32_t b = xcd_block();
+  const IterInstance& I = insts[block_inst[b]];
+  const InstState* st = I.state;
+  if (st->phase == 2 || I.ppt != 1) {  // converged (every stage skips it) or searched
+    if (threadIdx.x == 0) ps.done[b] = st->phase == 2 ? 1u : 0u;
+    return;
+  }
+  const uint32_t lb = b - I.block_begin;
+  const float move_rot = st->move_rot, move_const = st->move_const;
+  const uint32_t i = lb * kBlock + threadIdx.x;
+  const bool valid = i < I.n_src;
+  const float4 p = I.src[valid ? i : 0u];
+  const float4 ns = I.src_n[valid ? i : 0u];
+  const uint32_t prev = valid ? I.match[i] : kNone;
+  const float bud = valid ? I.budget[i] : 0.0f;
+  const float move =
+      move_rot * (sqrtf(p.x * p.x + p.y * p.y + p.z * p.z) * 1.000001f) + move_const;
+  const bool keep = prev != kNone && bud > move;
+  const bool settled = __syncthreads_and(!valid || keep);
+  if (threadIdx.x == 0) ps.done[b] = settled ? 1u : 0u;
+  if (!settled) return;
+  const InstConfig cfg = I.cfg;
+  const float beta = cfg.one_minus_eps, eps = 1.0f - cfg.one_minus_eps;
+  PassPose P;
+  load_pose(st, P);
+  float acc[kTerms];
+#pragma unroll
+  for (int t = 0; t < kTerms; ++t) acc[t] = 0.0f;
+  uint32_t count = 0;
+  if (valid) {
+    if constexpr (kStats) stats_add(stats, 0, 0, 0, kBlockDone);
+    float px, py, pz;
+    transform_point(P, p, px, py, pz);
+    const float4 q = I.tgt[prev];
+    const float4 nt = I.tgt_n[prev];
+    const float d2 = dist2(px, py, pz, q.x, q.y, q.z);
+    if (d2 <= cfg.dmax2) {
+      I.budget[i] = bud - move;
+      add_terms(acc, P, beta, eps, px, py, pz, q, nt, ns, d2);
+      ++count;
+    } else {  // the nearest point is beyond d_max: no correspondence, search next time
+      I.match[i] = kNone;
+      I.budget[i] = 0.0f;
+    }
+  }
+  __shared__ RedLdsHalf L;
+  write_partial(L, acc, count, I.partials + size_t(lb) * 64);
+}
+
+// One block of the search: every point's match (and movement budget) is
+// recorded and the uncertified ones are queued; a block none of whose points
+// was queued adds its terms and is flagged done.
+template <bool kStats>
+__device__ __forceinline__ void search_block(const IterInstance* __restrict__ insts,
+                                             const uint32_t* __restrict__ block_inst,
+                                             const PassScratch& ps, SearchStats* __restrict__ stats,
+                                             uint32_t b, RedLds& L) {
   const uint32_t inst_id = block_inst[b];
   const IterInstance& I = insts[inst_id];
   const InstState* st = I.state;
@@ -745,48 +817,6 @@ __global__ void __launch_bounds__(kBlock)
   const bool gap = move_rot * I.src_radius + move_const < kGapMove * g.h;
   const uint32_t ppt = I.ppt;
   const float beta = cfg.one_minus_eps, eps = 1.0f - cfg.one_minus_eps;
-  __shared__ RedLds L;
-
-  // Settled blocks: when every point of a one-point-per-thread block keeps its
-  // match, the search is skipped and the terms are added in the same sweep, so
-  // the block costs two rounds of loads (its own points, then the matched
-  // target points) instead of four. The terms are those loop 2 would add.
-  if (ppt == 1) {
-    const uint32_t i = lb * kBlock + threadIdx.x;
-    const bool valid = i < I.n_src;
-    const float4 p = I.src[valid ? i : 0u];
-    const float4 ns = I.src_n[valid ? i : 0u];
-    const uint32_t prev = valid ? I.match[i] : kNone;
-    const float bud = valid ? I.budget[i] : 0.0f;
-    const float move =
-        move_rot * (sqrtf(p.x * p.x + p.y * p.y + p.z * p.z) * 1.000001f) + move_const;
-    const bool keep = prev != kNone && bud > move;
-    if (__syncthreads_and(!valid || keep)) {
-      if (threadIdx.x == 0) ps.done[b] = 1u;
-      float acc[kTerms];
-#pragma unroll
-      for (int t = 0; t < kTerms; ++t) acc[t] = 0.0f;
-      uint32_t count = 0;
-      if (valid) {
-        if constexpr (kStats) stats_add(stats, 0, 0, 0, kBlockDone);
-        float px, py, pz;
-        transform_point(P, p, px, py, pz);
-        const float4 q = I.tgt[prev];
-        const float4 nt = I.tgt_n[prev];
-        const float d2 = dist2(px, py, pz, q.x, q.y, q.z);
-        if (d2 <= cfg.dmax2) {
-          I.budget[i] = bud - move;
-          add_terms(acc, P, beta, eps, px, py, pz, q, nt, ns, d2);
-          ++count;
-        } else {  // the nearest point is beyond d_max: no correspondence
-          I.match[i] = kNone;
-          I.budget[i] = 0.0f;
-        }
-      }
-      write_partial(L, acc, count, I.partials + size_t(lb) * 64);
-      return;
-    }
-  }
 
   // Loop 1, the search: every point's match (and movement budget) is recorded,
   // and the uncertified ones are queued.
@@ -873,6 +903,32 @@ __global__ void __launch_bounds__(kBlock)
     ++count;
   }
   write_partial(L, acc, count, I.partials + size_t(lb) * 64);
+}
+
+// The blocks k_gicp_settle left, a run of consecutive logical blocks per
+// workgroup (runs kept on one XCD, so neighbouring source tiles share an L2).
+template <bool kStats>
+__global__ void __launch_bounds__(kBlock)
+    __attribute__((amdgpu_waves_per_eu(SR_SEARCH_WAVES))) k_gicp_search(const IterInstance* __restrict__ insts,
+                                                        const uint32_t* __restrict__ block_inst,
+                                                        uint32_t total_blocks, PassScratch ps,
+                                                        SearchStats* __restrict__ stats) {
+#if SR_SEARCH_RUNS
+  const uint32_t w = xcd_block(), G = gridDim.x;
+  const uint32_t per = total_blocks / G, rem = total_blocks % G;
+  const uint32_t b0 = w * per + min(w, rem), b1 = b0 + per + (w < rem ? 1u : 0u);
+  __shared__ RedLds L;
+  for (uint32_t b = b0; b < b1; ++b) {
+    if (ps.done[b]) continue;
+    search_block<kStats>(insts, block_inst, ps, stats, b, L);
+    __syncthreads();  // the LDS is reused by the next block
+  }
+#else
+  const uint32_t b = xcd_block();
+  if (ps.done[b]) return;
+  __shared__ RedLds L;
+  search_block<kStats>(insts, block_inst, ps, stats, b, L);
+#endif
 }
 
 template <bool kStats>
@@ -1040,8 +1096,12 @@ uint32_t hard_stage_blocks(uint64_t points) {
   return uint32_t(std::min<uint64_t>(std::max<uint64_t>(want, 1), 2048));
 }
 
-// Workgroups of the terms stage: about two per CU at a time on 256 CUs, each
-// taking a run of blocks.
+// Workgroups of the search and terms stages, each taking a run of blocks:
+// several rounds of what 256 CUs hold at once, so runs stay short.
+#ifndef SR_SEARCH_RUNS
+#define SR_SEARCH_RUNS 0
+#endif
+constexpr uint32_t kSearchGrid = SR_SEARCH_RUNS ? 8192u : 0xFFFFFFFFu;
 constexpr uint32_t kTermsGrid = 4096;
 
 void gicp_pass(const IterInstance* insts, const uint32_t* block_inst, uint32_t total_blocks,
@@ -1051,13 +1111,18 @@ void gicp_pass(const IterInstance* insts, const uint32_t* block_inst, uint32_t t
   {
     ProfScope prof("gicp_pass", s);
     SearchStats* st = prof_stats("gicp_pass");
+    const dim3 search_grid(std::min<uint32_t>(total_blocks, kSearchGrid));
     if (st) {
-      hipLaunchKernelGGL(k_gicp_search<true>, dim3(total_blocks), dim3(kBlock), 0, s, insts,
+      hipLaunchKernelGGL(k_gicp_settle<true>, dim3(total_blocks), dim3(kBlock), 0, s, insts,
                          block_inst, ps, st);
+      hipLaunchKernelGGL(k_gicp_search<true>, search_grid, dim3(kBlock), 0, s, insts, block_inst,
+                         total_blocks, ps, st);
       hipLaunchKernelGGL(k_gicp_hard<true>, dim3(hard_blocks), dim3(kBlock), 0, s, insts, ps, st);
     } else {
-      hipLaunchKernelGGL(k_gicp_search<false>, dim3(total_blocks), dim3(kBlock), 0, s, insts,
+      hipLaunchKernelGGL(k_gicp_settle<false>, dim3(total_blocks), dim3(kBlock), 0, s, insts,
                          block_inst, ps, st);
+      hipLaunchKernelGGL(k_gicp_search<false>, search_grid, dim3(kBlock), 0, s, insts, block_inst,
+                         total_blocks, ps, st);
       hipLaunchKernelGGL(k_gicp_hard<false>, dim3(hard_blocks), dim3(kBlock), 0, s, insts, ps, st);
     }
     hipLaunchKernelGGL(k_gicp_terms, dim3(std::min<uint32_t>(total_blocks, kTermsGrid)), dim3(kBlock),

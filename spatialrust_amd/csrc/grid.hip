@@ -61,33 +61,27 @@ static void grid_desc(const uint32_t* hb, float h, uint32_t npoints, GridDesc& d
 
 __global__ void k_grid_count(const float4* __restrict__ pts, uint32_t n, GridDesc g,
                              uint32_t* __restrict__ counts, uint32_t* __restrict__ cid,
-                             uint32_t* __restrict__ nonempty) {
+                             uint32_t* __restrict__ rnk) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  bool first = false;
-  if (i < n) {
-    float4 p = pts[i];
-    int cx = cell_coord(p.x, g.ox, g.inv_h, g.dx);
-    int cy = cell_coord(p.y, g.oy, g.inv_h, g.dy);
-    int cz = cell_coord(p.z, g.oz, g.inv_h, g.dz);
-    uint32_t c = (uint32_t(cz) * uint32_t(g.dy) + uint32_t(cy)) * uint32_t(g.dx) + uint32_t(cx);
-    uint32_t old = atomicAdd(&counts[c], 1u);
-    if (cid) cid[i] = c;
-    first = old == 0;
-  }
-  if (nonempty) {
-    // one atomic per wave instead of one per newly touched cell
-    const unsigned long long m = __ballot(first);
-    if ((threadIdx.x & (kWave - 1)) == 0 && m) atomicAdd(nonempty, uint32_t(__popcll(m)));
-  }
+  if (i >= n) return;
+  float4 p = pts[i];
+  int cx = cell_coord(p.x, g.ox, g.inv_h, g.dx);
+  int cy = cell_coord(p.y, g.oy, g.inv_h, g.dy);
+  int cz = cell_coord(p.z, g.oz, g.inv_h, g.dz);
+  uint32_t c = (uint32_t(cz) * uint32_t(g.dy) + uint32_t(cy)) * uint32_t(g.dx) + uint32_t(cx);
+  rnk[i] = atomicAdd(&counts[c], 1u);
+  cid[i] = c;
 }
 
-__global__ void k_grid_scatter(const uint32_t* __restrict__ cid, uint32_t n,
-                               const uint32_t* __restrict__ start, uint32_t* __restrict__ cursor,
+// Point i goes to its cell's range at the slot its count increment returned
+// (arrival order; k_sort_cells restores index order within each cell).
+__global__ void k_grid_scatter(const uint32_t* __restrict__ cid, const uint32_t* __restrict__ rnk,
+                               uint32_t n, const uint32_t* __restrict__ start,
                                uint32_t* __restrict__ order) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint32_t c = cid[i];
-  order[start[c] + atomicAdd(&cursor[c], 1u)] = i;
+  order[start[c] + rnk[i]] = i;
 }
 
 constexpr uint32_t kSmallSeg = 48;
@@ -232,11 +226,10 @@ void gather_to_grid_order(const Grid& g, const float4* orig, float4* out, hipStr
                      g.order.p, n, out);
 }
 
-// Picks the cell edge from the measured mean occupancy of non-empty cells:
-// surface clouds fill cells in proportion to h^2, so one correction from a
-// first guess lands close to the requested occupancy.
-// The cell-edge choice in three host steps, shared by the one-cloud and the
-// many-cloud builds so both pick the same edge for the same cloud.
+// The cell edge is picked from the measured mean occupancy of non-empty cells
+// at candidate edges around a first guess (sketches below); the host steps are
+// shared by the one-cloud and the many-cloud builds, so both pick the same edge
+// for the same cloud.
 // First guess; returns false when the cloud has no extent (h is then final).
 static bool edge_first_guess(const uint32_t* hb, uint32_t n, float occupancy, float leaf_hint,
                              float& h) {
@@ -274,36 +267,197 @@ static float edge_fit_cap(const uint32_t* hb, float h, uint32_t n) {
   return h;
 }
 
-// One measurement: ne occupied cells at edge h. Returns true when h is kept.
-static bool edge_correct(uint32_t n, uint32_t ne, float occupancy, float& h) {
-  float occ = float(n) / float(std::max(ne, 1u));
-  float scale = std::sqrt(occupancy / occ);
-  scale = std::min(std::max(scale, 0.25f), 4.0f);
-  if (std::fabs(scale - 1.0f) < 0.15f) return true;
-  h *= scale;
-  return false;
+// ---- edge choice from distinct-cell sketches --------------------------------
+//
+// The occupancy of non-empty cells at edge h is n / (number of distinct cells),
+// which for a sparse cloud does not follow the h^2 surface law near the point
+// spacing, so a guess has to be measured. One pass over the points measures it
+// at kSketchEdges candidate edges at once: every point files itself into the
+// cell it falls in at each edge and updates a HyperLogLog sketch (Flajolet et
+// al., 2007) of that edge's distinct cells in LDS. A workgroup's registers are
+// stored as bytes; one workgroup per cloud takes the register-wise maximum over
+// its workgroups (order-free, so deterministic) and turns it into estimates.
+// The register arrays are small, so no atomics go to device memory (counting
+// distinct cells exactly needs a cell-sized array of device-wide atomics).
+
+constexpr int kSketchEdges = 8;         // candidate edges per cloud
+constexpr int kSketchRegs = 256;        // registers per sketch: ~6.5% standard error
+constexpr uint32_t kSketchChunk = 4096; // points per workgroup, at least
+constexpr uint32_t kSketchMaxWg = 64;   // workgroups per cloud, at most
+constexpr float kSketchStep = 1.6f;     // ratio of consecutive candidate edges
+constexpr int kSketchBelow = 2;         // candidates below the first guess
+
+struct SketchSeg {
+  float ox, oy, oz;
+  float inv_h[kSketchEdges];
+  uint32_t pt_off;  // first point
+  uint32_t n;       // points
+  uint32_t wg_off;  // first workgroup
+  uint32_t nwg;     // workgroups
+  uint32_t chunk;   // points per workgroup
+};
+
+// 64-bit finalizer of MurmurHash3 (a bijection that spreads every input bit).
+__device__ __forceinline__ unsigned long long sketch_mix(unsigned long long k) {
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdull;
+  k ^= k >> 33;
+  k *= 0xc4ceb9fe1a85ec53ull;
+  k ^= k >> 33;
+  return k;
+}
+
+__device__ __forceinline__ unsigned long long sketch_coord(float v, float o, float inv_h) {
+  const float f = floorf((v - o) * inv_h);
+  return f <= 0.0f ? 0ull : (f >= 2097151.0f ? 2097151ull : (unsigned long long)f);
+}
+
+__global__ void __launch_bounds__(kBlock) k_edge_sketch(const float4* __restrict__ pts,
+                                                        const SketchSeg* __restrict__ segs,
+                                                        uint32_t nseg,
+                                                        const uint32_t* __restrict__ wg_begin,
+                                                        uint8_t* __restrict__ regs_out) {
+  __shared__ uint32_t reg[kSketchEdges][kSketchRegs];
+  for (int t = threadIdx.x; t < kSketchEdges * kSketchRegs; t += blockDim.x) (&reg[0][0])[t] = 0u;
+  __syncthreads();
+  const uint32_t s = seg_of(wg_begin, nseg, blockIdx.x);
+  const SketchSeg& S = segs[s];
+  const uint32_t first = (blockIdx.x - S.wg_off) * S.chunk;
+  const uint32_t last = min(first + S.chunk, S.n);
+  for (uint32_t i = first + threadIdx.x; i < last; i += blockDim.x) {
+    const float4 p = pts[S.pt_off + i];
+#pragma unroll
+    for (int j = 0; j < kSketchEdges; ++j) {
+      const float ih = S.inv_h[j];
+      const unsigned long long key = sketch_coord(p.x, S.ox, ih) |
+                                     (sketch_coord(p.y, S.oy, ih) << 21) |
+                                     (sketch_coord(p.z, S.oz, ih) << 42);
+      const unsigned long long hv = sketch_mix(key);
+      const uint32_t idx = uint32_t(hv >> 56);
+      const unsigned long long rest = hv << 8;
+      const uint32_t rho = rest ? uint32_t(__clzll((long long)rest)) + 1u : 57u;
+      atomicMax(&reg[j][idx], rho);
+    }
+  }
+  __syncthreads();
+  uint8_t* out = regs_out + size_t(blockIdx.x) * (kSketchEdges * kSketchRegs);
+  for (int t = threadIdx.x; t < kSketchEdges * kSketchRegs; t += blockDim.x)
+    out[t] = uint8_t((&reg[0][0])[t]);
+}
+
+// One workgroup per (cloud, candidate edge), a thread per register: the
+// estimate of distinct cells, with the small-range correction (linear counting).
+__global__ void __launch_bounds__(kSketchRegs) k_edge_sketch_merge(
+    const uint8_t* __restrict__ regs, const SketchSeg* __restrict__ segs,
+    float* __restrict__ est) {
+  const uint32_t seg = blockIdx.x / kSketchEdges, j = blockIdx.x % kSketchEdges;
+  const SketchSeg& S = segs[seg];
+  __shared__ double inv_sum[kSketchRegs];
+  __shared__ uint32_t zeros[kSketchRegs];
+  const uint32_t m = threadIdx.x;
+  const uint8_t* base = regs + (size_t(S.wg_off) * kSketchEdges + j) * kSketchRegs + m;
+  uint32_t r = 0;
+#pragma unroll 8
+  for (uint32_t w = 0; w < S.nwg; ++w)
+    r = max(r, uint32_t(base[size_t(w) * (kSketchEdges * kSketchRegs)]));
+  inv_sum[m] = ldexp(1.0, -int(r));
+  zeros[m] = r == 0 ? 1u : 0u;
+  __syncthreads();
+  for (uint32_t h = kSketchRegs / 2; h > 0; h >>= 1) {
+    if (m < h) {
+      inv_sum[m] += inv_sum[m + h];
+      zeros[m] += zeros[m + h];
+    }
+    __syncthreads();
+  }
+  if (m == 0) {
+    const double M = double(kSketchRegs);
+    const double alpha = 0.7213 / (1.0 + 1.079 / M);
+    double e = alpha * M * M / inv_sum[0];
+    if (e <= 2.5 * M && zeros[0] > 0) e = M * log(M / double(zeros[0]));
+    est[blockIdx.x] = float(e);
+  }
+}
+
+// Candidate edges around a first guess g: g * kSketchStep^(j - kSketchBelow).
+static void sketch_candidates(float g, float* hc) {
+  for (int j = 0; j < kSketchEdges; ++j)
+    hc[j] = g * std::pow(kSketchStep, float(j - kSketchBelow));
+}
+
+// The edge whose estimated occupancy n / cells meets `occupancy`: log-log
+// interpolation between the bracketing candidates; beyond the candidates, the
+// surface law (occupancy ~ h^2) from the nearest one, within a factor of 4.
+static float edge_from_sketch(uint32_t n, const float* hc, const float* est, float occupancy) {
+  float o[kSketchEdges];
+  for (int j = 0; j < kSketchEdges; ++j) o[j] = float(n) / std::max(est[j], 1.0f);
+  if (o[0] >= occupancy) {
+    const float f = std::max(std::sqrt(occupancy / o[0]), 0.25f);
+    return hc[0] * f;
+  }
+  for (int j = 0; j + 1 < kSketchEdges; ++j) {
+    if (o[j + 1] < occupancy) continue;
+    const float lo = std::log(std::max(o[j], 1e-6f)), hi = std::log(o[j + 1]);
+    if (!(hi > lo)) return hc[j + 1];
+    const float t = (std::log(occupancy) - lo) / (hi - lo);
+    return hc[j] * std::pow(kSketchStep, std::min(std::max(t, 0.0f), 1.0f));
+  }
+  const float f = std::min(std::sqrt(occupancy / std::max(o[kSketchEdges - 1], 1e-6f)), 4.0f);
+  return hc[kSketchEdges - 1] * f;
+}
+
+// Sketches segments `ids` of a cloud at their candidate edges (hc: kSketchEdges
+// per listed segment) and returns the chosen edges in h[ids[k]].
+static void sketch_edges(const float4* pts, const std::vector<uint32_t>& off,
+                         const std::vector<uint32_t>& hb, const std::vector<uint32_t>& ids,
+                         const std::vector<float>& hc, float occupancy, std::vector<float>& h,
+                         hipStream_t s) {
+  const uint32_t m = uint32_t(ids.size());
+  if (m == 0) return;
+  std::vector<SketchSeg> segs(m);
+  std::vector<uint32_t> wg_begin(m);
+  uint32_t wgs = 0;
+  for (uint32_t k = 0; k < m; ++k) {
+    const uint32_t g = ids[k];
+    SketchSeg& S = segs[k];
+    S.ox = ord2f(hb[6 * g + 0]);
+    S.oy = ord2f(hb[6 * g + 1]);
+    S.oz = ord2f(hb[6 * g + 2]);
+    for (int j = 0; j < kSketchEdges; ++j) S.inv_h[j] = 1.0f / hc[size_t(k) * kSketchEdges + j];
+    S.pt_off = off[g];
+    S.n = off[g + 1] - off[g];
+    S.wg_off = wgs;
+    S.nwg = std::min(std::max<uint32_t>(1, (S.n + kSketchChunk - 1) / kSketchChunk), kSketchMaxWg);
+    S.chunk = (S.n + S.nwg - 1) / S.nwg;
+    wg_begin[k] = wgs;
+    wgs += S.nwg;
+  }
+  DevBuf<SketchSeg> d_segs(m);
+  DevBuf<uint32_t> d_wg(m);
+  DevBuf<uint8_t> regs(size_t(wgs) * kSketchEdges * kSketchRegs);
+  DevBuf<float> est(size_t(m) * kSketchEdges);
+  SR_HIP(hipMemcpyAsync(d_segs.p, segs.data(), sizeof(SketchSeg) * m, hipMemcpyHostToDevice, s));
+  SR_HIP(hipMemcpyAsync(d_wg.p, wg_begin.data(), sizeof(uint32_t) * m, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_edge_sketch, dim3(wgs), dim3(kBlock), 0, s, pts, d_segs.p, m, d_wg.p, regs.p);
+  hipLaunchKernelGGL(k_edge_sketch_merge, dim3(m * kSketchEdges), dim3(kSketchRegs), 0, s, regs.p,
+                     d_segs.p, est.p);
+  std::vector<float> host(size_t(m) * kSketchEdges);
+  SR_HIP(hipMemcpyAsync(host.data(), est.p, sizeof(float) * host.size(), hipMemcpyDeviceToHost, s));
+  SR_HIP(hipStreamSynchronize(s));  // also keeps the tables alive until the kernels end
+  for (uint32_t k = 0; k < m; ++k)
+    h[ids[k]] = edge_from_sketch(segs[k].n, &hc[size_t(k) * kSketchEdges],
+                                 &host[size_t(k) * kSketchEdges], occupancy);
 }
 
 float choose_cell_edge(const Cloud& c, const uint32_t* hb, float occupancy, float leaf_hint) {
-  hipStream_t s = thread_stream();
-  float h = 1.0f;
-  if (!edge_first_guess(hb, c.n, occupancy, leaf_hint, h)) return h;
-  DevBuf<uint32_t> nonempty(1);
-  for (int pass = 0; pass < 2; ++pass) {
-    h = edge_fit_cap(hb, h, c.n);  // keep the dense grid under the cell cap
-    GridDesc d;
-    grid_desc(hb, h, c.n, d);
-    DevBuf<uint32_t> counts(d.ncells);
-    fill_u32(counts.p, 0u, d.ncells, s);
-    fill_u32(nonempty.p, 0u, 1, s);
-    hipLaunchKernelGGL(k_grid_count, dim3((c.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
-                       c.pts.p, c.n, d, counts.p, (uint32_t*)nullptr, nonempty.p);
-    uint32_t ne = 0;
-    SR_HIP(hipMemcpyAsync(&ne, nonempty.p, 4, hipMemcpyDeviceToHost, s));
-    SR_HIP(hipStreamSynchronize(s));
-    if (edge_correct(c.n, ne, occupancy, h)) break;
-  }
-  return edge_fit_cap(hb, h, c.n);
+  std::vector<float> h(1, 1.0f);
+  if (!edge_first_guess(hb, c.n, occupancy, leaf_hint, h[0])) return h[0];
+  std::vector<float> hc(kSketchEdges);
+  sketch_candidates(h[0], hc.data());
+  const std::vector<uint32_t> off = {0u, c.n}, ids = {0u};
+  const std::vector<uint32_t> hbv(hb, hb + 6);
+  sketch_edges(c.pts.p, off, hbv, ids, hc, occupancy, h, thread_stream());
+  return edge_fit_cap(hb, h[0], c.n);
 }
 
 void build_grid(const Cloud& c, const uint32_t* hb, float h, Grid& g, hipStream_t s) {
@@ -317,12 +471,11 @@ void build_grid(const Cloud& c, const uint32_t* hb, float h, Grid& g, hipStream_
   g.cell_start.resize(size_t(nc) + 1);
   g.pts.resize(n > 0 ? n : 1);
   g.order.resize(n > 0 ? n : 1);
-  DevBuf<uint32_t> counts(size_t(nc) + 1), cid(n > 0 ? n : 1), cursor(nc), scal(2);
+  DevBuf<uint32_t> counts(size_t(nc) + 1), cid(n > 0 ? n : 1), rnk(n > 0 ? n : 1), scal(2);
   fill_u32(counts.p, 0u, size_t(nc) + 1, s);
-  fill_u32(cursor.p, 0u, nc, s);
   if (n > 0) {
     hipLaunchKernelGGL(k_grid_count, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, c.pts.p, n,
-                       g.desc, counts.p, cid.p, (uint32_t*)nullptr);
+                       g.desc, counts.p, cid.p, rnk.p);
   }
   exclusive_scan_u32(counts.p, g.cell_start.p, nc + 1, nullptr, s);
   {
@@ -340,8 +493,8 @@ void build_grid(const Cloud& c, const uint32_t* hb, float h, Grid& g, hipStream_
                        g.occ.p, g.desc, uint32_t(nrw), g.rowocc.p);
   }
   if (n == 0) return;
-  hipLaunchKernelGGL(k_grid_scatter, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, cid.p, n,
-                     g.cell_start.p, cursor.p, g.order.p);
+  hipLaunchKernelGGL(k_grid_scatter, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, cid.p,
+                     rnk.p, n, g.cell_start.p, g.order.p);
   fill_u32(scal.p, 0u, 1, s);
   DevBuf<uint32_t> big(std::max<uint32_t>(nc, 1));
   hipLaunchKernelGGL(k_sort_cells, dim3((nc + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
@@ -350,7 +503,7 @@ void build_grid(const Cloud& c, const uint32_t* hb, float h, Grid& g, hipStream_
                      big.p, scal.p);
   hipLaunchKernelGGL(k_grid_gather, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, c.pts.p,
                      g.order.p, n, g.pts.p);
-  // counts/cid/cursor/big return to the cache; ordering on s keeps reuse safe.
+  // counts/cid/rnk/big return to the cache; ordering on s keeps reuse safe.
 }
 
 // Chebyshev distance transform of occupancy, separable: along one axis
@@ -420,26 +573,19 @@ struct SegGrid {
 __global__ void k_seg_grid_count(const float4* __restrict__ pts, const SegGrid* __restrict__ sg,
                                  uint32_t nseg, const uint32_t* __restrict__ blk_begin,
                                  uint32_t* __restrict__ counts, uint32_t* __restrict__ cid,
-                                 uint32_t* __restrict__ nonempty) {
+                                 uint32_t* __restrict__ rnk) {
   const uint32_t s = seg_of(blk_begin, nseg, blockIdx.x);
   const SegGrid& G = sg[s];
   const uint32_t il = (blockIdx.x - blk_begin[s]) * blockDim.x + threadIdx.x;
-  bool first = false;
-  if (il < G.n) {
-    const GridDesc& g = G.d;
-    const float4 p = pts[G.pt_off + il];
-    int cx = cell_coord(p.x, g.ox, g.inv_h, g.dx);
-    int cy = cell_coord(p.y, g.oy, g.inv_h, g.dy);
-    int cz = cell_coord(p.z, g.oz, g.inv_h, g.dz);
-    uint32_t c = G.cell_off + (uint32_t(cz) * uint32_t(g.dy) + uint32_t(cy)) * uint32_t(g.dx) + uint32_t(cx);
-    uint32_t old = atomicAdd(&counts[c], 1u);
-    if (cid) cid[G.pt_off + il] = c;
-    first = old == 0;
-  }
-  if (nonempty) {
-    const unsigned long long m = __ballot(first);
-    if ((threadIdx.x & (kWave - 1)) == 0 && m) atomicAdd(&nonempty[s], uint32_t(__popcll(m)));
-  }
+  if (il >= G.n) return;
+  const GridDesc& g = G.d;
+  const float4 p = pts[G.pt_off + il];
+  int cx = cell_coord(p.x, g.ox, g.inv_h, g.dx);
+  int cy = cell_coord(p.y, g.oy, g.inv_h, g.dy);
+  int cz = cell_coord(p.z, g.oz, g.inv_h, g.dz);
+  uint32_t c = G.cell_off + (uint32_t(cz) * uint32_t(g.dy) + uint32_t(cy)) * uint32_t(g.dx) + uint32_t(cx);
+  rnk[G.pt_off + il] = atomicAdd(&counts[c], 1u);
+  cid[G.pt_off + il] = c;
 }
 
 // Per segment: cell_start entries from the flat exclusive scan, minus the
@@ -525,50 +671,22 @@ static std::vector<uint32_t> blocks_of(const std::vector<uint32_t>& off) {
 void choose_cell_edges_many(const Cloud& c, const std::vector<uint32_t>& off,
                             const std::vector<uint32_t>& hb, float occupancy, float leaf_hint,
                             std::vector<float>& h, hipStream_t s) {
+  // choose_cell_edge per segment, with the sketches of all segments in one launch
   const uint32_t nseg = uint32_t(off.size() - 1);
   h.assign(nseg, 1.0f);
-  std::vector<char> open(nseg, 0);
-  for (uint32_t g = 0; g < nseg; ++g)
-    open[g] = edge_first_guess(&hb[6 * g], off[g + 1] - off[g], occupancy, leaf_hint, h[g]) ? 1 : 0;
-  for (int pass = 0; pass < 2; ++pass) {
-    // one trial count over the segments still open, as choose_cell_edge's
-    std::vector<uint32_t> seg_ids, sub_off(1, 0);
-    std::vector<SegGrid> sg;
-    uint64_t cells = 0;
-    for (uint32_t g = 0; g < nseg; ++g) {
-      if (!open[g]) continue;
-      h[g] = edge_fit_cap(&hb[6 * g], h[g], off[g + 1] - off[g]);
-      SegGrid G{};
-      grid_desc(&hb[6 * g], h[g], off[g + 1] - off[g], G.d);
-      G.pt_off = off[g];
-      G.n = off[g + 1] - off[g];
-      G.cell_off = uint32_t(cells);
-      cells += G.d.ncells;
-      if (cells > 0xFFFFFFFFull) fail(3, "too many grid cells for one batch");
-      sg.push_back(G);
-      seg_ids.push_back(g);
-      sub_off.push_back(sub_off.back() + G.n);
-    }
-    if (sg.empty()) break;
-    const uint32_t m = uint32_t(sg.size());
-    std::vector<uint32_t> blk = blocks_of(sub_off);
-    DevBuf<SegGrid> d_sg;
-    DevBuf<uint32_t> d_blk, counts(std::max<uint64_t>(cells, 1)), nonempty(m);
-    upload_vec(d_sg, sg, s);
-    upload_vec(d_blk, blk, s);
-    fill_u32(counts.p, 0u, cells, s);
-    fill_u32(nonempty.p, 0u, m, s);
-    hipLaunchKernelGGL(k_seg_grid_count, dim3(blk.back()), dim3(kBlock), 0, s, c.pts.p, d_sg.p, m,
-                       d_blk.p, counts.p, (uint32_t*)nullptr, nonempty.p);
-    std::vector<uint32_t> ne(m);
-    SR_HIP(hipMemcpyAsync(ne.data(), nonempty.p, sizeof(uint32_t) * m, hipMemcpyDeviceToHost, s));
-    SR_HIP(hipStreamSynchronize(s));
-    for (uint32_t j = 0; j < m; ++j) {
-      const uint32_t g = seg_ids[j];
-      if (edge_correct(off[g + 1] - off[g], ne[j], occupancy, h[g])) open[g] = 0;
-    }
+  std::vector<uint32_t> ids;
+  std::vector<float> hc;
+  for (uint32_t g = 0; g < nseg; ++g) {
+    if (!edge_first_guess(&hb[6 * g], off[g + 1] - off[g], occupancy, leaf_hint, h[g])) continue;
+    ids.push_back(g);
+    hc.resize(hc.size() + kSketchEdges);
+    sketch_candidates(h[g], &hc[hc.size() - kSketchEdges]);
   }
-  for (uint32_t g = 0; g < nseg; ++g) h[g] = edge_fit_cap(&hb[6 * g], h[g], off[g + 1] - off[g]);
+  sketch_edges(c.pts.p, off, hb, ids, hc, occupancy, h, s);
+  for (uint32_t k = 0; k < ids.size(); ++k) {
+    const uint32_t g = ids[k];
+    h[g] = edge_fit_cap(&hb[6 * g], h[g], off[g + 1] - off[g]);
+  }
 }
 
 void build_grids_many(const Cloud& c, const std::vector<uint32_t>& off,
@@ -629,11 +747,10 @@ void build_grids_many(const Cloud& c, const std::vector<uint32_t>& off,
   upload_vec(d_cell, cell_begin, s);
   upload_vec(d_pt, off, s);
   DevBuf<uint32_t> counts(size_t(ncells) + 1), csg(size_t(ncells) + 1), cid(std::max<uint32_t>(n, 1)),
-      cursor(std::max<uint32_t>(ncells, 1)), order(std::max<uint32_t>(n, 1)), scal(2);
+      rnk(std::max<uint32_t>(n, 1)), order(std::max<uint32_t>(n, 1)), scal(2);
   fill_u32(counts.p, 0u, size_t(ncells) + 1, s);
-  fill_u32(cursor.p, 0u, ncells, s);
   hipLaunchKernelGGL(k_seg_grid_count, dim3(blk.back()), dim3(kBlock), 0, s, c.pts.p, d_sg.p, nseg,
-                     d_blk.p, counts.p, cid.p, (uint32_t*)nullptr);
+                     d_blk.p, counts.p, cid.p, rnk.p);
   exclusive_scan_u32(counts.p, csg.p, ncells + 1, nullptr, s);
   auto grid1 = [](uint64_t t) { return dim3(uint32_t(std::max<uint64_t>((t + kBlock - 1) / kBlock, 1))); };
   hipLaunchKernelGGL(k_seg_local_cs, grid1(ncs), dim3(kBlock), 0, s, csg.p, d_sg.p, nseg, d_cs.p, ncs,
@@ -645,7 +762,7 @@ void build_grids_many(const Cloud& c, const std::vector<uint32_t>& off,
     hipLaunchKernelGGL(k_seg_grid_rowbits, grid1(rws), dim3(kBlock), 0, s, out.occ.p, d_sg.p, nseg,
                        d_row.p, uint32_t(rws), out.rowocc.p);
   if (n) {
-    hipLaunchKernelGGL(k_grid_scatter, grid1(n), dim3(kBlock), 0, s, cid.p, n, csg.p, cursor.p, order.p);
+    hipLaunchKernelGGL(k_grid_scatter, grid1(n), dim3(kBlock), 0, s, cid.p, rnk.p, n, csg.p, order.p);
     fill_u32(scal.p, 0u, 1, s);
     DevBuf<uint32_t> big(std::max<uint32_t>(ncells, 1));
     hipLaunchKernelGGL(k_sort_cells, grid1(ncells), dim3(kBlock), 0, s, csg.p, ncells, order.p,

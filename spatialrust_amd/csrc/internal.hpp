@@ -200,7 +200,7 @@ struct GridSet {
   DevBuf<uint8_t> empty_radius;
 };
 // choose_cell_edge for every segment (hb: 6 bound words per segment), with the
-// trial counts of all segments in one launch per round; same edges.
+// occupancy sketches of all segments in one launch; same edges.
 void choose_cell_edges_many(const Cloud& c, const std::vector<uint32_t>& off,
                             const std::vector<uint32_t>& hb, float occupancy, float leaf_hint,
                             std::vector<float>& h, hipStream_t s);
@@ -212,8 +212,8 @@ void build_grids_many(const Cloud& c, const std::vector<uint32_t>& off,
                       uint32_t empty_segs, GridSet& out, hipStream_t s);
 
 // Chooses a cell edge so that occupied cells hold about `occupancy` points
-// (measured, starting from a guess; leaf_hint > 0 says the cloud holds one
-// point per voxel of that edge, which gives a closer first guess).
+// (estimated at candidate edges around a guess; leaf_hint > 0 says the cloud
+// holds at most one point per voxel of that edge, which centres the guess).
 float choose_cell_edge(const Cloud& c, const uint32_t* host_bounds, float occupancy,
                        float leaf_hint = 0.0f);
 // Builds a dense counting-sort grid with edge h over a device cloud (points in
