@@ -576,16 +576,20 @@ struct LoopArgs {
   DevBuf<uint32_t> ctl, done;
 };
 
-// Grows a pass scratch to `points` queue slots and `blocks` flags; the queue
-// length starts at zero (the finalize kernel keeps it so between passes).
+// Grows a pass scratch to `points` queue slots and `blocks` flags and masks
+// (the masks, four u64 words a block, follow the flags in the same buffer);
+// the queue length starts at zero (the finalize kernel keeps it so between
+// passes).
 static PassScratch grow_scratch(DevBuf<HardQuery>& queue, DevBuf<uint32_t>& ctl,
                                 DevBuf<uint32_t>& done, uint64_t points, uint32_t blocks,
                                 hipStream_t s) {
   queue.resize(std::max<uint64_t>(points, 1));
-  done.resize(std::max<uint32_t>(blocks, 1));
+  const uint64_t flags = (uint64_t(std::max<uint32_t>(blocks, 1)) + 1) & ~1ull;  // 8-byte aligned
+  done.resize(flags + 8 * uint64_t(std::max<uint32_t>(blocks, 1)));
   ctl.resize(1);
   fill_u32(ctl.p, 0u, 1, s);
-  return PassScratch{queue.p, ctl.p, done.p};
+  return PassScratch{queue.p, ctl.p, done.p,
+                     reinterpret_cast<unsigned long long*>(done.p + flags)};
 }
 static LoopArgs& thread_loop_args() {
   static thread_local LoopArgs* a = new LoopArgs();  // lives until the thread ends

@@ -545,12 +545,14 @@ __device__ __forceinline__ void combined_inverse(float ux, float uy, float uz, f
   M[5] = ia + k * wz * wz;
 }
 
-// ---- the pass in three stages ---------------------------------------------
+// ---- the pass in four stages ----------------------------------------------
 //
-// k_gicp_search: per source point, p' = R p + t and phase 1. Certified points
+// k_gicp_settle: per source point, the match the last search certified for
+//   the pose's movement so far is kept; the other points are marked for the
+//   search. A block whose points all keep their matches adds its terms, writes
+//   its partial and flags itself done.
+// k_gicp_search: per marked point, p' = R p + t and phase 1. Certified points
 //   record their match; uncertified ones are queued (wave-aggregated append).
-//   A block none of whose points was queued also adds its terms and writes its
-//   partial, and flags itself done.
 // k_gicp_hard: drains the queue, kWave/kNNGroup queries per wave, so the
 //   clustered hard queries spread over the whole chip instead of lengthening
 //   the waves that hold them.
@@ -733,12 +735,16 @@ constexpr uint32_t kHardBlockFlag = 0x80000000u;  // HardQuery.inst: block scann
 #define SR_SEARCH_WAVES 4
 #endif
 
-// Settled blocks: when every point of a one-point-per-thread block keeps its
-// match, its terms are added right away from two rounds of loads (its own
-// points, then the matched target points); the block is flagged done. Other
-// blocks are left to k_gicp_search. Compiled without the search, so more
-// workgroups are resident to hide the two rounds (the block reduction goes
-// through LDS in two halves for the same reason).
+// First stage. One-point-per-thread blocks: each point whose match the last
+// search certified for the pose's movement so far keeps it (budget spent, or
+// the match dropped when beyond d_max); the others are marked in the block's
+// mask for the search stage. A block whose points all keep their matches adds
+// its terms right away from two rounds of loads (its own points, then the
+// matched target points) and is flagged done; the other blocks get their
+// partials from k_gicp_terms. Blocks of several points per thread are left to
+// the search stage whole. Compiled without the search, so many workgroups are
+// resident to hide the two rounds (the block reduction goes through LDS in two
+// halves for the same reason).
 template <bool kStats>
 __global__ void __launch_bounds__(kBlock) k_gicp_settle(const IterInstance* __restrict__ insts,
                                                         const uint32_t* __restrict__ block_inst,
@@ -747,12 +753,15 @@ __global__ void __launch_bounds__(kBlock) k_gicp_settle(const IterInstance* __re
   const uint32_t b = xcd_block();
   const IterInstance& I = insts[block_inst[b]];
   const InstState* st = I.state;
-  if (st->phase == 2 || I.ppt != 1) {  // converged (every stage skips it) or searched
+  if (st->phase == 2 || I.ppt != 1) {  // converged (every stage skips it) or searched whole
     if (threadIdx.x == 0) ps.done[b] = st->phase == 2 ? 1u : 0u;
     return;
   }
   const uint32_t lb = b - I.block_begin;
   const float move_rot = st->move_rot, move_const = st->move_const;
+  const InstConfig cfg = I.cfg;
+  PassPose P;
+  load_pose(st, P);
   const uint32_t i = lb * kBlock + threadIdx.x;
   const bool valid = i < I.n_src;
   const float4 p = I.src[valid ? i : 0u];
@@ -761,46 +770,208 @@ __global__ void __launch_bounds__(kBlock) k_gicp_settle(const IterInstance* __re
   const float bud = valid ? I.budget[i] : 0.0f;
   const float move =
       move_rot * (sqrtf(p.x * p.x + p.y * p.y + p.z * p.z) * 1.000001f) + move_const;
-  const bool keep = prev != kNone && bud > move;
-  const bool settled = __syncthreads_and(!valid || keep);
-  if (threadIdx.x == 0) ps.done[b] = settled ? 1u : 0u;
-  if (!settled) return;
-  const InstConfig cfg = I.cfg;
-  const float beta = cfg.one_minus_eps, eps = 1.0f - cfg.one_minus_eps;
-  PassPose P;
-  load_pose(st, P);
-  float acc[kTerms];
-#pragma unroll
-  for (int t = 0; t < kTerms; ++t) acc[t] = 0.0f;
-  uint32_t count = 0;
-  if (valid) {
+  const bool keep = valid && prev != kNone && bud > move;
+  const bool search = valid && !keep;
+  const unsigned long long sm = __ballot(search);
+  if ((threadIdx.x & (kWave - 1)) == 0) ps.pend[size_t(b) * (kBlock / kWave) + threadIdx.x / kWave] = sm;
+  float px = 0.0f, py = 0.0f, pz = 0.0f, d2 = 0.0f;
+  float4 q = p, nt = p;
+  bool counted = false;
+  if (keep) {
     if constexpr (kStats) stats_add(stats, 0, 0, 0, kBlockDone);
-    float px, py, pz;
     transform_point(P, p, px, py, pz);
-    const float4 q = I.tgt[prev];
-    const float4 nt = I.tgt_n[prev];
-    const float d2 = dist2(px, py, pz, q.x, q.y, q.z);
+    q = I.tgt[prev];
+    nt = I.tgt_n[prev];
+    d2 = dist2(px, py, pz, q.x, q.y, q.z);
     if (d2 <= cfg.dmax2) {
       I.budget[i] = bud - move;
-      add_terms(acc, P, beta, eps, px, py, pz, q, nt, ns, d2);
-      ++count;
+      counted = true;
     } else {  // the nearest point is beyond d_max: no correspondence, search next time
       I.match[i] = kNone;
       I.budget[i] = 0.0f;
     }
   }
+  const bool settled = __syncthreads_and(search ? 0 : 1);
+  if (threadIdx.x == 0) ps.done[b] = settled ? 1u : 0u;
+  if (!settled) return;
+  const float beta = cfg.one_minus_eps, eps = 1.0f - cfg.one_minus_eps;
+  float acc[kTerms];
+#pragma unroll
+  for (int t = 0; t < kTerms; ++t) acc[t] = 0.0f;
+  uint32_t count = 0;
+  if (counted) {
+    add_terms(acc, P, beta, eps, px, py, pz, q, nt, ns, d2);
+    ++count;
+  }
   __shared__ RedLdsHalf L;
   write_partial(L, acc, count, I.partials + size_t(lb) * 64);
 }
 
-// One block of the search: every point's match (and movement budget) is
-// recorded and the uncertified ones are queued; a block none of whose points
-// was queued adds its terms and is flagged done.
+// One block of several points per thread: every point's match (and movement
+// budget) is kept or searched anew and recorded, and the uncertified ones are
+// queued; the block's terms come from k_gicp_terms.
 template <bool kStats>
 __device__ __forceinline__ void search_block(const IterInstance* __restrict__ insts,
                                              const uint32_t* __restrict__ block_inst,
                                              const PassScratch& ps, SearchStats* __restrict__ stats,
-                                             uint32_t b, RedLds& L) {
+                                             uint32_t b) {
+  const uint32_t inst_id = block_inst[b];
+  const IterInstance& I = insts[inst_id];
+  const InstState* st = I.state;
+  if (st->phase == 2) return;
+  const uint32_t lb = b - I.block_begin;
+  const GridDesc g = I.grid;
+  const InstConfig cfg = I.cfg;
+  PassPose P;
+  load_pose(st, P);
+  const int lane = int(threadIdx.x & (kWave - 1));
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const float move_rot = st->move_rot, move_const = st->move_const;
+  // budgets are worth computing once no source point moves kGapMove cells
+  const bool gap = move_rot * I.src_radius + move_const < kGapMove * g.h;
+  const uint32_t ppt = I.ppt;
+  for (uint32_t k = 0; k < ppt; ++k) {
+    const uint32_t i = (lb * ppt + k) * kBlock + threadIdx.x;
+    const bool valid = i < I.n_src;
+    const float4 p = I.src[valid ? i : 0u];
+    float px, py, pz;
+    transform_point(P, p, px, py, pz);
+    const uint32_t prev = valid ? I.match[i] : kNone;
+    // The last search left this point a movement budget (half its gap to the
+    // second-nearest point); while the pose moves it less, the match stands.
+    const float bud = valid ? I.budget[i] : 0.0f;
+    const float move =
+        move_rot * (sqrtf(p.x * p.x + p.y * p.y + p.z * p.z) * 1.000001f) + move_const;
+    const bool keep = prev != kNone && bud > move;
+    NNQuery Q;
+    Q.hard = false;
+    if (!keep) {
+      if (gap)
+        nn_phase1<kStats, true>(g, I.cell_start, I.empty_radius, I.tgt, px, py, pz, cfg.dmax,
+                                cfg.dmax2, prev, kGapMargin * g.h, Q, valid ? stats : nullptr);
+      else
+        nn_phase1<kStats, false>(g, I.cell_start, I.empty_radius, I.tgt, px, py, pz, cfg.dmax,
+                                 cfg.dmax2, prev, 0.0f, Q, valid ? stats : nullptr);
+    }
+    const bool hard = valid && Q.hard;
+    const unsigned long long hm = __ballot(hard);
+    if (hm) {
+      const int leader = __ffsll((long long)hm) - 1;
+      uint32_t base = 0;
+      if (lane == leader) base = atomicAdd(&ps.ctl[0], uint32_t(__popcll(hm)));
+      base = __shfl(base, leader);
+      if (hard) {
+        HardQuery h;
+        h.px = px;
+        h.py = py;
+        h.pz = pz;
+        h.inst = inst_id | (Q.skip_block ? kHardBlockFlag : 0u);
+        h.i = i;
+        h.pos = Q.best_pos;
+        h.key = Q.best_key;
+        ps.queue[base + uint32_t(__popcll(hm & below))] = h;
+        I.budget[i] = 0.0f;
+      }
+    }
+    if (!valid || hard) continue;
+    if (keep) {
+      if constexpr (kStats) stats_add(stats, 0, 0, 0, kBlockDone);
+      const float4 q = I.tgt[prev];
+      if (dist2(px, py, pz, q.x, q.y, q.z) <= cfg.dmax2) {
+        I.budget[i] = bud - move;
+      } else {  // the nearest point is beyond d_max: no correspondence, search next time
+        I.match[i] = kNone;
+        I.budget[i] = 0.0f;
+      }
+      continue;
+    }
+    I.budget[i] = Q.budget;
+    I.match[i] = Q.best_pos;
+  }
+}
+
+// Second stage, the blocks k_gicp_settle did not finish.
+template <bool kStats>
+__global__ void __launch_bounds__(kBlock)
+    __attribute__((amdgpu_waves_per_eu(SR_SEARCH_WAVES))) k_gicp_search(const IterInstance* __restrict__ insts,
+                                                        const uint32_t* __restrict__ block_inst,
+                                                        PassScratch ps,
+                                                        SearchStats* __restrict__ stats) {
+  const uint32_t b = xcd_block();
+  if (ps.done[b]) return;
+  const uint32_t inst_id = block_inst[b];
+  const IterInstance& I = insts[inst_id];
+  if (I.ppt != 1) {  // the whole block
+    search_block<kStats>(insts, block_inst, ps, stats, b);
+    return;
+  }
+  // The points k_gicp_settle marked, a wave per 64 threads of the block (waves
+  // with none leave at once); their terms come from k_gicp_terms.
+  const int lane = int(threadIdx.x & (kWave - 1));
+  const unsigned long long sm = ps.pend[size_t(b) * (kBlock / kWave) + threadIdx.x / kWave];
+  if (sm == 0) return;
+  const bool mine = (sm >> lane) & 1ull;
+  const InstState* st = I.state;
+  const GridDesc g = I.grid;
+  const InstConfig cfg = I.cfg;
+  PassPose P;
+  load_pose(st, P);
+  // budgets are worth computing once no source point moves kGapMove cells
+  const bool gap = st->move_rot * I.src_radius + st->move_const < kGapMove * g.h;
+  const uint32_t i = (b - I.block_begin) * kBlock + threadIdx.x;
+  float px = 0.0f, py = 0.0f, pz = 0.0f;
+  NNQuery Q;
+  Q.hard = false;
+  if (mine) {
+    transform_point(P, I.src[i], px, py, pz);
+    const uint32_t prev = I.match[i];
+    if (gap)
+      nn_phase1<kStats, true>(g, I.cell_start, I.empty_radius, I.tgt, px, py, pz, cfg.dmax,
+                              cfg.dmax2, prev, kGapMargin * g.h, Q, stats);
+    else
+      nn_phase1<kStats, false>(g, I.cell_start, I.empty_radius, I.tgt, px, py, pz, cfg.dmax,
+                               cfg.dmax2, prev, 0.0f, Q, stats);
+  }
+  const bool hard = mine && Q.hard;
+  const unsigned long long hm = __ballot(hard);
+  if (hm) {
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const int leader = __ffsll((long long)hm) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&ps.ctl[0], uint32_t(__popcll(hm)));
+    base = __shfl(base, leader);
+    if (hard) {
+      HardQuery h;
+      h.px = px;
+      h.py = py;
+      h.pz = pz;
+      h.inst = inst_id | (Q.skip_block ? kHardBlockFlag : 0u);
+      h.i = i;
+      h.pos = Q.best_pos;
+      h.key = Q.best_key;
+      ps.queue[base + uint32_t(__popcll(hm & below))] = h;
+      I.budget[i] = 0.0f;
+    }
+  }
+  if (mine && !hard) {
+    I.budget[i] = Q.budget;
+    I.match[i] = Q.best_pos;
+  }
+}
+
+// The first two stages in one kernel, for launches too small to fill the chip
+// (there, an extra launch per pass costs more than the settled blocks' higher
+// occupancy saves): settled one-point-per-thread blocks add their terms in one
+// sweep; other blocks keep or search each point, and a block none of whose
+// points was queued adds its terms from the matches just recorded. Matches,
+// budgets and sums are those of the two-kernel form.
+template <bool kStats>
+__global__ void __launch_bounds__(kBlock)
+    __attribute__((amdgpu_waves_per_eu(SR_SEARCH_WAVES))) k_gicp_search_fused(const IterInstance* __restrict__ insts,
+                                                        const uint32_t* __restrict__ block_inst,
+                                                        PassScratch ps,
+                                                        SearchStats* __restrict__ stats) {
+  const uint32_t b = xcd_block();
   const uint32_t inst_id = block_inst[b];
   const IterInstance& I = insts[inst_id];
   const InstState* st = I.state;
@@ -817,6 +988,48 @@ __device__ __forceinline__ void search_block(const IterInstance* __restrict__ in
   const bool gap = move_rot * I.src_radius + move_const < kGapMove * g.h;
   const uint32_t ppt = I.ppt;
   const float beta = cfg.one_minus_eps, eps = 1.0f - cfg.one_minus_eps;
+  __shared__ RedLds L;
+
+  // Settled blocks: when every point of a one-point-per-thread block keeps its
+  // match, the search is skipped and the terms are added in the same sweep, so
+  // the block costs two rounds of loads (its own points, then the matched
+  // target points) instead of four. The terms are those loop 2 would add.
+  if (ppt == 1) {
+    const uint32_t i = lb * kBlock + threadIdx.x;
+    const bool valid = i < I.n_src;
+    const float4 p = I.src[valid ? i : 0u];
+    const float4 ns = I.src_n[valid ? i : 0u];
+    const uint32_t prev = valid ? I.match[i] : kNone;
+    const float bud = valid ? I.budget[i] : 0.0f;
+    const float move =
+        move_rot * (sqrtf(p.x * p.x + p.y * p.y + p.z * p.z) * 1.000001f) + move_const;
+    const bool keep = prev != kNone && bud > move;
+    if (__syncthreads_and(!valid || keep)) {
+      if (threadIdx.x == 0) ps.done[b] = 1u;
+      float acc[kTerms];
+#pragma unroll
+      for (int t = 0; t < kTerms; ++t) acc[t] = 0.0f;
+      uint32_t count = 0;
+      if (valid) {
+        if constexpr (kStats) stats_add(stats, 0, 0, 0, kBlockDone);
+        float px, py, pz;
+        transform_point(P, p, px, py, pz);
+        const float4 q = I.tgt[prev];
+        const float4 nt = I.tgt_n[prev];
+        const float d2 = dist2(px, py, pz, q.x, q.y, q.z);
+        if (d2 <= cfg.dmax2) {
+          I.budget[i] = bud - move;
+          add_terms(acc, P, beta, eps, px, py, pz, q, nt, ns, d2);
+          ++count;
+        } else {  // the nearest point is beyond d_max: no correspondence
+          I.match[i] = kNone;
+          I.budget[i] = 0.0f;
+        }
+      }
+      write_partial(L, acc, count, I.partials + size_t(lb) * 64);
+      return;
+    }
+  }
 
   // Loop 1, the search: every point's match (and movement budget) is recorded,
   // and the uncertified ones are queued.
@@ -903,32 +1116,6 @@ __device__ __forceinline__ void search_block(const IterInstance* __restrict__ in
     ++count;
   }
   write_partial(L, acc, count, I.partials + size_t(lb) * 64);
-}
-
-// The blocks k_gicp_settle left, a run of consecutive logical blocks per
-// workgroup (runs kept on one XCD, so neighbouring source tiles share an L2).
-template <bool kStats>
-__global__ void __launch_bounds__(kBlock)
-    __attribute__((amdgpu_waves_per_eu(SR_SEARCH_WAVES))) k_gicp_search(const IterInstance* __restrict__ insts,
-                                                        const uint32_t* __restrict__ block_inst,
-                                                        uint32_t total_blocks, PassScratch ps,
-                                                        SearchStats* __restrict__ stats) {
-#if SR_SEARCH_RUNS
-  const uint32_t w = xcd_block(), G = gridDim.x;
-  const uint32_t per = total_blocks / G, rem = total_blocks % G;
-  const uint32_t b0 = w * per + min(w, rem), b1 = b0 + per + (w < rem ? 1u : 0u);
-  __shared__ RedLds L;
-  for (uint32_t b = b0; b < b1; ++b) {
-    if (ps.done[b]) continue;
-    search_block<kStats>(insts, block_inst, ps, stats, b, L);
-    __syncthreads();  // the LDS is reused by the next block
-  }
-#else
-  const uint32_t b = xcd_block();
-  if (ps.done[b]) return;
-  __shared__ RedLds L;
-  search_block<kStats>(insts, block_inst, ps, stats, b, L);
-#endif
 }
 
 template <bool kStats>
@@ -1096,13 +1283,15 @@ uint32_t hard_stage_blocks(uint64_t points) {
   return uint32_t(std::min<uint64_t>(std::max<uint64_t>(want, 1), 2048));
 }
 
-// Workgroups of the search and terms stages, each taking a run of blocks:
-// several rounds of what 256 CUs hold at once, so runs stay short.
-#ifndef SR_SEARCH_RUNS
-#define SR_SEARCH_RUNS 0
-#endif
-constexpr uint32_t kSearchGrid = SR_SEARCH_RUNS ? 8192u : 0xFFFFFFFFu;
+// Workgroups of the terms stage, each taking a run of blocks: several rounds
+// of what 256 CUs hold at once.
 constexpr uint32_t kTermsGrid = 4096;
+// Launches of at least this many blocks (several per CU) run the first two
+// stages as two kernels; smaller ones as one.
+#ifndef SR_SPLIT_BLOCKS
+#define SR_SPLIT_BLOCKS 4096
+#endif
+constexpr uint32_t kSplitBlocks = SR_SPLIT_BLOCKS;
 
 void gicp_pass(const IterInstance* insts, const uint32_t* block_inst, uint32_t total_blocks,
                uint32_t count, const PassScratch& ps, uint32_t hard_blocks, hipStream_t s,
@@ -1111,18 +1300,28 @@ void gicp_pass(const IterInstance* insts, const uint32_t* block_inst, uint32_t t
   {
     ProfScope prof("gicp_pass", s);
     SearchStats* st = prof_stats("gicp_pass");
-    const dim3 search_grid(std::min<uint32_t>(total_blocks, kSearchGrid));
+    const bool split = total_blocks >= kSplitBlocks;
     if (st) {
-      hipLaunchKernelGGL(k_gicp_settle<true>, dim3(total_blocks), dim3(kBlock), 0, s, insts,
-                         block_inst, ps, st);
-      hipLaunchKernelGGL(k_gicp_search<true>, search_grid, dim3(kBlock), 0, s, insts, block_inst,
-                         total_blocks, ps, st);
+      if (split) {
+        hipLaunchKernelGGL(k_gicp_settle<true>, dim3(total_blocks), dim3(kBlock), 0, s, insts,
+                           block_inst, ps, st);
+        hipLaunchKernelGGL(k_gicp_search<true>, dim3(total_blocks), dim3(kBlock), 0, s, insts,
+                           block_inst, ps, st);
+      } else {
+        hipLaunchKernelGGL(k_gicp_search_fused<true>, dim3(total_blocks), dim3(kBlock), 0, s,
+                           insts, block_inst, ps, st);
+      }
       hipLaunchKernelGGL(k_gicp_hard<true>, dim3(hard_blocks), dim3(kBlock), 0, s, insts, ps, st);
     } else {
-      hipLaunchKernelGGL(k_gicp_settle<false>, dim3(total_blocks), dim3(kBlock), 0, s, insts,
-                         block_inst, ps, st);
-      hipLaunchKernelGGL(k_gicp_search<false>, search_grid, dim3(kBlock), 0, s, insts, block_inst,
-                         total_blocks, ps, st);
+      if (split) {
+        hipLaunchKernelGGL(k_gicp_settle<false>, dim3(total_blocks), dim3(kBlock), 0, s, insts,
+                           block_inst, ps, st);
+        hipLaunchKernelGGL(k_gicp_search<false>, dim3(total_blocks), dim3(kBlock), 0, s, insts,
+                           block_inst, ps, st);
+      } else {
+        hipLaunchKernelGGL(k_gicp_search_fused<false>, dim3(total_blocks), dim3(kBlock), 0, s,
+                           insts, block_inst, ps, st);
+      }
       hipLaunchKernelGGL(k_gicp_hard<false>, dim3(hard_blocks), dim3(kBlock), 0, s, insts, ps, st);
     }
     hipLaunchKernelGGL(k_gicp_terms, dim3(std::min<uint32_t>(total_blocks, kTermsGrid)), dim3(kBlock),

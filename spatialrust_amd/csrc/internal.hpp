@@ -340,12 +340,14 @@ struct HardQuery {
 
 // Scratch of a pass launch: the queue of uncertified points (room for every
 // source point of the launch), its length ctl[0] (zero when a pass starts; the
-// finalize kernel zeroes it for the next) and one flag per block (the first
-// stage wrote the block's partial).
+// finalize kernel zeroes it for the next), one flag per block (a stage wrote
+// the block's partial) and four words per block (the first stage's mask of
+// the block's points left to search, one bit per thread).
 struct PassScratch {
   HardQuery* queue;
   uint32_t* ctl;
   uint32_t* done;
+  unsigned long long* pend;
 };
 
 // Workgroups of the queue-draining stage for a launch over `points` source points.

@@ -302,6 +302,31 @@ def test_batch_pairs_device_clouds_equal_plain_align(hip):
             c.close()
 
 
+def test_batch_pairs_two_kernel_first_stages_equal_plain_align(hip):
+    """A batch of ~1.7M source points (over 4096 blocks of 256) runs the first
+    two pass stages as two kernels (settled blocks apart from the search),
+    while each registration alone (a few hundred blocks) runs them as one; the
+    matches, budgets and sums are the same, so poses stay bit-identical."""
+    from spatialrust_amd import devmem
+    pairs = synth.batch_pairs(count=48, lo=20000, hi=60000, seed_offset=11)
+    assert sum(len(p.source) for p in pairs) * 0.85 > 4096 * 256
+    cfg = sr.GicpConfig(max_iterations=12, voxel_leaf=0.1, transformation_epsilon=0.0,
+                        fitness_epsilon=0.0)
+    srcs = [devmem.DeviceCloud(p.source) for p in pairs]
+    tgts = [devmem.DeviceCloud(p.target) for p in pairs]
+    try:
+        batched = sr.batch_align_pairs([c.view for c in srcs], [c.view for c in tgts], cfg)
+        reg = sr.GicpRegistration(cfg)
+        for i, (s_, t_) in enumerate(zip(srcs, tgts)):
+            alone = reg.align(s_.view, t_.view)
+            assert np.array_equal(alone.rotation64, batched[i].rotation64), i
+            assert np.array_equal(alone.translation64, batched[i].translation64), i
+            assert alone.iterations == batched[i].iterations
+    finally:
+        for c in srcs + tgts:
+            c.close()
+
+
 def test_receipt_records_stages_and_bytes(hip):
     w = synth.small_cube_pair()
     out = sr.GicpRegistration(sr.GicpConfig(max_iterations=3, voxel_leaf=0.1)).align_with_policy_and_receipt(
