@@ -1285,7 +1285,10 @@ uint32_t hard_stage_blocks(uint64_t points) {
 
 // Workgroups of the terms stage, each taking a run of blocks: several rounds
 // of what 256 CUs hold at once.
-constexpr uint32_t kTermsGrid = 4096;
+#ifndef SR_TERMS_GRID
+#define SR_TERMS_GRID 4096
+#endif
+constexpr uint32_t kTermsGrid = SR_TERMS_GRID;
 // Launches of at least this many blocks (several per CU) run the first two
 // stages as two kernels; smaller ones as one.
 #ifndef SR_SPLIT_BLOCKS

@@ -378,6 +378,32 @@ def test_pass_matches_fp64_oracle(hip, orc, pose):
     assert np.allclose(H, H.T)
 
 
+def test_pass_two_points_per_thread_equals_its_halves(hip):
+    """A source of 2.2M points runs two points per thread (the whole-block
+    search path and the terms stage); it is two copies of one scene 40 m apart
+    (farther than d_max and than any k-NN radius), so its matches and H, g are
+    those of the two copies alone (each 1.1M points, one point per thread):
+    matches equal, H and g within 1e-5 (f32 sums in another order)."""
+    w = synth.small_cube_pair(n_source=1_100_000, n_target=300_000)
+    off = np.array([40.0, 0.0, 0.0], np.float32)
+    s2, t2 = (w.source + off).astype(np.float32), (w.target + off).astype(np.float32)
+    src = np.concatenate([w.source, s2])
+    tgt = np.concatenate([w.target, t2])
+    assert len(src) >= 1 << 21
+    cfg = sr.GicpConfig(k_neighbors=20, max_correspondence_distance=1.0)
+    q, t = (0.0, 0.0, 0.0, 1.0), (0.0, 0.0, 0.0)
+    m, H, g, c, d2 = sr.gicp_normal_equations(src, tgt, cfg, q, t)
+    ma, Ha, ga, ca, d2a = sr.gicp_normal_equations(w.source, w.target, cfg, q, t)
+    mb, Hb, gb, cb, d2b = sr.gicp_normal_equations(s2, t2, cfg, q, t)
+    n = len(w.source)
+    assert np.array_equal(m[:n], ma)
+    assert np.array_equal(m[n:], np.where(mb >= 0, mb + len(w.target), mb))
+    assert c == ca + cb
+    assert np.linalg.norm(H - (Ha + Hb)) <= 1e-5 * np.linalg.norm(H)
+    assert np.linalg.norm(g - (ga + gb)) <= 1e-5 * np.linalg.norm(g)
+    assert abs(d2 - (d2a + d2b)) <= 1e-5 * d2
+
+
 def test_server_results_equal_alone(hip):
     """Continuous batching: requests submitted while others iterate, admitted as
     slots and budget free up, each finish with the pose sr_map_align gives the
