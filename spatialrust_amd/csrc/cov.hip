@@ -187,13 +187,16 @@ struct KDist {
 #pragma unroll
     for (int j = 0; j <= K; ++j) d[j] = INFINITY;
   }
+  // Distances are >= +0 (or NaN, mapped to +inf), whose IEEE bit patterns
+  // order like unsigned integers: integer min/max need no NaN canonicalizing
+  // of their operands, which float min/max would add to every step.
   __device__ __forceinline__ void add(float x) {
-    x = x < INFINITY ? x : INFINITY;  // NaN sorts as +inf
+    uint32_t xb = __float_as_uint(x < INFINITY ? x : INFINITY);  // NaN sorts as +inf
 #pragma unroll
     for (int j = 0; j <= K; ++j) {
-      const float lo = fminf(d[j], x);
-      x = fmaxf(d[j], x);
-      d[j] = lo;
+      const uint32_t db = __float_as_uint(d[j]);
+      d[j] = __uint_as_float(min(db, xb));
+      xb = max(db, xb);
     }
   }
   __device__ __forceinline__ float at(int j) const {  // d[j] for a runtime j <= K
@@ -244,9 +247,13 @@ __device__ __forceinline__ float kdist_slot(const KDist<K>& D, int j, int jfix) 
 // ranges, kKnnBlockW loads in flight: one lane's chain of dependent loads is
 // then the block's points / kKnnBlockW steps instead of a few steps per range.
 #ifndef SR_KNN_BLOCK_W
-#define SR_KNN_BLOCK_W 4
+#define SR_KNN_BLOCK_W 8
 #endif
 constexpr int kKnnBlockW = SR_KNN_BLOCK_W;
+#ifndef SR_KNN_CONCAT
+#define SR_KNN_CONCAT 1
+#endif
+constexpr bool kKnnConcat = SR_KNN_CONCAT != 0;  // one stream over the nine ranges
 
 
 // The candidates of an inside query: the 3x3x3 block, then rings 2..rend cut
@@ -256,7 +263,7 @@ __device__ __forceinline__ void knn_candidates(const GridDesc& g, const uint32_t
                                                const float4* __restrict__ gp, float px, float py,
                                                float pz, int hx, int hy, int hz, int rend,
                                                const float& thr, Visit&& visit) {
-  scan_block27<kKnnBlockW, true>(g, cs, gp, hx, hy, hz, visit);
+  scan_block27<kKnnBlockW, kKnnConcat>(g, cs, gp, hx, hy, hz, visit);
   uint32_t rows = 0;
   for (int r = 2; r <= rend; ++r) scan_ring<1>(g, cs, gp, px, py, pz, hx, hy, hz, r, thr, rows, visit);
 }
@@ -344,7 +351,7 @@ __global__ void __launch_bounds__(kBlock)
     KDist<K> D;
     D.init();
     auto net = [&](uint32_t, const float4& q) { D.add(dist2(q.x, q.y, q.z, p.x, p.y, p.z)); };
-    scan_block27<kKnnBlockW, true>(g, cs, gp, hx, hy, hz, net);
+    scan_block27<kKnnBlockW, kKnnConcat>(g, cs, gp, hx, hy, hz, net);
     float worst = kdist_slot<kExact>(D, k - 1, K - 1);
     const float cl = ring_clearance(g, p.x, p.y, p.z, hx, hy, hz, 1);
     if (!(worst < INFINITY && cl > 0.0f && worst < cl * cl)) {
