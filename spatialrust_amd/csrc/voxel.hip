@@ -326,7 +326,7 @@ __device__ __forceinline__ unsigned long long linear_voxel_key(const VoxelParams
 __global__ void k_voxel_insert(const float4* __restrict__ pts, uint32_t n,
                                const VoxelParams* __restrict__ Pp,
                                unsigned long long* __restrict__ tkeys, uint32_t* __restrict__ tfirst,
-                               uint32_t* __restrict__ tcount, uint32_t mask,
+                               uint32_t mask,
                                uint32_t* __restrict__ slot_of) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -341,7 +341,6 @@ __global__ void k_voxel_insert(const float4* __restrict__ pts, uint32_t n,
     h = (h + 1) & mask;
   }
   atomicMin(&tfirst[h], i);
-  atomicAdd(&tcount[h], 1u);
   slot_of[i] = h;
 }
 
@@ -354,15 +353,12 @@ __global__ void k_voxel_flags(const uint32_t* __restrict__ slot_of,
 
 __global__ void k_voxel_cellinfo(const float4* __restrict__ pts, const uint32_t* __restrict__ slot_of,
                                  const uint32_t* __restrict__ flag, const uint32_t* __restrict__ rank,
-                                 const uint32_t* __restrict__ tcount, uint32_t n,
-                                 const VoxelParams* __restrict__ Pp,
-                                 uint32_t* __restrict__ cell_of_slot, uint32_t* __restrict__ ocount,
-                                 int64_t* __restrict__ okeys) {
+                                 uint32_t n, const VoxelParams* __restrict__ Pp,
+                                 uint32_t* __restrict__ cell_of_slot, int64_t* __restrict__ okeys) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || !flag[i]) return;
   uint32_t c = rank[i], s = slot_of[i];
   cell_of_slot[s] = c;
-  ocount[c] = tcount[s];
   if (okeys) {
     int64_t k[3];
     float4 p = pts[i];
@@ -371,6 +367,16 @@ __global__ void k_voxel_cellinfo(const float4* __restrict__ pts, const uint32_t*
     okeys[3 * size_t(c) + 1] = k[1];
     okeys[3 * size_t(c) + 2] = k[2];
   }
+}
+
+// Points per output cell. The cells are numbered densely in first-occurrence
+// order, so these counters are a compact array (the hash table's slots are
+// spread over twice the points and their atomics cost several times more).
+__global__ void k_voxel_count(const uint32_t* __restrict__ slot_of,
+                              const uint32_t* __restrict__ cell_of_slot, uint32_t n,
+                              uint32_t* __restrict__ ocount) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) atomicAdd(&ocount[cell_of_slot[slot_of[i]]], 1u);
 }
 
 __global__ void k_voxel_scatter(const uint32_t* __restrict__ slot_of,
@@ -496,25 +502,25 @@ void voxel_downsample(const Cloud& in, float leaf, const float* origin, bool wan
   uint32_t cap = 1024;
   while (cap < 2u * n) cap <<= 1;
   DevBuf<unsigned long long> tkeys(cap);
-  DevBuf<uint32_t> tfirst(cap), tcount(cap), cell_of_slot(cap);
+  DevBuf<uint32_t> tfirst(cap), cell_of_slot(cap);
   hipLaunchKernelGGL(k_fill_u64, dim3(fill_blocks(cap)), dim3(kBlock), 0, s, tkeys.p, kEmptyKey,
                      size_t(cap));
   fill_u32(tfirst.p, 0xFFFFFFFFu, cap, s);
-  fill_u32(tcount.p, 0u, cap, s);
   DevBuf<uint32_t> slot_of(n), flag(n), rank(n), ocount(n), cstart(n), cursor(n), members(n),
       scal(4);
   const unsigned blocks = (n + kBlock - 1) / kBlock;
   ProfScope prof_insert("voxel_insert", s);
   hipLaunchKernelGGL(k_voxel_insert, dim3(blocks), dim3(kBlock), 0, s, in.pts.p, n, params.p,
-                     tkeys.p, tfirst.p, tcount.p, cap - 1, slot_of.p);
+                     tkeys.p, tfirst.p, cap - 1, slot_of.p);
   hipLaunchKernelGGL(k_voxel_flags, dim3(blocks), dim3(kBlock), 0, s, slot_of.p, tfirst.p, n, flag.p);
   exclusive_scan_u32(flag.p, rank.p, n, scal.p, s);  // scal[0] = cell count
   fill_u32(ocount.p, 0u, n, s);
   fill_u32(cursor.p, 0u, n, s);
   if (want_keys) out.keys.resize(size_t(3) * n);
   hipLaunchKernelGGL(k_voxel_cellinfo, dim3(blocks), dim3(kBlock), 0, s, in.pts.p, slot_of.p,
-                     flag.p, rank.p, tcount.p, n, params.p, cell_of_slot.p, ocount.p,
-                     want_keys ? out.keys.p : nullptr);
+                     flag.p, rank.p, n, params.p, cell_of_slot.p, want_keys ? out.keys.p : nullptr);
+  hipLaunchKernelGGL(k_voxel_count, dim3(blocks), dim3(kBlock), 0, s, slot_of.p, cell_of_slot.p, n,
+                     ocount.p);
   exclusive_scan_u32(ocount.p, cstart.p, n, nullptr, s);
   hipLaunchKernelGGL(k_voxel_scatter, dim3(blocks), dim3(kBlock), 0, s, slot_of.p, cell_of_slot.p,
                      cstart.p, cursor.p, n, members.p);
@@ -622,7 +628,7 @@ __global__ void k_seg_voxel_insert(const float4* __restrict__ pts, const uint32_
                                    const VoxelParams* __restrict__ Pp,
                                    const SegTable* __restrict__ tab,
                                    unsigned long long* __restrict__ tkeys,
-                                   uint32_t* __restrict__ tfirst, uint32_t* __restrict__ tcount,
+                                   uint32_t* __restrict__ tfirst,
                                    uint32_t* __restrict__ slot_of) {
   const uint32_t s = seg_of(blk_begin, nseg, blockIdx.x);
   const uint32_t i = off[s] + (blockIdx.x - blk_begin[s]) * blockDim.x + threadIdx.x;
@@ -638,7 +644,6 @@ __global__ void k_seg_voxel_insert(const float4* __restrict__ pts, const uint32_
     h = (h + 1) & T.mask;
   }
   atomicMin(&tfirst[T.base + h], i);
-  atomicAdd(&tcount[T.base + h], 1u);
   slot_of[i] = T.base + h;
 }
 
@@ -731,24 +736,24 @@ void voxel_downsample_many(const Cloud& in, const std::vector<uint32_t>& off, fl
   upload(d_off, off, s);
   upload(d_blk, blk, s);
   DevBuf<unsigned long long> tkeys(cap);
-  DevBuf<uint32_t> tfirst(cap), tcount(cap), cell_of_slot(cap);
+  DevBuf<uint32_t> tfirst(cap), cell_of_slot(cap);
   hipLaunchKernelGGL(k_fill_u64, dim3(fill_blocks(cap)), dim3(kBlock), 0, s, tkeys.p, kEmptyKey,
                      size_t(cap));
   fill_u32(tfirst.p, 0xFFFFFFFFu, cap, s);
-  fill_u32(tcount.p, 0u, cap, s);
   DevBuf<uint32_t> slot_of(n), flag(n), rank(n), ocount(n), cstart(n), cursor(n), members(n),
       scal(4), tailv(nseg + 2);
   const unsigned blocks = (n + kBlock - 1) / kBlock;
   ProfScope prof_insert("voxel_insert", s);
   hipLaunchKernelGGL(k_seg_voxel_insert, dim3(blk.back()), dim3(kBlock), 0, s, in.pts.p, d_off.p,
-                     nseg, d_blk.p, params.p, d_tab.p, tkeys.p, tfirst.p, tcount.p, slot_of.p);
+                     nseg, d_blk.p, params.p, d_tab.p, tkeys.p, tfirst.p, slot_of.p);
   hipLaunchKernelGGL(k_voxel_flags, dim3(blocks), dim3(kBlock), 0, s, slot_of.p, tfirst.p, n, flag.p);
   exclusive_scan_u32(flag.p, rank.p, n, scal.p, s);  // scal[0] = cell count
   fill_u32(ocount.p, 0u, n, s);
   fill_u32(cursor.p, 0u, n, s);
   hipLaunchKernelGGL(k_voxel_cellinfo, dim3(blocks), dim3(kBlock), 0, s, in.pts.p, slot_of.p,
-                     flag.p, rank.p, tcount.p, n, params.p, cell_of_slot.p, ocount.p,
-                     (int64_t*)nullptr);
+                     flag.p, rank.p, n, params.p, cell_of_slot.p, (int64_t*)nullptr);
+  hipLaunchKernelGGL(k_voxel_count, dim3(blocks), dim3(kBlock), 0, s, slot_of.p, cell_of_slot.p, n,
+                     ocount.p);
   exclusive_scan_u32(ocount.p, cstart.p, n, nullptr, s);
   hipLaunchKernelGGL(k_voxel_scatter, dim3(blocks), dim3(kBlock), 0, s, slot_of.p, cell_of_slot.p,
                      cstart.p, cursor.p, n, members.p);
