@@ -331,17 +331,35 @@ __global__ void k_voxel_insert(const float4* __restrict__ pts, uint32_t n,
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const VoxelParams P = *Pp;
-  if (P.error) return;
+  if (P.error) {  // the call fails; keep the later kernels' indices in bounds
+    slot_of[i] = 0;
+    return;
+  }
   float4 p = pts[i];
   const unsigned long long lin = linear_voxel_key(P, p);
   uint32_t h = hash64(lin, mask);
   while (true) {
     unsigned long long prev = atomicCAS(&tkeys[h], kEmptyKey, lin);
-    if (prev == kEmptyKey || prev == lin) break;
+    if (prev == kEmptyKey) {
+      tfirst[h] = i;  // a first guess at the slot's lowest index (k_voxel_first_fix)
+      break;
+    }
+    if (prev == lin) break;
     h = (h + 1) & mask;
   }
-  atomicMin(&tfirst[h], i);
   slot_of[i] = h;
+}
+
+// The point that claimed a slot wrote its index as the slot's first; the
+// slot's lower indices (points of the same voxel that came later) lower it.
+// Most voxels hold one point, or were claimed by their lowest index, so few
+// points take the atomic.
+__global__ void k_voxel_first_fix(const uint32_t* __restrict__ slot_of, uint32_t n,
+                                  uint32_t* __restrict__ tfirst) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t* f = &tfirst[slot_of[i]];
+  if (*f > i) atomicMin(f, i);
 }
 
 __global__ void k_voxel_flags(const uint32_t* __restrict__ slot_of,
@@ -376,7 +394,8 @@ __global__ void k_voxel_count(const uint32_t* __restrict__ slot_of,
                               const uint32_t* __restrict__ cell_of_slot, uint32_t n,
                               uint32_t* __restrict__ ocount) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) atomicAdd(&ocount[cell_of_slot[slot_of[i]]], 1u);
+  // (the clamp only matters when a key-range error has left slots unnumbered)
+  if (i < n) atomicAdd(&ocount[min(cell_of_slot[slot_of[i]], n - 1)], 1u);
 }
 
 __global__ void k_voxel_scatter(const uint32_t* __restrict__ slot_of,
@@ -385,8 +404,8 @@ __global__ void k_voxel_scatter(const uint32_t* __restrict__ slot_of,
                                 uint32_t n, uint32_t* __restrict__ members) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  uint32_t c = cell_of_slot[slot_of[i]];
-  uint32_t pos = cstart[c] + atomicAdd(&cursor[c], 1u);
+  uint32_t c = min(cell_of_slot[slot_of[i]], n - 1);  // see k_voxel_count
+  uint32_t pos = min(cstart[c] + atomicAdd(&cursor[c], 1u), n - 1);
   members[pos] = i;
 }
 
@@ -505,13 +524,13 @@ void voxel_downsample(const Cloud& in, float leaf, const float* origin, bool wan
   DevBuf<uint32_t> tfirst(cap), cell_of_slot(cap);
   hipLaunchKernelGGL(k_fill_u64, dim3(fill_blocks(cap)), dim3(kBlock), 0, s, tkeys.p, kEmptyKey,
                      size_t(cap));
-  fill_u32(tfirst.p, 0xFFFFFFFFu, cap, s);
   DevBuf<uint32_t> slot_of(n), flag(n), rank(n), ocount(n), cstart(n), cursor(n), members(n),
       scal(4);
   const unsigned blocks = (n + kBlock - 1) / kBlock;
   ProfScope prof_insert("voxel_insert", s);
   hipLaunchKernelGGL(k_voxel_insert, dim3(blocks), dim3(kBlock), 0, s, in.pts.p, n, params.p,
                      tkeys.p, tfirst.p, cap - 1, slot_of.p);
+  hipLaunchKernelGGL(k_voxel_first_fix, dim3(blocks), dim3(kBlock), 0, s, slot_of.p, n, tfirst.p);
   hipLaunchKernelGGL(k_voxel_flags, dim3(blocks), dim3(kBlock), 0, s, slot_of.p, tfirst.p, n, flag.p);
   exclusive_scan_u32(flag.p, rank.p, n, scal.p, s);  // scal[0] = cell count
   fill_u32(ocount.p, 0u, n, s);
@@ -634,16 +653,22 @@ __global__ void k_seg_voxel_insert(const float4* __restrict__ pts, const uint32_
   const uint32_t i = off[s] + (blockIdx.x - blk_begin[s]) * blockDim.x + threadIdx.x;
   if (i >= off[s + 1]) return;
   const VoxelParams P = Pp[s];
-  if (P.error) return;
+  if (P.error) {  // the call fails; keep the later kernels' indices in bounds
+    slot_of[i] = 0;
+    return;
+  }
   const SegTable T = tab[s];
   const unsigned long long lin = linear_voxel_key(P, pts[i]);
   uint32_t h = hash64(lin, T.mask);
   while (true) {
     unsigned long long prev = atomicCAS(&tkeys[T.base + h], kEmptyKey, lin);
-    if (prev == kEmptyKey || prev == lin) break;
+    if (prev == kEmptyKey) {
+      tfirst[T.base + h] = i;  // a first guess at the slot's lowest index (k_voxel_first_fix)
+      break;
+    }
+    if (prev == lin) break;
     h = (h + 1) & T.mask;
   }
-  atomicMin(&tfirst[T.base + h], i);
   slot_of[i] = T.base + h;
 }
 
@@ -739,13 +764,13 @@ void voxel_downsample_many(const Cloud& in, const std::vector<uint32_t>& off, fl
   DevBuf<uint32_t> tfirst(cap), cell_of_slot(cap);
   hipLaunchKernelGGL(k_fill_u64, dim3(fill_blocks(cap)), dim3(kBlock), 0, s, tkeys.p, kEmptyKey,
                      size_t(cap));
-  fill_u32(tfirst.p, 0xFFFFFFFFu, cap, s);
   DevBuf<uint32_t> slot_of(n), flag(n), rank(n), ocount(n), cstart(n), cursor(n), members(n),
       scal(4), tailv(nseg + 2);
   const unsigned blocks = (n + kBlock - 1) / kBlock;
   ProfScope prof_insert("voxel_insert", s);
   hipLaunchKernelGGL(k_seg_voxel_insert, dim3(blk.back()), dim3(kBlock), 0, s, in.pts.p, d_off.p,
                      nseg, d_blk.p, params.p, d_tab.p, tkeys.p, tfirst.p, slot_of.p);
+  hipLaunchKernelGGL(k_voxel_first_fix, dim3(blocks), dim3(kBlock), 0, s, slot_of.p, n, tfirst.p);
   hipLaunchKernelGGL(k_voxel_flags, dim3(blocks), dim3(kBlock), 0, s, slot_of.p, tfirst.p, n, flag.p);
   exclusive_scan_u32(flag.p, rank.p, n, scal.p, s);  // scal[0] = cell count
   fill_u32(ocount.p, 0u, n, s);
