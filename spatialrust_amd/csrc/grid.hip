@@ -87,6 +87,39 @@ __global__ void k_grid_scatter(const uint32_t* __restrict__ cid, const uint32_t*
 constexpr uint32_t kSmallSeg = 48;
 constexpr uint32_t kBigSeg = 4096;
 
+// Bitonic sorting network over N (a power of two) values in registers; every
+// index is a compile-time constant once unrolled.
+template <int N>
+__device__ __forceinline__ void sort_net(uint32_t (&v)[N]) {
+#pragma unroll
+  for (int k = 2; k <= N; k <<= 1)
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const int l = i ^ j;
+        if (l > i) {
+          const uint32_t a = v[i], b = v[l];
+          const bool up = (i & k) == 0;
+          v[i] = up ? min(a, b) : max(a, b);
+          v[l] = up ? max(a, b) : min(a, b);
+        }
+      }
+}
+
+// A cell of at most N members: loaded into registers (padded with ~0, which
+// sorts last), sorted by the network, stored back.
+template <int N>
+__device__ __forceinline__ void sort_small_cell(uint32_t* seg, uint32_t cnt) {
+  uint32_t v[N];
+#pragma unroll
+  for (int t = 0; t < N; ++t) v[t] = uint32_t(t) < cnt ? seg[t] : 0xFFFFFFFFu;
+  sort_net<N>(v);
+#pragma unroll
+  for (int t = 0; t < N; ++t)
+    if (uint32_t(t) < cnt) seg[t] = v[t];
+}
+
 __global__ void k_sort_cells(const uint32_t* __restrict__ start, uint32_t ncells,
                              uint32_t* __restrict__ order, uint32_t* __restrict__ big_list,
                              uint32_t* __restrict__ big_count) {
@@ -100,6 +133,14 @@ __global__ void k_sort_cells(const uint32_t* __restrict__ start, uint32_t ncells
     return;
   }
   uint32_t* seg = order + b;
+  if (cnt <= 8) {
+    sort_small_cell<8>(seg, cnt);
+    return;
+  }
+  if (cnt <= 16) {
+    sort_small_cell<16>(seg, cnt);
+    return;
+  }
   for (uint32_t a = 1; a < cnt; ++a) {
     uint32_t v = seg[a];
     uint32_t k = a;
