@@ -103,33 +103,37 @@ void init_states(InstState* states, const double* init7_dev, uint32_t count, hip
 
 // FP64 LDL^T solve of H x = -g (H symmetric, packed upper). false if a pivot
 // falls below 1e-12 — the singularity threshold of solve_linear_system.
-// The factor lives in LDS scratch `w` (>= 48 doubles) rather than registers:
-// this runs on one thread of one workgroup per pass, and keeping 6x6 FP64
-// arrays in VGPRs would cap the occupancy of the whole point-parallel kernel.
-__device__ bool ldlt_solve(const double* Hp, const double* g, double* x, double* w) {
-  double* L = w;       // 36, row-major, unit lower
-  double* D = w + 36;  // 6
-  double* y = w + 42;  // 6
+// Every loop bound is a constant, so the factor stays in registers: this runs
+// on one thread of the finalize kernel, whose registers cost nothing else.
+__device__ __forceinline__ bool ldlt_solve(const double* Hp, const double* g, double* x) {
+  double L[6][6], D[6], y[6];
+#pragma unroll
   for (int j = 0; j < 6; ++j) {
     double d = Hp[ut(j, j)];
-    for (int k = 0; k < j; ++k) d -= L[6 * j + k] * L[6 * j + k] * D[k];
+#pragma unroll
+    for (int k = 0; k < j; ++k) d -= L[j][k] * L[j][k] * D[k];
     if (!(d >= 1e-12)) return false;
     D[j] = d;
-    L[6 * j + j] = 1.0;
+#pragma unroll
     for (int i = j + 1; i < 6; ++i) {
       double v = Hp[ut(j, i)];
-      for (int k = 0; k < j; ++k) v -= L[6 * i + k] * L[6 * j + k] * D[k];
-      L[6 * i + j] = v / d;
+#pragma unroll
+      for (int k = 0; k < j; ++k) v -= L[i][k] * L[j][k] * D[k];
+      L[i][j] = v / d;
     }
   }
+#pragma unroll
   for (int i = 0; i < 6; ++i) {
     double v = -g[i];
-    for (int k = 0; k < i; ++k) v -= L[6 * i + k] * y[k];
+#pragma unroll
+    for (int k = 0; k < i; ++k) v -= L[i][k] * y[k];
     y[i] = v;
   }
+#pragma unroll
   for (int i = 5; i >= 0; --i) {
     double v = y[i] / D[i];
-    for (int k = i + 1; k < 6; ++k) v -= L[6 * k + i] * x[k];
+#pragma unroll
+    for (int k = i + 1; k < 6; ++k) v -= L[k][i] * x[k];
     x[i] = v;
   }
   return true;
@@ -137,8 +141,7 @@ __device__ bool ldlt_solve(const double* Hp, const double* g, double* x, double*
 
 // Gauss-Newton bookkeeping of one registration after a pass (one thread).
 __device__ __forceinline__ void finalize(InstState* st, const InstConfig& cfg, const double* Hp,
-                                      const double* g, double fit_sum, unsigned long long cnt,
-                                      double* w) {
+                                      const double* g, double fit_sum, unsigned long long cnt) {
   const double F = cnt > 0 ? fit_sum / double(cnt) : DBL_MAX;
   st->count = cnt;
   st->move_rot = 0.0f;  // unchanged pose unless updated below
@@ -165,8 +168,8 @@ __device__ __forceinline__ void finalize(InstState* st, const InstConfig& cfg, c
     st->phase = 2;
     return;
   }
-  double* x = w + 48;
-  if (!ldlt_solve(Hp, g, x, w)) {  // gicp.rs:190-194
+  double x[6];
+  if (!ldlt_solve(Hp, g, x)) {  // gicp.rs:190-194
     st->status = 2;
     st->phase = 2;
     return;
@@ -1231,7 +1234,6 @@ __global__ void __launch_bounds__(kFinBlock) k_gicp_finalize(const IterInstance*
   __shared__ double gsum[kFinGroups][32];
   __shared__ unsigned long long gcnt[kFinGroups];
   __shared__ double dtot[kTerms];
-  __shared__ double work[64];
   const uint32_t nb = I.nblocks;
   const int slot = threadIdx.x & 31, grp = threadIdx.x >> 5;
   const unsigned long long* pp = reinterpret_cast<const unsigned long long*>(I.partials);
@@ -1266,7 +1268,7 @@ __global__ void __launch_bounds__(kFinBlock) k_gicp_finalize(const IterInstance*
   if (threadIdx.x == 0) {
     unsigned long long c = 0;
     for (int w = 0; w < kFinGroups; ++w) c += gcnt[w];
-    finalize(&sst, cfg, dtot, dtot + 21, dtot[27], c, work);
+    finalize(&sst, cfg, dtot, dtot + 21, dtot[27], c);
   }
   __syncthreads();
   if (threadIdx.x < kStateWords)
