@@ -187,17 +187,17 @@ struct KDist {
 #pragma unroll
     for (int j = 0; j <= K; ++j) d[j] = INFINITY;
   }
-  // Distances are >= +0 (or NaN, mapped to +inf), whose IEEE bit patterns
-  // order like unsigned integers: integer min/max need no NaN canonicalizing
-  // of their operands, which float min/max would add to every step.
+  // Inserting x into the sorted d: slot j becomes min(d[j], max(d[j-1], x)),
+  // the median of (d[j-1], d[j], x) since d[j-1] <= d[j] (slot 0: the median of
+  // (d[0], x, -inf), their minimum). One v_med3_f32 per slot, every slot
+  // independent of the others (no chain through x), written from the top so
+  // each reads the old d[j-1]. The distances are >= +0 or +inf (NaN is mapped
+  // to +inf), so the median is the ordered one.
   __device__ __forceinline__ void add(float x) {
-    uint32_t xb = __float_as_uint(x < INFINITY ? x : INFINITY);  // NaN sorts as +inf
+    x = x < INFINITY ? x : INFINITY;  // NaN sorts as +inf
 #pragma unroll
-    for (int j = 0; j <= K; ++j) {
-      const uint32_t db = __float_as_uint(d[j]);
-      d[j] = __uint_as_float(min(db, xb));
-      xb = max(db, xb);
-    }
+    for (int j = K; j >= 1; --j) d[j] = __builtin_amdgcn_fmed3f(d[j - 1], d[j], x);
+    d[0] = __builtin_amdgcn_fmed3f(d[0], x, -INFINITY);
   }
   __device__ __forceinline__ float at(int j) const {  // d[j] for a runtime j <= K
     float v = d[0];
