@@ -176,35 +176,44 @@ __device__ __forceinline__ void stream_ranges(const uint32_t (&lo)[N], const uin
 }
 
 // Visits the points of N contiguous ranges [lo[k], hi[k]) as one sequence:
-// each step loads W consecutive entries of the concatenation (range picked by
-// a compare chain on the running starts), so short ranges share steps and the
-// loads of several ranges are in flight together.
+// each step loads W consecutive entries of the concatenation, so short ranges
+// share the steps and the loads of several ranges are in flight together.
+// Each range is padded to a multiple of W in the concatenation, so a step lies
+// in one range: its range is picked once per step (a compare chain on the
+// padded starts), not once per entry, and the padding entries are skipped.
 template <int W, int N, typename VisitPoint>
 __device__ __forceinline__ void stream_concat(const uint32_t (&lo)[N], const uint32_t (&hi)[N],
                                               const float4* __restrict__ pts, VisitPoint&& visit) {
-  uint32_t start[N], delta[N];
+  uint32_t start[N], delta[N], end[N];
   uint32_t total = 0;
 #pragma unroll
   for (int k = 0; k < N; ++k) {
+    const uint32_t len = hi[k] - lo[k];
     start[k] = total;
     delta[k] = lo[k] - total;  // position = offset + delta (mod 2^32)
-    total += hi[k] - lo[k];
+    end[k] = total + len;      // first padding offset of range k
+    total += (len + uint32_t(W - 1)) & ~uint32_t(W - 1);
   }
   for (uint32_t c = 0; c < total; c += W) {
+    // the last range starting at or before c (an empty range's successor
+    // starts at the same offset and wins)
+    uint32_t d = delta[0], e = end[0];
+#pragma unroll
+    for (int k = 1; k < N; ++k) {
+      const bool in = c >= start[k];
+      d = in ? delta[k] : d;
+      e = in ? end[k] : e;
+    }
     uint32_t pos[W];
     float4 q[W];
 #pragma unroll
     for (int w = 0; w < W; ++w) {
-      const uint32_t o = min(c + uint32_t(w), total - 1);
-      uint32_t d = delta[0];
-#pragma unroll
-      for (int k = 1; k < N; ++k) d = o >= start[k] ? delta[k] : d;
-      pos[w] = o + d;
+      pos[w] = min(c + uint32_t(w), e - 1u) + d;  // padding reloads the range's last point
       q[w] = pts[pos[w]];
     }
 #pragma unroll
     for (int w = 0; w < W; ++w)
-      if (c + uint32_t(w) < total) visit(pos[w], q[w]);
+      if (c + uint32_t(w) < e) visit(pos[w], q[w]);
   }
 }
 
