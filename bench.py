@@ -98,11 +98,19 @@ def _batched_vs_plain(count):
                         transformation_epsilon=0.0, fitness_epsilon=0.0, voxel_leaf=0.1)
     reg = sr.GicpRegistration(cfg)
     sv, tv = [s.view for s in srcs], [t.view for t in tgts]
-    # warm-up of both legs (allocator, graph cache, worker streams)
+    # warm-up of both legs (allocator, graph cache, worker streams); the
+    # batched leg's allocator is warmed at its full size by a batch of other
+    # pairs of the same generator (so the timed call reuses no graph of its own)
     for s_, t_ in zip(sv[:8], tv[:8]):
         reg.align(s_, t_)
-    sr.batch_align_pairs(sv[:8], tv[:8], cfg)
+    warm = synth.batch_pairs(count=count, seed_offset=1)
+    ws = [devmem.DeviceCloud(p.source) for p in warm]
+    wt = [devmem.DeviceCloud(p.target) for p in warm]
+    sr.batch_align_pairs([c.view for c in ws], [c.view for c in wt], cfg)
     devmem.synchronize()
+    for c in ws + wt:
+        c.close()
+    del warm
     t0 = time.perf_counter()
     plain = [reg.align(s_, t_) for s_, t_ in zip(sv, tv)]
     devmem.synchronize()
