@@ -369,7 +369,7 @@ __device__ __forceinline__ void nn_phase1(const GridDesc& g, const uint32_t* __r
 }
 
 #ifndef SR_NN_GROUP
-#define SR_NN_GROUP 16
+#define SR_NN_GROUP 8
 #endif
 constexpr int kNNGroup = SR_NN_GROUP;  // lanes per query in phase 2 (a power of two <= 64)
 static_assert(kNNGroup >= 1 && kNNGroup <= 64 && (kNNGroup & (kNNGroup - 1)) == 0, "group size");
