@@ -374,7 +374,7 @@ __device__ __forceinline__ void nn_phase1(const GridDesc& g, const uint32_t* __r
 constexpr int kNNGroup = SR_NN_GROUP;  // lanes per query in phase 2 (a power of two <= 64)
 static_assert(kNNGroup >= 1 && kNNGroup <= 64 && (kNNGroup & (kNNGroup - 1)) == 0, "group size");
 #ifndef SR_ROW_BATCH
-#define SR_ROW_BATCH 2
+#define SR_ROW_BATCH 1
 #endif
 constexpr int kRowBatch = SR_ROW_BATCH;  // rows per bound-loading batch in phase 2
 #ifndef SR_STREAM_W
