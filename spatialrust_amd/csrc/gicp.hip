@@ -737,6 +737,13 @@ constexpr uint32_t kHardBlockFlag = 0x80000000u;  // HardQuery.inst: block scann
 #ifndef SR_SEARCH_WAVES
 #define SR_SEARCH_WAVES 4
 #endif
+// The search stage over marked points (batches) gains from a fifth wave per
+// SIMD although registers then spill: its waves mostly wait on the dependent
+// loads of a few marked lanes (measured on 256 pairs: 31 passes 13.7 -> 13.3 ms;
+// the one-kernel form is slower at 5).
+#ifndef SR_MARKED_WAVES
+#define SR_MARKED_WAVES 5
+#endif
 
 // First stage. One-point-per-thread blocks: each point whose match the last
 // search certified for the pose's movement so far keeps it (budget spent, or
@@ -896,7 +903,7 @@ __device__ __forceinline__ void search_block(const IterInstance* __restrict__ in
 // Second stage, the blocks k_gicp_settle did not finish.
 template <bool kStats>
 __global__ void __launch_bounds__(kBlock)
-    __attribute__((amdgpu_waves_per_eu(SR_SEARCH_WAVES))) k_gicp_search(const IterInstance* __restrict__ insts,
+    __attribute__((amdgpu_waves_per_eu(SR_MARKED_WAVES))) k_gicp_search(const IterInstance* __restrict__ insts,
                                                         const uint32_t* __restrict__ block_inst,
                                                         PassScratch ps,
                                                         SearchStats* __restrict__ stats) {
