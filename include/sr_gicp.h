@@ -130,6 +130,18 @@ int32_t sr_version(void);
 /* GicpConfig::default() (crates/spatialrust-registration/src/gicp.rs:46-60). */
 void sr_gicp_config_default(sr_gicp_config* out);
 
+/* ---- runtime options -----------------------------------------------------
+ * No counterpart in the reference (its GICP has no device-side state to
+ * tune). Process-wide switches read when a registration is set up:
+ *   "match_reuse" (default 1): a Gauss-Newton pass may keep a source point's
+ *     correspondence without searching when the last search proved that no
+ *     other target point can have become nearer (half the gap to the
+ *     second-nearest point exceeds the pose's movement). 0 searches every
+ *     point in every pass; results are the same either way.
+ * Unknown names and values out of range are InvalidArgument. */
+int32_t sr_option_set(const char* name, int64_t value);
+int32_t sr_option_get(const char* name, int64_t* value);
+
 /* ---- kernel timing ------------------------------------------------------
  * No counterpart in the reference (it records no GPU timestamps,
  * `timestamp_writes: None`). When enabled, launches of the named kernels are

@@ -233,6 +233,13 @@ ProfScope::~ProfScope() {
   (void)hipEventRecord(b, stream);
 }
 
+// -------------------------------------------------------------- options ----
+
+Options& options() {
+  static Options o;
+  return o;
+}
+
 // --------------------------------------------------------- preparation ----
 
 struct Prepared {
@@ -317,6 +324,7 @@ static InstConfig inst_config(const sr_gicp_config& c) {
   ic.one_minus_eps = float(1.0 - c.epsilon);
   ic.max_iter = c.max_iterations;
   ic.min_corr = c.min_correspondences;
+  ic.flags = options().match_reuse.load() ? kCfgMatchReuse : 0u;
   ic.trans_eps = c.transformation_epsilon;
   ic.fit_eps = c.fitness_epsilon;
   return ic;
@@ -809,13 +817,35 @@ static int32_t state_to_result(const InstState& st, const sr_gicp_config& cfg, c
   return SR_OK;
 }
 
+// k_neighbors bounds: the reference's lower bound (gicp.rs:113-117) and this
+// backend's upper bound on the k-NN covariance networks (the radius path does
+// not use k).
+static void check_k(uint32_t k, float covariance_radius) {
+  if (k < 3) fail(SR_ERR_INVALID_ARGUMENT, "GICP needs k_neighbors >= 3 for covariance estimation");
+  if (k > kMaxKnnK && !(covariance_radius > 0.0f))
+    fail(SR_ERR_INVALID_ARGUMENT, "GICP k-NN covariances on the HIP backend support k_neighbors <= " +
+                                      std::to_string(kMaxKnnK));
+}
+
+// A request against a resident map must describe the map's target the way the
+// map was built: the target normals were computed with the map's k_neighbors
+// / covariance_radius and voxel leaf, and the combined covariance uses one
+// epsilon for both clouds (gicp.rs:119-123 computes both with one config).
+static void check_map_config(const sr_gicp_config& map_cfg, const sr_gicp_config& cfg) {
+  const bool radius = map_cfg.covariance_radius > 0.0f, r2 = cfg.covariance_radius > 0.0f;
+  if (map_cfg.epsilon != cfg.epsilon || map_cfg.voxel_leaf != cfg.voxel_leaf || radius != r2 ||
+      (radius && map_cfg.covariance_radius != cfg.covariance_radius) ||
+      (!radius && map_cfg.k_neighbors != cfg.k_neighbors))
+    fail(SR_ERR_INVALID_ARGUMENT,
+         "config must match the map's epsilon, voxel_leaf, k_neighbors and covariance_radius");
+}
+
 static void validate_cfg(const sr_xyz_view* source, const sr_xyz_view* target,
                          const sr_gicp_config* cfg) {
   if (!cfg) fail(SR_ERR_INVALID_ARGUMENT, "config must not be null");
   if ((source && source->n == 0) || (target && target->n == 0) || !source || !target)
     fail(SR_ERR_INVALID_ARGUMENT, "GICP requires non-empty source and target point clouds");
-  if (cfg->k_neighbors < 3)
-    fail(SR_ERR_INVALID_ARGUMENT, "GICP needs k_neighbors >= 3 for covariance estimation");
+  check_k(cfg->k_neighbors, cfg->covariance_radius);
   if (!(cfg->voxel_leaf <= 0.0f) && !(cfg->voxel_leaf > 0.0f))
     fail(SR_ERR_INVALID_ARGUMENT, "leaf_size must be greater than zero");
 }
@@ -1199,6 +1229,28 @@ int64_t sr_profile_read_each(const char* name, double* out_ms, uint64_t cap) {
   return int64_t(cnt);
 }
 
+int32_t sr_option_set(const char* name, int64_t value) {
+  return guarded([&] {
+    if (!name) fail(SR_ERR_INVALID_ARGUMENT, "option name must not be null");
+    const std::string n(name);
+    if (n == "match_reuse") {
+      if (value != 0 && value != 1) fail(SR_ERR_INVALID_ARGUMENT, "match_reuse takes 0 or 1");
+      options().match_reuse.store(value);
+    } else {
+      fail(SR_ERR_INVALID_ARGUMENT, "unknown option " + n);
+    }
+  });
+}
+
+int32_t sr_option_get(const char* name, int64_t* value) {
+  return guarded([&] {
+    if (!name || !value) fail(SR_ERR_INVALID_ARGUMENT, "null option argument");
+    const std::string n(name);
+    if (n == "match_reuse") *value = options().match_reuse.load();
+    else fail(SR_ERR_INVALID_ARGUMENT, "unknown option " + n);
+  });
+}
+
 int32_t sr_device_available(void) { return device_state().status == 0 ? 1 : 0; }
 
 int32_t sr_version(void) { return 100; }
@@ -1489,6 +1541,7 @@ int32_t sr_map_align(const sr_map* map, const sr_xyz_view* source, const sr_gicp
     validate_view(source);
     const sr_xyz_view dummy{nullptr, nullptr, nullptr, map->tgt.n, 0};
     validate_cfg(source, &dummy, config);
+    check_map_config(map->cfg, *config);
     hipStream_t s = thread_stream();
     Receipt rc;
     Prepared src;
@@ -1522,6 +1575,7 @@ int32_t sr_batch_align(const sr_map* const* maps, const sr_xyz_view* sources, ui
       validate_view(&sources[i]);
       const sr_xyz_view dummy{nullptr, nullptr, nullptr, maps[i]->tgt.n, 0};
       validate_cfg(&sources[i], &dummy, config);
+      check_map_config(maps[i]->cfg, *config);
       views[i] = &sources[i];
       sp[i] = &srcs[i];
       tp[i] = &maps[i]->tgt;
@@ -1599,8 +1653,7 @@ int32_t sr_server_create(const sr_server_config* config, const sr_gicp_config* g
     *out = nullptr;
     if (!config || !gicp) fail(SR_ERR_INVALID_ARGUMENT, "server config must not be null");
     if (config->max_active == 0) fail(SR_ERR_INVALID_ARGUMENT, "max_active must be at least 1");
-    if (gicp->k_neighbors < 3)
-      fail(SR_ERR_INVALID_ARGUMENT, "GICP needs k_neighbors >= 3 for covariance estimation");
+    check_k(gicp->k_neighbors, gicp->covariance_radius);
     ensure_device();
     auto sv = std::make_unique<sr_server>();
     sv->cfg = *config;
@@ -1624,6 +1677,7 @@ int32_t sr_server_submit(sr_server* server, const sr_map* map, const sr_xyz_view
     validate_view(source);
     const sr_xyz_view dummy{nullptr, nullptr, nullptr, map->tgt.n, 0};
     validate_cfg(source, &dummy, &server->gicp);
+    check_map_config(map->cfg, server->gicp);
     if (source->mem != SR_MEM_HOST)
       fail(SR_ERR_INVALID_ARGUMENT, "server requests take host xyz columns");
     const uint64_t need = request_bytes(source->n);
@@ -1694,8 +1748,7 @@ int32_t sr_odometry_create(const sr_odometry_config* config, const sr_gicp_confi
     if (!(config->voxel_leaf > 0.0f)) fail(SR_ERR_INVALID_ARGUMENT, "leaf_size must be greater than zero");
     if (config->max_map_points == 0 || config->max_map_points > 0x7FFFFFFFull)
       fail(SR_ERR_INVALID_ARGUMENT, "max_map_points must be in [1, 2^31)");
-    if (gicp->k_neighbors < 3)
-      fail(SR_ERR_INVALID_ARGUMENT, "GICP needs k_neighbors >= 3 for covariance estimation");
+    check_k(gicp->k_neighbors, gicp->covariance_radius);
     ensure_device();
     auto o = std::make_unique<sr_odometry>();
     o->cfg = *config;

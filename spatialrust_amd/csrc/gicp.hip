@@ -838,7 +838,8 @@ __device__ __forceinline__ void search_block(const IterInstance* __restrict__ in
   const unsigned long long below = (1ull << lane) - 1ull;
   const float move_rot = st->move_rot, move_const = st->move_const;
   // budgets are worth computing once no source point moves kGapMove cells
-  const bool gap = move_rot * I.src_radius + move_const < kGapMove * g.h;
+  // (and only with match reuse on: without a budget every point is searched)
+  const bool gap = (cfg.flags & kCfgMatchReuse) && move_rot * I.src_radius + move_const < kGapMove * g.h;
   const uint32_t ppt = I.ppt;
   for (uint32_t k = 0; k < ppt; ++k) {
     const uint32_t i = (lb * ppt + k) * kBlock + threadIdx.x;
@@ -927,7 +928,9 @@ __global__ void __launch_bounds__(kBlock)
   PassPose P;
   load_pose(st, P);
   // budgets are worth computing once no source point moves kGapMove cells
-  const bool gap = st->move_rot * I.src_radius + st->move_const < kGapMove * g.h;
+  // (and only with match reuse on: without a budget every point is searched)
+  const bool gap = (cfg.flags & kCfgMatchReuse) &&
+                   st->move_rot * I.src_radius + st->move_const < kGapMove * g.h;
   const uint32_t i = (b - I.block_begin) * kBlock + threadIdx.x;
   float px = 0.0f, py = 0.0f, pz = 0.0f;
   NNQuery Q;
@@ -995,7 +998,8 @@ __global__ void __launch_bounds__(kBlock)
   const unsigned long long below = (1ull << lane) - 1ull;
   const float move_rot = st->move_rot, move_const = st->move_const;
   // budgets are worth computing once no source point moves kGapMove cells
-  const bool gap = move_rot * I.src_radius + move_const < kGapMove * g.h;
+  // (and only with match reuse on: without a budget every point is searched)
+  const bool gap = (cfg.flags & kCfgMatchReuse) && move_rot * I.src_radius + move_const < kGapMove * g.h;
   const uint32_t ppt = I.ppt;
   const float beta = cfg.one_minus_eps, eps = 1.0f - cfg.one_minus_eps;
   __shared__ RedLds L;

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <cstddef>
 #include <string>
 #include <vector>
@@ -224,6 +225,9 @@ void host_bounds(const Cloud& c, uint32_t* host6, hipStream_t s);
 
 // ---- covariances ---------------------------------------------------------
 
+// Largest k of the k-NN kernels (their per-lane sorting networks).
+constexpr uint32_t kMaxKnnK = 64;
+
 // Exact k-NN normals of the grid's own points, in grid order: float4(n, a) with
 // C = a*I - (1-eps) n n^T (a = 1). If cov6 is non-null also writes the 6
 // regularized covariance entries per point in ORIGINAL index order.
@@ -293,7 +297,7 @@ struct InstConfig {
   float one_minus_eps;  // 1 - epsilon of the plane regularization
   uint32_t max_iter;
   uint32_t min_corr;
-  uint32_t pad_;
+  uint32_t flags;   // kCfgMatchReuse: certified matches may be kept without a search
   double trans_eps;
   double fit_eps;
 };
@@ -321,6 +325,14 @@ struct IterInstance {
   uint32_t ppt;           // source points per thread (from this instance's size only,
                           // so its sums do not depend on the rest of a batch)
 };
+
+constexpr uint32_t kCfgMatchReuse = 1u;  // InstConfig.flags
+
+// Library-wide runtime options (sr_option_set / sr_option_get).
+struct Options {
+  std::atomic<int64_t> match_reuse{1};  // certified match reuse in the Gauss-Newton passes
+};
+Options& options();
 
 constexpr int kTerms = 28;  // 21 H (upper triangle) + 6 g + fitness sum
 

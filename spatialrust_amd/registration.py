@@ -221,6 +221,17 @@ def device_available() -> bool:
     return bool(_n.load().sr_device_available())
 
 
+def set_option(name: str, value: int) -> None:
+    """Process-wide runtime option (sr_option_set), e.g. "match_reuse"."""
+    _check(_n.load().sr_option_set(name.encode(), int(value)))
+
+
+def get_option(name: str) -> int:
+    v = ctypes.c_int64(0)
+    _check(_n.load().sr_option_get(name.encode(), ctypes.byref(v)))
+    return int(v.value)
+
+
 class GicpRegistration:
     """Generalized ICP on the HIP device (GicpRegistration, gicp.rs:70-235)."""
 

@@ -392,11 +392,13 @@ def invert_motion(q, t):
 
 
 def rotation_angle_between(qa, qb):
-    """Angle (rad) of the relative rotation between two unit quaternions."""
+    """Angle (rad) of the relative rotation between two unit quaternions:
+    2 atan2(|v|, |w|) of conj(qa) * qb = (v, w), which stays accurate for tiny
+    angles (an arccos of the dot product rounds everything below ~2e-8 rad to 0)."""
     qa = np.asarray(qa, np.float64) / np.linalg.norm(qa)
     qb = np.asarray(qb, np.float64) / np.linalg.norm(qb)
-    d = abs(float(np.dot(qa, qb)))
-    return 2.0 * np.arccos(min(1.0, d))
+    rel = quat_mul(np.array([-qa[0], -qa[1], -qa[2], qa[3]]), qb)
+    return float(2.0 * np.arctan2(np.linalg.norm(rel[:3]), abs(rel[3])))
 
 
 # --------------------------------------------------------------- workloads --

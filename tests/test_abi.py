@@ -130,3 +130,18 @@ def test_pipeline_rejects_non_gicp_methods():
     from spatialrust_amd.registration import InvalidArgument
     with pytest.raises(InvalidArgument, match="no HIP backend"):
         pl.registration_stage(np.zeros((10, 3), np.float32), pl.MvpIcpConfig())
+
+
+def test_runtime_options_round_trip_and_errors():
+    """sr_option_set / sr_option_get need no device: known names round-trip,
+    unknown names and out-of-range values are InvalidArgument."""
+    assert sr.get_option("match_reuse") == 1
+    sr.set_option("match_reuse", 0)
+    try:
+        assert sr.get_option("match_reuse") == 0
+    finally:
+        sr.set_option("match_reuse", 1)
+    with pytest.raises(sr.InvalidArgument, match="unknown option"):
+        sr.set_option("no_such_option", 1)
+    with pytest.raises(sr.InvalidArgument, match="0 or 1"):
+        sr.set_option("match_reuse", 2)

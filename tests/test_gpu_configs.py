@@ -1,0 +1,230 @@
+"""HIP path at the benchmark configurations and on the claims that a single
+pass cannot show (gpu).
+
+  large block (2M-point target, 200k source; leaf 0.2, k 15, d_max 2.0, 50
+    iterations)           pose vs the FP64 oracle golden: <= 1e-4 rad and
+                          <= 1 mm; voxel counts exact; fitness within 1e-4 relative
+  256-pair batch (small-cube generator, sources log-uniform in [5k, 120k])
+                          batched == one-at-a-time, bit for bit; eight golden
+                          instances vs the FP64 oracle at <= 1e-4 rad, <= 1 mm
+  certified match reuse   on == off, bit for bit (poses, fitness, iterations),
+                          and on does less search work
+  singular normal equations (gicp.rs:190-194, linalg.rs:25-27)
+                          the reference's error on an input whose H has an
+                          exactly zero row
+  radius covariances      pose vs the FP64 oracle fed the oracle's radius
+                          covariances: <= 1e-4 rad and <= 1 mm
+"""
+import ctypes
+import json
+import os
+
+import numpy as np
+import pytest
+
+import spatialrust_amd as sr
+from spatialrust_amd import devmem, synth
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _fixed(**kw):
+    return sr.GicpConfig(transformation_epsilon=0.0, fitness_epsilon=0.0, **kw)
+
+
+def _pose_err(res, q, t):
+    return (synth.rotation_angle_between(res.rotation64, q),
+            float(np.linalg.norm(res.translation64 - np.asarray(t))))
+
+
+def _same(a, b):
+    return (np.array_equal(a.rotation64, b.rotation64) and
+            np.array_equal(a.translation64, b.translation64) and a.fitness == b.fitness and
+            a.iterations == b.iterations and a.converged == b.converged)
+
+
+def test_large_block_matches_fp64_golden(hip):
+    w = synth.large_block_pair()
+    ref = json.load(open(os.path.join(GOLDEN, "fp64_poses.json")))[w.name]
+    print(f"large block: {len(w.source)} source, {len(w.target)} target points", flush=True)
+    src, tgt = devmem.DeviceCloud(w.source), devmem.DeviceCloud(w.target)
+    try:
+        cfg = _fixed(max_iterations=w.max_iterations, k_neighbors=w.k_neighbors,
+                     max_correspondence_distance=w.max_correspondence_distance,
+                     voxel_leaf=w.voxel_leaf)
+        r = sr.GicpRegistration(cfg).align(src.view, tgt.view)
+        host = sr.GicpRegistration(cfg).align(w.source, w.target)  # host views: same result
+    finally:
+        src.close()
+        tgt.close()
+    assert (r.source_points, r.target_points) == (182352, 897684)
+    assert (r.source_points, r.target_points) == (ref["source_points"], ref["target_points"])
+    assert r.iterations == ref["iterations"] == 50
+    rot, tr = _pose_err(r, ref["q"], ref["t"])
+    print(f"large block vs FP64 oracle: {rot:.3e} rad, {tr:.3e} m", flush=True)
+    assert rot <= 1e-4 and tr <= 1e-3 and tr <= 1e-4 * w.extent
+    assert abs(r.fitness - ref["fitness"]) <= 1e-4 * ref["fitness"]
+    assert _same(r, host)
+
+
+@pytest.fixture(scope="module")
+def batch256():
+    pairs = synth.batch_pairs(count=256)
+    srcs = [devmem.DeviceCloud(p.source) for p in pairs]
+    tgts = [devmem.DeviceCloud(p.target) for p in pairs]
+    yield pairs, srcs, tgts
+    for c in srcs + tgts:
+        c.close()
+
+
+BATCH_CFG = dict(max_iterations=30, k_neighbors=20, max_correspondence_distance=1.0, voxel_leaf=0.1)
+
+
+def test_batch_256_equals_plain_and_fp64_golden(hip, batch256):
+    pairs, srcs, tgts = batch256
+    cfg = _fixed(**BATCH_CFG)
+    gold = json.load(open(os.path.join(GOLDEN, "batch_poses.json")))
+    assert gold["count"] == len(pairs) == 256
+    batched = sr.batch_align_pairs([c.view for c in srcs], [c.view for c in tgts], cfg)
+    reg = sr.GicpRegistration(cfg)
+    for i in range(len(pairs)):
+        alone = reg.align(srcs[i].view, tgts[i].view)
+        assert _same(alone, batched[i]), i
+    worst = (0.0, 0.0)
+    for key, ref in gold["instances"].items():
+        i = int(key)
+        r = batched[i]
+        assert len(pairs[i].source) == ref["raw_source_points"]
+        assert (r.source_points, r.target_points) == (ref["source_points"], ref["target_points"])
+        rot, tr = _pose_err(r, ref["q"], ref["t"])
+        worst = (max(worst[0], rot), max(worst[1], tr))
+        assert rot <= 1e-4 and tr <= 1e-3, (i, rot, tr)
+        assert abs(r.fitness - ref["fitness"]) <= 1e-4 * ref["fitness"], i
+    print(f"batch golden instances: worst {worst[0]:.3e} rad, {worst[1]:.3e} m", flush=True)
+
+
+def _search_points(fn):
+    """Runs fn() with search statistics on; returns (its result, points tested
+    by the Gauss-Newton passes)."""
+    from spatialrust_amd import native
+    lib = native.load()
+    st = (ctypes.c_uint64 * 10)()
+    lib.sr_profile_read_stats(b"gicp_pass", st)  # clears
+    lib.sr_profile_enable(2)
+    try:
+        out = fn()
+    finally:
+        lib.sr_profile_enable(0)
+    lib.sr_profile_read_stats(b"gicp_pass", st)
+    lib.sr_profile_reset()
+    return out, int(st[3])
+
+
+@pytest.fixture
+def reuse_restored():
+    yield
+    sr.set_option("match_reuse", 1)
+
+
+def test_match_reuse_off_is_bit_identical(hip, reuse_restored, batch256):
+    assert sr.get_option("match_reuse") == 1
+    w = synth.small_cube_pair()
+    pairs, srcs, tgts = batch256
+    # a subset of over 4096 blocks of 256 source points: the batch runs the
+    # two-kernel first stage (settle + marked search)
+    sub = list(range(0, 256, 3))
+    assert sum(len(pairs[i].source) for i in sub) * 0.85 > 4096 * 256
+    runs = {}
+    for reuse in (1, 0):
+        sr.set_option("match_reuse", reuse)
+        fixed = sr.GicpRegistration(_fixed(max_iterations=30, voxel_leaf=0.1))
+        dflt = sr.GicpRegistration(sr.GicpConfig(max_iterations=30, voxel_leaf=0.1))
+        a, pts = _search_points(lambda: fixed.align(w.source, w.target))
+        b = dflt.align(w.source, w.target)
+        c = sr.batch_align_pairs([srcs[i].view for i in sub], [tgts[i].view for i in sub],
+                                 _fixed(**BATCH_CFG))
+        runs[reuse] = (a, b, c, pts)
+    on, off = runs[1], runs[0]
+    assert _same(on[0], off[0])
+    assert _same(on[1], off[1])
+    assert all(_same(x, y) for x, y in zip(on[2], off[2]))
+    print(f"points tested over 31 passes: reuse on {on[3]}, off {off[3]}", flush=True)
+    assert on[3] < off[3]
+
+
+def test_singular_normal_equations(hip, orc):
+    """Source and target on the x axis: with the identity pose every
+    transformed point has y = z = 0 exactly, so the rotation about x moves no
+    point, the first row and column of H are exactly zero, and both the
+    reference's partial-pivot elimination and the LDL^T stop on that pivot."""
+    x = np.arange(200, dtype=np.float32) * np.float32(0.05)
+    z0 = np.zeros_like(x)
+    tgt = np.stack([x, z0, z0], 1)
+    src = np.stack([x + np.float32(0.01), z0, z0], 1)
+    cfg = sr.GicpConfig(k_neighbors=10, max_iterations=10)
+    for mode in ("faithful", "fp64"):
+        with pytest.raises(ValueError, match="GICP normal equations were singular"):
+            orc.gicp_align(src, tgt, cfg._c(), mode=mode)
+    with pytest.raises(sr.InvalidArgument, match="^GICP normal equations were singular$"):
+        sr.GicpRegistration(cfg).align(src, tgt)
+
+
+def _cov6_to_9(c6):
+    m = np.zeros((len(c6), 3, 3))
+    m[:, 0, 0], m[:, 1, 1], m[:, 2, 2] = c6[:, 0], c6[:, 1], c6[:, 2]
+    m[:, 0, 1] = m[:, 1, 0] = c6[:, 3]
+    m[:, 0, 2] = m[:, 2, 0] = c6[:, 4]
+    m[:, 1, 2] = m[:, 2, 1] = c6[:, 5]
+    return m
+
+
+def test_radius_covariance_pose_matches_fp64_oracle(hip, orc):
+    """covariance_radius set (gicp.rs:91-100 GPU covariance branch): the pose
+    vs the FP64 oracle loop fed the oracle's radius covariances (the CPU
+    evaluation of COV_GRID_WGSL, bit-exact to the device's)."""
+    w = synth.small_cube_pair(n_source=30000, n_target=30000)
+    sv, _, _ = orc.voxel_centroid(w.source, 0.1)
+    tv, _, _ = orc.voxel_centroid(w.target, 0.1)
+    radius = 0.3
+    sc, _ = orc.radius_covariances(sv, radius, 1e-3)
+    tc, _ = orc.radius_covariances(tv, radius, 1e-3)
+    cfg = _fixed(max_iterations=20, covariance_radius=radius)
+    ref = orc.gicp_align(sv, tv, cfg._c(), mode="fp64", src_cov=_cov6_to_9(sc),
+                         tgt_cov=_cov6_to_9(tc))
+    r = sr.GicpRegistration(cfg).align(sv, tv)
+    assert r.iterations == ref["iterations"] == 20
+    rot, tr = _pose_err(r, ref["q"], ref["t"])
+    print(f"radius covariances: {rot:.3e} rad, {tr:.3e} m vs FP64", flush=True)
+    assert rot <= 1e-4 and tr <= 1e-3
+    assert abs(r.fitness - ref["fitness"]) <= 1e-4 * ref["fitness"]
+
+
+def test_map_rejects_mismatched_config(hip):
+    w = synth.small_cube_pair(n_source=5000, n_target=8000)
+    base = sr.GicpConfig(voxel_leaf=0.1, max_iterations=5)
+    m = sr.TargetMap(w.target, base)
+    try:
+        m.align(w.source, sr.GicpConfig(voxel_leaf=0.1, max_iterations=3))  # iteration cap may differ
+        for bad in (sr.GicpConfig(voxel_leaf=0.1, epsilon=1e-2), sr.GicpConfig(voxel_leaf=0.2),
+                    sr.GicpConfig(voxel_leaf=0.1, k_neighbors=12),
+                    sr.GicpConfig(voxel_leaf=0.1, covariance_radius=0.3)):
+            with pytest.raises(sr.InvalidArgument, match="must match the map"):
+                m.align(w.source, bad)
+            with pytest.raises(sr.InvalidArgument, match="must match the map"):
+                sr.batch_align([m], [w.source], bad)
+    finally:
+        m.close()
+
+
+def test_k_above_network_size_is_rejected_before_work(hip):
+    cloud = synth.small_cube_pair(n_source=3000, n_target=3000).target
+    with pytest.raises(sr.InvalidArgument, match="k_neighbors <= 64"):
+        sr.GicpRegistration(sr.GicpConfig(k_neighbors=65)).align(cloud, cloud)
+    with pytest.raises(sr.InvalidArgument, match="k_neighbors <= 64"):
+        sr.Server(sr.GicpConfig(k_neighbors=80), hbm_budget_bytes=1 << 30, max_active=2)
+    # the radius path does not use k
+    r = sr.GicpRegistration(sr.GicpConfig(k_neighbors=65, covariance_radius=0.3,
+                                          max_iterations=2)).align(cloud, cloud)
+    assert r.iterations >= 1

@@ -450,3 +450,17 @@ def test_grid_normals_match_bruteforce(orc):
         assert abs(curv[i] - w[0] / w.sum()) < 1e-4 * max(1.0, w[0] / w.sum()) + 1e-6, i
         checked += 1
     assert checked > 1000
+
+
+def test_gicp_singular_on_collinear_clouds(orc):
+    """Known answer of linalg.rs:25-27 through gicp.rs:190-194: clouds on the x
+    axis at the identity pose give an H whose first row is exactly zero (the
+    rotation about x moves no point), so elimination reports singular."""
+    from spatialrust_amd.registration import GicpConfig
+    x = np.arange(200, dtype=np.float32) * np.float32(0.05)
+    z0 = np.zeros_like(x)
+    tgt = np.stack([x, z0, z0], 1)
+    src = np.stack([x + np.float32(0.01), z0, z0], 1)
+    for mode in ("faithful", "fp64"):
+        with pytest.raises(ValueError, match="GICP normal equations were singular"):
+            orc.gicp_align(src, tgt, GicpConfig(k_neighbors=10, max_iterations=10)._c(), mode=mode)
