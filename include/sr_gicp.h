@@ -171,6 +171,19 @@ int32_t sr_gicp_align(const sr_xyz_view* source, const sr_xyz_view* target,
                       const sr_gicp_config* config, sr_policy policy,
                       sr_result* out, sr_receipt* receipt);
 
+/* sr_gicp_align without device allocation. No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * semantics and bit-identical results as sr_gicp_align with policy
+ * Gpu(Hip); every device buffer of the call comes from a 256 MiB scratch array
+ * the library holds on each GPU it runs on until the process ends (query it
+ * with the read-only options "noalloc_scratch_bytes" and "noalloc_peak_bytes").
+ * Each cloud may hold at most 2,000,000 points; a call whose buffers would
+ * not fit the scratch fails with InvalidArgument. Calls on one GPU share its
+ * array and are serialised (a second caller waits); no graph is captured. */
+int32_t sr_gicp_align_noalloc(const sr_xyz_view* source, const sr_xyz_view* target,
+                              const sr_gicp_config* config, sr_result* out);
+
 /* One correspondence-and-accumulate pass of the Gauss-Newton loop at a given
  * FP64 pose (q = x y z w, t), without the solve: the loop body of
  * GicpRegistration::align_with_diagnostics (crates/spatialrust-registration/src/gicp.rs:137-179)

@@ -120,7 +120,72 @@ static size_t size_class(size_t bytes) {
   return c;
 }
 
+// Arena of the allocation-free entry point (sr_gicp_align_noalloc): while a
+// thread has one installed, its device buffers come from that arena instead
+// of the caching allocator.
+struct Arena {
+  char* base = nullptr;
+  size_t size = 0;
+  std::map<size_t, size_t> free_;           // offset -> length, coalesced
+  std::unordered_map<void*, size_t> live;   // pointer -> length
+  size_t in_use = 0, peak = 0;
+
+  void reset(char* b, size_t n) {
+    base = b;
+    size = n;
+    free_.clear();
+    free_[0] = n;
+    live.clear();
+    in_use = peak = 0;
+  }
+  // First fit, 256-byte granules.
+  void* alloc(size_t bytes) {
+    const size_t len = (std::max<size_t>(bytes, 1) + 255) & ~size_t(255);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < len) continue;
+      const size_t off = it->first, have = it->second;
+      free_.erase(it);
+      if (have > len) free_[off + len] = have - len;
+      void* p = base + off;
+      live[p] = len;
+      in_use += len;
+      peak = std::max(peak, in_use);
+      return p;
+    }
+    fail(SR_ERR_INVALID_ARGUMENT,
+         "the buffers of this call need more than the " + std::to_string(size >> 20) +
+             " MiB scratch of the allocation-free entry point (" + std::to_string(in_use) +
+             " bytes held, " + std::to_string(len) + " more requested)");
+  }
+  // false: p is not an arena block.
+  bool release(void* p) {
+    auto it = live.find(p);
+    if (it == live.end()) return false;
+    size_t off = size_t(static_cast<char*>(p) - base), len = it->second;
+    live.erase(it);
+    in_use -= len;
+    auto nx = free_.lower_bound(off);
+    if (nx != free_.end() && off + len == nx->first) {
+      len += nx->second;
+      nx = free_.erase(nx);
+    }
+    if (nx != free_.begin()) {
+      auto pv = std::prev(nx);
+      if (pv->first + pv->second == off) {
+        pv->second += len;
+        return true;
+      }
+    }
+    free_[off] = len;
+    return true;
+  }
+};
+static thread_local Arena* t_arena = nullptr;
+
+bool arena_active() { return t_arena != nullptr; }
+
 void* dev_alloc(size_t bytes) {
+  if (t_arena) return t_arena->alloc(bytes);
   Pool& P = pool();
   const size_t cls = size_class(bytes);
   const uint64_t tok = thread_token();
@@ -157,6 +222,7 @@ void* dev_alloc(size_t bytes) {
 
 void dev_free(void* p) {
   if (!p) return;
+  if (t_arena && t_arena->release(p)) return;
   Pool& P = pool();
   std::lock_guard<std::mutex> lk(P.mu);
   auto it = P.live.find(p);
@@ -201,7 +267,7 @@ static hipEvent_t prof_event() {
 
 SearchStats* prof_stats(const char* name) {
   Profiler& P = profiler();
-  if (P.on.load() != 2) return nullptr;
+  if (P.on.load() != 2 || t_arena) return nullptr;
   std::lock_guard<std::mutex> lk(P.mu);
   auto it = P.stats.find(name);
   if (it != P.stats.end()) return it->second;
@@ -214,7 +280,7 @@ SearchStats* prof_stats(const char* name) {
 
 ProfScope::ProfScope(const char* name, hipStream_t s) : stream(s) {
   Profiler& P = profiler();
-  if (!P.on.load()) return;
+  if (!P.on.load() || t_arena) return;  // the allocation-free entry point records nothing
   hipEvent_t a = prof_event(), b = prof_event();
   SR_HIP(hipEventRecord(a, s));
   std::lock_guard<std::mutex> lk(P.mu);
@@ -605,8 +671,13 @@ static LoopArgs& thread_loop_args() {
 }
 
 struct Batch {
-  DevBuf<IterInstance>& insts = thread_loop_args().insts;
-  DevBuf<uint32_t>& block_inst = thread_loop_args().block_inst;
+  // Launch descriptors and pass scratch: the thread's persistent buffers, or,
+  // inside the allocation-free entry point, buffers of this call (from its
+  // arena, released with the batch).
+  std::unique_ptr<LoopArgs> own = arena_active() ? std::make_unique<LoopArgs>() : nullptr;
+  LoopArgs& la = own ? *own : thread_loop_args();
+  DevBuf<IterInstance>& insts = la.insts;
+  DevBuf<uint32_t>& block_inst = la.block_inst;
   DevBuf<InstState> states;
   DevBuf<float> partials;
   DevBuf<double> init;
@@ -700,7 +771,7 @@ static void batch_setup(const std::vector<const Prepared*>& src, const std::vect
   }
   B.total_blocks = blocks;
   {
-    LoopArgs& la = thread_loop_args();
+    LoopArgs& la = B.la;
     B.scratch = grow_scratch(la.queue, la.ctl, la.done, total_pts, blocks, s);
     B.hard_blocks = hard_stage_blocks(total_pts);
   }
@@ -731,7 +802,9 @@ struct GraphEntry {
 };
 
 static void batch_run(Batch& B, hipStream_t s) {
-  if (profiler().on.load() != 0) {  // eager, so each launch can be timed
+  // eager when profiling (so each launch can be timed) and inside the
+  // allocation-free entry point (instantiating a graph allocates)
+  if (profiler().on.load() != 0 || arena_active()) {
     for (uint32_t k = 0; k < B.max_passes; ++k)
       gicp_pass(B.insts.p, B.block_inst.p, B.total_blocks, B.count, B.scratch, B.hard_blocks, s);
     return;
@@ -878,6 +951,69 @@ static sr_policy resolve(sr_policy p) {
   }
   return sr_policy{SR_GPU, SR_DEV_HIP};
 }
+
+}  // namespace sr
+
+// ------------------------------------------------- allocation-free entry ----
+//
+// sr_gicp_align_noalloc runs sr_gicp_align's kernels with every device buffer
+// carved from `scratch`, a file-scope device array each GPU holds for the
+// life of the process, so the call makes no hipMalloc / hipFree and no
+// allocating library call (and captures no graph, since instantiating one
+// allocates). Calls on one device are serialised by a lock and each ends with
+// its stream drained, so they never share the array's contents at the same
+// time; the array keeps the last call's data, and every kernel of the path
+// clears or overwrites what it reads (as with reused allocator blocks).
+
+namespace sr {
+constexpr size_t kScratchBytes = size_t(256) << 20;  // per device
+constexpr uint64_t kNoAllocMaxPoints = 2000000;     // per cloud
+constexpr int kMaxDevices = 64;
+}  // namespace sr
+
+static __device__ __align__(256) unsigned char scratch[sr::kScratchBytes];
+
+namespace sr {
+
+struct ScratchDevice {
+  std::mutex call;      // one allocation-free call on the device at a time
+  char* ptr = nullptr;  // device address of `scratch` on this device
+  Arena arena;
+  size_t last_peak = 0;
+};
+
+static ScratchDevice& scratch_device(int dev) {
+  static ScratchDevice devs[kMaxDevices];
+  static std::mutex setup;
+  if (dev < 0 || dev >= kMaxDevices) fail(SR_ERR_INVALID_ARGUMENT, "device id out of range");
+  ScratchDevice& d = devs[dev];
+  std::lock_guard<std::mutex> lk(setup);
+  if (!d.ptr) {
+    // Once per device, with that device current: the host name `scratch` is
+    // only a stand-in; kernels and copies take this address.
+    void* p = nullptr;
+    SR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) fail(SR_ERR_INVALID_ARGUMENT, "scratch array has no device address");
+    d.ptr = static_cast<char*>(p);
+  }
+  return d;
+}
+
+// Installs the device's arena on this thread for one call; on exit (normal or
+// error) the stream is drained before the lock is released.
+struct ArenaScope {
+  ScratchDevice& d;
+  hipStream_t s;
+  ArenaScope(ScratchDevice& dev, hipStream_t st) : d(dev), s(st) {
+    d.arena.reset(d.ptr, kScratchBytes);
+    t_arena = &d.arena;
+  }
+  ~ArenaScope() {
+    (void)hipStreamSynchronize(s);
+    d.last_peak = d.arena.peak;
+    t_arena = nullptr;
+  }
+};
 
 }  // namespace sr
 
@@ -1246,8 +1382,19 @@ int32_t sr_option_get(const char* name, int64_t* value) {
   return guarded([&] {
     if (!name || !value) fail(SR_ERR_INVALID_ARGUMENT, "null option argument");
     const std::string n(name);
-    if (n == "match_reuse") *value = options().match_reuse.load();
-    else fail(SR_ERR_INVALID_ARGUMENT, "unknown option " + n);
+    if (n == "match_reuse") {
+      *value = options().match_reuse.load();
+    } else if (n == "noalloc_scratch_bytes") {
+      *value = int64_t(kScratchBytes);
+    } else if (n == "noalloc_peak_bytes") {  // of the last allocation-free call on this device
+      int dev = 0;
+      SR_HIP(hipGetDevice(&dev));
+      ScratchDevice& sd = scratch_device(dev);
+      std::lock_guard<std::mutex> lk(sd.call);
+      *value = int64_t(sd.last_peak);
+    } else {
+      fail(SR_ERR_INVALID_ARGUMENT, "unknown option " + n);
+    }
   });
 }
 
@@ -1305,6 +1452,39 @@ int32_t sr_gicp_align(const sr_xyz_view* source, const sr_xyz_view* target,
     }
     std::string msg;
     int32_t code = state_to_result(st, *config, src, tgt, out, &msg);
+    if (code) fail(code, msg);
+  });
+}
+
+int32_t sr_gicp_align_noalloc(const sr_xyz_view* source, const sr_xyz_view* target,
+                              const sr_gicp_config* config, sr_result* out) {
+  return guarded([&] {
+    resolve(sr_policy{SR_GPU, SR_DEV_HIP});
+    validate_view(source);
+    validate_view(target);
+    validate_cfg(source, target, config);
+    if (source->n > kNoAllocMaxPoints || target->n > kNoAllocMaxPoints)
+      fail(SR_ERR_INVALID_ARGUMENT,
+           "sr_gicp_align_noalloc takes clouds of at most " + std::to_string(kNoAllocMaxPoints) +
+               " points (got " + std::to_string(source->n) + " and " + std::to_string(target->n) + ")");
+    hipStream_t s = thread_stream();
+    int dev = 0;
+    SR_HIP(hipGetDevice(&dev));
+    ScratchDevice& sd = scratch_device(dev);
+    std::lock_guard<std::mutex> lk(sd.call);
+    ArenaScope scope(sd, s);  // declared before every buffer of the call
+    Receipt rc;
+    Prepared tgt, src;
+    prepare(*target, *config, tgt, rc, s, true);
+    prepare(*source, *config, src, rc, s, false);
+    Batch B;
+    batch_setup({&src}, {&tgt}, {*config}, B, s);
+    batch_run(B, s);
+    InstState st;
+    SR_HIP(hipMemcpyAsync(&st, B.states.p, sizeof(st), hipMemcpyDeviceToHost, s));
+    SR_HIP(hipStreamSynchronize(s));
+    std::string msg;
+    const int32_t code = state_to_result(st, *config, src, tgt, out, &msg);
     if (code) fail(code, msg);
   });
 }

@@ -35,6 +35,9 @@ struct Error {
 // on the per-registration path).
 void* dev_alloc(size_t bytes);
 void dev_free(void* p);
+// True while the calling thread runs the allocation-free entry point: its
+// buffers then come from the library's static scratch array.
+bool arena_active();
 
 // RAII buffer from the caching allocator.
 template <typename T>

@@ -96,6 +96,8 @@ SIGNATURES = {
     "sr_gicp_align": (_I32, [ctypes.POINTER(XyzView), ctypes.POINTER(XyzView),
                              ctypes.POINTER(GicpConfigC), PolicyC, ctypes.POINTER(ResultC),
                              ctypes.POINTER(ReceiptC)]),
+    "sr_gicp_align_noalloc": (_I32, [ctypes.POINTER(XyzView), ctypes.POINTER(XyzView),
+                                     ctypes.POINTER(GicpConfigC), ctypes.POINTER(ResultC)]),
     "sr_grid_normals": (_I32, [ctypes.POINTER(XyzView), ctypes.c_float, _P, _P, _P]),
     "sr_batch_align_pairs": (_I32, [ctypes.POINTER(XyzView), ctypes.POINTER(XyzView), ctypes.c_uint32,
                                     ctypes.POINTER(GicpConfigC), ctypes.POINTER(ResultC)]),
