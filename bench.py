@@ -52,34 +52,81 @@ def _config(w):
                          transformation_epsilon=0.0, fitness_epsilon=0.0, voxel_leaf=w.voxel_leaf)
 
 
-def _cpu_baseline(w, budget_iters=(1, 3)):
-    """Faithful CPU oracle, one thread: voxel filter + (tree, covariances,
-    iterations) at two iteration counts; the per-iteration cost is extrapolated
-    to the workload's iteration count."""
+def _cpu_baseline(w, all_core_threads):
+    """Faithful CPU oracle (the f32/f64 restatement of the reference's CPU
+    path), timed in full on this host: one registration of the workload (voxel
+    filter of both clouds, KD-trees, k-NN covariances and every Gauss-Newton
+    iteration) on one thread, as the reference runs; the same registration with
+    its per-point loops on all cores (informational); and the reference's own
+    bench case, the 7,500-point box corner (benches/registration.rs:23-63,
+    81-86; published ~26 ms and a 0.00057 m probe error in
+    notes/2026-06-15_registration_bench.md:11,36-38)."""
     import oracle
     from spatialrust_amd.registration import GicpConfig
-    t0 = time.perf_counter()
-    sv, _, _ = oracle.voxel_centroid(w.source, w.voxel_leaf)
-    tv, _, _ = oracle.voxel_centroid(w.target, w.voxel_leaf)
-    t_vox = time.perf_counter() - t0
-    times = []
-    for iters in budget_iters:
-        cfg = GicpConfig(max_iterations=iters, k_neighbors=w.k_neighbors,
-                         max_correspondence_distance=w.max_correspondence_distance,
-                         transformation_epsilon=0.0, fitness_epsilon=0.0)
+    cfg = GicpConfig(max_iterations=w.max_iterations, k_neighbors=w.k_neighbors,
+                     max_correspondence_distance=w.max_correspondence_distance,
+                     transformation_epsilon=0.0, fitness_epsilon=0.0)
+
+    def one(threads):
         t0 = time.perf_counter()
-        oracle.gicp_align(sv, tv, cfg._c(), mode="faithful")
-        times.append(time.perf_counter() - t0)
-    per_iter = (times[1] - times[0]) / (budget_iters[1] - budget_iters[0])
-    fixed = times[0] - per_iter * budget_iters[0]
-    est = t_vox + fixed + per_iter * w.max_iterations
-    sample = (f"{w.name}: voxel filter of both clouds + trees + covariances in full, "
-              f"{budget_iters[0]} and {budget_iters[1]} Gauss-Newton iterations timed, "
-              f"per-iteration cost ({per_iter * 1e3:.1f} ms) extrapolated to {w.max_iterations}; "
-              f"measured {t_vox + sum(times):.1f} s of CPU work")
-    return {"value": 1.0 / est, "unit": "registrations/s", "cores": 1, "kind": "port",
-            "sample": sample, "seconds_per_registration": est,
+        sv, _, _ = oracle.voxel_centroid(w.source, w.voxel_leaf)
+        tv, _, _ = oracle.voxel_centroid(w.target, w.voxel_leaf)
+        r = oracle.gicp_align(sv, tv, cfg._c(), mode="faithful", threads=threads)
+        return time.perf_counter() - t0, r
+
+    t1, r1 = one(1)
+    tn, _ = one(all_core_threads)
+    corner = _reference_corner_cpu()
+    return {"value": 1.0 / t1, "unit": "registrations/s", "cores": 1, "kind": "port",
+            "sample": (f"{w.name}: one full registration (voxel filter of both clouds, trees, "
+                       f"covariances, {r1['iterations']} Gauss-Newton iterations), measured, "
+                       "not extrapolated"),
+            "seconds_per_registration": t1,
+            "all_core": {"threads": all_core_threads, "seconds_per_registration": tn,
+                         "registrations_per_s": 1.0 / tn,
+                         "note": "informational: the reference's GICP is single-threaded"},
+            "reference_corner": corner,
             "host_cpu": _cpu_model(), "host_nproc": os.cpu_count()}
+
+
+def _reference_corner_cpu(reps=9):
+    """The reference's registration bench case on the CPU oracle: median time
+    of `reps` runs and the probe error."""
+    import oracle
+    from spatialrust_amd.registration import GicpConfig
+    from tests import refinputs
+    tgt, src, mq, mt = refinputs.bench_corner_pair()
+    cfg = GicpConfig(max_correspondence_distance=0.3, max_iterations=40, k_neighbors=12)._c()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        r = oracle.gicp_align(src, tgt, cfg, mode="faithful")
+        ts.append(time.perf_counter() - t0)
+    return {"points": int(len(tgt)), "median_ms": float(np.median(ts) * 1e3),
+            "probe_error_m": refinputs.corner_probe_error(r["q"], r["t"], mq, mt),
+            "iterations": r["iterations"],
+            "published": {"median_ms": 26.0, "probe_error_m": 0.00057,
+                          "source": "notes/2026-06-15_registration_bench.md:11,36-38"}}
+
+
+def _reference_corner_gpu(reps=20):
+    """The same corner case through the HIP path (host views, as the
+    reference's bench passes its clouds): median time and probe error."""
+    import spatialrust_amd as sr
+    from tests import refinputs
+    tgt, src, mq, mt = refinputs.bench_corner_pair()
+    reg = sr.GicpRegistration(sr.GicpConfig(max_correspondence_distance=0.3, max_iterations=40,
+                                            k_neighbors=12))
+    for _ in range(3):
+        reg.align(src, tgt)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        r = reg.align(src, tgt)
+        ts.append(time.perf_counter() - t0)
+    return {"points": int(len(tgt)), "median_ms": float(np.median(ts) * 1e3),
+            "iterations": r.iterations, "converged": r.converged,
+            "probe_error_m": refinputs.corner_probe_error(r.rotation64, r.translation64, mq, mt)}
 
 
 def _batched_vs_plain(count):
@@ -123,7 +170,20 @@ def _batched_vs_plain(count):
                     np.array_equal(a.translation64, b.translation64) for a, b in zip(plain, batched))
     for c in srcs + tgts:
         c.close()
+    fp64 = None
+    gpath = os.path.join(ROOT, "tests", "golden", "batch_poses.json")
+    if os.path.exists(gpath):
+        gold = json.load(open(gpath))
+        if gold["count"] == count:
+            rot = [synth.rotation_angle_between(batched[int(i)].rotation64, g["q"])
+                   for i, g in gold["instances"].items()]
+            tr = [float(np.linalg.norm(batched[int(i)].translation64 - np.array(g["t"])))
+                  for i, g in gold["instances"].items()]
+            fp64 = {"instances": sorted(int(i) for i in gold["instances"]),
+                    "max_rot_rad": float(max(rot)), "max_trans_m": float(max(tr)),
+                    "vs": "FP64 CPU oracle (tests/golden/batch_poses.json)"}
     return {"workload": f"batch_{count}_small_cube_pairs", "pairs": count,
+            "error_vs_fp64_oracle": fp64,
             "source_points": int(sum(len(p.source) for p in pairs)),
             "target_points": int(sum(len(p.target) for p in pairs)),
             "plain_registrations_per_s": count / t_plain,
@@ -132,6 +192,21 @@ def _batched_vs_plain(count):
             "identical_poses": bool(identical),
             "max_rot_err_vs_ground_truth_rad": float(max(
                 synth.rotation_angle_between(b.rotation64, p.gt_q) for b, p in zip(batched, pairs)))}
+
+
+def _host_view_rate(w, cfg, steps):
+    """The drop-in call as a Rust caller makes it, with host (numpy) clouds:
+    the raw xyz columns (12 B per point) are copied to the device inside every
+    registration. Not the headline (which has the clouds resident in HBM)."""
+    import spatialrust_amd as sr
+    reg = sr.GicpRegistration(cfg)
+    reg.align(w.source, w.target)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        reg.align(w.source, w.target)
+    dt = (time.perf_counter() - t0) / steps
+    return {"registrations_per_s": 1.0 / dt, "ms_per_registration": dt * 1e3, "steps": steps,
+            "h2d_bytes_per_registration": 12 * (len(w.source) + len(w.target))}
 
 
 def _traffic(kernel, workload):
@@ -302,9 +377,13 @@ def main():
     if not args.no_batch:
         print("batched vs plain leg ...", file=sys.stderr, flush=True)
         line["batched_vs_plain"] = _batched_vs_plain(args.batch_count)
+    print("host-view and reference-corner legs ...", file=sys.stderr, flush=True)
+    line["host_view"] = _host_view_rate(w, cfg, steps=max(3, args.steps // 4))
+    line["reference_corner_gpu"] = _reference_corner_gpu()
     if not args.no_cpu_baseline:
-        print("timing the CPU oracle sample ...", file=sys.stderr, flush=True)
-        line["cpu_baseline"] = _cpu_baseline(w)
+        print("timing the CPU oracle (full registration, 1 thread, then all cores) ...",
+              file=sys.stderr, flush=True)
+        line["cpu_baseline"] = _cpu_baseline(w, min(16, os.cpu_count() or 1))
     print(json.dumps(line), flush=True)
 
 

@@ -67,6 +67,9 @@ def _declare(L):
     L.or_gicp_align.argtypes = [_F, _F, _F, ctypes.c_int64, _F, _F, _F, ctypes.c_int64,
                                 ctypes.c_void_p, ctypes.c_int32, _D, _D,
                                 ctypes.POINTER(OrAlignResult), _D, ctypes.c_int64]
+    L.or_gicp_align_mt.restype = ctypes.c_int32
+    L.or_gicp_align_mt.argtypes = [_F, _F, _F, ctypes.c_int64, _F, _F, _F, ctypes.c_int64,
+                                   ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(OrAlignResult)]
     L.or_gicp_normal_equations.restype = ctypes.c_int64
     L.or_gicp_normal_equations.argtypes = [_F, _F, _F, ctypes.c_int64, _F, _F, _F, ctypes.c_int64,
                                            _D, _D, _D, _D, ctypes.c_double, _D, _D, _I64]
@@ -277,15 +280,27 @@ _ERRORS = {
 
 
 def gicp_align(source, target, config, mode="faithful", src_cov=None, tgt_cov=None,
-               trace_cap=0):
+               trace_cap=0, threads=1):
     """GicpRegistration::align_with_diagnostics (gicp.rs:103-224).
 
     `config` is a ctypes sr_gicp_config (spatialrust_amd.native.GicpConfigC).
     Returns dict(q, t, fitness, iterations, converged, trace) or raises ValueError
-    with the reference's message."""
+    with the reference's message. threads > 1 runs the faithful loop with its
+    per-point work split over that many threads (an informational all-core
+    timing; the reference is single-threaded)."""
     sx, sy, sz = _cols(source)
     tx, ty, tz = _cols(target)
     res = OrAlignResult()
+    if threads > 1:
+        if mode != "faithful" or src_cov is not None or tgt_cov is not None or trace_cap:
+            raise ValueError("the multi-threaded form runs the faithful k-NN path only")
+        st = lib().or_gicp_align_mt(_p(sx, _F), _p(sy, _F), _p(sz, _F), len(sx), _p(tx, _F),
+                                    _p(ty, _F), _p(tz, _F), len(tx), ctypes.byref(config),
+                                    int(threads), ctypes.byref(res))
+        if st != 0:
+            raise ValueError(_ERRORS.get(res.error_kind, "invalid argument"))
+        return dict(q=np.array(res.q[:]), t=np.array(res.t[:]), fitness=res.fitness,
+                    iterations=int(res.iterations), converged=bool(res.converged), trace=None)
     trace = np.zeros((max(trace_cap, 1), 10), np.float64)
     sc = None if src_cov is None else np.ascontiguousarray(src_cov, dtype=np.float64)
     tc = None if tgt_cov is None else np.ascontiguousarray(tgt_cov, dtype=np.float64)

@@ -2,7 +2,9 @@
 
 Compiles oracle/oracle.cpp into oracle/liboracle.so with g++. -ffp-contract=off
 keeps every f32/f64 operation rounded separately, as in the reference's Rust
-code (rustc never contracts a*b+c into an FMA).
+code (rustc never contracts a*b+c into an FMA). -fopenmp serves only the
+informational all-core timing (or_gicp_align_mt); every other entry point runs
+on one thread, as the reference does.
 """
 import os
 import subprocess
@@ -14,7 +16,7 @@ LIB = os.path.join(HERE, "liboracle.so")
 def build(verbose: bool = False) -> str:
     src = os.path.join(HERE, "oracle.cpp")
     cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
-           "-fno-fast-math", "-Wall", "-Wextra", "-o", LIB + ".tmp", src]
+           "-fno-fast-math", "-fopenmp", "-Wall", "-Wextra", "-o", LIB + ".tmp", src]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -34,6 +34,7 @@
 #include <limits>
 #include <unordered_map>
 #include <vector>
+#include <omp.h>
 
 #include "../include/sr_gicp.h"
 
@@ -451,13 +452,18 @@ void regularize(const double cov[3][3], double eps, double out[3][3], double nor
 
 // covariances(): per point, k-NN (self included) -> accumulator -> regularize
 // (gicp.rs:259-274). Count < 2 gives the identity before regularization.
+// threads > 1 (the informational all-core timing only) splits the points over
+// OpenMP threads; each point's result is the same either way.
 template <typename P>
 void point_covariances(const KdTree<P>& tree, size_t k, double eps, std::vector<double>& out9,
-                       double* normals) {
+                       double* normals, int threads = 1) {
   size_t n = tree.x.size();
   out9.assign(n * 9, 0.0);
+#pragma omp parallel num_threads(threads) if (threads > 1)
+  {
   std::vector<Neighbor<P>> nb;
   nb.reserve(k);
+#pragma omp for schedule(dynamic, 512)
   for (size_t i = 0; i < n; ++i) {
     tree.nearest_k(tree.x[i], tree.y[i], tree.z[i], k, nb);
     CovAcc<P> acc;
@@ -471,6 +477,7 @@ void point_covariances(const KdTree<P>& tree, size_t k, double eps, std::vector<
     regularize(c, eps, r, normals ? normals + 3 * i : nullptr);
     for (int a = 0; a < 3; ++a)
       for (int b = 0; b < 3; ++b) out9[i * 9 + a * 3 + b] = r[a][b];
+  }
   }
 }
 
@@ -514,21 +521,42 @@ struct IterTrace {
 
 // euclidean_fitness (gicp.rs:330-357): mean squared inlier distance (f64 of
 // the P-precision differences); f64::MAX when there are no inliers.
+// threads > 1 (all-core timing only): per-thread sums added in thread order.
 template <typename P>
 double euclidean_fitness(const std::vector<V3<P>>& pts, const KdTree<P>& tree, P maxd2,
-                         std::vector<Neighbor<P>>& scratch) {
-  double sum = 0.0;
-  uint64_t count = 0;
-  for (auto& p : pts) {
-    tree.nearest_k(p.x, p.y, p.z, 1, scratch);
-    if (scratch.empty()) continue;
-    if (scratch[0].d2 > maxd2) continue;
-    uint32_t j = scratch[0].index;
+                         std::vector<Neighbor<P>>& scratch, int threads = 1) {
+  auto one = [&](const V3<P>& p, std::vector<Neighbor<P>>& nb, double& sum, uint64_t& count) {
+    tree.nearest_k(p.x, p.y, p.z, 1, nb);
+    if (nb.empty()) return;
+    if (nb[0].d2 > maxd2) return;
+    uint32_t j = nb[0].index;
     double dx = double(P(p.x - tree.x[j]));
     double dy = double(P(p.y - tree.y[j]));
     double dz = double(P(p.z - tree.z[j]));
     sum += dx * dx + dy * dy + dz * dz;
     ++count;
+  };
+  double sum = 0.0;
+  uint64_t count = 0;
+  if (threads <= 1) {
+    for (auto& p : pts) one(p, scratch, sum, count);
+  } else {
+    std::vector<double> ts(size_t(threads), 0.0);
+    std::vector<uint64_t> tc(size_t(threads), 0);
+#pragma omp parallel num_threads(threads)
+    {
+      std::vector<Neighbor<P>> nb;
+      double ls = 0.0;
+      uint64_t lc = 0;
+#pragma omp for schedule(static)
+      for (size_t i = 0; i < pts.size(); ++i) one(pts[i], nb, ls, lc);
+      ts[size_t(omp_get_thread_num())] = ls;
+      tc[size_t(omp_get_thread_num())] = lc;
+    }
+    for (int t = 0; t < threads; ++t) {
+      sum += ts[size_t(t)];
+      count += tc[size_t(t)];
+    }
   }
   if (count == 0) return std::numeric_limits<double>::max();
   return sum / double(count);
@@ -549,7 +577,7 @@ template <typename P>
 AlignOut gicp_align(const float* sx, const float* sy, const float* sz, size_t ns, const float* tx,
                     const float* ty, const float* tz, size_t nt, const sr_gicp_config& cfg,
                     const double* src_cov9_in, const double* tgt_cov9_in, IterTrace* trace,
-                    size_t trace_cap) {
+                    size_t trace_cap, int threads = 1) {
   AlignOut out{};
   if (ns == 0 || nt == 0) {
     out.status = SR_ERR_INVALID_ARGUMENT;
@@ -569,13 +597,13 @@ AlignOut gicp_align(const float* sx, const float* sy, const float* sz, size_t ns
     scov.assign(src_cov9_in, src_cov9_in + 9 * ns);
   } else {
     KdTree<P> st(Sx.data(), Sy.data(), Sz.data(), ns);
-    point_covariances(st, cfg.k_neighbors, cfg.epsilon, scov, nullptr);
+    point_covariances(st, cfg.k_neighbors, cfg.epsilon, scov, nullptr, threads);
   }
   if (tgt_cov9_in) {
     tcov.assign(tgt_cov9_in, tgt_cov9_in + 9 * nt);
   } else {
     KdTree<P> tt(Tx.data(), Ty.data(), Tz.data(), nt);  // the reference builds a second target tree
-    point_covariances(tt, cfg.k_neighbors, cfg.epsilon, tcov, nullptr);
+    point_covariances(tt, cfg.k_neighbors, cfg.epsilon, tcov, nullptr, threads);
   }
 
   const P dmax = P(cfg.max_correspondence_distance);
@@ -601,11 +629,13 @@ AlignOut gicp_align(const float* sx, const float* sy, const float* sz, size_t ns
     double H[6][6] = {};
     double g[6] = {};
     uint64_t count = 0;
-    for (size_t i = 0; i < ns; ++i) {
+    // One correspondence's terms (gicp.rs:144-178) added to (H, g, count).
+    auto body = [&](size_t i, std::vector<Neighbor<P>>& nb, double (&H)[6][6], double (&g)[6],
+                    uint64_t& count) {
       const V3<P>& p = tr[i];
       target_tree.nearest_k(p.x, p.y, p.z, 1, nb);
-      if (nb.empty()) continue;
-      if (nb[0].d2 > maxd2) continue;
+      if (nb.empty()) return;
+      if (nb[0].d2 > maxd2) return;
       uint32_t j = nb[0].index;
       Mat cs, tmp, rs, comb, m;
       std::memcpy(cs, &scov[9 * i], sizeof(cs));
@@ -613,7 +643,7 @@ AlignOut gicp_align(const float* sx, const float* sy, const float* sz, size_t ns
       mat_mul(tmp, rot_t, rs);
       for (int a = 0; a < 3; ++a)
         for (int b = 0; b < 3; ++b) comb[a][b] = tcov[9 * size_t(j) + a * 3 + b] + rs[a][b];
-      if (!inverse3(comb, m)) continue;
+      if (!inverse3(comb, m)) return;
       double e[3] = {double(p.x) - double(Tx[j]), double(p.y) - double(Ty[j]),
                      double(p.z) - double(Tz[j])};
       double px = double(p.x), py = double(p.y), pz = double(p.z);
@@ -636,6 +666,30 @@ AlignOut gicp_align(const float* sx, const float* sy, const float* sz, size_t ns
           for (int b = 0; b < 6; ++b) H[a][b] += jac[k][a] * mj[k][b];
         }
       ++count;
+    };
+    if (threads <= 1) {
+      for (size_t i = 0; i < ns; ++i) body(i, nb, H, g, count);
+    } else {  // all-core timing only: per-thread sums added in thread order
+      std::vector<double> th(size_t(threads) * 42, 0.0);
+      std::vector<uint64_t> tcnt(size_t(threads), 0);
+#pragma omp parallel num_threads(threads)
+      {
+        std::vector<Neighbor<P>> lnb;
+        double lH[6][6] = {}, lg[6] = {};
+        uint64_t lc = 0;
+#pragma omp for schedule(static)
+        for (size_t i = 0; i < ns; ++i) body(i, lnb, lH, lg, lc);
+        double* o = th.data() + 42 * size_t(omp_get_thread_num());
+        std::memcpy(o, lH, sizeof(lH));
+        std::memcpy(o + 36, lg, sizeof(lg));
+        tcnt[size_t(omp_get_thread_num())] = lc;
+      }
+      for (int t = 0; t < threads; ++t) {
+        const double* o = th.data() + 42 * size_t(t);
+        for (int a = 0; a < 36; ++a) H[a / 6][a % 6] += o[a];
+        for (int a = 0; a < 6; ++a) g[a] += o[36 + a];
+        count += tcnt[size_t(t)];
+      }
     }
     if (count < cfg.min_correspondences) {
       out.status = SR_ERR_INVALID_ARGUMENT;
@@ -684,7 +738,7 @@ AlignOut gicp_align(const float* sx, const float* sy, const float* sz, size_t ns
       converged = true;
       break;
     }
-    double fit = euclidean_fitness(tr, target_tree, maxd2, nb);
+    double fit = euclidean_fitness(tr, target_tree, maxd2, nb, threads);
     if (tr_it) tr_it->fitness = fit;
     if (fit < cfg.fitness_epsilon) {
       converged = true;
@@ -989,6 +1043,26 @@ int32_t or_gicp_align(const float* sx, const float* sy, const float* sz, int64_t
       d[9] = double(tr[i].count);
     }
   }
+  return o.status;
+}
+
+// or_gicp_align in faithful precision with the per-point loops (covariances,
+// correspondences, fitness) split over `threads` OpenMP threads. The
+// reference is single-threaded; this exists only for an informational
+// all-core CPU timing (its sums are added in another order).
+int32_t or_gicp_align_mt(const float* sx, const float* sy, const float* sz, int64_t ns,
+                         const float* tx, const float* ty, const float* tz, int64_t nt,
+                         const sr_gicp_config* cfg, int32_t threads, or_align_result* res) {
+  AlignOut o = gicp_align<float>(sx, sy, sz, size_t(ns), tx, ty, tz, size_t(nt), *cfg, nullptr,
+                                 nullptr, nullptr, 0, std::max(1, int(threads)));
+  res->status = o.status;
+  res->error_kind = o.error_kind;
+  res->error_count = o.error_count;
+  std::memcpy(res->q, o.q, sizeof(o.q));
+  std::memcpy(res->t, o.t, sizeof(o.t));
+  res->fitness = o.fitness;
+  res->iterations = o.iterations;
+  res->converged = o.converged;
   return o.status;
 }
 

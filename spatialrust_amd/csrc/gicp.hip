@@ -382,6 +382,7 @@ constexpr int kRowBatch = SR_ROW_BATCH;  // rows per bound-loading batch in phas
 #endif
 constexpr int kStreamW = SR_STREAM_W;  // point loads in flight per lane in phase 2
 
+
 // Row t of Chebyshev ring r (r >= 1) of (y,z) rows around the query's row:
 // the perimeter of the (2r+1)^2 square, each row once.
 __device__ __forceinline__ void ring_row(int r, int t, int& dy, int& dz) {
@@ -412,11 +413,12 @@ __device__ __forceinline__ void ring_row(int r, int t, int& dy, int& dz) {
 // beside the grid still reaches every grid row within max(dy, dz) rings (the
 // gap and clearance tests use the true position). On return every lane of the
 // group holds the group's best key and position.
-template <bool kStats>
+template <bool kStats, int kG = kNNGroup>
 __device__ __forceinline__ void group_search(const GridDesc& g, const uint32_t* __restrict__ cs,
                                              const float4* __restrict__ tp, float qx, float qy,
                                              float qz, bool skip, NNQuery& W) {
-  const int sub = int(threadIdx.x & (kWave - 1)) % kNNGroup;
+  static_assert(kG >= 1 && kG <= kWave && (kG & (kG - 1)) == 0, "group size");
+  const int sub = int(threadIdx.x & (kWave - 1)) % kG;
   const int hx = cell_coord_free(qx, g.ox, g.inv_h);
   const int hy = min(max(cell_coord_free(qy, g.oy, g.inv_h), -1), g.dy);
   const int hz = min(max(cell_coord_free(qz, g.oz, g.inv_h), -1), g.dz);
@@ -431,12 +433,12 @@ __device__ __forceinline__ void group_search(const GridDesc& g, const uint32_t* 
     // batch's ranges are loaded together and then its points streamed, so a
     // batch costs one bound round trip instead of one per row. A row whose
     // (y,z) gap exceeds the radius contributes an empty range.
-    for (int t0 = sub; t0 < nrows; t0 += kNNGroup * kRowBatch) {
+    for (int t0 = sub; t0 < nrows; t0 += kG * kRowBatch) {
       uint32_t ia[2 * kRowBatch], ib[2 * kRowBatch];
 #pragma unroll
       for (int m = 0; m < kRowBatch; ++m) {
         ia[2 * m] = ib[2 * m] = ia[2 * m + 1] = ib[2 * m + 1] = 0;
-        const int t = t0 + m * kNNGroup;
+        const int t = t0 + m * kG;
         if (t >= nrows) continue;
         int dy, dz;
         ring_row(r, t, dy, dz);
@@ -481,11 +483,11 @@ __device__ __forceinline__ void group_search(const GridDesc& g, const uint32_t* 
     }
     // the group agrees on the smallest radius before the next ring
 #pragma unroll
-    for (int off = 1; off < kNNGroup; off <<= 1) W.thr = fminf(W.thr, __shfl_xor(W.thr, off));
+    for (int off = 1; off < kG; off <<= 1) W.thr = fminf(W.thr, __shfl_xor(W.thr, off));
   }
   // group minimum of (key, position)
 #pragma unroll
-  for (int off = 1; off < kNNGroup; off <<= 1) {
+  for (int off = 1; off < kG; off <<= 1) {
     const uint32_t ohi = __shfl_xor(uint32_t(W.best_key >> 32), off);
     const uint32_t olo = __shfl_xor(uint32_t(W.best_key), off);
     const uint32_t opos = __shfl_xor(W.best_pos, off);
@@ -745,6 +747,32 @@ constexpr uint32_t kHardBlockFlag = 0x80000000u;  // HardQuery.inst: block scann
 #define SR_MARKED_WAVES 5
 #endif
 
+// Queues this lane's uncertified point (hard) for the ring search: one atomic
+// per wave (the leader reserves the wave's slots).
+__device__ __forceinline__ void enqueue_hard(const PassScratch& ps, bool hard, const NNQuery& Q,
+                                             float px, float py, float pz, uint32_t inst_id,
+                                             uint32_t i, float* budget) {
+  const int lane = int(threadIdx.x & (kWave - 1));
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const unsigned long long hm = __ballot(hard);
+  if (!hm) return;
+  const int leader = __ffsll((long long)hm) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(&ps.ctl[0], uint32_t(__popcll(hm)));
+  base = __shfl(base, leader);
+  if (!hard) return;
+  HardQuery h;
+  h.px = px;
+  h.py = py;
+  h.pz = pz;
+  h.inst = inst_id | (Q.skip_block ? kHardBlockFlag : 0u);
+  h.i = i;
+  h.pos = Q.best_pos;
+  h.key = Q.best_key;
+  ps.queue[base + uint32_t(__popcll(hm & below))] = h;
+  budget[i] = 0.0f;
+}
+
 // First stage. One-point-per-thread blocks: each point whose match the last
 // search certified for the pose's movement so far keeps it (budget spent, or
 // the match dropped when beyond d_max); the others are marked in the block's
@@ -865,25 +893,7 @@ __device__ __forceinline__ void search_block(const IterInstance* __restrict__ in
                                  cfg.dmax2, prev, 0.0f, Q, valid ? stats : nullptr);
     }
     const bool hard = valid && Q.hard;
-    const unsigned long long hm = __ballot(hard);
-    if (hm) {
-      const int leader = __ffsll((long long)hm) - 1;
-      uint32_t base = 0;
-      if (lane == leader) base = atomicAdd(&ps.ctl[0], uint32_t(__popcll(hm)));
-      base = __shfl(base, leader);
-      if (hard) {
-        HardQuery h;
-        h.px = px;
-        h.py = py;
-        h.pz = pz;
-        h.inst = inst_id | (Q.skip_block ? kHardBlockFlag : 0u);
-        h.i = i;
-        h.pos = Q.best_pos;
-        h.key = Q.best_key;
-        ps.queue[base + uint32_t(__popcll(hm & below))] = h;
-        I.budget[i] = 0.0f;
-      }
-    }
+    enqueue_hard(ps, hard, Q, px, py, pz, inst_id, i, I.budget);
     if (!valid || hard) continue;
     if (keep) {
       if constexpr (kStats) stats_add(stats, 0, 0, 0, kBlockDone);
@@ -946,26 +956,7 @@ __global__ void __launch_bounds__(kBlock)
                                cfg.dmax2, prev, 0.0f, Q, stats);
   }
   const bool hard = mine && Q.hard;
-  const unsigned long long hm = __ballot(hard);
-  if (hm) {
-    const unsigned long long below = (1ull << lane) - 1ull;
-    const int leader = __ffsll((long long)hm) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(&ps.ctl[0], uint32_t(__popcll(hm)));
-    base = __shfl(base, leader);
-    if (hard) {
-      HardQuery h;
-      h.px = px;
-      h.py = py;
-      h.pz = pz;
-      h.inst = inst_id | (Q.skip_block ? kHardBlockFlag : 0u);
-      h.i = i;
-      h.pos = Q.best_pos;
-      h.key = Q.best_key;
-      ps.queue[base + uint32_t(__popcll(hm & below))] = h;
-      I.budget[i] = 0.0f;
-    }
-  }
+  enqueue_hard(ps, hard, Q, px, py, pz, inst_id, i, I.budget);
   if (mine && !hard) {
     I.budget[i] = Q.budget;
     I.match[i] = Q.best_pos;
@@ -1072,26 +1063,8 @@ __global__ void __launch_bounds__(kBlock)
                                  cfg.dmax2, prev, 0.0f, Q, valid ? stats : nullptr);
     }
     const bool hard = valid && Q.hard;
-    const unsigned long long hm = __ballot(hard);
-    if (hm) {
-      queued = true;
-      const int leader = __ffsll((long long)hm) - 1;
-      uint32_t base = 0;
-      if (lane == leader) base = atomicAdd(&ps.ctl[0], uint32_t(__popcll(hm)));
-      base = __shfl(base, leader);
-      if (hard) {
-        HardQuery h;
-        h.px = px;
-        h.py = py;
-        h.pz = pz;
-        h.inst = inst_id | (Q.skip_block ? kHardBlockFlag : 0u);
-        h.i = i;
-        h.pos = Q.best_pos;
-        h.key = Q.best_key;
-        ps.queue[base + uint32_t(__popcll(hm & below))] = h;
-        I.budget[i] = 0.0f;
-      }
-    }
+    queued = queued || __ballot(hard) != 0ull;
+    enqueue_hard(ps, hard, Q, px, py, pz, inst_id, i, I.budget);
     if (!valid || hard) continue;
     if (keep) {
       if constexpr (kStats) stats_add(stats, 0, 0, 0, kBlockDone);
@@ -1132,7 +1105,34 @@ __global__ void __launch_bounds__(kBlock)
   write_partial(L, acc, count, I.partials + size_t(lb) * 64);
 }
 
-template <bool kStats>
+// One queued point's ring search by a group of kG lanes (all of them call it).
+template <bool kStats, bool kSingle, int kG>
+__device__ __forceinline__ void hard_query(const IterInstance* __restrict__ insts, const HardQuery& h,
+                                           bool active, SearchStats* __restrict__ stats) {
+  const IterInstance& I = insts[kSingle ? 0u : (h.inst & ~kHardBlockFlag)];
+  NNQuery W;
+  W.best_key = h.key;
+  W.best_pos = h.pos;
+  W.thr = __uint_as_float(uint32_t(h.key >> 32));
+  W.rows = W.pts = 0;
+  if (!active) return;
+  const GridDesc g = I.grid;
+  group_search<kStats, kG>(g, I.cell_start, I.tgt, h.px, h.py, h.pz,
+                           (h.inst & kHardBlockFlag) != 0, W);
+  if (int(threadIdx.x & (kWave - 1)) % kG == 0) {
+    I.match[h.i] = W.best_pos;
+    if constexpr (kStats)
+      stats_add(stats, 2, W.rows, W.pts, W.best_pos == kNone ? kRingNone : kRingFound);
+  }
+}
+
+// kSingle: the launch holds one registration, so every queued point belongs to
+// insts[0] and its grid descriptor and pointers are wave-uniform (scalar
+// registers) instead of per-lane copies (cfg1: 64 -> 50 VGPRs, no spill).
+// Static split, kWave/kNNGroup queries per wave and round: the grid holds a
+// wave per such slice of a fully queued launch (capped at what the chip runs
+// at once), so every wave takes few slices (no shared cursor to contend on).
+template <bool kStats, bool kSingle>
 __global__ void __launch_bounds__(kBlock)
     __attribute__((amdgpu_waves_per_eu(SR_HARD_WAVES))) k_gicp_hard(const IterInstance* __restrict__ insts,
                                                       PassScratch ps,
@@ -1141,31 +1141,13 @@ __global__ void __launch_bounds__(kBlock)
   const int lane = int(threadIdx.x & (kWave - 1));
   constexpr uint32_t G = kWave / kNNGroup;
   constexpr uint32_t kWavesPerBlock = kBlock / kWave;
-  // Static split, kWave/kNNGroup queries per wave and round: the grid holds a
-  // wave per such slice of a fully queued launch, so every wave takes at most
-  // one slice (no shared cursor to contend on).
   const uint32_t waves = gridDim.x * kWavesPerBlock;
   const uint32_t wid = xcd_block() * kWavesPerBlock + threadIdx.x / kWave;
   for (uint32_t base = wid * G; base < qlen; base += waves * G) {
     const uint32_t e = base + uint32_t(lane / kNNGroup);
     const bool active = e < qlen;  // the same on a group's lanes
     const HardQuery h = ps.queue[active ? e : base];
-    const IterInstance& I = insts[h.inst & ~kHardBlockFlag];
-    NNQuery W;
-    W.best_key = h.key;
-    W.best_pos = h.pos;
-    W.thr = __uint_as_float(uint32_t(h.key >> 32));
-    W.rows = W.pts = 0;
-    if (active) {
-      const GridDesc g = I.grid;
-      group_search<kStats>(g, I.cell_start, I.tgt, h.px, h.py, h.pz,
-                           (h.inst & kHardBlockFlag) != 0, W);
-      if (lane % kNNGroup == 0) {
-        I.match[h.i] = W.best_pos;
-        if constexpr (kStats)
-          stats_add(stats, 2, W.rows, W.pts, W.best_pos == kNone ? kRingNone : kRingFound);
-      }
-    }
+    hard_query<kStats, kSingle, kNNGroup>(insts, h, active, stats);
   }
 }
 
@@ -1327,7 +1309,10 @@ void gicp_pass(const IterInstance* insts, const uint32_t* block_inst, uint32_t t
         hipLaunchKernelGGL(k_gicp_search_fused<true>, dim3(total_blocks), dim3(kBlock), 0, s,
                            insts, block_inst, ps, st);
       }
-      hipLaunchKernelGGL(k_gicp_hard<true>, dim3(hard_blocks), dim3(kBlock), 0, s, insts, ps, st);
+      if (count == 1)
+        hipLaunchKernelGGL((k_gicp_hard<true, true>), dim3(hard_blocks), dim3(kBlock), 0, s, insts, ps, st);
+      else
+        hipLaunchKernelGGL((k_gicp_hard<true, false>), dim3(hard_blocks), dim3(kBlock), 0, s, insts, ps, st);
     } else {
       if (split) {
         hipLaunchKernelGGL(k_gicp_settle<false>, dim3(total_blocks), dim3(kBlock), 0, s, insts,
@@ -1338,7 +1323,10 @@ void gicp_pass(const IterInstance* insts, const uint32_t* block_inst, uint32_t t
         hipLaunchKernelGGL(k_gicp_search_fused<false>, dim3(total_blocks), dim3(kBlock), 0, s,
                            insts, block_inst, ps, st);
       }
-      hipLaunchKernelGGL(k_gicp_hard<false>, dim3(hard_blocks), dim3(kBlock), 0, s, insts, ps, st);
+      if (count == 1)
+        hipLaunchKernelGGL((k_gicp_hard<false, true>), dim3(hard_blocks), dim3(kBlock), 0, s, insts, ps, st);
+      else
+        hipLaunchKernelGGL((k_gicp_hard<false, false>), dim3(hard_blocks), dim3(kBlock), 0, s, insts, ps, st);
     }
     hipLaunchKernelGGL(k_gicp_terms, dim3(std::min<uint32_t>(total_blocks, kTermsGrid)), dim3(kBlock),
                        0, s, insts, block_inst, total_blocks, ps);

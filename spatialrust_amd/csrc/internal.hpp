@@ -360,7 +360,7 @@ struct HardQuery {
 // the block's points left to search, one bit per thread).
 struct PassScratch {
   HardQuery* queue;
-  uint32_t* ctl;
+  uint32_t* ctl;   // [0]: queue length
   uint32_t* done;
   unsigned long long* pend;
 };

@@ -63,3 +63,35 @@ def sphere_surface(n=300, radius=1.0):
     theta = np.pi * (1.0 + 5.0 ** 0.5) * i
     pts = np.stack([np.cos(theta) * np.sin(phi), np.sin(theta) * np.sin(phi), np.cos(phi)], 1)
     return (pts * radius).astype(F)
+
+
+def bench_corner_pair(side: int = 50, yaw: float = 0.03, t=(0.02, -0.015, 0.01)):
+    """The registration bench's case (benches/registration.rs:23-63, 81-86):
+    target = bench_corner(side), source = target moved by a yaw of `yaw` rad
+    about z and translation t, in f32. Returns (target, source, q, t)."""
+    tgt = bench_corner(side)
+    h = F(F(yaw) * F(0.5))
+    q = np.array([0.0, 0.0, np.sin(h), np.cos(h)], dtype=F)
+    tv = np.array(t, dtype=F)
+    x, y, z, w = (np.float64(v) for v in q)
+    R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                  [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                  [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+    src = (tgt.astype(np.float64) @ R.T + tv).astype(F)
+    return tgt, src, q, tv
+
+
+def corner_probe_error(q, t, mq, mt, probe=(0.4, 0.5, 0.3)):
+    """|T_est ∘ T_mis (p) - p| at the bench's probe point (FP64): how far the
+    estimated transform is from undoing the misalignment."""
+    def mat(qq):
+        x, y, z, w = (float(v) for v in qq)
+        n = (x * x + y * y + z * z + w * w) ** 0.5
+        x, y, z, w = x / n, y / n, z / n, w / n
+        return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                         [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                         [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+    p = np.asarray(probe, np.float64)
+    moved = mat(mq) @ p + np.asarray(mt, np.float64)
+    back = mat(q) @ moved + np.asarray(t, np.float64)
+    return float(np.linalg.norm(back - p))
