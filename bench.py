@@ -194,6 +194,144 @@ def _batched_vs_plain(count):
                 synth.rotation_angle_between(b.rotation64, p.gt_q) for b, p in zip(batched, pairs)))}
 
 
+def _gen_map(m):
+    from spatialrust_amd import synth
+    return synth.serving_map(synth.serving_pool(), m)
+
+
+def _gen_source(mj):
+    from spatialrust_amd import synth
+    w = synth.serving_source(synth.serving_pool(), *mj)
+    return w.source, w.gt_q, w.gt_t
+
+
+def _serving(rates, seconds, max_requests, max_active):
+    """Continuous-batching server under Poisson arrivals. 32 device-resident
+    maps of 0.5M-4M points (large-block generator, one seeded scene each),
+    request sources log-uniform in [2k, 200k] points sampled from the map's
+    scene and offset by up to 15 deg / 2 m; leaf 0.2, k 15, d_max 2.0, run until
+    the pose update is below 1e-6 or 40 iterations, 200 GB HBM budget. Latency
+    is from each request's scheduled arrival to the step that saw it finished;
+    throughput is requests over the time from the first arrival to the last
+    completion. Each served result is checked bit for bit against the same
+    registration run alone (sr_map_align) on a sample of requests, and against
+    FP64 oracle poses where tests/golden/serving_poses.json holds one."""
+    import concurrent.futures
+    import multiprocessing
+    import spatialrust_amd as sr
+    from spatialrust_amd import synth
+    pool = synth.serving_pool()
+    nm, spm = len(pool.map_sizes), len(pool.source_sizes[0])
+    t0 = time.perf_counter()
+    ctx = multiprocessing.get_context("spawn")
+    with concurrent.futures.ProcessPoolExecutor(max_workers=min(12, os.cpu_count() or 1),
+                                                mp_context=ctx) as ex:
+        maps_host = list(ex.map(_gen_map, range(nm)))
+        srcs = list(ex.map(_gen_source, [(m, j) for m in range(nm) for j in range(spm)]))
+    t_gen = time.perf_counter() - t0
+    cfg = sr.GicpConfig(voxel_leaf=0.2, k_neighbors=15, max_correspondence_distance=2.0,
+                        max_iterations=40, transformation_epsilon=1e-6, fitness_epsilon=0.0)
+    t0 = time.perf_counter()
+    maps = [sr.TargetMap(mh, cfg) for mh in maps_host]
+    t_maps = time.perf_counter() - t0
+    map_points = [m.points() for m in maps]
+    del maps_host
+    gold = {}
+    gpath = os.path.join(ROOT, "tests", "golden", "serving_poses.json")
+    if os.path.exists(gpath):
+        gold = json.load(open(gpath))["requests"]
+    out = {"workload": "serving_32_maps", "maps": nm, "map_points_raw": [int(v) for v in pool.map_sizes],
+           "map_points_after_voxel": map_points, "sources_per_map": spm,
+           "source_points_raw_mean": float(np.mean([len(s[0]) for s in srcs])),
+           "max_active": max_active, "hbm_budget_bytes": 200 << 30, "setup_s": {
+               "generate": t_gen, "maps": t_maps}, "rates": []}
+    rng = np.random.default_rng(12345)
+    for rate, secs in zip(rates, seconds):
+        n = int(min(max_requests, rate * secs))
+        gaps = rng.exponential(1.0 / rate, n)
+        arrive = np.cumsum(gaps) - gaps[0]
+        req = [(int(rng.integers(nm)), int(rng.integers(spm))) for _ in range(n)]
+        server = sr.Server(cfg, hbm_budget_bytes=200 << 30, max_active=max_active,
+                           max_iterations=40, delta_pose_epsilon=1e-6)
+        # warm-up: a few requests through the same server
+        for k in range(4):
+            m, j = req[k]
+            server.submit(maps[m], srcs[m * spm + j][0])
+        while server.step():
+            pass
+        for t in server.finished():
+            server.poll(t)
+        ticket_of = {}
+        done_at = np.zeros(n)
+        results = {}
+        nxt = 0
+        ndone = 0
+        steps = 0
+        peak_bytes = 0
+        start = time.perf_counter()
+        while ndone < n:
+            now = time.perf_counter() - start
+            while nxt < n and arrive[nxt] <= now:
+                m, j = req[nxt]
+                ticket_of[server.submit(maps[m], srcs[m * spm + j][0])] = nxt
+                nxt += 1
+            pending = server.step()
+            steps += 1
+            peak_bytes = max(peak_bytes, server.device_bytes())
+            fin = server.finished()
+            if fin:
+                tnow = time.perf_counter() - start
+                for t in fin:
+                    i = ticket_of.pop(t)
+                    results[i] = server.poll(t)
+                    done_at[i] = tnow
+                    ndone += 1
+            if not pending and nxt < n:  # idle until the next arrival
+                wait = arrive[nxt] - (time.perf_counter() - start)
+                if wait > 0:
+                    time.sleep(min(wait, 0.001))
+        server.close()
+        lat = (done_at - arrive) * 1e3
+        span = float(done_at.max() - arrive[0])
+        its = [results[i].iterations for i in range(n)]
+        gt_rot = max(synth.rotation_angle_between(results[i].rotation64, srcs[req[i][0] * spm + req[i][1]][1])
+                     for i in range(n))
+        # bit-identity with the same registration alone, on a sample of requests
+        sample = list(range(0, n, max(1, n // 6)))[:6]
+        ident = True
+        for i in sample:
+            m, j = req[i]
+            alone = maps[m].align(srcs[m * spm + j][0], cfg)
+            ident = ident and (np.array_equal(alone.rotation64, results[i].rotation64) and
+                               np.array_equal(alone.translation64, results[i].translation64) and
+                               alone.iterations == results[i].iterations)
+        fp64 = []
+        for key, g in gold.items():
+            m, j = (int(v) for v in key.split("_"))
+            hits = [i for i in range(n) if req[i] == (m, j)]
+            if hits:
+                r = results[hits[0]]
+                fp64.append((synth.rotation_angle_between(r.rotation64, g["q"]),
+                             float(np.linalg.norm(r.translation64 - np.array(g["t"])))))
+        out["rates"].append({
+            "offered_per_s": rate, "requests": n, "achieved_per_s": n / span,
+            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                           "max": float(lat.max())},
+            "mean_iterations": float(np.mean(its)),
+            "converged": float(np.mean([results[i].converged for i in range(n)])),
+            "steps": steps, "peak_charged_bytes": int(peak_bytes),
+            "max_rot_err_vs_ground_truth_rad": float(gt_rot),
+            "alone_equals_served": {"checked": len(sample), "identical": bool(ident)},
+            "error_vs_fp64_oracle": ({"checked": len(fp64), "max_rot_rad": max(e[0] for e in fp64),
+                                      "max_trans_m": max(e[1] for e in fp64)} if fp64 else None)})
+        print(f"serving {rate}/s: {out['rates'][-1]['achieved_per_s']:.0f} reg/s, "
+              f"p50 {out['rates'][-1]['latency_ms']['p50']:.1f} ms, "
+              f"p99 {out['rates'][-1]['latency_ms']['p99']:.1f} ms", file=sys.stderr, flush=True)
+    for m in maps:
+        m.close()
+    return out
+
+
 def _host_view_rate(w, cfg, steps):
     """The drop-in call as a Rust caller makes it, with host (numpy) clouds:
     the raw xyz columns (12 B per point) are copied to the device inside every
@@ -247,6 +385,14 @@ def main():
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched-vs-plain leg (256 independent small-cube pairs)")
     ap.add_argument("--batch-count", type=int, default=256)
+    ap.add_argument("--no-extra-legs", action="store_true",
+                    help="skip the host-view rate and the reference-corner GPU timing")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the serving leg (Poisson arrivals against 32 resident maps)")
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[200.0, 1000.0, 4000.0])
+    ap.add_argument("--serve-seconds", type=float, nargs="+", default=[5.0, 3.0, 1.0])
+    ap.add_argument("--serve-max-requests", type=int, default=4000)
+    ap.add_argument("--serve-max-active", type=int, default=256)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (a plain copy "
                          "kernel of known size for calibrating hardware byte counters)")
@@ -377,9 +523,14 @@ def main():
     if not args.no_batch:
         print("batched vs plain leg ...", file=sys.stderr, flush=True)
         line["batched_vs_plain"] = _batched_vs_plain(args.batch_count)
-    print("host-view and reference-corner legs ...", file=sys.stderr, flush=True)
-    line["host_view"] = _host_view_rate(w, cfg, steps=max(3, args.steps // 4))
-    line["reference_corner_gpu"] = _reference_corner_gpu()
+    if not args.no_serve:
+        print("serving leg ...", file=sys.stderr, flush=True)
+        line["serving"] = _serving(args.serve_rates, args.serve_seconds, args.serve_max_requests,
+                                   args.serve_max_active)
+    if not args.no_extra_legs:
+        print("host-view and reference-corner legs ...", file=sys.stderr, flush=True)
+        line["host_view"] = _host_view_rate(w, cfg, steps=max(3, args.steps // 4))
+        line["reference_corner_gpu"] = _reference_corner_gpu()
     if not args.no_cpu_baseline:
         print("timing the CPU oracle (full registration, 1 thread, then all cores) ...",
               file=sys.stderr, flush=True)

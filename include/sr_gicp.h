@@ -296,7 +296,9 @@ int32_t sr_batch_align_pairs(const sr_xyz_view* sources, const sr_xyz_view* targ
  * requests. A request's result is bit-identical to sr_map_align of its source
  * against its map with the request config (gicp + max_iterations and
  * delta_pose_epsilon as transformation_epsilon). Submit and poll may be called
- * from other threads while a step runs; steps must not overlap. */
+ * from other threads while a step runs; steps must not overlap. A step does
+ * not wait for its own pass: the states return asynchronously and a later
+ * step retires what they show finished, with at most two steps in flight. */
 typedef struct sr_server sr_server;
 
 typedef struct {
@@ -322,7 +324,12 @@ int32_t sr_server_step(sr_server* server, uint32_t* pending);
 int32_t sr_server_poll(sr_server* server, uint64_t ticket, sr_result* out, uint8_t* ready);
 /* Seconds from submit to the step that finished `ticket` (before polling it). */
 int32_t sr_server_latency(sr_server* server, uint64_t ticket, double* seconds);
-/* Device bytes currently charged against the budget. */
+/* Lists up to `cap` finished tickets not listed before (in finishing order;
+ * *n receives the count), so a caller polls only finished requests. */
+int32_t sr_server_finished(sr_server* server, uint64_t* tickets, uint64_t cap, uint64_t* n);
+/* Device bytes currently charged against the budget: for a queued or
+ * preparing request an estimate of its preparation peak, for an active one
+ * the measured footprint of its prepared source, state and pass buffers. */
 uint64_t sr_server_device_bytes(const sr_server* server);
 
 /* ---- scan-to-map odometry (rolling device-resident voxel map) ------------ */

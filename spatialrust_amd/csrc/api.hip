@@ -1030,13 +1030,25 @@ struct sr_map {
 //
 // Requests queue on the host; each scheduling step (server_step) runs one
 // Gauss-Newton pass (+ finalize) over every active request in a single launch
-// pair, while the requests admitted for the next step are preprocessed on the
-// worker streams, so the two overlap on the device. A request owns its
-// prepared source, state, partials and match buffer from admission to
-// retirement, and its sums never depend on who else is active: its result is
-// the one sr_map_align returns for it alone.
+// set, while the requests admitted in the same step are prepared on a worker
+// stream, so the two overlap on the device. A request owns its prepared
+// source, state, partials and match buffer from admission to retirement, and
+// its sums never depend on who else is active: its result is the one
+// sr_map_align returns for it alone.
+//
+// The host does not wait for each pass: the states come back through an
+// asynchronous copy into pinned memory after every step, and a step retires
+// the requests a completed snapshot shows finished. At most kServerDepth steps
+// are in flight; a step waits only for the one kServerDepth steps back.
 
 namespace sr {
+
+constexpr int kServerDepth = 2;
+// Upper bound of the transient device bytes per raw point while a request is
+// prepared: host columns staged on the device (12), the packed cloud (16), the
+// voxel table (up to 4 slots of 16 B per point), its per-point work arrays
+// (9 x 4) and the centroids (16).
+constexpr double kPrepBytesPerPoint = 12 + 16 + 64 + 36 + 16;
 
 struct ServerRequest {
   uint64_t ticket = 0;
@@ -1045,6 +1057,7 @@ struct ServerRequest {
   uint64_t reserve = 0;  // device bytes charged against the budget
   std::chrono::steady_clock::time_point submitted;
   Prepared src;
+  std::shared_ptr<SegHolder> hold;  // flat buffers the prepared source views into
   uint32_t slot = 0;  // index of its state in the server's state array
   DevBuf<float> partials;
   DevBuf<uint32_t> match;
@@ -1058,17 +1071,28 @@ struct ServerDone {
   double latency_s = 0.0;
 };
 
+// One in-flight step's state snapshot: the states copied after its launches,
+// the ticket each slot held then, and the event that marks the copy done.
+struct ServerSnap {
+  InstState* states = nullptr;  // pinned host memory, one per slot
+  std::vector<uint64_t> ticket;  // per slot (0: free)
+  hipEvent_t done = nullptr;
+  bool pending = false;
+};
+
 }  // namespace sr
 
 struct sr_server {
   sr_server_config cfg{};
   sr_gicp_config gicp{};
-  std::mutex mu;  // queue, done, next_ticket (submit/poll may run beside a step)
+  std::mutex mu;  // queue, done, finished, next_ticket (submit/poll may run beside a step)
   std::deque<std::unique_ptr<sr::ServerRequest>> queue;
   std::vector<std::unique_ptr<sr::ServerRequest>> active;
   std::unordered_map<uint64_t, sr::ServerDone> done;
+  std::deque<uint64_t> finished;  // finished tickets not yet listed by sr_server_finished
   uint64_t next_ticket = 1;
   uint64_t bytes_active = 0;
+  double bytes_per_point = 64.0;  // admission estimate per raw source point (learned)
   sr::DevBuf<sr::InstState> states;  // one slot per possible active request
   std::vector<uint32_t> free_slots;
   sr::DevBuf<sr::IterInstance> insts;
@@ -1076,6 +1100,14 @@ struct sr_server {
   sr::DevBuf<double> init;
   sr::DevBuf<sr::HardQuery> pass_queue;  // pass scratch
   sr::DevBuf<uint32_t> pass_ctl, pass_done;
+  sr::ServerSnap snaps[sr::kServerDepth];
+  uint64_t steps = 0;
+  ~sr_server() {
+    for (auto& sn : snaps) {
+      if (sn.states) (void)hipHostFree(sn.states);
+      if (sn.done) (void)hipEventDestroy(sn.done);
+    }
+  }
 };
 
 namespace sr {
@@ -1089,13 +1121,108 @@ static sr_gicp_config request_config(const sr_server_config& sc, const sr_gicp_c
   return c;
 }
 
-// Upper bound of the device bytes a request of n raw source points holds while
-// active (packed cloud, voxel and grid scratch at their peak, grid, normals,
-// match, partials).
-static uint64_t request_bytes(uint64_t n) { return 256 * n + (1u << 20); }
+// Device bytes a prepared request holds while active: its grid (cell starts,
+// points, order, bitmaps), normals, match and budget words, block partials, and
+// its share of the pass queue.
+static uint64_t resident_bytes(const Prepared& p) {
+  const uint64_t n = p.n;
+  const GridDesc& d = p.grid.desc;
+  const uint64_t words = uint64_t(d.wpr) * uint64_t(d.dy) * uint64_t(d.dz) +
+                         uint64_t(d.wps) * uint64_t(d.dz);
+  return 4 * (uint64_t(d.ncells) + 1) + n * (16 + 4 + 16 + 4 + 4 + sizeof(HardQuery)) + 4 * words +
+         uint64_t(instance_blocks(p.n)) * 64 * sizeof(float);
+}
+
+// Retires the requests that snapshot `sn` shows finished (and that still hold
+// the slot they held when it was taken).
+static void server_retire(sr_server& S, ServerSnap& sn) {
+  if (!sn.pending) return;
+  sn.pending = false;
+  const auto now = std::chrono::steady_clock::now();
+  std::vector<std::unique_ptr<ServerRequest>> still;
+  std::lock_guard<std::mutex> lk(S.mu);
+  for (auto& rp : S.active) {
+    ServerRequest& r = *rp;
+    const InstState& st = sn.states[r.slot];
+    if (sn.ticket[r.slot] != r.ticket || st.phase != 2) {
+      still.push_back(std::move(rp));
+      continue;
+    }
+    ServerDone d;
+    d.code = state_to_result(st, S.gicp, r.src, r.map->tgt, &d.result, &d.msg);
+    d.latency_s = std::chrono::duration<double>(now - r.submitted).count();
+    S.done[r.ticket] = std::move(d);
+    S.finished.push_back(r.ticket);
+    S.bytes_active -= r.reserve;
+    S.free_slots.push_back(r.slot);
+  }
+  S.active = std::move(still);
+}
+
+// Prepares the admitted sources. With a voxel leaf and k-NN covariances they
+// are uploaded and prepared together as segments of flat buffers on one worker
+// stream (one launch set for the group); otherwise each on the worker pool.
+static void server_prepare(sr_server& S, std::vector<std::unique_ptr<ServerRequest>>& admit) {
+  std::vector<Prepared> prepared(admit.size());
+  std::shared_ptr<SegHolder> hold;
+  Receipt rc;
+  const bool seg = S.gicp.voxel_leaf > 0.0f && !(S.gicp.covariance_radius > 0.0f);
+  if (seg) {
+    Workers::get().run(1, [&](size_t, hipStream_t ws) {
+      // device copies of the host columns, one flat buffer per axis
+      std::vector<uint32_t> off(admit.size() + 1, 0);
+      for (size_t i = 0; i < admit.size(); ++i) off[i + 1] = off[i] + uint32_t(admit[i]->x.size());
+      const size_t total = off.back();
+      DevBuf<float> dx(std::max<size_t>(total, 1)), dy(std::max<size_t>(total, 1)),
+          dz(std::max<size_t>(total, 1));
+      std::vector<sr_xyz_view> views(admit.size());
+      std::vector<const sr_xyz_view*> vp(admit.size());
+      std::vector<bool> is_target(admit.size(), false);
+      std::vector<Prepared*> outs(admit.size());
+      for (size_t i = 0; i < admit.size(); ++i) {
+        ServerRequest& r = *admit[i];
+        const size_t n = r.x.size(), o = off[i];
+        SR_HIP(hipMemcpyAsync(dx.p + o, r.x.data(), 4 * n, hipMemcpyHostToDevice, ws));
+        SR_HIP(hipMemcpyAsync(dy.p + o, r.y.data(), 4 * n, hipMemcpyHostToDevice, ws));
+        SR_HIP(hipMemcpyAsync(dz.p + o, r.z.data(), 4 * n, hipMemcpyHostToDevice, ws));
+        views[i] = sr_xyz_view{dx.p + o, dy.p + o, dz.p + o, n, SR_MEM_DEVICE};
+        vp[i] = &views[i];
+        outs[i] = &prepared[i];
+      }
+      hold = prepare_segmented(vp, is_target, S.gicp, outs, rc, ws);
+      if (!hold) fail(SR_ERR_INVALID_ARGUMENT, "server preparation did not apply");
+      SR_HIP(hipStreamSynchronize(ws));  // before the staging columns return to the cache
+    });
+  } else {
+    std::vector<sr_xyz_view> views(admit.size());
+    std::vector<const sr_xyz_view*> vp(admit.size());
+    std::vector<sr_gicp_config> cfgs(admit.size(), S.gicp);
+    for (size_t i = 0; i < admit.size(); ++i) {
+      ServerRequest& r = *admit[i];
+      views[i] = sr_xyz_view{r.x.data(), r.y.data(), r.z.data(), r.x.size(), SR_MEM_HOST};
+      vp[i] = &views[i];
+    }
+    prepare_many(vp, cfgs, prepared, false, rc);
+  }
+  for (size_t i = 0; i < admit.size(); ++i) {
+    admit[i]->src = std::move(prepared[i]);
+    admit[i]->hold = hold;
+  }
+}
 
 static void server_step(sr_server& S) {
   hipStream_t s = thread_stream();
+  ServerSnap& sn = S.snaps[S.steps % kServerDepth];
+  // The step kServerDepth back used this snapshot: wait for its copy, retire.
+  if (sn.pending) {
+    SR_HIP(hipEventSynchronize(sn.done));
+    server_retire(S, sn);
+  }
+  // Newer snapshots that have landed retire their finished requests too.
+  for (int k = 1; k < kServerDepth; ++k) {
+    ServerSnap& o = S.snaps[(S.steps + k) % kServerDepth];
+    if (o.pending && hipEventQuery(o.done) == hipSuccess) server_retire(S, o);
+  }
   // Admission: FIFO while slots and budget allow.
   std::vector<std::unique_ptr<ServerRequest>> admit;
   {
@@ -1131,20 +1258,10 @@ static void server_step(sr_server& S) {
     const PassScratch ps = grow_scratch(S.pass_queue, S.pass_ctl, S.pass_done, points, begin, s);
     gicp_pass(S.insts.p, S.block_inst.p, begin, count, ps, hard_stage_blocks(points), s);
   }
-  // Meanwhile: preprocess the admitted sources on the worker streams.
+  // Meanwhile: prepare the admitted sources on a worker stream.
   if (!admit.empty()) {
-    std::vector<sr_xyz_view> views(admit.size());
-    std::vector<const sr_xyz_view*> vp(admit.size());
-    std::vector<Prepared> prepared(admit.size());
-    std::vector<sr_gicp_config> cfgs(admit.size(), S.gicp);
-    for (size_t i = 0; i < admit.size(); ++i) {
-      ServerRequest& r = *admit[i];
-      views[i] = sr_xyz_view{r.x.data(), r.y.data(), r.z.data(), r.x.size(), SR_MEM_HOST};
-      vp[i] = &views[i];
-    }
-    Receipt rc;
     try {
-      prepare_many(vp, cfgs, prepared, false, rc);
+      server_prepare(S, admit);
     } catch (const Error& e) {
       // The whole admission group fails alike; report it on each ticket.
       std::lock_guard<std::mutex> lk(S.mu);
@@ -1153,6 +1270,7 @@ static void server_step(sr_server& S) {
         d.code = e.code;
         d.msg = e.msg;
         S.done[r->ticket] = d;
+        S.finished.push_back(r->ticket);
         S.bytes_active -= r->reserve;
       }
       admit.clear();
@@ -1160,7 +1278,16 @@ static void server_step(sr_server& S) {
     std::vector<double> init(7 * admit.size());
     for (size_t i = 0; i < admit.size(); ++i) {
       ServerRequest& r = *admit[i];
-      r.src = std::move(prepared[i]);
+      // the budget charge becomes the measured footprint; later estimates
+      // follow the largest bytes per raw point seen
+      const uint64_t raw = std::max<uint64_t>(r.x.size(), 1);
+      const uint64_t have = resident_bytes(r.src);
+      S.bytes_per_point = std::max(S.bytes_per_point, double(have) / double(raw));
+      {
+        std::lock_guard<std::mutex> lk(S.mu);
+        S.bytes_active = S.bytes_active - r.reserve + have;
+      }
+      r.reserve = have;
       std::vector<float>().swap(r.x);  // the host copy is no longer needed
       std::vector<float>().swap(r.y);
       std::vector<float>().swap(r.z);
@@ -1181,31 +1308,17 @@ static void server_step(sr_server& S) {
         init_states(S.states.p + admit[i]->slot, S.init.p + 7 * i, 1, s);
     }
   }
-  // Retire finished requests (after the pass), then activate the admitted ones.
-  std::vector<InstState> slots(count ? S.states.n : 0);
-  if (count)
-    SR_HIP(hipMemcpyAsync(slots.data(), S.states.p, sizeof(InstState) * slots.size(),
+  for (auto& r : admit) S.active.push_back(std::move(r));
+  // Snapshot of every slot after this step's work, copied asynchronously.
+  if (!S.active.empty()) {
+    std::fill(sn.ticket.begin(), sn.ticket.end(), 0ull);
+    for (auto& r : S.active) sn.ticket[r->slot] = r->ticket;
+    SR_HIP(hipMemcpyAsync(sn.states, S.states.p, sizeof(InstState) * S.states.n,
                           hipMemcpyDeviceToHost, s));
-  SR_HIP(hipStreamSynchronize(s));
-  const auto now = std::chrono::steady_clock::now();
-  std::vector<std::unique_ptr<ServerRequest>> still;
-  std::lock_guard<std::mutex> lk(S.mu);
-  for (uint32_t i = 0; i < count; ++i) {
-    ServerRequest& r = *S.active[i];
-    const InstState& st = slots[r.slot];
-    if (st.phase != 2) {
-      still.push_back(std::move(S.active[i]));
-      continue;
-    }
-    ServerDone d;
-    d.code = state_to_result(st, S.gicp, r.src, r.map->tgt, &d.result, &d.msg);
-    d.latency_s = std::chrono::duration<double>(now - r.submitted).count();
-    S.done[r.ticket] = std::move(d);
-    S.bytes_active -= r.reserve;
-    S.free_slots.push_back(r.slot);
+    SR_HIP(hipEventRecord(sn.done, s));
+    sn.pending = true;
   }
-  for (auto& r : admit) still.push_back(std::move(r));
-  S.active = std::move(still);
+  ++S.steps;
 }
 
 }  // namespace sr
@@ -1840,6 +1953,12 @@ int32_t sr_server_create(const sr_server_config* config, const sr_gicp_config* g
     sv->gicp = request_config(*config, *gicp);
     sv->states.resize(config->max_active);
     for (uint32_t i = config->max_active; i > 0; --i) sv->free_slots.push_back(i - 1);
+    for (auto& sn : sv->snaps) {
+      SR_HIP(hipHostMalloc(reinterpret_cast<void**>(&sn.states),
+                           sizeof(InstState) * config->max_active, hipHostMallocDefault));
+      SR_HIP(hipEventCreateWithFlags(&sn.done, hipEventDisableTiming));
+      sn.ticket.assign(config->max_active, 0ull);
+    }
     *out = sv.release();
   });
 }
@@ -1860,7 +1979,11 @@ int32_t sr_server_submit(sr_server* server, const sr_map* map, const sr_xyz_view
     check_map_config(map->cfg, server->gicp);
     if (source->mem != SR_MEM_HOST)
       fail(SR_ERR_INVALID_ARGUMENT, "server requests take host xyz columns");
-    const uint64_t need = request_bytes(source->n);
+    // admission charge until the request is prepared and measured: its
+    // preparation's transient buffers (kPrepBytesPerPoint) on top of the largest
+    // resident bytes per raw point measured so far (a 64-byte floor)
+    const uint64_t need =
+        uint64_t((server->bytes_per_point + kPrepBytesPerPoint) * double(source->n)) + (1u << 20);
     if (need > server->cfg.hbm_budget_bytes)
       fail(SR_ERR_INVALID_ARGUMENT, "request needs " + std::to_string(need) +
                                         " device bytes, above the server's HBM budget");
@@ -1912,6 +2035,19 @@ int32_t sr_server_latency(sr_server* server, uint64_t ticket, double* seconds) {
     auto it = server->done.find(ticket);
     if (it == server->done.end()) fail(SR_ERR_INVALID_ARGUMENT, "ticket has not finished");
     *seconds = it->second.latency_s;
+  });
+}
+
+int32_t sr_server_finished(sr_server* server, uint64_t* tickets, uint64_t cap, uint64_t* n) {
+  return guarded([&] {
+    if (!server || !n) fail(SR_ERR_INVALID_ARGUMENT, "null server argument");
+    *n = 0;
+    if (cap && !tickets) fail(SR_ERR_INVALID_ARGUMENT, "tickets must not be null");
+    std::lock_guard<std::mutex> lk(server->mu);
+    while (*n < cap && !server->finished.empty()) {
+      tickets[(*n)++] = server->finished.front();
+      server->finished.pop_front();
+    }
   });
 }
 

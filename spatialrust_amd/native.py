@@ -127,6 +127,7 @@ SIGNATURES = {
                               ctypes.POINTER(ctypes.c_uint8)]),
     "sr_server_latency": (_I32, [_P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]),
     "sr_server_device_bytes": (ctypes.c_uint64, [_P]),
+    "sr_server_finished": (_I32, [_P, _P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),
     "sr_odometry_create": (_I32, [ctypes.POINTER(OdometryConfigC), ctypes.POINTER(GicpConfigC),
                                   ctypes.POINTER(_P)]),
     "sr_odometry_destroy": (None, [_P]),

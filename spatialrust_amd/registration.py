@@ -549,6 +549,13 @@ class Server:
     def device_bytes(self) -> int:
         return int(_n.load().sr_server_device_bytes(self._h))
 
+    def finished(self, cap: int = 4096):
+        """Tickets finished since the last call (at most `cap`)."""
+        buf = (ctypes.c_uint64 * max(cap, 1))()
+        n = ctypes.c_uint64(0)
+        _check(_n.load().sr_server_finished(self._h, buf, cap, ctypes.byref(n)))
+        return [int(buf[i]) for i in range(n.value)]
+
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
             _n.load().sr_server_destroy(self._h)

@@ -467,3 +467,48 @@ def batch_pairs(count: int = 256, lo: int = 5_000, hi: int = 120_000, seed_offse
         out.append(Workload(f"batch_{i}", apply_motion(q, t, src0), tgt, gq, gt, 0.1, 20, 1.0, 30,
                             20.0))
     return out
+
+
+# ------------------------------------------------------------------ serving --
+
+@dataclass
+class ServingPool:
+    """Maps and request sources of the serving workload: `map_sizes[m]` target
+    points of the large-block generator with scene seed `map_seeds[m]`;
+    `source_sizes[m][j]` points for source j of map m."""
+    map_seeds: list
+    map_sizes: list
+    source_sizes: list
+
+
+def serving_pool(n_maps: int = 32, sources_per_map: int = 8, map_lo: int = 500_000,
+                 map_hi: int = 4_000_000, src_lo: int = 2_000, src_hi: int = 200_000) -> ServingPool:
+    """32 maps of 0.5M-4M points (log-uniform) from the large-block generator
+    (each its own seeded scene) and, per map, request sources log-uniform in
+    [2k, 200k] points."""
+    rng = Rng(SEED_BASE + 1000 * 3)
+    msz = np.exp(rng.uniform(n_maps, np.log(map_lo), np.log(map_hi))).astype(np.int64)
+    ssz = np.exp(rng.uniform(n_maps * sources_per_map, np.log(src_lo), np.log(src_hi))).astype(np.int64)
+    seeds = [SEED_BASE + 1000 * 3 + 101 * m for m in range(n_maps)]
+    return ServingPool(seeds, [int(v) for v in msz],
+                       [[int(v) for v in ssz[m * sources_per_map:(m + 1) * sources_per_map]]
+                        for m in range(n_maps)])
+
+
+def serving_map(pool: ServingPool, m: int) -> np.ndarray:
+    """Target cloud of map m (sigma 0.02 m)."""
+    seed = pool.map_seeds[m]
+    return sample_scene(large_block_scene(seed), pool.map_sizes[m], 0.02, seed * 7 + 1)
+
+
+def serving_source(pool: ServingPool, m: int, j: int) -> Workload:
+    """Source j of map m: an independent sample of map m's scene moved by a
+    seeded offset of up to 15 deg / 2 m (the large-block generator's); leaf
+    0.2, k 15, d_max 2.0, at most 40 iterations."""
+    seed = pool.map_seeds[m]
+    sseed = seed * 7 + 11 + 13 * j
+    src0 = sample_scene(large_block_scene(seed), pool.source_sizes[m][j], 0.02, sseed)
+    q, t = random_motion(sseed, 15.0, 2.0, per_axis=False)
+    gq, gt = invert_motion(q, t)
+    return Workload(f"serve_{m}_{j}", apply_motion(q, t, src0), np.zeros((0, 3), np.float32), gq,
+                    gt, 0.2, 15, 2.0, 40, 100.0)

@@ -430,6 +430,8 @@ def test_server_results_equal_alone(hip):
         steps += 1
         assert steps < 500
     assert server.device_bytes() == 0
+    fin = server.finished()
+    assert sorted(fin) == sorted(tickets) and server.finished() == []  # each listed once
     for t, idx in tickets.items():
         lat = server.latency(t)
         got = server.poll(t)
