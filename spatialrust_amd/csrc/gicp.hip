@@ -413,20 +413,36 @@ __device__ __forceinline__ void ring_row(int r, int t, int& dy, int& dz) {
 // beside the grid still reaches every grid row within max(dy, dz) rings (the
 // gap and clearance tests use the true position). On return every lane of the
 // group holds the group's best key and position.
-template <bool kStats, int kG = kNNGroup>
+//
+// kGap (converged passes, as in phase 1): the scan radius is widened by
+// `margin` beyond the best distance and the smallest distance of the other
+// points is kept in W.second, so that on return every point other than the
+// best is known to lie at least min(sqrt(W.second), best + margin) away (see
+// hard_query for the movement budget this gives). The group's lanes scan
+// disjoint rows, so the other lanes' bests count as other points too.
+template <bool kStats, int kG = kNNGroup, bool kGap = false>
 __device__ __forceinline__ void group_search(const GridDesc& g, const uint32_t* __restrict__ cs,
                                              const float4* __restrict__ tp, float qx, float qy,
-                                             float qz, bool skip, NNQuery& W) {
+                                             float qz, bool skip, float margin, NNQuery& W) {
   static_assert(kG >= 1 && kG <= kWave && (kG & (kG - 1)) == 0, "group size");
   const int sub = int(threadIdx.x & (kWave - 1)) % kG;
   const int hx = cell_coord_free(qx, g.ox, g.inv_h);
   const int hy = min(max(cell_coord_free(qy, g.oy, g.inv_h), -1), g.dy);
   const int hz = min(max(cell_coord_free(qz, g.oz, g.inv_h), -1), g.dz);
   const int rmax = max(max(hy, g.dy - 1 - hy), max(hz, g.dz - 1 - hz));
+  // squared scan radius: the best distance, widened by margin under kGap
+  auto scan2 = [&](float thr) {
+    if constexpr (kGap) {
+      const float r = __builtin_amdgcn_sqrtf(thr) + margin;
+      return r * r;
+    } else {
+      return thr;
+    }
+  };
   for (int r = 0; r <= rmax; ++r) {
     if (r > 0) {  // W.thr is the same on the group's lanes here
       const float c = clearance_yz(g, qy, qz, hy, hz, r - 1);
-      if (c > 0.0f && c * c > W.thr) break;
+      if (c > 0.0f && c * c > scan2(W.thr)) break;
     }
     const int nrows = r == 0 ? 1 : 8 * r;
     // This lane's rows of the ring, kRowBatch at a time: the bounds of a
@@ -447,8 +463,9 @@ __device__ __forceinline__ void group_search(const GridDesc& g, const uint32_t* 
         const float gz = axis_gap(qz, g.oz, g.h, g.margin, cz);
         const float gy = axis_gap(qy, g.oy, g.h, g.margin, cy);
         const float gyz = gz * gz + gy * gy;
-        if (gyz > W.thr) continue;
-        const float rx = __builtin_amdgcn_sqrtf(W.thr - gyz) + g.margin;
+        const float R2 = scan2(W.thr);
+        if (gyz > R2) continue;
+        const float rx = __builtin_amdgcn_sqrtf(R2 - gyz) + g.margin;
         const int x0 = max(cell_coord_free(qx - rx, g.ox, g.inv_h) - 1, 0);
         const int x1 = min(cell_coord_free(qx + rx, g.ox, g.inv_h) + 1, g.dx - 1);
         if (x0 > x1) continue;
@@ -477,7 +494,7 @@ __device__ __forceinline__ void group_search(const GridDesc& g, const uint32_t* 
       }
       auto visit = [&](uint32_t s, const float4& q) {
         if constexpr (kStats) ++W.pts;
-        nn_consider(W, qx, qy, qz, s, q);
+        nn_consider<kGap>(W, qx, qy, qz, s, q);
       };
       stream_ranges<kStreamW>(lo, hi, tp, visit);
     }
@@ -485,6 +502,8 @@ __device__ __forceinline__ void group_search(const GridDesc& g, const uint32_t* 
 #pragma unroll
     for (int off = 1; off < kG; off <<= 1) W.thr = fminf(W.thr, __shfl_xor(W.thr, off));
   }
+  const uint64_t own_key = W.best_key;
+  const bool own_point = W.best_pos != kNone;
   // group minimum of (key, position)
 #pragma unroll
   for (int off = 1; off < kG; off <<= 1) {
@@ -500,6 +519,13 @@ __device__ __forceinline__ void group_search(const GridDesc& g, const uint32_t* 
       W.rows += __shfl_xor(W.rows, off);
       W.pts += __shfl_xor(W.pts, off);
     }
+  }
+  if constexpr (kGap) {
+    // a lane's own best that lost the group minimum is another point
+    if (own_point && own_key != W.best_key)
+      W.second = fminf(W.second, __uint_as_float(uint32_t(own_key >> 32)));
+#pragma unroll
+    for (int off = 1; off < kG; off <<= 1) W.second = fminf(W.second, __shfl_xor(W.second, off));
   }
 }
 
@@ -1114,13 +1140,35 @@ __device__ __forceinline__ void hard_query(const IterInstance* __restrict__ inst
   W.best_key = h.key;
   W.best_pos = h.pos;
   W.thr = __uint_as_float(uint32_t(h.key >> 32));
+  W.second = INFINITY;
   W.rows = W.pts = 0;
   if (!active) return;
   const GridDesc g = I.grid;
-  group_search<kStats, kG>(g, I.cell_start, I.tgt, h.px, h.py, h.pz,
-                           (h.inst & kHardBlockFlag) != 0, W);
+  const InstState* st = I.state;
+  // the same test as the first stage's, on the same state: budgets are worth
+  // computing once no source point moves kGapMove cells
+  const bool gap = (I.cfg.flags & kCfgMatchReuse) &&
+                   st->move_rot * I.src_radius + st->move_const < kGapMove * g.h;
+  const float margin = kGapMargin * g.h;
+  if (gap)  // the 3x3x3 block is scanned again: its second-nearest counts
+    group_search<kStats, kG, true>(g, I.cell_start, I.tgt, h.px, h.py, h.pz, false, margin, W);
+  else
+    group_search<kStats, kG, false>(g, I.cell_start, I.tgt, h.px, h.py, h.pz,
+                                    (h.inst & kHardBlockFlag) != 0, 0.0f, W);
   if (int(threadIdx.x & (kWave - 1)) % kG == 0) {
     I.match[h.i] = W.best_pos;
+    if (gap && W.best_pos != kNone) {
+      // Every other point lies at least `other` away: scanned ones at
+      // sqrt(second), unscanned ones beyond the widened radius (rows cut to
+      // it, rings stopped once their clearance exceeded it). While the point
+      // moves less than half the gap (less a rounding slack), its nearest
+      // point stays the same: the movement budget phase 1 gives its own
+      // certified points.
+      const float best = sqrtf(W.thr);
+      const float other = fminf(sqrtf(W.second), best + margin);
+      const float slack = 1e-6f * (fabsf(h.px) + fabsf(h.py) + fabsf(h.pz) + best + 1.0f);
+      I.budget[h.i] = fmaxf(0.5f * (other - best) - slack, 0.0f);
+    }
     if constexpr (kStats)
       stats_add(stats, 2, W.rows, W.pts, W.best_pos == kNone ? kRingNone : kRingFound);
   }
