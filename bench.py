@@ -332,6 +332,46 @@ def _serving(rates, seconds, max_requests, max_active):
     return out
 
 
+def _radius_leg(w, reps=5):
+    """Radius covariances (covariance_radius, COV_GRID_WGSL semantics) and grid
+    normals of the workload's target after the voxel filter, radius 0.4 m,
+    with the neighbourhoods read from LDS-staged cell tiles and from device
+    memory per point: kernel times (HIP events) and bit identity."""
+    import ctypes
+    import spatialrust_amd as sr
+    from spatialrust_amd import native
+    lib = native.load()
+    vox = sr.VoxelGridDownsample(sr.VoxelGridDownsampleConfig(w.voxel_leaf)).filter_with_policy(
+        w.target, sr.ExecutionPolicy.gpu()).points
+    radius = 0.4
+    est = sr.GpuNormalEstimator(sr.NormalEstimationConfig(search_radius=radius))
+    out, res = {}, {}
+    for tiles in (1, 0):
+        sr.set_option("radius_tiles", tiles)
+        cov, _ = sr.plane_covariances(vox, radius=radius)  # warm-up
+        lib.sr_profile_reset()
+        lib.sr_profile_enable(1)
+        for _ in range(reps):
+            cov, _ = sr.plane_covariances(vox, radius=radius)
+            nrm, curv = est.estimate(vox)
+        lib.sr_profile_enable(0)
+        t = {}
+        for name in ("radius_cov", "grid_normals"):
+            ms, cnt = ctypes.c_double(0.0), ctypes.c_uint64(0)
+            lib.sr_profile_read(name.encode(), ctypes.byref(ms), ctypes.byref(cnt))
+            t[name] = ms.value / max(cnt.value, 1)
+        lib.sr_profile_reset()
+        out[tiles] = t
+        res[tiles] = (cov, nrm, curv)
+    sr.set_option("radius_tiles", 0)
+    n = len(vox)
+    return {"points": int(n), "radius": radius,
+            "radius_cov_ms": {"tiles": out[1]["radius_cov"], "per_point": out[0]["radius_cov"]},
+            "grid_normals_ms": {"tiles": out[1]["grid_normals"], "per_point": out[0]["grid_normals"]},
+            "radius_cov_gbs_per_point": 36.0 * n / (out[0]["radius_cov"] * 1e-3) / 1e9,
+            "identical": bool(all(np.array_equal(a, b) for a, b in zip(res[1], res[0])))}
+
+
 def _host_view_rate(w, cfg, steps):
     """The drop-in call as a Rust caller makes it, with host (numpy) clouds:
     the raw xyz columns (12 B per point) are copied to the device inside every
@@ -387,6 +427,8 @@ def main():
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--no-extra-legs", action="store_true",
                     help="skip the host-view rate and the reference-corner GPU timing")
+    ap.add_argument("--no-radius", action="store_true",
+                    help="skip the radius covariance / grid normal leg (LDS tiles vs per point)")
     ap.add_argument("--no-serve", action="store_true",
                     help="skip the serving leg (Poisson arrivals against 32 resident maps)")
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[200.0, 1000.0, 4000.0])
@@ -527,6 +569,9 @@ def main():
         print("serving leg ...", file=sys.stderr, flush=True)
         line["serving"] = _serving(args.serve_rates, args.serve_seconds, args.serve_max_requests,
                                    args.serve_max_active)
+    if not args.no_radius:
+        print("radius covariance leg ...", file=sys.stderr, flush=True)
+        line["radius_covariances"] = _radius_leg(w)
     if not args.no_extra_legs:
         print("host-view and reference-corner legs ...", file=sys.stderr, flush=True)
         line["host_view"] = _host_view_rate(w, cfg, steps=max(3, args.steps // 4))

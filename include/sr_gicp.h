@@ -138,6 +138,10 @@ void sr_gicp_config_default(sr_gicp_config* out);
  *     other target point can have become nearer (half the gap to the
  *     second-nearest point exceeds the pose's movement). 0 searches every
  *     point in every pass; results are the same either way.
+ *   "radius_tiles" (default 0): 1 makes the radius covariances and grid
+ *     normals read each point's 27-cell neighbourhood from LDS-staged cell
+ *     tiles instead of from device memory per point (slower on gfx950, where
+ *     the per-point reads hit L2). Results are bit-identical either way.
  * Unknown names and values out of range are InvalidArgument. */
 int32_t sr_option_set(const char* name, int64_t value);
 int32_t sr_option_get(const char* name, int64_t* value);

@@ -1482,9 +1482,9 @@ int32_t sr_option_set(const char* name, int64_t value) {
   return guarded([&] {
     if (!name) fail(SR_ERR_INVALID_ARGUMENT, "option name must not be null");
     const std::string n(name);
-    if (n == "match_reuse") {
-      if (value != 0 && value != 1) fail(SR_ERR_INVALID_ARGUMENT, "match_reuse takes 0 or 1");
-      options().match_reuse.store(value);
+    if (n == "match_reuse" || n == "radius_tiles") {
+      if (value != 0 && value != 1) fail(SR_ERR_INVALID_ARGUMENT, n + " takes 0 or 1");
+      (n == "match_reuse" ? options().match_reuse : options().radius_tiles).store(value);
     } else {
       fail(SR_ERR_INVALID_ARGUMENT, "unknown option " + n);
     }
@@ -1497,6 +1497,8 @@ int32_t sr_option_get(const char* name, int64_t* value) {
     const std::string n(name);
     if (n == "match_reuse") {
       *value = options().match_reuse.load();
+    } else if (n == "radius_tiles") {
+      *value = options().radius_tiles.load();
     } else if (n == "noalloc_scratch_bytes") {
       *value = int64_t(kScratchBytes);
     } else if (n == "noalloc_peak_bytes") {  // of the last allocation-free call on this device

@@ -333,7 +333,8 @@ constexpr uint32_t kCfgMatchReuse = 1u;  // InstConfig.flags
 
 // Library-wide runtime options (sr_option_set / sr_option_get).
 struct Options {
-  std::atomic<int64_t> match_reuse{1};  // certified match reuse in the Gauss-Newton passes
+  std::atomic<int64_t> match_reuse{1};   // certified match reuse in the Gauss-Newton passes
+  std::atomic<int64_t> radius_tiles{0};  // radius covariances / grid normals from LDS cell tiles
 };
 Options& options();
 
