@@ -580,6 +580,7 @@ def main():
         print("timing the CPU oracle (full registration, 1 thread, then all cores) ...",
               file=sys.stderr, flush=True)
         line["cpu_baseline"] = _cpu_baseline(w, min(16, os.cpu_count() or 1))
+    line["launch_tuning"] = sr.tune_describe().strip().splitlines()
     print(json.dumps(line), flush=True)
 
 

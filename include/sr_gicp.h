@@ -142,9 +142,24 @@ void sr_gicp_config_default(sr_gicp_config* out);
  *     normals read each point's 27-cell neighbourhood from LDS-staged cell
  *     tiles instead of from device memory per point (slower on gfx950, where
  *     the per-point reads hit L2). Results are bit-identical either way.
+ *   "autotune" (default 1): the launch choices of the Gauss-Newton pass loop
+ *     (lanes per queued point in the ring search, that stage's workgroup cap,
+ *     the launch size from which the first stage splits in two kernels) are
+ *     picked per problem class (architecture; one or many registrations; log2
+ *     of the source points) by timing each candidate on the class's first
+ *     loop, which then costs a few loops (one registration under 128k source
+ *     points is not tuned); 0 uses the defaults. Results are
+ *     bit-identical whatever is picked. SR_TUNE_CACHE (environment) names a
+ *     file the decisions are read from and appended to.
+ *   "tune_force" (default -1): >= 0 uses that candidate for every loop (read
+ *     "tune_candidates_single" / "tune_candidates_batch" for their counts).
  * Unknown names and values out of range are InvalidArgument. */
 int32_t sr_option_set(const char* name, int64_t value);
 int32_t sr_option_get(const char* name, int64_t* value);
+/* The tuner's decisions so far as text, one line per problem class (the
+ * chosen launch choices and each candidate's measured loop time in ms); at
+ * most cap - 1 bytes plus a NUL go to out, *len receives the full length. */
+int32_t sr_tune_describe(char* out, uint64_t cap, uint64_t* len);
 
 /* ---- kernel timing ------------------------------------------------------
  * No counterpart in the reference (it records no GPU timestamps,

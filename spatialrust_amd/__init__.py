@@ -13,7 +13,7 @@ from .registration import (DeviceKind, DeviceUnavailable, ExecutionOutput, Execu
                            VoxelGridDownsample, VoxelGridDownsampleConfig, VoxelOutput,
                            batch_align, batch_align_pairs, device_available, device_view, get_option,
                            gicp_normal_equations, knn, plane_covariances, register_gicp,
-                           set_option)
+                           set_option, tune_describe)
 
 __all__ = [
     "DeviceKind", "DeviceUnavailable", "ExecutionOutput", "ExecutionPolicy", "ExecutionReceipt",
@@ -21,5 +21,5 @@ __all__ = [
     "NormalEstimationConfig", "Isometry3", "RegistrationResult",
     "RollingMapOdometry", "Server", "SpatialError", "TargetMap", "VoxelGridDownsample", "VoxelGridDownsampleConfig", "VoxelOutput",
     "batch_align", "batch_align_pairs", "device_available", "device_view", "gicp_normal_equations", "knn",
-    "plane_covariances", "register_gicp", "get_option", "set_option",
+    "plane_covariances", "register_gicp", "get_option", "set_option", "tune_describe",
 ]

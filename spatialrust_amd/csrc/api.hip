@@ -7,6 +7,7 @@
 #include <cmath>
 #include <condition_variable>
 #include <exception>
+#include <fstream>
 #include <functional>
 #include <thread>
 #include <cstdlib>
@@ -684,6 +685,7 @@ struct Batch {
   DevBuf<uint32_t> match;
   DevBuf<float> budget;
   uint32_t total_blocks = 0, count = 0, max_passes = 0;
+  uint64_t total_pts = 0;
   PassScratch scratch{};
   uint32_t hard_blocks = 1;
 };
@@ -770,6 +772,7 @@ static void batch_setup(const std::vector<const Prepared*>& src, const std::vect
     B.max_passes = std::max(B.max_passes, cfgs[i].max_iterations + 1);
   }
   B.total_blocks = blocks;
+  B.total_pts = total_pts;
   {
     LoopArgs& la = B.la;
     B.scratch = grow_scratch(la.queue, la.ctl, la.done, total_pts, blocks, s);
@@ -782,6 +785,152 @@ static void batch_setup(const std::vector<const Prepared*>& src, const std::vect
   SR_HIP(hipMemcpyAsync(B.block_inst.p, hbi.data(), sizeof(uint32_t) * blocks, hipMemcpyHostToDevice, s));
   SR_HIP(hipMemcpyAsync(B.init.p, init.data(), sizeof(double) * init.size(), hipMemcpyHostToDevice, s));
   init_states(B.states.p, B.init.p, count, s);
+}
+
+// ---- launch tuner -------------------------------------------------------------
+//
+// The pass loop's launch choices (PassTuning: lanes per queued point in the
+// ring search, that stage's workgroup cap, the size from which the first stage
+// splits into two kernels) change no result: matches, budgets and sums are the
+// same whichever is used (the match-reuse and two-kernel tests show the same
+// for the choices they cover). Their speed depends on the problem, so the
+// first pass loop of each problem class (device architecture; one or many
+// registrations; log2 of the source points) runs once per candidate from the
+// same initial state, timed with HIP events, and the fastest is kept for the
+// class. Loops too small for the choice to matter (one registration under
+// 128k source points) use the defaults. SR_TUNE_CACHE names a file the decisions
+// are read from and appended to, so a process can start tuned.
+
+struct TuneEntry {
+  PassTuning best;
+  std::vector<float> ms;  // per candidate
+};
+
+static std::vector<PassTuning> tune_candidates(bool batch) {
+  std::vector<PassTuning> c;
+  auto add = [&](int g, uint32_t cap, uint32_t split) { c.push_back(PassTuning{g, cap, split}); };
+  add(8, 2048, 4096);  // the defaults first
+  add(4, 2048, 4096);
+  add(16, 2048, 4096);
+  add(8, 4096, 4096);
+  add(8, 1024, 4096);
+  if (batch) {
+    add(8, 2048, 1u << 30);  // never split
+    add(8, 2048, 16384);
+  }
+  return c;
+}
+
+struct Tuner {
+  std::mutex mu;
+  std::map<std::pair<std::string, uint32_t>, TuneEntry> done;
+  bool loaded = false;
+};
+static Tuner& tuner() {
+  static Tuner* t = new Tuner();
+  return *t;
+}
+
+static std::string device_arch() {
+  static std::string arch = [] {
+    hipDeviceProp_t p;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess) return std::string("unknown");
+    std::string a = p.gcnArchName;
+    return a.substr(0, a.find(':'));
+  }();
+  return arch;
+}
+
+static uint32_t tune_class(const Batch& B) {
+  uint32_t l = 0;
+  while ((uint64_t(1) << (l + 1)) <= B.total_pts) ++l;
+  return (B.count > 1 ? 0x100u : 0u) | l;
+}
+
+static void tuner_load(Tuner& T) {
+  if (T.loaded) return;
+  T.loaded = true;
+  const char* path = std::getenv("SR_TUNE_CACHE");
+  if (!path || !*path) return;
+  std::ifstream f(path);
+  std::string arch;
+  uint32_t cls, cap, split;
+  int group;
+  while (f >> arch >> cls >> group >> cap >> split) T.done[{arch, cls}].best = PassTuning{group, cap, split};
+}
+
+static void tuner_store(const std::string& arch, uint32_t cls, const PassTuning& t) {
+  const char* path = std::getenv("SR_TUNE_CACHE");
+  if (!path || !*path) return;
+  std::ofstream f(path, std::ios::app);
+  f << arch << ' ' << cls << ' ' << t.group << ' ' << t.hard_cap << ' ' << t.split_blocks << '\n';
+}
+
+static void run_passes(Batch& B, hipStream_t s, const PassTuning& t) {
+  const uint32_t hb = hard_stage_blocks(B.total_pts, t.hard_cap);
+  for (uint32_t k = 0; k < B.max_passes; ++k)
+    gicp_pass(B.insts.p, B.block_inst.p, B.total_blocks, B.count, B.scratch, hb, s, true, t);
+}
+
+// The registrations' starting state (as batch_setup leaves it).
+static void reset_batch(Batch& B, hipStream_t s) {
+  init_states(B.states.p, B.init.p, B.count, s);
+  fill_u32(B.match.p, kNone, B.total_pts, s);
+  fill_u32(reinterpret_cast<uint32_t*>(B.budget.p), 0u, B.total_pts, s);
+}
+
+// Times every candidate on this batch (eager launches) and returns the
+// fastest; the batch is left as the last run finished it, which is the result
+// every candidate gives.
+static PassTuning tune_batch(Batch& B, hipStream_t s, TuneEntry& e) {
+  const std::vector<PassTuning> cand = tune_candidates(B.count > 1);
+  hipEvent_t a, b;
+  SR_HIP(hipEventCreate(&a));
+  SR_HIP(hipEventCreate(&b));
+  e.ms.assign(cand.size(), 0.0f);
+  size_t best = 0;
+  for (size_t c = 0; c < cand.size(); ++c) {
+    if (c > 0) reset_batch(B, s);
+    SR_HIP(hipEventRecord(a, s));
+    run_passes(B, s, cand[c]);
+    SR_HIP(hipEventRecord(b, s));
+    SR_HIP(hipEventSynchronize(b));
+    SR_HIP(hipEventElapsedTime(&e.ms[c], a, b));
+    if (e.ms[c] < e.ms[best]) best = c;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  e.best = cand[best];
+  return e.best;
+}
+
+// The launch choices for this batch's pass loop (tuning the class first if it
+// is new; `tuned` reports whether the loop has already run doing so).
+static PassTuning choose_tuning(Batch& B, hipStream_t s, bool eager_ok, bool& tuned) {
+  tuned = false;
+  const int64_t forced = options().tune_force.load();
+  if (forced >= 0) {
+    const std::vector<PassTuning> cand = tune_candidates(B.count > 1);
+    return cand[size_t(forced) % cand.size()];
+  }
+  if (!options().autotune.load()) return PassTuning{};
+  if (B.count == 1 && B.total_pts < (1u << 17)) return PassTuning{};
+  Tuner& T = tuner();
+  const std::pair<std::string, uint32_t> key{device_arch(), tune_class(B)};
+  {
+    std::lock_guard<std::mutex> lk(T.mu);
+    tuner_load(T);
+    auto it = T.done.find(key);
+    if (it != T.done.end()) return it->second.best;
+  }
+  if (!eager_ok) return PassTuning{};
+  TuneEntry e;
+  const PassTuning best = tune_batch(B, s, e);
+  tuned = true;
+  std::lock_guard<std::mutex> lk(T.mu);
+  if (T.done.emplace(key, e).second) tuner_store(key.first, key.second, best);
+  return best;
 }
 
 // The Gauss-Newton loop of a batch: max_passes pairs of (pass, finalize)
@@ -798,15 +947,21 @@ struct GraphEntry {
   const void* queue;
   const void* done;
   uint32_t blocks, count, passes, hard_blocks;
+  int group;
+  uint32_t hard_cap, split;
   hipGraphExec_t exec;
 };
 
 static void batch_run(Batch& B, hipStream_t s) {
   // eager when profiling (so each launch can be timed) and inside the
   // allocation-free entry point (instantiating a graph allocates)
-  if (profiler().on.load() != 0 || arena_active()) {
-    for (uint32_t k = 0; k < B.max_passes; ++k)
-      gicp_pass(B.insts.p, B.block_inst.p, B.total_blocks, B.count, B.scratch, B.hard_blocks, s);
+  const bool eager = profiler().on.load() != 0 || arena_active();
+  bool tuned = false;
+  const PassTuning tune = choose_tuning(B, s, !eager, tuned);
+  if (tuned) return;  // the tuning runs finished the loop
+  B.hard_blocks = hard_stage_blocks(B.total_pts, tune.hard_cap);
+  if (eager) {
+    run_passes(B, s, tune);
     return;
   }
   static thread_local std::vector<GraphEntry>* cache = new std::vector<GraphEntry>();
@@ -814,12 +969,14 @@ static void batch_run(Batch& B, hipStream_t s) {
   // eager launches of one loop, so a shape is captured on its second run (a
   // stream of one-off shapes, such as odometry scans, stays eager).
   static thread_local std::vector<GraphEntry>* seen = new std::vector<GraphEntry>();
-  const GraphEntry key{B.insts.p, B.block_inst.p, B.scratch.queue, B.scratch.done,
-                       B.total_blocks, B.count, B.max_passes, B.hard_blocks, nullptr};
+  const GraphEntry key{B.insts.p, B.block_inst.p, B.scratch.queue, B.scratch.done, B.total_blocks,
+                       B.count, B.max_passes, B.hard_blocks, tune.group, tune.hard_cap,
+                       tune.split_blocks, nullptr};
   auto same = [&](const GraphEntry& e) {
     return e.insts == key.insts && e.block_inst == key.block_inst && e.queue == key.queue &&
            e.done == key.done && e.blocks == key.blocks && e.count == key.count &&
-           e.passes == key.passes && e.hard_blocks == key.hard_blocks;
+           e.passes == key.passes && e.hard_blocks == key.hard_blocks && e.group == key.group &&
+           e.hard_cap == key.hard_cap && e.split == key.split;
   };
   hipGraphExec_t exec = nullptr;
   for (const GraphEntry& e : *cache)
@@ -830,15 +987,13 @@ static void batch_run(Batch& B, hipStream_t s) {
     if (!again) {
       if (seen->size() >= 16) seen->erase(seen->begin());
       seen->push_back(key);
-      for (uint32_t k = 0; k < B.max_passes; ++k)
-        gicp_pass(B.insts.p, B.block_inst.p, B.total_blocks, B.count, B.scratch, B.hard_blocks, s);
+      run_passes(B, s, tune);
       return;
     }
     SR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     hipGraph_t graph = nullptr;
     try {
-      for (uint32_t k = 0; k < B.max_passes; ++k)
-        gicp_pass(B.insts.p, B.block_inst.p, B.total_blocks, B.count, B.scratch, B.hard_blocks, s);
+      run_passes(B, s, tune);
     } catch (...) {
       (void)hipStreamEndCapture(s, &graph);
       if (graph) (void)hipGraphDestroy(graph);
@@ -853,8 +1008,9 @@ static void batch_run(Batch& B, hipStream_t s) {
       (void)hipGraphExecDestroy(cache->front().exec);
       cache->erase(cache->begin());
     }
-    cache->push_back(GraphEntry{B.insts.p, B.block_inst.p, B.scratch.queue, B.scratch.done,
-                                B.total_blocks, B.count, B.max_passes, B.hard_blocks, exec});
+    GraphEntry ent = key;
+    ent.exec = exec;
+    cache->push_back(ent);
   }
   SR_HIP(hipGraphLaunch(exec, s));
 }
@@ -1482,9 +1638,13 @@ int32_t sr_option_set(const char* name, int64_t value) {
   return guarded([&] {
     if (!name) fail(SR_ERR_INVALID_ARGUMENT, "option name must not be null");
     const std::string n(name);
-    if (n == "match_reuse" || n == "radius_tiles") {
+    if (n == "match_reuse" || n == "radius_tiles" || n == "autotune") {
       if (value != 0 && value != 1) fail(SR_ERR_INVALID_ARGUMENT, n + " takes 0 or 1");
-      (n == "match_reuse" ? options().match_reuse : options().radius_tiles).store(value);
+      (n == "match_reuse" ? options().match_reuse
+                          : n == "radius_tiles" ? options().radius_tiles : options().autotune).store(value);
+    } else if (n == "tune_force") {
+      if (value < -1) fail(SR_ERR_INVALID_ARGUMENT, "tune_force takes -1 or a candidate index");
+      options().tune_force.store(value);
     } else {
       fail(SR_ERR_INVALID_ARGUMENT, "unknown option " + n);
     }
@@ -1499,6 +1659,14 @@ int32_t sr_option_get(const char* name, int64_t* value) {
       *value = options().match_reuse.load();
     } else if (n == "radius_tiles") {
       *value = options().radius_tiles.load();
+    } else if (n == "autotune") {
+      *value = options().autotune.load();
+    } else if (n == "tune_force") {
+      *value = options().tune_force.load();
+    } else if (n == "tune_candidates_single") {
+      *value = int64_t(tune_candidates(false).size());
+    } else if (n == "tune_candidates_batch") {
+      *value = int64_t(tune_candidates(true).size());
     } else if (n == "noalloc_scratch_bytes") {
       *value = int64_t(kScratchBytes);
     } else if (n == "noalloc_peak_bytes") {  // of the last allocation-free call on this device
@@ -1509,6 +1677,31 @@ int32_t sr_option_get(const char* name, int64_t* value) {
       *value = int64_t(sd.last_peak);
     } else {
       fail(SR_ERR_INVALID_ARGUMENT, "unknown option " + n);
+    }
+  });
+}
+
+int32_t sr_tune_describe(char* out, uint64_t cap, uint64_t* len) {
+  return guarded([&] {
+    std::string text;
+    {
+      Tuner& T = tuner();
+      std::lock_guard<std::mutex> lk(T.mu);
+      for (const auto& kv : T.done) {
+        const PassTuning& b = kv.second.best;
+        text += kv.first.first + " class " + std::string(kv.first.second & 0x100u ? "batch" : "single") +
+                " log2_points " + std::to_string(kv.first.second & 0xFFu) + ": group " +
+                std::to_string(b.group) + " hard_cap " + std::to_string(b.hard_cap) + " split " +
+                std::to_string(b.split_blocks) + " ms";
+        for (float m : kv.second.ms) text += " " + std::to_string(m);
+        text += "\n";
+      }
+    }
+    if (len) *len = text.size();
+    if (out && cap) {
+      const size_t n = std::min<size_t>(cap - 1, text.size());
+      std::memcpy(out, text.data(), n);
+      out[n] = '\0';
     }
   });
 }

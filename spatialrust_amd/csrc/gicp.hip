@@ -1180,22 +1180,25 @@ __device__ __forceinline__ void hard_query(const IterInstance* __restrict__ inst
 // Static split, kWave/kNNGroup queries per wave and round: the grid holds a
 // wave per such slice of a fully queued launch (capped at what the chip runs
 // at once), so every wave takes few slices (no shared cursor to contend on).
-template <bool kStats, bool kSingle>
+// kG: lanes per queued point (a launch-tuning choice: any group size gives the
+// same matches, since every point of a ring is visited by exactly one lane and
+// the group takes the minimum packed key).
+template <bool kStats, bool kSingle, int kG>
 __global__ void __launch_bounds__(kBlock)
     __attribute__((amdgpu_waves_per_eu(SR_HARD_WAVES))) k_gicp_hard(const IterInstance* __restrict__ insts,
                                                       PassScratch ps,
                                                       SearchStats* __restrict__ stats) {
   const uint32_t qlen = ps.ctl[0];
   const int lane = int(threadIdx.x & (kWave - 1));
-  constexpr uint32_t G = kWave / kNNGroup;
+  constexpr uint32_t G = kWave / kG;
   constexpr uint32_t kWavesPerBlock = kBlock / kWave;
   const uint32_t waves = gridDim.x * kWavesPerBlock;
   const uint32_t wid = xcd_block() * kWavesPerBlock + threadIdx.x / kWave;
   for (uint32_t base = wid * G; base < qlen; base += waves * G) {
-    const uint32_t e = base + uint32_t(lane / kNNGroup);
+    const uint32_t e = base + uint32_t(lane / kG);
     const bool active = e < qlen;  // the same on a group's lanes
     const HardQuery h = ps.queue[active ? e : base];
-    hard_query<kStats, kSingle, kNNGroup>(insts, h, active, stats);
+    hard_query<kStats, kSingle, kG>(insts, h, active, stats);
   }
 }
 
@@ -1317,13 +1320,13 @@ __global__ void __launch_bounds__(kFinBlock) k_gicp_finalize(const IterInstance*
         reinterpret_cast<const unsigned long long*>(&sst)[threadIdx.x];
 }
 
-uint32_t hard_stage_blocks(uint64_t points) {
-  // a wave per kWave/kNNGroup queries of a fully queued launch, capped at what
-  // the chip holds at once (256 CUs x 8 waves per SIMD; larger queues take
-  // several rounds per wave), so an empty queue costs a small launch
+uint32_t hard_stage_blocks(uint64_t points, uint32_t cap) {
+  // a wave per kWave/kNNGroup queries of a fully queued launch, capped (by
+  // default at what the chip holds at once: 256 CUs x 8 waves per SIMD; larger
+  // queues take several rounds per wave), so an empty queue costs a small launch
   const uint64_t per_block = uint64_t(kBlock / kWave) * (kWave / kNNGroup);
   const uint64_t want = (points + per_block - 1) / per_block;
-  return uint32_t(std::min<uint64_t>(std::max<uint64_t>(want, 1), 2048));
+  return uint32_t(std::min<uint64_t>(std::max<uint64_t>(want, 1), std::max<uint32_t>(cap, 1)));
 }
 
 // Workgroups of the terms stage, each taking a run of blocks: several rounds
@@ -1332,21 +1335,32 @@ uint32_t hard_stage_blocks(uint64_t points) {
 #define SR_TERMS_GRID 4096
 #endif
 constexpr uint32_t kTermsGrid = SR_TERMS_GRID;
-// Launches of at least this many blocks (several per CU) run the first two
-// stages as two kernels; smaller ones as one.
-#ifndef SR_SPLIT_BLOCKS
-#define SR_SPLIT_BLOCKS 4096
-#endif
-constexpr uint32_t kSplitBlocks = SR_SPLIT_BLOCKS;
+
+template <bool kStats, bool kSingle>
+static void launch_hard(int group, uint32_t hard_blocks, hipStream_t s, const IterInstance* insts,
+                        const PassScratch& ps, SearchStats* st) {
+  switch (group) {
+    case 4:
+      hipLaunchKernelGGL((k_gicp_hard<kStats, kSingle, 4>), dim3(hard_blocks), dim3(kBlock), 0, s, insts, ps, st);
+      break;
+    case 16:
+      hipLaunchKernelGGL((k_gicp_hard<kStats, kSingle, 16>), dim3(hard_blocks), dim3(kBlock), 0, s, insts, ps, st);
+      break;
+    default:
+      hipLaunchKernelGGL((k_gicp_hard<kStats, kSingle, kNNGroup>), dim3(hard_blocks), dim3(kBlock), 0, s, insts, ps, st);
+  }
+}
 
 void gicp_pass(const IterInstance* insts, const uint32_t* block_inst, uint32_t total_blocks,
                uint32_t count, const PassScratch& ps, uint32_t hard_blocks, hipStream_t s,
-               bool finalize) {
+               bool finalize, const PassTuning& tune) {
   if (total_blocks == 0) return;
   {
     ProfScope prof("gicp_pass", s);
     SearchStats* st = prof_stats("gicp_pass");
-    const bool split = total_blocks >= kSplitBlocks;
+    // Launches of at least tune.split_blocks blocks (several per CU) run the
+    // first two stages as two kernels; smaller ones as one.
+    const bool split = total_blocks >= tune.split_blocks;
     if (st) {
       if (split) {
         hipLaunchKernelGGL(k_gicp_settle<true>, dim3(total_blocks), dim3(kBlock), 0, s, insts,
@@ -1357,10 +1371,8 @@ void gicp_pass(const IterInstance* insts, const uint32_t* block_inst, uint32_t t
         hipLaunchKernelGGL(k_gicp_search_fused<true>, dim3(total_blocks), dim3(kBlock), 0, s,
                            insts, block_inst, ps, st);
       }
-      if (count == 1)
-        hipLaunchKernelGGL((k_gicp_hard<true, true>), dim3(hard_blocks), dim3(kBlock), 0, s, insts, ps, st);
-      else
-        hipLaunchKernelGGL((k_gicp_hard<true, false>), dim3(hard_blocks), dim3(kBlock), 0, s, insts, ps, st);
+      if (count == 1) launch_hard<true, true>(tune.group, hard_blocks, s, insts, ps, st);
+      else launch_hard<true, false>(tune.group, hard_blocks, s, insts, ps, st);
     } else {
       if (split) {
         hipLaunchKernelGGL(k_gicp_settle<false>, dim3(total_blocks), dim3(kBlock), 0, s, insts,
@@ -1371,10 +1383,8 @@ void gicp_pass(const IterInstance* insts, const uint32_t* block_inst, uint32_t t
         hipLaunchKernelGGL(k_gicp_search_fused<false>, dim3(total_blocks), dim3(kBlock), 0, s,
                            insts, block_inst, ps, st);
       }
-      if (count == 1)
-        hipLaunchKernelGGL((k_gicp_hard<false, true>), dim3(hard_blocks), dim3(kBlock), 0, s, insts, ps, st);
-      else
-        hipLaunchKernelGGL((k_gicp_hard<false, false>), dim3(hard_blocks), dim3(kBlock), 0, s, insts, ps, st);
+      if (count == 1) launch_hard<false, true>(tune.group, hard_blocks, s, insts, ps, st);
+      else launch_hard<false, false>(tune.group, hard_blocks, s, insts, ps, st);
     }
     hipLaunchKernelGGL(k_gicp_terms, dim3(std::min<uint32_t>(total_blocks, kTermsGrid)), dim3(kBlock),
                        0, s, insts, block_inst, total_blocks, ps);

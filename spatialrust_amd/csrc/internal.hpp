@@ -335,6 +335,8 @@ constexpr uint32_t kCfgMatchReuse = 1u;  // InstConfig.flags
 struct Options {
   std::atomic<int64_t> match_reuse{1};   // certified match reuse in the Gauss-Newton passes
   std::atomic<int64_t> radius_tiles{0};  // radius covariances / grid normals from LDS cell tiles
+  std::atomic<int64_t> autotune{1};      // pick pass launch choices per problem class by timing
+  std::atomic<int64_t> tune_force{-1};   // >= 0: use that candidate for every pass loop
 };
 Options& options();
 
@@ -366,15 +368,26 @@ struct PassScratch {
   unsigned long long* pend;
 };
 
-// Workgroups of the queue-draining stage for a launch over `points` source points.
-uint32_t hard_stage_blocks(uint64_t points);
+// Launch choices of a pass that leave every result unchanged (the tuner picks
+// them per problem class): lanes per queued point in the ring search (4, 8 or
+// 16), the workgroup cap of that stage, and the launch size (blocks) from
+// which the first stage runs as two kernels (settle + marked search).
+struct PassTuning {
+  int group = 8;
+  uint32_t hard_cap = 2048;
+  uint32_t split_blocks = 4096;
+};
+
+// Workgroups of the queue-draining stage for a launch over `points` source
+// points, at most `cap`.
+uint32_t hard_stage_blocks(uint64_t points, uint32_t cap = 2048);
 
 // Launches one Gauss-Newton pass over `count` instances (device array `insts`),
 // total_blocks blocks; block_inst maps block -> instance. Without `finalize`
 // only the per-block partial sums are written (no solve, no state change).
 void gicp_pass(const IterInstance* insts, const uint32_t* block_inst, uint32_t total_blocks,
                uint32_t count, const PassScratch& ps, uint32_t hard_blocks, hipStream_t s,
-               bool finalize = true);
+               bool finalize = true, const PassTuning& tune = PassTuning{});
 // Initialises instance states from poses (device array of 7 doubles per
 // instance: quaternion x y z w, translation).
 void init_states(InstState* states, const double* init7, uint32_t count, hipStream_t s);

@@ -86,6 +86,7 @@ SIGNATURES = {
     "sr_gicp_config_default": (None, [ctypes.POINTER(GicpConfigC)]),
     "sr_option_set": (_I32, [ctypes.c_char_p, ctypes.c_int64]),
     "sr_option_get": (_I32, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]),
+    "sr_tune_describe": (_I32, [ctypes.c_char_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),
     "sr_profile_enable": (None, [_I32]),
     "sr_profile_reset": (None, []),
     "sr_profile_read": (_I32, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),

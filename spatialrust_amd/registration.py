@@ -232,6 +232,15 @@ def get_option(name: str) -> int:
     return int(v.value)
 
 
+def tune_describe() -> str:
+    """The launch tuner's decisions so far (sr_tune_describe)."""
+    n = ctypes.c_uint64(0)
+    _check(_n.load().sr_tune_describe(None, 0, ctypes.byref(n)))
+    buf = ctypes.create_string_buffer(int(n.value) + 1)
+    _check(_n.load().sr_tune_describe(buf, len(buf), ctypes.byref(n)))
+    return buf.value.decode()
+
+
 class GicpRegistration:
     """Generalized ICP on the HIP device (GicpRegistration, gicp.rs:70-235)."""
 

@@ -262,3 +262,42 @@ def test_radius_tiles_equal_per_point_form(hip, orc, tiles_restored, case):
     if case == "dense":
         o, _ = orc.radius_covariances(pts, radius, 1e-3)
         assert np.array_equal(out[1][0], o)
+
+
+@pytest.fixture
+def tune_restored():
+    yield
+    sr.set_option("tune_force", -1)
+
+
+def test_autotune_candidates_bit_identical(hip, tune_restored):
+    """Every launch-tuning candidate gives the pose, fitness and iteration
+    count of the tuner's own choice, bit for bit: for one registration large
+    enough to be tuned (over 128k source points after the voxel filter), and for
+    a batch of ~1.8M source points, where the candidates run the first stage
+    as two kernels or as one."""
+    w = synth.small_cube_pair(n_source=200000, n_target=200000)
+    cfg = _fixed(max_iterations=20, voxel_leaf=0.1)
+    reg = sr.GicpRegistration(cfg)
+    ref = reg.align(w.source, w.target)
+    assert ref.source_points >= 1 << 17
+    assert "single" in sr.tune_describe()
+    for c in range(sr.get_option("tune_candidates_single")):
+        sr.set_option("tune_force", c)
+        assert _same(reg.align(w.source, w.target), ref), c
+    sr.set_option("tune_force", -1)
+    pairs = synth.batch_pairs(count=40, lo=30000, hi=60000, seed_offset=13)
+    srcs = [devmem.DeviceCloud(p.source) for p in pairs]
+    tgts = [devmem.DeviceCloud(p.target) for p in pairs]
+    try:
+        sv, tv = [c.view for c in srcs], [c.view for c in tgts]
+        bref = sr.batch_align_pairs(sv, tv, cfg)
+        assert "batch" in sr.tune_describe()
+        for c in range(sr.get_option("tune_candidates_batch")):
+            sr.set_option("tune_force", c)
+            got = sr.batch_align_pairs(sv, tv, cfg)
+            assert all(_same(a, b) for a, b in zip(got, bref)), c
+    finally:
+        for c in srcs + tgts:
+            c.close()
+    print(sr.tune_describe(), flush=True)
