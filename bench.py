@@ -372,6 +372,93 @@ def _radius_leg(w, reps=5):
             "identical": bool(all(np.array_equal(a, b) for a, b in zip(res[1], res[0])))}
 
 
+ODO_SCANS = 250
+
+
+def _gen_scans(n):
+    """All scans of the odometry leg, ray-cast in FP64 torch on the GPU when
+    torch sees one (this runs in a child process), else on the CPU."""
+    import torch
+    from spatialrust_amd import synth
+    dev = "cuda" if torch.cuda.is_available() else "cpu"
+    torch.set_num_threads(min(8, os.cpu_count() or 1))
+    scene = synth.odometry_scene(synth.SEED_BASE + 4000)
+    poses = synth.odometry_trajectory(ODO_SCANS, seed=1)[:n]
+    out = []
+    for k, (q, t) in enumerate(poses):
+        out.append(synth.lidar_scan(scene, q, t, seed=100 + k, device=dev))
+        if (k + 1) % 25 == 0:
+            print(f"  scans ray-cast: {k + 1}/{n} ({dev})", file=sys.stderr, flush=True)
+    return out
+
+
+def _odometry(n_scans):
+    """Scan-to-map odometry at full scan size: a spinning LiDAR of 64 beams
+    (elevations -25..+15 deg) x 2048 azimuth bins ray-cast into the large-block
+    scene (range noise 0.02 m, 5% dropout, ~120k points per scan) along a
+    smooth 10 m/s loop at 10 Hz; each scan voxel-filtered at 0.25 m and
+    registered (k 15, d_max 1.0, 30 iterations) against the rolling
+    device-resident voxel map (4M voxels at most), which it then extends.
+    Per-scan latency is the wall time of sr_odometry_push (host scan in, pose
+    out); drift is the estimated pose relative to the first scan against the
+    ground truth."""
+    import concurrent.futures
+    import ctypes
+    import multiprocessing
+    import spatialrust_amd as sr
+    from spatialrust_amd import native, synth
+    lib = native.load()
+    t0 = time.perf_counter()
+    ctx = multiprocessing.get_context("spawn")
+    with concurrent.futures.ProcessPoolExecutor(max_workers=1, mp_context=ctx) as ex:
+        scans = ex.submit(_gen_scans, n_scans).result()
+    t_gen = time.perf_counter() - t0
+    poses = synth.odometry_trajectory(ODO_SCANS, seed=1)[:n_scans]
+    cfg = sr.GicpConfig(max_iterations=30, k_neighbors=15, max_correspondence_distance=1.0)
+    odo = sr.RollingMapOdometry(cfg, voxel_leaf=0.25, max_map_points=4_000_000)
+    lat, res = [], []
+    lib.sr_profile_reset()
+    lib.sr_profile_enable(1)
+    for k in range(n_scans):
+        t1 = time.perf_counter()
+        res.append(odo.push(scans[k]))
+        lat.append(time.perf_counter() - t1)
+        if (k + 1) % 50 == 0:
+            print(f"  scans registered: {k + 1}/{n_scans}, map {odo.map_points()} voxels",
+                  file=sys.stderr, flush=True)
+    lib.sr_profile_enable(0)
+    spans = {}
+    for name in ("odometry_register", "odometry_map"):
+        ms, cnt = ctypes.c_double(0.0), ctypes.c_uint64(0)
+        lib.sr_profile_read(name.encode(), ctypes.byref(ms), ctypes.byref(cnt))
+        spans[name] = ms.value / max(cnt.value, 1)
+    lib.sr_profile_reset()
+    map_points = odo.map_points()
+    odo.close()
+    q0i, t0i = synth.invert_motion(*poses[0])
+    R0i = synth.quat_to_mat(q0i)
+    terr, rerr = [], []
+    for k in range(n_scans):
+        gq = synth.quat_mul(q0i, poses[k][0])
+        gt = R0i @ np.asarray(poses[k][1]) + t0i
+        terr.append(float(np.linalg.norm(res[k].translation64 - gt)))
+        rerr.append(synth.rotation_angle_between(res[k].rotation64, gq))
+    path = float(sum(np.linalg.norm(np.asarray(poses[k + 1][1]) - np.asarray(poses[k][1]))
+                     for k in range(n_scans - 1)))
+    lat_ms = np.array(lat[1:]) * 1e3  # the first scan only seeds the map
+    return {"workload": "lidar_64x2048_odometry", "scans": n_scans,
+            "points_per_scan_mean": float(np.mean([len(s) for s in scans])),
+            "scan_voxels_mean": float(np.mean([r.source_points for r in res[1:]])),
+            "map_points_final": int(map_points), "map_capacity": 4_000_000,
+            "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99)),
+                           "max": float(lat_ms.max())},
+            "scans_per_s": float(1e3 / np.mean(lat_ms)),
+            "gpu_span_ms_per_scan": {"register": spans["odometry_register"], "map_update": spans["odometry_map"]},
+            "drift": {"path_m": path, "final_trans_m": terr[-1], "max_trans_m": max(terr),
+                      "final_rot_rad": rerr[-1], "max_rot_rad": max(rerr)},
+            "generate_s": t_gen}
+
+
 def _host_view_rate(w, cfg, steps):
     """The drop-in call as a Rust caller makes it, with host (numpy) clouds:
     the raw xyz columns (12 B per point) are copied to the device inside every
@@ -427,6 +514,9 @@ def main():
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--no-extra-legs", action="store_true",
                     help="skip the host-view rate and the reference-corner GPU timing")
+    ap.add_argument("--no-odometry", action="store_true",
+                    help="skip the LiDAR odometry leg (64 x 2048-beam scans, rolling 4M map)")
+    ap.add_argument("--odometry-scans", type=int, default=ODO_SCANS)
     ap.add_argument("--no-radius", action="store_true",
                     help="skip the radius covariance / grid normal leg (LDS tiles vs per point)")
     ap.add_argument("--no-serve", action="store_true",
@@ -569,6 +659,9 @@ def main():
         print("serving leg ...", file=sys.stderr, flush=True)
         line["serving"] = _serving(args.serve_rates, args.serve_seconds, args.serve_max_requests,
                                    args.serve_max_active)
+    if not args.no_odometry:
+        print("odometry leg ...", file=sys.stderr, flush=True)
+        line["odometry"] = _odometry(args.odometry_scans)
     if not args.no_radius:
         print("radius covariance leg ...", file=sys.stderr, flush=True)
         line["radius_covariances"] = _radius_leg(w)

@@ -2301,6 +2301,7 @@ int32_t sr_odometry_push(sr_odometry* odo, const sr_xyz_view* scan, sr_result* o
       double pose7[7] = {guess.q[0], guess.q[1], guess.q[2], guess.q[3],
                          guess.t[0], guess.t[1], guess.t[2]};
       Batch B;
+      ProfScope prof("odometry_register", s);
       batch_setup({&src}, {&odo->map_prep}, {odo->gicp}, B, s, pose7);
       batch_run(B, s);
       InstState st;
@@ -2320,6 +2321,7 @@ int32_t sr_odometry_push(sr_odometry* odo, const sr_xyz_view* scan, sr_result* o
     MapPose mp;
     quat_mat64(pose.q, mp.r);
     for (int a = 0; a < 3; ++a) mp.t[a] = pose.t[a];
+    ProfScope prof_map("odometry_map", s);
     odo->map.insert(vox.centroids.pts.p, vox.cells, mp, s);
     Prepared fresh;
     prepare_cloud(odo->map.list, odo->gicp, fresh, rc, s, true, odo->cfg.voxel_leaf);

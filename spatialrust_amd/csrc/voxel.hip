@@ -437,11 +437,19 @@ __device__ __forceinline__ float4 cell_centroid(const float4* __restrict__ pts,
   return make_float4(__fmul_rn(sx, inv), __fmul_rn(sy, inv), __fmul_rn(sz, inv), 0.0f);
 }
 
+// Cells of at most kTinyCell members are sorted and summed by one thread here;
+// up to kSmallCell members by one wave (k_voxel_medium_cells); larger ones by
+// a workgroup (k_voxel_big_cells). The per-thread insertion sort works in
+// device memory, so it is kept to a few members (a 64-member cell took that
+// thread ~1000 dependent memory round trips).
+constexpr uint32_t kTinyCell = 4;
+
 __global__ void k_voxel_centroids(const float4* __restrict__ pts, const uint32_t* __restrict__ ncells,
                                   const uint32_t* __restrict__ cstart,
                                   const uint32_t* __restrict__ ocount, uint32_t* __restrict__ members,
                                   float4* __restrict__ out, uint32_t* __restrict__ big_list,
-                                  uint32_t* __restrict__ big_count) {
+                                  uint32_t* __restrict__ big_count,
+                                  uint32_t* __restrict__ med_count) {
   uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= *ncells) return;
   uint32_t cnt = ocount[c];
@@ -450,8 +458,55 @@ __global__ void k_voxel_centroids(const float4* __restrict__ pts, const uint32_t
     big_list[atomicAdd(big_count, 1u)] = c;
     return;
   }
+  if (cnt > kTinyCell) {  // medium list: filled from the end of the same buffer
+    big_list[*ncells - 1u - atomicAdd(med_count, 1u)] = c;
+    return;
+  }
   sort_segment(seg, cnt);
   out[c] = cell_centroid(pts, seg, cnt);
+}
+
+// One wave per medium cell (kTinyCell < members <= kSmallCell = 64): the
+// member indices, one per lane, are put in ascending order by a bitonic
+// network across the lanes, the points loaded in parallel, and their
+// coordinates added in index order (lane reads), which is the sequential f32
+// sum cell_centroid forms.
+static_assert(kSmallCell <= 64, "one member per lane");
+__global__ void __launch_bounds__(kBlock) k_voxel_medium_cells(
+    const float4* __restrict__ pts, const uint32_t* __restrict__ ncells,
+    const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ ocount,
+    const uint32_t* __restrict__ members, const uint32_t* __restrict__ list,
+    const uint32_t* __restrict__ med_count, float4* __restrict__ out) {
+  const uint32_t nmed = *med_count, nc = *ncells;
+  const int lane = int(threadIdx.x & 63u);
+  const uint32_t waves = gridDim.x * (blockDim.x / 64u);
+  for (uint32_t w = blockIdx.x * (blockDim.x / 64u) + threadIdx.x / 64u; w < nmed; w += waves) {
+    const uint32_t c = list[nc - 1u - w];
+    const uint32_t cnt = ocount[c];
+    uint32_t v = uint32_t(lane) < cnt ? members[cstart[c] + uint32_t(lane)] : 0xFFFFFFFFu;
+#pragma unroll
+    for (int size = 2; size <= 64; size <<= 1) {
+#pragma unroll
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        const uint32_t o = uint32_t(__shfl_xor(int(v), stride));
+        const bool up = (lane & size) == 0, lower = (lane & stride) == 0;
+        // the lower lane of a pair keeps the smaller key in an ascending run
+        v = (lower == up) ? min(v, o) : max(v, o);
+      }
+    }
+    float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (uint32_t(lane) < cnt) p = pts[v];
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    for (uint32_t a = 0; a < cnt; ++a) {
+      sx = __fadd_rn(sx, __shfl(p.x, int(a)));
+      sy = __fadd_rn(sy, __shfl(p.y, int(a)));
+      sz = __fadd_rn(sz, __shfl(p.z, int(a)));
+    }
+    if (lane == 0) {
+      const float inv = __fdiv_rn(1.0f, (float)cnt);
+      out[c] = make_float4(__fmul_rn(sx, inv), __fmul_rn(sy, inv), __fmul_rn(sz, inv), 0.0f);
+    }
+  }
 }
 
 // Large cells: one workgroup per cell sorts its member indices (bitonic in LDS
@@ -525,7 +580,7 @@ void voxel_downsample(const Cloud& in, float leaf, const float* origin, bool wan
   hipLaunchKernelGGL(k_fill_u64, dim3(fill_blocks(cap)), dim3(kBlock), 0, s, tkeys.p, kEmptyKey,
                      size_t(cap));
   DevBuf<uint32_t> slot_of(n), flag(n), rank(n), ocount(n), cstart(n), cursor(n), members(n),
-      scal(4);
+      scal(5);
   const unsigned blocks = (n + kBlock - 1) / kBlock;
   ProfScope prof_insert("voxel_insert", s);
   hipLaunchKernelGGL(k_voxel_insert, dim3(blocks), dim3(kBlock), 0, s, in.pts.p, n, params.p,
@@ -545,9 +600,13 @@ void voxel_downsample(const Cloud& in, float leaf, const float* origin, bool wan
                      cstart.p, cursor.p, n, members.p);
   out.centroids.pts.resize(n);
   fill_u32(scal.p + 1, 0u, 1, s);
+  fill_u32(scal.p + 4, 0u, 1, s);  // medium-cell count
   DevBuf<uint32_t> big_list(n);
   hipLaunchKernelGGL(k_voxel_centroids, dim3(blocks), dim3(kBlock), 0, s, in.pts.p, scal.p,
-                     cstart.p, ocount.p, members.p, out.centroids.pts.p, big_list.p, scal.p + 1);
+                     cstart.p, ocount.p, members.p, out.centroids.pts.p, big_list.p, scal.p + 1,
+                     scal.p + 4);
+  hipLaunchKernelGGL(k_voxel_medium_cells, dim3(1024), dim3(kBlock), 0, s, in.pts.p, scal.p,
+                     cstart.p, ocount.p, members.p, big_list.p, scal.p + 4, out.centroids.pts.p);
   hipLaunchKernelGGL(k_voxel_big_cells, dim3(256), dim3(kBlock), 0, s, in.pts.p, cstart.p,
                      ocount.p, members.p, big_list.p, scal.p + 1, out.centroids.pts.p);
   hipLaunchKernelGGL(k_error_check, dim3(1), dim3(64), 0, s, params.p, scal.p + 2);
@@ -765,7 +824,7 @@ void voxel_downsample_many(const Cloud& in, const std::vector<uint32_t>& off, fl
   hipLaunchKernelGGL(k_fill_u64, dim3(fill_blocks(cap)), dim3(kBlock), 0, s, tkeys.p, kEmptyKey,
                      size_t(cap));
   DevBuf<uint32_t> slot_of(n), flag(n), rank(n), ocount(n), cstart(n), cursor(n), members(n),
-      scal(4), tailv(nseg + 2);
+      scal(5), tailv(nseg + 2);
   const unsigned blocks = (n + kBlock - 1) / kBlock;
   ProfScope prof_insert("voxel_insert", s);
   hipLaunchKernelGGL(k_seg_voxel_insert, dim3(blk.back()), dim3(kBlock), 0, s, in.pts.p, d_off.p,
@@ -784,9 +843,13 @@ void voxel_downsample_many(const Cloud& in, const std::vector<uint32_t>& off, fl
                      cstart.p, cursor.p, n, members.p);
   out.centroids.pts.resize(n);
   fill_u32(scal.p + 1, 0u, 1, s);
+  fill_u32(scal.p + 4, 0u, 1, s);  // medium-cell count
   DevBuf<uint32_t> big_list(n);
   hipLaunchKernelGGL(k_voxel_centroids, dim3(blocks), dim3(kBlock), 0, s, in.pts.p, scal.p,
-                     cstart.p, ocount.p, members.p, out.centroids.pts.p, big_list.p, scal.p + 1);
+                     cstart.p, ocount.p, members.p, out.centroids.pts.p, big_list.p, scal.p + 1,
+                     scal.p + 4);
+  hipLaunchKernelGGL(k_voxel_medium_cells, dim3(1024), dim3(kBlock), 0, s, in.pts.p, scal.p,
+                     cstart.p, ocount.p, members.p, big_list.p, scal.p + 4, out.centroids.pts.p);
   hipLaunchKernelGGL(k_voxel_big_cells, dim3(256), dim3(kBlock), 0, s, in.pts.p, cstart.p,
                      ocount.p, members.p, big_list.p, scal.p + 1, out.centroids.pts.p);
   fill_u32(tailv.p + nseg + 1, 0u, 1, s);
