@@ -416,13 +416,15 @@ def _odometry(n_scans):
     poses = synth.odometry_trajectory(ODO_SCANS, seed=1)[:n_scans]
     cfg = sr.GicpConfig(max_iterations=30, k_neighbors=15, max_correspondence_distance=1.0)
     odo = sr.RollingMapOdometry(cfg, voxel_leaf=0.25, max_map_points=4_000_000)
-    lat, res = [], []
+    lat, res, refreshed = [], [], []
     lib.sr_profile_reset()
     lib.sr_profile_enable(1)
     for k in range(n_scans):
         t1 = time.perf_counter()
         res.append(odo.push(scans[k]))
         lat.append(time.perf_counter() - t1)
+        st = odo.map_stats()
+        refreshed.append(st["refreshed"] / max(st["map_points"], 1))
         if (k + 1) % 50 == 0:
             print(f"  scans registered: {k + 1}/{n_scans}, map {odo.map_points()} voxels",
                   file=sys.stderr, flush=True)
@@ -434,6 +436,7 @@ def _odometry(n_scans):
         spans[name] = ms.value / max(cnt.value, 1)
     lib.sr_profile_reset()
     map_points = odo.map_points()
+    rebuilds = odo.map_stats()["rebuilds"]
     odo.close()
     q0i, t0i = synth.invert_motion(*poses[0])
     R0i = synth.quat_to_mat(q0i)
@@ -454,6 +457,9 @@ def _odometry(n_scans):
                            "max": float(lat_ms.max())},
             "scans_per_s": float(1e3 / np.mean(lat_ms)),
             "gpu_span_ms_per_scan": {"register": spans["odometry_register"], "map_update": spans["odometry_map"]},
+            "map_upkeep": {"full_rebuilds": int(rebuilds),
+                           "normals_recomputed_frac_mean": float(np.mean(refreshed[1:])),
+                           "normals_recomputed_frac_p50": float(np.median(refreshed[1:]))},
             "drift": {"path_m": path, "final_trans_m": terr[-1], "max_trans_m": max(terr),
                       "final_rot_rad": rerr[-1], "max_rot_rad": max(rerr)},
             "generate_s": t_gen}

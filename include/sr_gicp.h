@@ -153,6 +153,11 @@ void sr_gicp_config_default(sr_gicp_config* out);
  *     file the decisions are read from and appended to.
  *   "tune_force" (default -1): >= 0 uses that candidate for every loop (read
  *     "tune_candidates_single" / "tune_candidates_batch" for their counts).
+ *   "odometry_incremental" (default 1): after a scan is merged, the odometry
+ *     map recomputes the k-NN normals only of the new voxels and of the voxels
+ *     whose k-th nearest neighbour is not provably closer than every new voxel
+ *     (from their grid cells' distance to the new voxels' cells); 0 recomputes
+ *     every normal. Poses are bit-identical either way.
  * Unknown names and values out of range are InvalidArgument. */
 int32_t sr_option_set(const char* name, int64_t value);
 int32_t sr_option_get(const char* name, int64_t* value);
@@ -378,6 +383,12 @@ void sr_odometry_destroy(sr_odometry* odometry);
  * the merge. */
 int32_t sr_odometry_push(sr_odometry* odometry, const sr_xyz_view* scan, sr_result* out);
 uint64_t sr_odometry_map_points(const sr_odometry* odometry);
+/* Map upkeep counters (no counterpart in the reference): out4 = {map points,
+ * normals computed by the last push, full grid rebuilds so far, scans}. The
+ * map's grid keeps its geometry from one full rebuild to the next; a rebuild
+ * follows a drop of old voxels, a new voxel outside the grid's box, or the map
+ * doubling since the last rebuild. */
+int32_t sr_odometry_map_stats(const sr_odometry* odometry, uint64_t* out4);
 /* Copies up to `cap` map points (interleaved xyz, insertion order) to host
  * memory; *n receives the map size. */
 int32_t sr_odometry_map_read(const sr_odometry* odometry, float* xyz, uint64_t cap, uint64_t* n);

@@ -1540,6 +1540,17 @@ struct sr_odometry {
   sr::Prepared map_prep;  // grid, empty field and normals of map.list
   sr::Pose64 pose, prev;  // map_T_scan of the last two scans
   uint64_t scans = 0;
+  // Grid geometry of the map (bounds as ordered words, cell edge), kept from
+  // one full rebuild to the next so unchanged neighbourhoods keep their normals.
+  bool geo_valid = false;
+  uint32_t geo_hb[6] = {};
+  float geo_h = 0.0f;
+  uint32_t geo_points = 0;  // map size at the last full rebuild
+  uint64_t geo_drops = 0;   // map.drops at the last full rebuild
+  sr::DevBuf<float4> lnrm;  // normals of the map points, list order
+  sr::DevBuf<float> ltau;   // k-th neighbour squared distance of each, list order
+  sr::DevBuf<uint32_t> refresh, refresh_n;
+  uint64_t refreshed = 0, rebuilds = 0;  // normals computed by the last push; full rebuilds
 };
 
 template <typename F>
@@ -1634,14 +1645,27 @@ int64_t sr_profile_read_each(const char* name, double* out_ms, uint64_t cap) {
   return int64_t(cnt);
 }
 
+}  // extern "C"
+
+// The 0/1 switches among the options.
+static std::atomic<int64_t>* switch_option(const std::string& n) {
+  Options& o = options();
+  if (n == "match_reuse") return &o.match_reuse;
+  if (n == "radius_tiles") return &o.radius_tiles;
+  if (n == "autotune") return &o.autotune;
+  if (n == "odometry_incremental") return &o.odometry_incremental;
+  return nullptr;
+}
+
+extern "C" {
+
 int32_t sr_option_set(const char* name, int64_t value) {
   return guarded([&] {
     if (!name) fail(SR_ERR_INVALID_ARGUMENT, "option name must not be null");
     const std::string n(name);
-    if (n == "match_reuse" || n == "radius_tiles" || n == "autotune") {
+    if (std::atomic<int64_t>* sw = switch_option(n)) {
       if (value != 0 && value != 1) fail(SR_ERR_INVALID_ARGUMENT, n + " takes 0 or 1");
-      (n == "match_reuse" ? options().match_reuse
-                          : n == "radius_tiles" ? options().radius_tiles : options().autotune).store(value);
+      sw->store(value);
     } else if (n == "tune_force") {
       if (value < -1) fail(SR_ERR_INVALID_ARGUMENT, "tune_force takes -1 or a candidate index");
       options().tune_force.store(value);
@@ -1655,12 +1679,8 @@ int32_t sr_option_get(const char* name, int64_t* value) {
   return guarded([&] {
     if (!name || !value) fail(SR_ERR_INVALID_ARGUMENT, "null option argument");
     const std::string n(name);
-    if (n == "match_reuse") {
-      *value = options().match_reuse.load();
-    } else if (n == "radius_tiles") {
-      *value = options().radius_tiles.load();
-    } else if (n == "autotune") {
-      *value = options().autotune.load();
+    if (std::atomic<int64_t>* sw = switch_option(n)) {
+      *value = sw->load();
     } else if (n == "tune_force") {
       *value = options().tune_force.load();
     } else if (n == "tune_candidates_single") {
@@ -2250,6 +2270,100 @@ uint64_t sr_server_device_bytes(const sr_server* server) {
   return server ? server->bytes_active : 0;
 }
 
+}  // extern "C"
+
+// Grid, empty field and normals of the odometry map after points [n_old,
+// list.n) were appended (n_old = 0 after a drop renumbered the list). A full
+// rebuild chooses the grid geometry: the list's bounds padded by an eighth of
+// their span plus 16 leaves per side, and the cell edge a fresh grid of the
+// list would get. Until the next full rebuild (a drop, a new point outside the
+// box, or the map doubling) every scan re-sorts the grid with that geometry and
+// recomputes the k-NN normals of map_refresh_set's points only; the others keep
+// theirs, which equal what a full computation on that geometry gives.
+static void odometry_update_map(sr_odometry* o, uint32_t n_old, Receipt& rc, hipStream_t s) {
+  const Cloud& list = o->map.list;
+  const uint32_t n = list.n;
+  if (o->gicp.covariance_radius > 0.0f) {  // radius covariances: the whole map each scan
+    Prepared fresh;
+    prepare_cloud(list, o->gicp, fresh, rc, s, true, o->cfg.voxel_leaf);
+    SR_HIP(hipStreamSynchronize(s));
+    o->map_prep = std::move(fresh);
+    o->refreshed = n;
+    ++o->rebuilds;
+    return;
+  }
+  bool full = !o->geo_valid || o->map.drops != o->geo_drops || n_old == 0 ||
+              uint64_t(n) >= 2ull * o->geo_points;
+  if (!full && n > n_old) {
+    DevBuf<uint32_t> b(8);
+    uint32_t nb[6];
+    cloud_bounds(list.pts.p + n_old, n - n_old, b.p, s);
+    SR_HIP(hipMemcpyAsync(nb, b.p, sizeof(nb), hipMemcpyDeviceToHost, s));
+    SR_HIP(hipStreamSynchronize(s));
+    for (int a = 0; a < 3; ++a) full = full || nb[a] < o->geo_hb[a] || nb[3 + a] > o->geo_hb[3 + a];
+  }
+  const float leaf = o->cfg.voxel_leaf;
+  if (full) {
+    uint32_t hb[6];
+    host_bounds(list, hb, s);
+    float h = choose_cell_edge(list, hb, knn_occupancy(o->gicp.k_neighbors), leaf);
+    double span_cells[3];
+    for (int a = 0; a < 3; ++a) {
+      const float lo = ord2f(hb[a]), hi = ord2f(hb[3 + a]);
+      const float pad = 0.125f * (hi - lo) + 16.0f * leaf;
+      o->geo_hb[a] = f2ord(lo - pad);
+      o->geo_hb[3 + a] = f2ord(hi + pad);
+      span_cells[a] = double(ord2f(o->geo_hb[3 + a]) - ord2f(o->geo_hb[a]));
+    }
+    // the padded box stays within a quarter of the 64M-cell grid cap
+    auto cells = [&](float e) {
+      double c = 1.0;
+      for (int a = 0; a < 3; ++a) c *= std::floor(span_cells[a] / double(e)) + 1.0;
+      return c;
+    };
+    while (cells(h) > 16.0e6) h *= 1.125f;
+    o->geo_h = h;
+    o->geo_valid = true;
+    o->geo_points = n;
+    o->geo_drops = o->map.drops;
+    ++o->rebuilds;
+    // list-order buffers for every map size up to the next full rebuild (< 2n)
+    const size_t cap = std::max<size_t>(std::min<uint64_t>(2ull * n, o->cfg.max_map_points), 1);
+    if (o->lnrm.n < cap) {
+      o->lnrm.resize(cap);
+      o->ltau.resize(cap);
+      o->refresh.resize(cap);
+    }
+    o->refresh_n.resize(1);
+  }
+  Prepared fresh;
+  fresh.n = n;
+  build_grid(list, o->geo_hb, o->geo_h, fresh.grid, s);
+  build_empty_radius(fresh.grid, s);
+  rc.stages |= SR_STAGE_GRID;
+  fresh.normals.resize(std::max<uint32_t>(n, 1));
+  KnnCloud kc = knn_cloud(fresh.grid, fresh.normals.p, nullptr, nullptr);
+  kc.nrm_orig = o->lnrm.p;
+  kc.tau_orig = o->ltau.p;
+  const bool incremental = !full && options().odometry_incremental.load() != 0;
+  if (incremental) {
+    map_refresh_set(fresh.grid, list.pts.p, n_old, o->ltau.p, o->refresh.p, o->refresh_n.p, s);
+    kc.subset = o->refresh.p;
+    kc.subset_n = o->refresh_n.p;
+  }
+  knn_normals_many({kc}, o->gicp.k_neighbors, o->gicp.epsilon, s);
+  if (incremental) gather_list_normals(fresh.grid, o->lnrm.p, fresh.normals.p, s);
+  rc.stages |= SR_STAGE_COVARIANCE;
+  uint32_t refreshed = n;
+  if (incremental)
+    SR_HIP(hipMemcpyAsync(&refreshed, o->refresh_n.p, sizeof(refreshed), hipMemcpyDeviceToHost, s));
+  SR_HIP(hipStreamSynchronize(s));
+  o->refreshed = refreshed;
+  o->map_prep = std::move(fresh);
+}
+
+extern "C" {
+
 int32_t sr_odometry_create(const sr_odometry_config* config, const sr_gicp_config* gicp,
                            sr_odometry** out) {
   return guarded([&] {
@@ -2322,11 +2436,10 @@ int32_t sr_odometry_push(sr_odometry* odo, const sr_xyz_view* scan, sr_result* o
     quat_mat64(pose.q, mp.r);
     for (int a = 0; a < 3; ++a) mp.t[a] = pose.t[a];
     ProfScope prof_map("odometry_map", s);
+    const uint64_t drops = odo->map.drops;
+    const uint32_t n_old = odo->map.list.n;
     odo->map.insert(vox.centroids.pts.p, vox.cells, mp, s);
-    Prepared fresh;
-    prepare_cloud(odo->map.list, odo->gicp, fresh, rc, s, true, odo->cfg.voxel_leaf);
-    SR_HIP(hipStreamSynchronize(s));
-    odo->map_prep = std::move(fresh);
+    odometry_update_map(odo, odo->map.drops == drops ? n_old : 0u, rc, s);
     odo->prev = odo->scans > 0 ? odo->pose : pose;
     odo->pose = pose;
     ++odo->scans;
@@ -2336,6 +2449,16 @@ int32_t sr_odometry_push(sr_odometry* odo, const sr_xyz_view* scan, sr_result* o
 }
 
 uint64_t sr_odometry_map_points(const sr_odometry* odo) { return odo ? odo->map.list.n : 0; }
+
+int32_t sr_odometry_map_stats(const sr_odometry* odo, uint64_t* out4) {
+  return guarded([&] {
+    if (!odo || !out4) fail(SR_ERR_INVALID_ARGUMENT, "null odometry argument");
+    out4[0] = odo->map.list.n;
+    out4[1] = odo->refreshed;
+    out4[2] = odo->rebuilds;
+    out4[3] = odo->scans;
+  });
+}
 
 int32_t sr_odometry_map_read(const sr_odometry* odo, float* xyz, uint64_t cap, uint64_t* n) {
   return guarded([&] {

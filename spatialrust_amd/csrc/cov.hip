@@ -287,8 +287,13 @@ constexpr int kKnnLaneRings = 4;
 __device__ __forceinline__ void write_knn_normal(uint32_t i, uint32_t orig, const float n[3],
                                                  float b, float4* __restrict__ nrm_grid,
                                                  float* __restrict__ cov6,
-                                                 float* __restrict__ normal3) {
+                                                 float* __restrict__ normal3,
+                                                 float4* __restrict__ nrm_orig = nullptr,
+                                                 float* __restrict__ tau_orig = nullptr,
+                                                 float tau = 0.0f) {
   nrm_grid[i] = make_float4(n[0], n[1], n[2], 1.0f);
+  if (nrm_orig) nrm_orig[orig] = make_float4(n[0], n[1], n[2], 1.0f);
+  if (tau_orig) tau_orig[orig] = tau;
   if (cov6) {
     float* o = cov6 + 6 * size_t(orig);
     o[0] = 1.0f - b * n[0] * n[0];
@@ -337,7 +342,11 @@ __global__ void __launch_bounds__(kBlock)
   float4* __restrict__ nrm_grid = C.nrm_grid;
   float* __restrict__ cov6 = C.cov6;
   float* __restrict__ normal3 = C.normal3;
-  const uint32_t i = (blockIdx.x - C.block_begin) * blockDim.x + threadIdx.x;
+  uint32_t i = (blockIdx.x - C.block_begin) * blockDim.x + threadIdx.x;
+  if (C.subset) {
+    if (i >= *C.subset_n) return;
+    i = C.subset[i];
+  }
   if (i >= g.npoints) return;
   const float4 p = gp[i];
   const int hx = cell_coord_free(p.x, g.ox, g.inv_h), hy = cell_coord_free(p.y, g.oy, g.inv_h),
@@ -426,7 +435,8 @@ __global__ void __launch_bounds__(kBlock)
   }
 
   if (found >= 2) smallest_eigvec(c00, c11, c22, c01, c02, c12, n);
-  write_knn_normal(i, __float_as_uint(p.w), n, one_minus_eps, nrm_grid, cov6, normal3);
+  write_knn_normal(i, __float_as_uint(p.w), n, one_minus_eps, nrm_grid, cov6, normal3, C.nrm_orig,
+                   C.tau_orig, tau);
 }
 
 // ---- far points: one wave per queued point --------------------------------
@@ -610,7 +620,8 @@ __global__ void __launch_bounds__(kBlock) k_knn_normals_far(const KnnCloud* __re
       smallest_eigvec(c00, c11, c22, c01, c02, c12, nv);
     }
     if (lane == 0)
-      write_knn_normal(i, __float_as_uint(p.w), nv, one_minus_eps, C.nrm_grid, C.cov6, C.normal3);
+      write_knn_normal(i, __float_as_uint(p.w), nv, one_minus_eps, C.nrm_grid, C.cov6, C.normal3,
+                       C.nrm_orig, C.tau_orig, tau);
   }
 }
 

@@ -578,17 +578,24 @@ __global__ void k_empty_pass(const uint8_t* __restrict__ in, uint8_t* __restrict
   if (c < g.ncells) empty_pass_cell(in, out, g, axis, c);
 }
 
+void chebyshev_field(uint8_t* field, const GridDesc& d, hipStream_t s) {
+  const uint32_t nc = d.ncells;
+  if (nc == 0) return;
+  DevBuf<uint8_t> tmp(nc);
+  const dim3 grid((nc + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(k_empty_pass, grid, dim3(kBlock), 0, s, field, tmp.p, d, 0);
+  hipLaunchKernelGGL(k_empty_pass, grid, dim3(kBlock), 0, s, tmp.p, field, d, 1);
+  hipLaunchKernelGGL(k_empty_pass, grid, dim3(kBlock), 0, s, field, tmp.p, d, 2);
+  SR_HIP(hipMemcpyAsync(field, tmp.p, nc, hipMemcpyDeviceToDevice, s));
+}
+
 void build_empty_radius(Grid& g, hipStream_t s) {
   const uint32_t nc = g.desc.ncells;
   g.empty_radius.resize(std::max<uint32_t>(nc, 1));
   if (nc == 0) return;
-  DevBuf<uint8_t> tmp(nc);
   const dim3 grid((nc + kBlock - 1) / kBlock);
   hipLaunchKernelGGL(k_empty_init, grid, dim3(kBlock), 0, s, g.cell_start.p, nc, g.empty_radius.p);
-  hipLaunchKernelGGL(k_empty_pass, grid, dim3(kBlock), 0, s, g.empty_radius.p, tmp.p, g.desc, 0);
-  hipLaunchKernelGGL(k_empty_pass, grid, dim3(kBlock), 0, s, tmp.p, g.empty_radius.p, g.desc, 1);
-  hipLaunchKernelGGL(k_empty_pass, grid, dim3(kBlock), 0, s, g.empty_radius.p, tmp.p, g.desc, 2);
-  SR_HIP(hipMemcpyAsync(g.empty_radius.p, tmp.p, nc, hipMemcpyDeviceToDevice, s));
+  chebyshev_field(g.empty_radius.p, g.desc, s);
 }
 
 // ---- many clouds in one set of launches -----------------------------------

@@ -188,6 +188,10 @@ constexpr int kEmptyCap = 14;
 
 // Fills g.empty_radius (Chebyshev distance transform of cell occupancy, capped).
 void build_empty_radius(Grid& g, hipStream_t s);
+// In place, over the cells of d: field[c] = min over cells c' of
+// max(Chebyshev distance(c, c'), field[c']), for a field of 0 (seed cells) and
+// kEmptyCap + 1 (the rest): the distance in cells to the nearest seed, capped.
+void chebyshev_field(uint8_t* field, const GridDesc& d, hipStream_t s);
 
 // Grids of many segments of one cloud (segment s = points [pt_off[s],
 // pt_off[s+1])), each the grid build_grid makes for the segment alone, in
@@ -246,6 +250,14 @@ struct KnnCloud {
   float* normal3;
   uint32_t block_begin;  // first workgroup of this cloud in the launch (set by the launcher)
   uint32_t pad_;
+  // Optional: only the grid positions subset[0 .. *subset_n) are computed
+  // (launched for up to g.npoints of them); nrm_orig / tau_orig, when set,
+  // receive float4(n, 1) and the k-th neighbour's squared distance in
+  // original point order.
+  const uint32_t* subset = nullptr;
+  const uint32_t* subset_n = nullptr;
+  float4* nrm_orig = nullptr;
+  float* tau_orig = nullptr;
 };
 KnnCloud knn_cloud(const Grid& g, float4* normals_grid_order, float* cov6, float* normal3);
 // knn_normals over many clouds in one launch (one K for all): a cloud's
@@ -337,6 +349,7 @@ struct Options {
   std::atomic<int64_t> radius_tiles{0};  // radius covariances / grid normals from LDS cell tiles
   std::atomic<int64_t> autotune{1};      // pick pass launch choices per problem class by timing
   std::atomic<int64_t> tune_force{-1};   // >= 0: use that candidate for every pass loop
+  std::atomic<int64_t> odometry_incremental{1};  // odometry map: recompute affected normals only
 };
 Options& options();
 
@@ -408,6 +421,7 @@ struct VoxelMap {
   DevBuf<uint64_t> keys;
   DevBuf<uint32_t> val, claim, flags, scal;  // scal: [new count, overflow flag]
   Cloud list;             // map points, list.n of them
+  uint64_t drops = 0;     // times old voxels were dropped (list indices renumbered)
 
   void init(float leaf, uint32_t capacity, hipStream_t s);
   // Transforms n scan points (sensor frame) by `pose` and adds the voxels the
@@ -416,5 +430,18 @@ struct VoxelMap {
   uint32_t insert(const float4* scan_local, uint32_t n, const MapPose& pose, hipStream_t s);
   void drop_oldest(uint32_t count, hipStream_t s);
 };
+
+// Grid positions of g (the grid of the map list's points [0, npoints)) whose
+// k-NN normal can differ from the one computed before the points [n_old,
+// npoints) were appended: the new points, and every point whose k-th
+// neighbour's squared distance ltau (list order, from that computation) is not
+// below the squared lower bound on its distance to the new points that the
+// cell distance to the nearest new point's cell gives. Every other point keeps
+// its k nearest, and visits them in the same order when g has the same
+// geometry as before. Writes subset[0 .. *count) (device).
+void map_refresh_set(const Grid& g, const float4* list, uint32_t n_old, const float* ltau,
+                     uint32_t* subset, uint32_t* count, hipStream_t s);
+// nrm_grid[i] = lnrm[g.order[i]] for every grid position.
+void gather_list_normals(const Grid& g, const float4* lnrm, float4* nrm_grid, hipStream_t s);
 
 }  // namespace sr

@@ -136,7 +136,89 @@ void fill_u64(unsigned long long* p, unsigned long long v, size_t n, hipStream_t
                      0, s, p, v, n);
 }
 
+// ---- incremental upkeep of the map's grid and normals ----------------------
+
+// field[cell] = 0 for the cells of the list points [n_old, n) (the new voxels).
+__global__ void k_mark_new_cells(const float4* __restrict__ list, uint32_t n_old, uint32_t n,
+                                 GridDesc g, uint8_t* __restrict__ field) {
+  const uint32_t i = n_old + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 p = list[i];
+  const int cx = cell_coord(p.x, g.ox, g.inv_h, g.dx), cy = cell_coord(p.y, g.oy, g.inv_h, g.dy),
+            cz = cell_coord(p.z, g.oz, g.inv_h, g.dz);
+  field[(uint32_t(cz) * uint32_t(g.dy) + uint32_t(cy)) * uint32_t(g.dx) + uint32_t(cx)] = 0;
+}
+
+// Appends the grid positions whose normal must be computed again. A point in
+// a cell at Chebyshev distance m >= 1 (cells) from every new point's cell lies
+// more than (m - 1) h - 2 margin from each new point (cells are filed with at
+// most `margin` of rounding; m is capped, and the cap is a lower bound); its k
+// nearest stay when its k-th neighbour is closer than that (2 more margins
+// cover the rounding of the squared distances).
+__global__ void k_map_affected(const float4* __restrict__ gp, uint32_t n, GridDesc g,
+                               const uint8_t* __restrict__ field, const float* __restrict__ ltau,
+                               uint32_t n_old, uint32_t* __restrict__ out,
+                               uint32_t* __restrict__ count) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool aff = false;
+  if (i < n) {
+    const float4 p = gp[i];
+    const uint32_t orig = __float_as_uint(p.w);
+    if (orig >= n_old) {
+      aff = true;
+    } else {
+      const int cx = cell_coord(p.x, g.ox, g.inv_h, g.dx), cy = cell_coord(p.y, g.oy, g.inv_h, g.dy),
+                cz = cell_coord(p.z, g.oz, g.inv_h, g.dz);
+      const int m = field[(uint32_t(cz) * uint32_t(g.dy) + uint32_t(cy)) * uint32_t(g.dx) + uint32_t(cx)];
+      const float lim = float(m - 1) * g.h - 4.0f * g.margin;
+      aff = !(lim > 0.0f && ltau[orig] < lim * lim);
+    }
+  }
+  const unsigned long long mask = __ballot(aff);
+  if (!mask) return;
+  const int lane = int(threadIdx.x & 63u);
+  const int leader = __ffsll((long long)mask) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(count, uint32_t(__popcll(mask)));
+  base = __shfl(base, leader);
+  if (aff) out[base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)))] = i;
+}
+
+// nrm_grid[i] = lnrm[order[i]]: list-order normals into grid order.
+__global__ void k_gather_normals(const uint32_t* __restrict__ order, const float4* __restrict__ lnrm,
+                                 uint32_t n, float4* __restrict__ nrm_grid) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) nrm_grid[i] = lnrm[order[i]];
+}
+
 }  // namespace
+
+void map_refresh_set(const Grid& g, const float4* list, uint32_t n_old, const float* ltau,
+                     uint32_t* subset, uint32_t* count, hipStream_t s) {
+  const GridDesc& d = g.desc;
+  const uint32_t n = d.npoints;
+  fill_u32(count, 0u, 1, s);
+  if (n == 0) return;
+  const size_t words = (size_t(d.ncells) + 3) / 4;
+  DevBuf<uint8_t> field(4 * words);
+  static_assert(kEmptyCap + 1 < 256, "distance field is one byte per cell");
+  fill_u32(reinterpret_cast<uint32_t*>(field.p), 0x01010101u * uint32_t(kEmptyCap + 1), words, s);
+  if (n > n_old) {
+    hipLaunchKernelGGL(k_mark_new_cells, dim3(blocks_for(n - n_old)), dim3(kBlk), 0, s, list, n_old,
+                       n, d, field.p);
+    chebyshev_field(field.p, d, s);
+  }
+  hipLaunchKernelGGL(k_map_affected, dim3(blocks_for(n)), dim3(kBlk), 0, s, g.pts.p, n, d, field.p,
+                     ltau, n_old, subset, count);
+  // the field returns to the cache; ordering on s keeps its reuse safe
+}
+
+void gather_list_normals(const Grid& g, const float4* lnrm, float4* nrm_grid, hipStream_t s) {
+  const uint32_t n = g.desc.npoints;
+  if (n)
+    hipLaunchKernelGGL(k_gather_normals, dim3(blocks_for(n)), dim3(kBlk), 0, s, g.order.p, lnrm, n,
+                       nrm_grid);
+}
 
 void VoxelMap::init(float leaf_, uint32_t capacity_, hipStream_t s) {
   leaf = leaf_;
@@ -160,6 +242,7 @@ void VoxelMap::init(float leaf_, uint32_t capacity_, hipStream_t s) {
 
 void VoxelMap::drop_oldest(uint32_t count, hipStream_t s) {
   if (count == 0) return;
+  ++drops;
   const uint32_t keep = list.n > count ? list.n - count : 0;
   if (keep) {
     DevBuf<float4> tmp(keep);

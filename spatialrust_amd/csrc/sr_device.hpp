@@ -64,8 +64,13 @@ __device__ __forceinline__ uint32_t seg_of(const uint32_t* __restrict__ begins, 
 }
 
 // Order-preserving map of f32 onto u32 so atomicMin/atomicMax work on floats.
-__device__ __forceinline__ uint32_t f2ord(float f) {
+__host__ __device__ __forceinline__ uint32_t f2ord(float f) {
+#ifdef __HIP_DEVICE_COMPILE__
   uint32_t u = __float_as_uint(f);
+#else
+  uint32_t u;
+  __builtin_memcpy(&u, &f, 4);
+#endif
   return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 __host__ __device__ __forceinline__ float ord2f(uint32_t u) {

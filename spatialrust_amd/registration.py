@@ -497,6 +497,14 @@ class RollingMapOdometry:
     def map_points(self) -> int:
         return int(_n.load().sr_odometry_map_points(self._h))
 
+    def map_stats(self) -> dict:
+        """Map upkeep counters: map points, normals computed by the last
+        push, full grid rebuilds so far, scans pushed."""
+        out = (ctypes.c_uint64 * 4)()
+        _check(_n.load().sr_odometry_map_stats(self._h, out))
+        return {"map_points": int(out[0]), "refreshed": int(out[1]),
+                "rebuilds": int(out[2]), "scans": int(out[3])}
+
     def map_cloud(self) -> np.ndarray:
         n = self.map_points()
         out = np.zeros((max(n, 1), 3), np.float32)

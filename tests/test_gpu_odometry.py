@@ -135,3 +135,38 @@ def test_odometry_drift(hip):
     rot = synth.rotation_angle_between(res[-1].rotation64, q_gt)
     trans = float(np.linalg.norm(res[-1].translation64 - t_gt))
     assert trans < 0.1 and rot < np.deg2rad(0.5), (trans, rot)
+
+
+@pytest.mark.parametrize("capacity", [4_000_000, 9000])
+def test_odometry_incremental_map_matches_full_recompute(hip, capacity):
+    """The map keeps its grid geometry between full rebuilds and recomputes
+    only the normals map_refresh_set names; every other normal then equals a
+    full recompute on that geometry, so the poses are bit-identical with the
+    incremental upkeep on and off (the capacity case drops old voxels, which
+    forces full rebuilds)."""
+    scans, _ = _scans(30)
+    runs = []
+    for inc in (1, 0):
+        sr.set_option("odometry_incremental", inc)
+        try:
+            odo = sr.RollingMapOdometry(_cfg(), voxel_leaf=LEAF, max_map_points=capacity)
+            res, counts = [], []
+            for s in scans:
+                res.append(odo.push(s))
+                st = odo.map_stats()
+                counts.append((st["refreshed"], st["map_points"]))
+            stats = odo.map_stats()
+            odo.close()
+        finally:
+            sr.set_option("odometry_incremental", 1)
+        runs.append((res, counts, stats))
+    (a, cnt_a, st_a), (b, cnt_b, st_b) = runs
+    for k, (x, y) in enumerate(zip(a, b)):
+        assert np.array_equal(x.rotation64, y.rotation64), k
+        assert np.array_equal(x.translation64, y.translation64), k
+        assert (x.fitness, x.iterations, x.correspondences) == (y.fitness, y.iterations, y.correspondences), k
+    assert st_a["rebuilds"] == st_b["rebuilds"] and st_a["scans"] == 30
+    assert all(r == n for r, n in cnt_b)
+    if capacity == 4_000_000:
+        assert st_a["rebuilds"] < 30
+        assert any(r < n for r, n in cnt_a), cnt_a
