@@ -14,6 +14,12 @@ pass cannot show (gpu).
                           exactly zero row
   radius covariances      pose vs the FP64 oracle fed the oracle's radius
                           covariances: <= 1e-4 rad and <= 1 mm
+  serving workload (the three smallest of its 32 maps, 0.50M-0.54M points, all
+    their 24 sources of 2k-200k points; leaf 0.2, k 15, d_max 2.0, 40
+    iterations or a pose update below 1e-6)
+                          served while others iterate == alone, bit for bit;
+                          the golden requests vs the FP64 oracle at <= 1e-4
+                          rad, <= 1 mm
 """
 import ctypes
 import json
@@ -301,3 +307,43 @@ def test_autotune_candidates_bit_identical(hip, tune_restored):
         for c in srcs + tgts:
             c.close()
     print(sr.tune_describe(), flush=True)
+
+
+def test_serving_workload_maps_served_equal_alone_and_fp64_golden(hip):
+    pool = synth.serving_pool()
+    ms = [int(m) for m in np.argsort(pool.map_sizes, kind="stable")[:3]]
+    cfg = sr.GicpConfig(voxel_leaf=0.2, k_neighbors=15, max_correspondence_distance=2.0,
+                        max_iterations=40, transformation_epsilon=1e-6, fitness_epsilon=0.0)
+    maps = {m: sr.TargetMap(synth.serving_map(pool, m), cfg) for m in ms}
+    reqs = [(m, j) for m in ms for j in range(len(pool.source_sizes[m]))]
+    srcs = {mj: synth.serving_source(pool, *mj) for mj in reqs}
+    order = np.random.default_rng(7).permutation(len(reqs))
+    server = sr.Server(cfg, hbm_budget_bytes=8 << 30, max_active=256, max_iterations=40,
+                       delta_pose_epsilon=1e-6)
+    tickets = {}
+    for k, r in enumerate(order):  # arrivals spread over the steps
+        mj = reqs[r]
+        tickets[server.submit(maps[mj[0]], srcs[mj].source)] = mj
+        if k % 3 == 2:
+            server.step()
+    steps = 0
+    while server.step():
+        steps += 1
+        assert steps < 2000
+    assert sorted(server.finished()) == sorted(tickets)
+    gold = json.load(open(os.path.join(GOLDEN, "serving_poses.json")))["requests"]
+    checked = 0
+    for t, mj in tickets.items():
+        got = server.poll(t)
+        alone = maps[mj[0]].align(srcs[mj].source, cfg)
+        assert _same(got, alone), mj
+        g = gold.get(f"{mj[0]}_{mj[1]}")
+        if g is not None:
+            rot, trans = _pose_err(got, g["q"], g["t"])
+            assert rot <= 1e-4 and trans <= 1e-3, (mj, rot, trans)
+            assert abs(got.iterations - g["iterations"]) <= 1, (mj, got.iterations)
+            checked += 1
+    assert checked == len(gold)
+    server.close()
+    for m in maps.values():
+        m.close()
