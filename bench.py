@@ -378,18 +378,8 @@ ODO_SCANS = 250
 def _gen_scans(n):
     """All scans of the odometry leg, ray-cast in FP64 torch on the GPU when
     torch sees one (this runs in a child process), else on the CPU."""
-    import torch
     from spatialrust_amd import synth
-    dev = "cuda" if torch.cuda.is_available() else "cpu"
-    torch.set_num_threads(min(8, os.cpu_count() or 1))
-    scene = synth.odometry_scene(synth.SEED_BASE + 4000)
-    poses = synth.odometry_trajectory(ODO_SCANS, seed=1)[:n]
-    out = []
-    for k, (q, t) in enumerate(poses):
-        out.append(synth.lidar_scan(scene, q, t, seed=100 + k, device=dev))
-        if (k + 1) % 25 == 0:
-            print(f"  scans ray-cast: {k + 1}/{n} ({dev})", file=sys.stderr, flush=True)
-    return out
+    return synth.odometry_scans(n, ODO_SCANS, progress=True)
 
 
 def _odometry(n_scans):

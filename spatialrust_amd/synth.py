@@ -330,6 +330,28 @@ def lidar_scan(scene: Scene, q, t, seed: int, beams: int = 64, azimuth_bins: int
 
 # ------------------------------------------------------------ rigid motion --
 
+def odometry_scans(n: int, trajectory_scans: int = 0, device: str = "", progress: bool = False):
+    """The first n full-size scans (64 x 2048 beams, seeds 100 + k) of the
+    odometry workload along odometry_trajectory(max(n, trajectory_scans),
+    seed=1) through odometry_scene(SEED_BASE + 4000). The ray casting runs on
+    `device` ("" picks cuda when torch sees a GPU, else cpu). torch must own
+    the process's first HIP initialisation: run this in a fresh process (e.g.
+    a spawned worker) when another HIP library is already loaded."""
+    import sys
+    import torch
+    dev = device or ("cuda" if torch.cuda.is_available() else "cpu")
+    if dev == "cpu":
+        torch.set_num_threads(min(8, torch.get_num_threads()))
+    scene = odometry_scene(SEED_BASE + 4000)
+    poses = odometry_trajectory(max(n, trajectory_scans), seed=1)[:n]
+    out = []
+    for k, (q, t) in enumerate(poses):
+        out.append(lidar_scan(scene, q, t, seed=100 + k, device=dev))
+        if progress and (k + 1) % 25 == 0:
+            print(f"  scans ray-cast: {k + 1}/{n} ({dev})", file=sys.stderr, flush=True)
+    return out
+
+
 def quat_axis_angle(axis, angle):
     axis = np.asarray(axis, np.float64)
     axis = axis / np.linalg.norm(axis)

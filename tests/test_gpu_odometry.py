@@ -8,6 +8,10 @@ Bars:
                                  the poses the device returned (voxel filter by the oracle)
   odometry                       deterministic (identical poses on a re-run); drift vs the
                                  ground-truth trajectory below 0.1 m / 0.5 deg over 30 m
+  incremental map upkeep         poses bit-identical to recomputing every normal
+  full scan size (64 x 2048 beams, ~118k points per scan, 40 scans, 4M-voxel map)
+                                 map contents bit-exact vs the insertion rule,
+                                 incremental == full recompute, drift below 0.1 m / 0.5 deg
 """
 import numpy as np
 import pytest
@@ -170,3 +174,36 @@ def test_odometry_incremental_map_matches_full_recompute(hip, capacity):
     if capacity == 4_000_000:
         assert st_a["rebuilds"] < 30
         assert any(r < n for r, n in cnt_a), cnt_a
+
+
+def test_odometry_full_scan_size(hip, orc):
+    import concurrent.futures
+    import multiprocessing
+    n = 40
+    # ray casting in a spawned process: torch there is the first to initialise HIP
+    with concurrent.futures.ProcessPoolExecutor(
+            max_workers=1, mp_context=multiprocessing.get_context("spawn")) as ex:
+        scans = ex.submit(synth.odometry_scans, n).result()
+    poses = synth.odometry_trajectory(n, seed=1)
+    assert min(len(s) for s in scans) > 100_000
+    runs = []
+    for inc in (1, 0):
+        sr.set_option("odometry_incremental", inc)
+        try:
+            odo = sr.RollingMapOdometry(_cfg(), voxel_leaf=LEAF, max_map_points=4_000_000)
+            res = [odo.push(s) for s in scans]
+            runs.append((res, odo.map_cloud(), odo.map_stats()))
+            odo.close()
+        finally:
+            sr.set_option("odometry_incremental", 1)
+    (res, cloud, st), (res0, cloud0, _) = runs
+    for k, (a, b) in enumerate(zip(res, res0)):
+        assert np.array_equal(a.rotation64, b.rotation64) and np.array_equal(a.translation64, b.translation64), k
+    assert np.array_equal(cloud, cloud0)
+    assert st["rebuilds"] < n
+    want = _expected_map(orc, scans, res, LEAF, 4_000_000)
+    assert np.array_equal(cloud, want)
+    q_gt, t_gt = _rel(poses[0], poses[-1])
+    rot = synth.rotation_angle_between(res[-1].rotation64, q_gt)
+    trans = float(np.linalg.norm(res[-1].translation64 - t_gt))
+    assert trans < 0.1 and rot < np.deg2rad(0.5), (trans, rot)
