@@ -372,7 +372,8 @@ def _radius_leg(w, reps=5):
             "identical": bool(all(np.array_equal(a, b) for a, b in zip(res[1], res[0])))}
 
 
-ODO_SCANS = 250
+ODO_SCANS = 2000
+ODO_STREAMS = 8
 
 
 def _gen_scans(n):
@@ -380,6 +381,44 @@ def _gen_scans(n):
     torch sees one (this runs in a child process), else on the CPU."""
     from spatialrust_amd import synth
     return synth.odometry_scans(n, ODO_SCANS, progress=True)
+
+
+def _odometry_streams(sr, cfg, scans, streams):
+    """The scans cut into `streams` consecutive segments, each its own
+    odometry (its own map): pushed one stream after another with
+    sr_odometry_push, then all streams in lockstep with sr_odometry_push_many
+    (one batched registration per step). Rates are scans over wall time."""
+    seg = len(scans) // streams
+    parts = [scans[j * seg:(j + 1) * seg] for j in range(streams)]
+    alone = []
+    t0 = time.perf_counter()
+    for part in parts:
+        odo = sr.RollingMapOdometry(cfg, voxel_leaf=0.25, max_map_points=4_000_000)
+        alone.append([odo.push(s) for s in part])
+        odo.close()
+    t_alone = time.perf_counter() - t0
+    print(f"  {streams} streams one after another: {t_alone:.2f} s", file=sys.stderr, flush=True)
+    odos = [sr.RollingMapOdometry(cfg, voxel_leaf=0.25, max_map_points=4_000_000) for _ in parts]
+    got = [[] for _ in parts]
+    step_ms = []
+    t0 = time.perf_counter()
+    for k in range(seg):
+        t1 = time.perf_counter()
+        for j, r in enumerate(sr.RollingMapOdometry.push_many(odos, [p[k] for p in parts])):
+            got[j].append(r)
+        step_ms.append((time.perf_counter() - t1) * 1e3)
+    t_many = time.perf_counter() - t0
+    for o in odos:
+        o.close()
+    same = all(np.array_equal(a.rotation64, b.rotation64) and np.array_equal(a.translation64, b.translation64)
+               for ra, rb in zip(alone, got) for a, b in zip(ra, rb))
+    step = np.array(step_ms[1:])
+    return {"streams": streams, "scans_per_stream": seg,
+            "sequential_scans_per_s": streams * seg / t_alone,
+            "batched_scans_per_s": streams * seg / t_many,
+            "batched_over_sequential": t_alone / t_many,
+            "batched_step_ms": {"p50": float(np.percentile(step, 50)), "p99": float(np.percentile(step, 99))},
+            "identical_poses": bool(same)}
 
 
 def _odometry(n_scans):
@@ -391,7 +430,7 @@ def _odometry(n_scans):
     device-resident voxel map (4M voxels at most), which it then extends.
     Per-scan latency is the wall time of sr_odometry_push (host scan in, pose
     out); drift is the estimated pose relative to the first scan against the
-    ground truth."""
+    ground truth. The same scans then run as ODO_STREAMS streams, batched."""
     import concurrent.futures
     import ctypes
     import multiprocessing
@@ -415,7 +454,7 @@ def _odometry(n_scans):
         lat.append(time.perf_counter() - t1)
         st = odo.map_stats()
         refreshed.append(st["refreshed"] / max(st["map_points"], 1))
-        if (k + 1) % 50 == 0:
+        if (k + 1) % 250 == 0:
             print(f"  scans registered: {k + 1}/{n_scans}, map {odo.map_points()} voxels",
                   file=sys.stderr, flush=True)
     lib.sr_profile_enable(0)
@@ -452,6 +491,7 @@ def _odometry(n_scans):
                            "normals_recomputed_frac_p50": float(np.median(refreshed[1:]))},
             "drift": {"path_m": path, "final_trans_m": terr[-1], "max_trans_m": max(terr),
                       "final_rot_rad": rerr[-1], "max_rot_rad": max(rerr)},
+            "batched_streams": _odometry_streams(sr, cfg, scans, ODO_STREAMS) if n_scans >= 2 * ODO_STREAMS else None,
             "generate_s": t_gen}
 
 

@@ -382,6 +382,16 @@ void sr_odometry_destroy(sr_odometry* odometry);
  * map_T_scan; source_points = scan voxels, target_points = map voxels after
  * the merge. */
 int32_t sr_odometry_push(sr_odometry* odometry, const sr_xyz_view* scan, sr_result* out);
+/* sr_odometry_push for one scan of each of `count` distinct odometries (one
+ * stream each, e.g. one vehicle each): the scans are filtered and prepared
+ * concurrently, registered in one batch (a set of pass launches over all of
+ * them) and merged into their maps concurrently. out[i] (optional) is what
+ * sr_odometry_push(odometries[i], &scans[i], ...) would return, bit for bit.
+ * A registration error fails the call, naming the scan, before any map
+ * changes. No counterpart in the reference (ScanMatcher matches one scan per
+ * call, crates/spatialrust-mapping/src/scan.rs:12-20). */
+int32_t sr_odometry_push_many(sr_odometry* const* odometries, const sr_xyz_view* scans,
+                              uint32_t count, sr_result* out);
 uint64_t sr_odometry_map_points(const sr_odometry* odometry);
 /* Map upkeep counters (no counterpart in the reference): out4 = {map points,
  * normals computed by the last push, full grid rebuilds so far, scans}. The

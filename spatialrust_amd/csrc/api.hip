@@ -2362,6 +2362,84 @@ static void odometry_update_map(sr_odometry* o, uint32_t n_old, Receipt& rc, hip
   o->map_prep = std::move(fresh);
 }
 
+// One scan on its way through an odometry push.
+struct OdoScan {
+  VoxelOut vox;
+  Prepared src;
+  Pose64 pose;  // map_T_scan (the first scan: identity)
+  sr_result r{};
+};
+
+static void odometry_check_scan(const sr_xyz_view* scan) {
+  validate_view(scan);
+  if (scan->n == 0) fail(SR_ERR_INVALID_ARGUMENT, "GICP requires non-empty source and target point clouds");
+  if (scan->n > 0xFFFFFFF0ull) fail(SR_ERR_INVALID_ARGUMENT, "point clouds above 4G points are not supported");
+}
+
+// Voxel filter of the scan and, from the second scan on, its grid and normals.
+static void odometry_prepare(sr_odometry* o, const sr_xyz_view& scan, OdoScan& sc, hipStream_t s) {
+  Receipt rc;
+  Cloud raw;
+  pack_cloud(scan.x, scan.y, scan.z, uint32_t(scan.n), scan.mem == SR_MEM_DEVICE, raw, s, &rc.h2d);
+  voxel_downsample(raw, o->cfg.voxel_leaf, nullptr, false, sc.vox, s);
+  std::memset(&sc.r, 0, sizeof(sc.r));
+  if (o->scans > 0) prepare_cloud(sc.vox.centroids, o->gicp, sc.src, rc, s, false, o->cfg.voxel_leaf);
+}
+
+// Registers each scan against its odometry's map in one batch, from the
+// constant-velocity guess (last pose composed with the last scan-to-scan
+// motion). Fails, naming the scan, before any map changes.
+static void odometry_register(const std::vector<sr_odometry*>& o, const std::vector<OdoScan*>& sc,
+                              Batch& B, hipStream_t s) {
+  const size_t n = o.size();
+  std::vector<const Prepared*> src(n), tgt(n);
+  std::vector<sr_gicp_config> cfgs(n);
+  std::vector<double> pose7(7 * n);
+  for (size_t i = 0; i < n; ++i) {
+    const Pose64 guess = compose64(o[i]->pose, compose64(inverse64(o[i]->prev), o[i]->pose));
+    for (int a = 0; a < 4; ++a) pose7[7 * i + a] = guess.q[a];
+    for (int a = 0; a < 3; ++a) pose7[7 * i + 4 + a] = guess.t[a];
+    src[i] = &sc[i]->src;
+    tgt[i] = &o[i]->map_prep;
+    cfgs[i] = o[i]->gicp;
+  }
+  batch_setup(src, tgt, cfgs, B, s, pose7.data());
+  batch_run(B, s);
+  std::vector<InstState> st(n);
+  SR_HIP(hipMemcpyAsync(st.data(), B.states.p, sizeof(InstState) * n, hipMemcpyDeviceToHost, s));
+  SR_HIP(hipStreamSynchronize(s));
+  for (size_t i = 0; i < n; ++i) {
+    std::string msg;
+    const int32_t code = state_to_result(st[i], o[i]->gicp, sc[i]->src, o[i]->map_prep, &sc[i]->r, &msg);
+    if (code) fail(code, "scan " + std::to_string(o[i]->scans) + ": " + msg);
+    for (int a = 0; a < 4; ++a) sc[i]->pose.q[a] = st[i].q[a];
+    for (int a = 0; a < 3; ++a) sc[i]->pose.t[a] = st[i].t[a];
+  }
+}
+
+// Merges the registered scan into the map and advances the odometry.
+static void odometry_commit(sr_odometry* o, OdoScan& sc, hipStream_t s) {
+  sr_result& r = sc.r;
+  if (o->scans == 0) {
+    r.q[3] = 1.0f;
+    r.q64[3] = 1.0;
+    r.converged = 1;
+    r.source_points = sc.vox.cells;
+  }
+  MapPose mp;
+  quat_mat64(sc.pose.q, mp.r);
+  for (int a = 0; a < 3; ++a) mp.t[a] = sc.pose.t[a];
+  Receipt rc;
+  const uint64_t drops = o->map.drops;
+  const uint32_t n_old = o->map.list.n;
+  o->map.insert(sc.vox.centroids.pts.p, sc.vox.cells, mp, s);
+  odometry_update_map(o, o->map.drops == drops ? n_old : 0u, rc, s);
+  o->prev = o->scans > 0 ? o->pose : sc.pose;
+  o->pose = sc.pose;
+  ++o->scans;
+  r.target_points = o->map.list.n;
+}
+
 extern "C" {
 
 int32_t sr_odometry_create(const sr_odometry_config* config, const sr_gicp_config* gicp,
@@ -2395,56 +2473,53 @@ void sr_odometry_destroy(sr_odometry* odo) {
 int32_t sr_odometry_push(sr_odometry* odo, const sr_xyz_view* scan, sr_result* out) {
   return guarded([&] {
     if (!odo) fail(SR_ERR_INVALID_ARGUMENT, "odometry must not be null");
-    validate_view(scan);
-    if (scan->n == 0) fail(SR_ERR_INVALID_ARGUMENT, "GICP requires non-empty source and target point clouds");
-    if (scan->n > 0xFFFFFFF0ull) fail(SR_ERR_INVALID_ARGUMENT, "point clouds above 4G points are not supported");
+    odometry_check_scan(scan);
     hipStream_t s = thread_stream();
-    Receipt rc;
-    Cloud raw;
-    pack_cloud(scan->x, scan->y, scan->z, uint32_t(scan->n), scan->mem == SR_MEM_DEVICE, raw, s, &rc.h2d);
-    VoxelOut vox;
-    voxel_downsample(raw, odo->cfg.voxel_leaf, nullptr, false, vox, s);
-    sr_result r;
-    std::memset(&r, 0, sizeof(r));
-    Pose64 pose;  // map_T_scan; the first scan defines the map frame
+    OdoScan sc;
+    odometry_prepare(odo, *scan, sc, s);
     if (odo->scans > 0) {
-      // constant velocity: last pose composed with the last scan-to-scan motion
-      const Pose64 guess = compose64(odo->pose, compose64(inverse64(odo->prev), odo->pose));
-      Prepared src;
-      prepare_cloud(vox.centroids, odo->gicp, src, rc, s, false, odo->cfg.voxel_leaf);
-      double pose7[7] = {guess.q[0], guess.q[1], guess.q[2], guess.q[3],
-                         guess.t[0], guess.t[1], guess.t[2]};
       Batch B;
       ProfScope prof("odometry_register", s);
-      batch_setup({&src}, {&odo->map_prep}, {odo->gicp}, B, s, pose7);
-      batch_run(B, s);
-      InstState st;
-      SR_HIP(hipMemcpyAsync(&st, B.states.p, sizeof(st), hipMemcpyDeviceToHost, s));
-      SR_HIP(hipStreamSynchronize(s));
-      std::string msg;
-      const int32_t code = state_to_result(st, odo->gicp, src, odo->map_prep, &r, &msg);
-      if (code) fail(code, "scan " + std::to_string(odo->scans) + ": " + msg);
-      for (int a = 0; a < 4; ++a) pose.q[a] = st.q[a];
-      for (int a = 0; a < 3; ++a) pose.t[a] = st.t[a];
-    } else {
-      r.q[3] = 1.0f;
-      r.q64[3] = 1.0;
-      r.converged = 1;
-      r.source_points = vox.cells;
+      odometry_register({odo}, {&sc}, B, s);
     }
-    MapPose mp;
-    quat_mat64(pose.q, mp.r);
-    for (int a = 0; a < 3; ++a) mp.t[a] = pose.t[a];
     ProfScope prof_map("odometry_map", s);
-    const uint64_t drops = odo->map.drops;
-    const uint32_t n_old = odo->map.list.n;
-    odo->map.insert(vox.centroids.pts.p, vox.cells, mp, s);
-    odometry_update_map(odo, odo->map.drops == drops ? n_old : 0u, rc, s);
-    odo->prev = odo->scans > 0 ? odo->pose : pose;
-    odo->pose = pose;
-    ++odo->scans;
-    r.target_points = odo->map.list.n;
-    if (out) *out = r;
+    odometry_commit(odo, sc, s);
+    if (out) *out = sc.r;
+  });
+}
+
+int32_t sr_odometry_push_many(sr_odometry* const* odos, const sr_xyz_view* scans, uint32_t count,
+                              sr_result* out) {
+  return guarded([&] {
+    if (count == 0) return;
+    if (!odos || !scans) fail(SR_ERR_INVALID_ARGUMENT, "odometry arguments must not be null");
+    for (uint32_t i = 0; i < count; ++i) {
+      if (!odos[i]) fail(SR_ERR_INVALID_ARGUMENT, "odometry must not be null");
+      for (uint32_t j = 0; j < i; ++j)
+        if (odos[j] == odos[i]) fail(SR_ERR_INVALID_ARGUMENT, "an odometry appears twice in one call");
+      odometry_check_scan(&scans[i]);
+    }
+    ensure_device();
+    std::vector<OdoScan> sc(count);
+    // scans filtered and prepared on the workers, registered in one batch,
+    // merged into their maps on the workers
+    Workers::get().run(count, [&](size_t i, hipStream_t s) { odometry_prepare(odos[i], scans[i], sc[i], s); });
+    std::vector<sr_odometry*> reg;
+    std::vector<OdoScan*> reg_sc;
+    for (uint32_t i = 0; i < count; ++i)
+      if (odos[i]->scans > 0) {
+        reg.push_back(odos[i]);
+        reg_sc.push_back(&sc[i]);
+      }
+    if (!reg.empty()) {
+      hipStream_t s = thread_stream();
+      Batch B;
+      ProfScope prof("odometry_register", s);
+      odometry_register(reg, reg_sc, B, s);
+    }
+    Workers::get().run(count, [&](size_t i, hipStream_t s) { odometry_commit(odos[i], sc[i], s); });
+    if (out)
+      for (uint32_t i = 0; i < count; ++i) out[i] = sc[i].r;
   });
 }
 

@@ -133,6 +133,8 @@ SIGNATURES = {
                                   ctypes.POINTER(_P)]),
     "sr_odometry_destroy": (None, [_P]),
     "sr_odometry_push": (_I32, [_P, ctypes.POINTER(XyzView), ctypes.POINTER(ResultC)]),
+    "sr_odometry_push_many": (_I32, [_P, ctypes.POINTER(XyzView), ctypes.c_uint32,
+                                     ctypes.POINTER(ResultC)]),
     "sr_odometry_map_points": (ctypes.c_uint64, [_P]),
     "sr_odometry_map_stats": (_I32, [_P, ctypes.POINTER(ctypes.c_uint64)]),
     "sr_odometry_map_read": (_I32, [_P, _P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),

@@ -497,6 +497,22 @@ class RollingMapOdometry:
     def map_points(self) -> int:
         return int(_n.load().sr_odometry_map_points(self._h))
 
+    @staticmethod
+    def push_many(odometries, scans) -> list:
+        """One scan for each of several distinct odometries, registered in one
+        batch (sr_odometry_push_many); the results equal push() on each."""
+        if len(odometries) != len(scans):
+            raise InvalidArgument("one scan per odometry")
+        n = len(odometries)
+        if n == 0:
+            return []
+        views = [_view(s) for s in scans]
+        arr = (_n.XyzView * n)(*[v for v, _ in views])
+        hs = (ctypes.c_void_p * n)(*[o._h.value for o in odometries])
+        out = (_n.ResultC * n)()
+        _check(_n.load().sr_odometry_push_many(hs, arr, n, out))
+        return [RegistrationResult._from_c(out[i]) for i in range(n)]
+
     def map_stats(self) -> dict:
         """Map upkeep counters: map points, normals computed by the last
         push, full grid rebuilds so far, scans pushed."""

@@ -207,3 +207,31 @@ def test_odometry_full_scan_size(hip, orc):
     rot = synth.rotation_angle_between(res[-1].rotation64, q_gt)
     trans = float(np.linalg.norm(res[-1].translation64 - t_gt))
     assert trans < 0.1 and rot < np.deg2rad(0.5), (trans, rot)
+
+
+def test_odometry_push_many_equals_push(hip):
+    """Several odometry streams pushed together (one batched registration per
+    step) give each stream the poses and map that pushing it alone gives."""
+    scans, _ = _scans(30)
+    streams = [scans[0:10], scans[10:20], scans[20:30]]
+    alone = []
+    for st in streams:
+        odo = sr.RollingMapOdometry(_cfg(), voxel_leaf=LEAF)
+        alone.append(([odo.push(s) for s in st], odo.map_cloud()))
+        odo.close()
+    odos = [sr.RollingMapOdometry(_cfg(), voxel_leaf=LEAF) for _ in streams]
+    got = [[] for _ in streams]
+    for k in range(10):
+        for j, r in enumerate(sr.RollingMapOdometry.push_many(odos, [st[k] for st in streams])):
+            got[j].append(r)
+    for j, (res, cloud) in enumerate(alone):
+        for k, (a, b) in enumerate(zip(res, got[j])):
+            assert np.array_equal(a.rotation64, b.rotation64), (j, k)
+            assert np.array_equal(a.translation64, b.translation64), (j, k)
+            assert (a.fitness, a.iterations, a.source_points, a.target_points) == \
+                (b.fitness, b.iterations, b.source_points, b.target_points), (j, k)
+        assert np.array_equal(cloud, odos[j].map_cloud()), j
+    with pytest.raises(sr.InvalidArgument, match="twice"):
+        sr.RollingMapOdometry.push_many([odos[0], odos[0]], [streams[0][0], streams[0][1]])
+    for o in odos:
+        o.close()
