@@ -548,7 +548,8 @@ static std::shared_ptr<SegHolder> prepare_segmented(const std::vector<const sr_x
   {
     DevBuf<uint32_t> b(6 * size_t(nseg));
     seg_bounds(vox.centroids.pts.p, voff, b.p, s);
-    SR_HIP(hipMemcpy(hb.data(), b.p, sizeof(uint32_t) * hb.size(), hipMemcpyDeviceToHost));
+    SR_HIP(hipMemcpyAsync(hb.data(), b.p, sizeof(uint32_t) * hb.size(), hipMemcpyDeviceToHost, s));
+    SR_HIP(hipStreamSynchronize(s));
   }
   std::vector<float> h;
   choose_cell_edges_many(vox.centroids, voff, hb, knn_occupancy(cfg.k_neighbors), cfg.voxel_leaf,
@@ -1760,8 +1761,16 @@ int32_t sr_gicp_align(const sr_xyz_view* source, const sr_xyz_view* target,
     hipStream_t s = thread_stream();
     Receipt rc;
     Prepared tgt, src;
-    prepare(*target, *config, tgt, rc, s, true);
-    prepare(*source, *config, src, rc, s, false);
+    // device views with a voxel leaf and k-NN covariances: both clouds as two
+    // segments of one preparation (its host waits shared by the two clouds);
+    // the results are those of preparing each alone
+    std::vector<Prepared*> outs = {&tgt, &src};
+    const std::shared_ptr<SegHolder> hold =
+        prepare_segmented({target, source}, {true, false}, *config, outs, rc, s);
+    if (!hold) {
+      prepare(*target, *config, tgt, rc, s, true);
+      prepare(*source, *config, src, rc, s, false);
+    }
     Batch B;
     batch_setup({&src}, {&tgt}, {*config}, B, s);
     batch_run(B, s);

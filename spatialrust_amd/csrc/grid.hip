@@ -87,26 +87,6 @@ __global__ void k_grid_scatter(const uint32_t* __restrict__ cid, const uint32_t*
 constexpr uint32_t kSmallSeg = 48;
 constexpr uint32_t kBigSeg = 4096;
 
-// Bitonic sorting network over N (a power of two) values in registers; every
-// index is a compile-time constant once unrolled.
-template <int N>
-__device__ __forceinline__ void sort_net(uint32_t (&v)[N]) {
-#pragma unroll
-  for (int k = 2; k <= N; k <<= 1)
-#pragma unroll
-    for (int j = k >> 1; j > 0; j >>= 1)
-#pragma unroll
-      for (int i = 0; i < N; ++i) {
-        const int l = i ^ j;
-        if (l > i) {
-          const uint32_t a = v[i], b = v[l];
-          const bool up = (i & k) == 0;
-          v[i] = up ? min(a, b) : max(a, b);
-          v[l] = up ? max(a, b) : min(a, b);
-        }
-      }
-}
-
 // A cell of at most N members: loaded into registers (padded with ~0, which
 // sorts last), sorted by the network, stored back.
 template <int N>
@@ -239,10 +219,24 @@ __device__ __forceinline__ void grid_rowbits_word(const uint32_t* __restrict__ b
   rows[wi] = m;
 }
 
+// The same words with one thread per row (32 per word): each lane ORs its
+// row's occupancy words and a ballot forms the word, so a grid of few z-slabs
+// still spreads over many threads.
 __global__ void k_grid_rowbits(const uint32_t* __restrict__ bits, GridDesc g, uint32_t nwords,
                                uint32_t* __restrict__ rows) {
-  const uint32_t wi = blockIdx.x * blockDim.x + threadIdx.x;
-  if (wi < nwords) grid_rowbits_word(bits, g, wi, rows);
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t wi = t >> 5;
+  uint32_t any = 0;
+  if (wi < nwords) {
+    const uint32_t cz = wi / g.wps, cy = (wi % g.wps) * 32u + (t & 31u);
+    if (cy < uint32_t(g.dy)) {
+      const uint32_t* rb = bits + (size_t(cz) * uint32_t(g.dy) + cy) * g.wpr;
+      for (uint32_t w = 0; w < g.wpr; ++w) any |= rb[w];
+    }
+  }
+  const unsigned long long m = __ballot(any != 0u);
+  const uint32_t lane = threadIdx.x & 63u;
+  if ((lane & 31u) == 0u && wi < nwords) rows[wi] = uint32_t(m >> lane);
 }
 
 __global__ void k_grid_gather(const float4* __restrict__ pts, const uint32_t* __restrict__ order,
@@ -530,7 +524,7 @@ void build_grid(const Cloud& c, const uint32_t* hb, float h, Grid& g, hipStream_
     const uint64_t nrw = uint64_t(g.desc.wps) * uint64_t(g.desc.dz);
     g.rowocc.resize(std::max<uint64_t>(nrw, 1));
     g.desc.rows = g.rowocc.p;
-    hipLaunchKernelGGL(k_grid_rowbits, dim3(uint32_t((nrw + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+    hipLaunchKernelGGL(k_grid_rowbits, dim3(uint32_t((32 * nrw + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
                        g.occ.p, g.desc, uint32_t(nrw), g.rowocc.p);
   }
   if (n == 0) return;

@@ -63,6 +63,26 @@ __device__ __forceinline__ uint32_t seg_of(const uint32_t* __restrict__ begins, 
   return lo;
 }
 
+// Bitonic sorting network over N (a power of two) values in registers; every
+// index is a compile-time constant once unrolled.
+template <int N>
+__device__ __forceinline__ void sort_net(uint32_t (&v)[N]) {
+#pragma unroll
+  for (int k = 2; k <= N; k <<= 1)
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const int l = i ^ j;
+        if (l > i) {
+          const uint32_t a = v[i], b = v[l];
+          const bool up = (i & k) == 0;
+          v[i] = up ? min(a, b) : max(a, b);
+          v[l] = up ? max(a, b) : min(a, b);
+        }
+      }
+}
+
 // Order-preserving map of f32 onto u32 so atomicMin/atomicMax work on floats.
 __host__ __device__ __forceinline__ uint32_t f2ord(float f) {
 #ifdef __HIP_DEVICE_COMPILE__

@@ -437,12 +437,33 @@ __device__ __forceinline__ float4 cell_centroid(const float4* __restrict__ pts,
   return make_float4(__fmul_rn(sx, inv), __fmul_rn(sy, inv), __fmul_rn(sz, inv), 0.0f);
 }
 
-// Cells of at most kTinyCell members are sorted and summed by one thread here;
-// up to kSmallCell members by one wave (k_voxel_medium_cells); larger ones by
-// a workgroup (k_voxel_big_cells). The per-thread insertion sort works in
-// device memory, so it is kept to a few members (a 64-member cell took that
-// thread ~1000 dependent memory round trips).
-constexpr uint32_t kTinyCell = 4;
+// Cells of at most kTinyCell members are sorted and summed by one thread here,
+// their member indices sorted in registers by a network of the next power of
+// two; up to kSmallCell members by one wave (k_voxel_medium_cells); larger
+// ones by a workgroup (k_voxel_big_cells). (A per-thread insertion sort in
+// device memory took a 64-member cell ~1000 dependent memory round trips.)
+constexpr uint32_t kTinyCell = 16;
+
+template <int N>
+__device__ __forceinline__ float4 small_cell_centroid(const float4* __restrict__ pts,
+                                                      const uint32_t* __restrict__ seg,
+                                                      uint32_t cnt) {
+  uint32_t v[N];
+#pragma unroll
+  for (int t = 0; t < N; ++t) v[t] = uint32_t(t) < cnt ? seg[t] : 0xFFFFFFFFu;
+  sort_net<N>(v);
+  float sx = 0.0f, sy = 0.0f, sz = 0.0f;  // sequential sums in index order
+#pragma unroll
+  for (int t = 0; t < N; ++t)
+    if (uint32_t(t) < cnt) {
+      const float4 q = pts[v[t]];
+      sx = __fadd_rn(sx, q.x);
+      sy = __fadd_rn(sy, q.y);
+      sz = __fadd_rn(sz, q.z);
+    }
+  const float inv = __fdiv_rn(1.0f, (float)cnt);
+  return make_float4(__fmul_rn(sx, inv), __fmul_rn(sy, inv), __fmul_rn(sz, inv), 0.0f);
+}
 
 __global__ void k_voxel_centroids(const float4* __restrict__ pts, const uint32_t* __restrict__ ncells,
                                   const uint32_t* __restrict__ cstart,
@@ -450,20 +471,35 @@ __global__ void k_voxel_centroids(const float4* __restrict__ pts, const uint32_t
                                   float4* __restrict__ out, uint32_t* __restrict__ big_list,
                                   uint32_t* __restrict__ big_count,
                                   uint32_t* __restrict__ med_count) {
-  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= *ncells) return;
-  uint32_t cnt = ocount[c];
-  uint32_t* seg = members + cstart[c];
-  if (cnt > kSmallCell) {
-    big_list[atomicAdd(big_count, 1u)] = c;
-    return;
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t nc = *ncells;
+  const uint32_t cnt = c < nc ? ocount[c] : 0u;
+  // list appends aggregated per wave: one atomic per wave and list
+  const bool big = cnt > kSmallCell, med = cnt > kTinyCell && !big;
+  const int lane = int(threadIdx.x & 63u);
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const unsigned long long mb = __ballot(big), mm = __ballot(med);
+  if (mb) {
+    const int leader = __ffsll((long long)mb) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(big_count, uint32_t(__popcll(mb)));
+    base = __shfl(base, leader);
+    if (big) big_list[base + uint32_t(__popcll(mb & below))] = c;
   }
-  if (cnt > kTinyCell) {  // medium list: filled from the end of the same buffer
-    big_list[*ncells - 1u - atomicAdd(med_count, 1u)] = c;
-    return;
+  if (mm) {  // medium list: filled from the end of the same buffer
+    const int leader = __ffsll((long long)mm) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(med_count, uint32_t(__popcll(mm)));
+    base = __shfl(base, leader);
+    if (med) big_list[nc - 1u - (base + uint32_t(__popcll(mm & below)))] = c;
   }
-  sort_segment(seg, cnt);
-  out[c] = cell_centroid(pts, seg, cnt);
+  if (c >= nc || big || med) return;
+  const uint32_t* seg = members + cstart[c];
+  float4 r;
+  if (cnt <= 4) r = small_cell_centroid<4>(pts, seg, cnt);
+  else if (cnt <= 8) r = small_cell_centroid<8>(pts, seg, cnt);
+  else r = small_cell_centroid<16>(pts, seg, cnt);
+  out[c] = r;
 }
 
 // One wave per medium cell (kTinyCell < members <= kSmallCell = 64): the
