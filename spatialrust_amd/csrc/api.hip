@@ -233,6 +233,99 @@ void dev_free(void* p) {
   P.free_by_owner[{cls, thread_token()}].push_back(p);
 }
 
+// Small host-to-device copies (descriptor tables, offsets) go through a
+// per-thread ring of pinned staging chunks: from pinned memory hipMemcpyAsync
+// is asynchronous, while from pageable memory it stages the data and the host
+// waits ~10 us per call. A chunk is written again only after an event recorded
+// behind its last copy has completed. Inside the allocation-free entry point
+// (which must not allocate pinned memory either) and for large copies the
+// source is used directly.
+namespace {
+struct Staging {
+  static constexpr size_t kChunk = size_t(1) << 20;
+  static constexpr int kChunks = 4;
+  struct Chunk {
+    char* p = nullptr;
+    hipEvent_t ev = nullptr;
+    hipStream_t s = nullptr;  // stream of the copies from the chunk's current round
+    bool pending = false;     // ev recorded behind them and not yet waited for
+  };
+  Chunk c[kChunks];
+  int cur = 0;
+  size_t used = 0;
+  ~Staging() {
+    for (Chunk& ch : c) {
+      if (ch.ev) (void)hipEventDestroy(ch.ev);
+      if (ch.p) (void)hipHostFree(ch.p);
+    }
+  }
+  Chunk& open(int i) {
+    Chunk& ch = c[i];
+    if (!ch.p) {
+      SR_HIP(hipHostMalloc(reinterpret_cast<void**>(&ch.p), kChunk, hipHostMallocDefault));
+      SR_HIP(hipEventCreateWithFlags(&ch.ev, hipEventDisableTiming));
+    } else if (ch.pending) {
+      SR_HIP(hipEventSynchronize(ch.ev));
+      ch.pending = false;
+    }
+    ch.s = nullptr;
+    return ch;
+  }
+};
+}  // namespace
+
+void upload(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return;
+  if (t_arena || bytes > Staging::kChunk / 4) {
+    SR_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    return;
+  }
+  static thread_local Staging st;
+  Staging::Chunk* ch = &st.c[st.cur];
+  if (!ch->p) ch = &st.open(st.cur);
+  const size_t len = (bytes + 255) & ~size_t(255);
+  if (st.used + len > Staging::kChunk || (ch->s && ch->s != s)) {
+    if (ch->s) {  // close this chunk's round behind its last copy
+      SR_HIP(hipEventRecord(ch->ev, ch->s));
+      ch->pending = true;
+    }
+    st.cur = (st.cur + 1) % Staging::kChunks;
+    ch = &st.open(st.cur);
+    st.used = 0;
+  }
+  char* p = ch->p + st.used;
+  st.used += len;
+  std::memcpy(p, src, bytes);
+  ch->s = s;
+  SR_HIP(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, s));
+}
+
+// A small device-to-host read that the host waits for: through a per-thread
+// pinned buffer (a direct DMA), then copied out; the stream is synchronised.
+void download(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  constexpr size_t kPinned = size_t(256) << 10;
+  if (bytes == 0) {
+    SR_HIP(hipStreamSynchronize(s));
+    return;
+  }
+  if (t_arena || bytes > kPinned) {
+    SR_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+    SR_HIP(hipStreamSynchronize(s));
+    return;
+  }
+  struct Pinned {
+    void* p = nullptr;
+    ~Pinned() {
+      if (p) (void)hipHostFree(p);
+    }
+  };
+  static thread_local Pinned pb;
+  if (!pb.p) SR_HIP(hipHostMalloc(&pb.p, kPinned, hipHostMallocDefault));
+  SR_HIP(hipMemcpyAsync(pb.p, src, bytes, hipMemcpyDeviceToHost, s));
+  SR_HIP(hipStreamSynchronize(s));
+  std::memcpy(dst, pb.p, bytes);
+}
+
 // --------------------------------------------------------------- timing ----
 
 struct Profiler {
@@ -548,8 +641,7 @@ static std::shared_ptr<SegHolder> prepare_segmented(const std::vector<const sr_x
   {
     DevBuf<uint32_t> b(6 * size_t(nseg));
     seg_bounds(vox.centroids.pts.p, voff, b.p, s);
-    SR_HIP(hipMemcpyAsync(hb.data(), b.p, sizeof(uint32_t) * hb.size(), hipMemcpyDeviceToHost, s));
-    SR_HIP(hipStreamSynchronize(s));
+    download(hb.data(), b.p, sizeof(uint32_t) * hb.size(), s);
   }
   std::vector<float> h;
   choose_cell_edges_many(vox.centroids, voff, hb, knn_occupancy(cfg.k_neighbors), cfg.voxel_leaf,
@@ -782,9 +874,9 @@ static void batch_setup(const std::vector<const Prepared*>& src, const std::vect
   B.insts.resize(count);
   B.block_inst.resize(blocks);
   B.init.resize(init.size());
-  SR_HIP(hipMemcpyAsync(B.insts.p, hi.data(), sizeof(IterInstance) * count, hipMemcpyHostToDevice, s));
-  SR_HIP(hipMemcpyAsync(B.block_inst.p, hbi.data(), sizeof(uint32_t) * blocks, hipMemcpyHostToDevice, s));
-  SR_HIP(hipMemcpyAsync(B.init.p, init.data(), sizeof(double) * init.size(), hipMemcpyHostToDevice, s));
+  upload(B.insts.p, hi.data(), sizeof(IterInstance) * count, s);
+  upload(B.block_inst.p, hbi.data(), sizeof(uint32_t) * blocks, s);
+  upload(B.init.p, init.data(), sizeof(double) * init.size(), s);
   init_states(B.states.p, B.init.p, count, s);
 }
 
@@ -1410,8 +1502,8 @@ static void server_step(sr_server& S) {
     }
     S.insts.resize(count);
     S.block_inst.resize(begin);
-    SR_HIP(hipMemcpyAsync(S.insts.p, hi.data(), sizeof(IterInstance) * count, hipMemcpyHostToDevice, s));
-    SR_HIP(hipMemcpyAsync(S.block_inst.p, hbi.data(), sizeof(uint32_t) * begin, hipMemcpyHostToDevice, s));
+    upload(S.insts.p, hi.data(), sizeof(IterInstance) * count, s);
+    upload(S.block_inst.p, hbi.data(), sizeof(uint32_t) * begin, s);
     const PassScratch ps = grow_scratch(S.pass_queue, S.pass_ctl, S.pass_done, points, begin, s);
     gicp_pass(S.insts.p, S.block_inst.p, begin, count, ps, hard_stage_blocks(points), s);
   }
@@ -1460,7 +1552,7 @@ static void server_step(sr_server& S) {
     }
     if (!admit.empty()) {
       S.init.resize(init.size());
-      SR_HIP(hipMemcpyAsync(S.init.p, init.data(), sizeof(double) * init.size(), hipMemcpyHostToDevice, s));
+      upload(S.init.p, init.data(), sizeof(double) * init.size(), s);
       for (size_t i = 0; i < admit.size(); ++i)
         init_states(S.states.p + admit[i]->slot, S.init.p + 7 * i, 1, s);
     }
@@ -1776,8 +1868,7 @@ int32_t sr_gicp_align(const sr_xyz_view* source, const sr_xyz_view* target,
     batch_run(B, s);
     rc.stages |= SR_STAGE_ITERATE;
     InstState st;
-    SR_HIP(hipMemcpyAsync(&st, B.states.p, sizeof(st), hipMemcpyDeviceToHost, s));
-    SR_HIP(hipStreamSynchronize(s));
+    download(&st, B.states.p, sizeof(st), s);
     rc.d2h += sizeof(st);
     rc.stages |= SR_STAGE_READBACK;
     if (receipt) {
@@ -1818,8 +1909,7 @@ int32_t sr_gicp_align_noalloc(const sr_xyz_view* source, const sr_xyz_view* targ
     batch_setup({&src}, {&tgt}, {*config}, B, s);
     batch_run(B, s);
     InstState st;
-    SR_HIP(hipMemcpyAsync(&st, B.states.p, sizeof(st), hipMemcpyDeviceToHost, s));
-    SR_HIP(hipStreamSynchronize(s));
+    download(&st, B.states.p, sizeof(st), s);
     std::string msg;
     const int32_t code = state_to_result(st, *config, src, tgt, out, &msg);
     if (code) fail(code, msg);
@@ -1852,8 +1942,7 @@ int32_t sr_gicp_normal_equations(const sr_xyz_view* source, const sr_xyz_view* t
     SR_HIP(hipMemcpyAsync(part.data(), B.partials.p, part.size() * sizeof(float), hipMemcpyDeviceToHost, s));
     SR_HIP(hipMemcpyAsync(match.data(), B.match.p, match.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     SR_HIP(hipMemcpyAsync(sorder.data(), src.grid.order.p, sorder.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    SR_HIP(hipMemcpyAsync(torder.data(), tgt.grid.order.p, torder.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    SR_HIP(hipStreamSynchronize(s));
+    download(torder.data(), tgt.grid.order.p, torder.size() * sizeof(uint32_t), s);
     // Block partials in block order, each term as its (hi, lo) pair, in FP64:
     // the same sums the finalize kernel forms.
     double tot[kTerms] = {};
@@ -1974,8 +2063,7 @@ int32_t sr_grid_normals(const sr_xyz_view* cloud, float radius, const float* vie
     DevBuf<float4> out(n);
     grid_normals(rg, radius, viewpoint, out.p, s);
     std::vector<float4> host(n);
-    SR_HIP(hipMemcpyAsync(host.data(), out.p, sizeof(float4) * n, hipMemcpyDeviceToHost, s));
-    SR_HIP(hipStreamSynchronize(s));
+    download(host.data(), out.p, sizeof(float4) * n, s);
     for (uint32_t i = 0; i < n; ++i) {
       if (out_normals) {
         out_normals[3 * size_t(i)] = host[i].x;
@@ -2014,8 +2102,7 @@ int32_t sr_knn(const sr_xyz_view* points, const sr_xyz_view* queries, uint32_t k
     DevBuf<float> d2(size_t(qc.n) * k);
     knn_query(g, qc.pts.p, qc.n, k, idx.p, d2.p, s);
     SR_HIP(hipMemcpyAsync(out_index, idx.p, sizeof(int32_t) * qc.n * k, hipMemcpyDeviceToHost, s));
-    SR_HIP(hipMemcpyAsync(out_d2, d2.p, sizeof(float) * qc.n * k, hipMemcpyDeviceToHost, s));
-    SR_HIP(hipStreamSynchronize(s));
+    download(out_d2, d2.p, sizeof(float) * qc.n * k, s);
   });
 }
 
@@ -2067,8 +2154,7 @@ int32_t sr_map_align(const sr_map* map, const sr_xyz_view* source, const sr_gicp
     batch_setup({&src}, {&map->tgt}, {*config}, B, s);
     batch_run(B, s);
     InstState st;
-    SR_HIP(hipMemcpyAsync(&st, B.states.p, sizeof(st), hipMemcpyDeviceToHost, s));
-    SR_HIP(hipStreamSynchronize(s));
+    download(&st, B.states.p, sizeof(st), s);
     std::string msg;
     int32_t code = state_to_result(st, *config, src, map->tgt, out, &msg);
     if (code) fail(code, msg);
@@ -2102,8 +2188,7 @@ int32_t sr_batch_align(const sr_map* const* maps, const sr_xyz_view* sources, ui
     batch_setup(sp, tp, cfgs, B, s);
     batch_run(B, s);
     std::vector<InstState> st(count);
-    SR_HIP(hipMemcpyAsync(st.data(), B.states.p, sizeof(InstState) * count, hipMemcpyDeviceToHost, s));
-    SR_HIP(hipStreamSynchronize(s));
+    download(st.data(), B.states.p, sizeof(InstState) * count, s);
     int32_t first_code = 0;
     std::string first_msg;
     for (uint32_t i = 0; i < count; ++i) {
@@ -2147,8 +2232,7 @@ int32_t sr_batch_align_pairs(const sr_xyz_view* sources, const sr_xyz_view* targ
     batch_setup(sp, tp, cfgs, B, s);
     batch_run(B, s);
     std::vector<InstState> st(count);
-    SR_HIP(hipMemcpyAsync(st.data(), B.states.p, sizeof(InstState) * count, hipMemcpyDeviceToHost, s));
-    SR_HIP(hipStreamSynchronize(s));
+    download(st.data(), B.states.p, sizeof(InstState) * count, s);
     int32_t first_code = 0;
     std::string first_msg;
     for (uint32_t i = 0; i < count; ++i) {
@@ -2307,8 +2391,7 @@ static void odometry_update_map(sr_odometry* o, uint32_t n_old, Receipt& rc, hip
     DevBuf<uint32_t> b(8);
     uint32_t nb[6];
     cloud_bounds(list.pts.p + n_old, n - n_old, b.p, s);
-    SR_HIP(hipMemcpyAsync(nb, b.p, sizeof(nb), hipMemcpyDeviceToHost, s));
-    SR_HIP(hipStreamSynchronize(s));
+    download(nb, b.p, sizeof(nb), s);
     for (int a = 0; a < 3; ++a) full = full || nb[a] < o->geo_hb[a] || nb[3 + a] > o->geo_hb[3 + a];
   }
   const float leaf = o->cfg.voxel_leaf;
@@ -2364,9 +2447,8 @@ static void odometry_update_map(sr_odometry* o, uint32_t n_old, Receipt& rc, hip
   if (incremental) gather_list_normals(fresh.grid, o->lnrm.p, fresh.normals.p, s);
   rc.stages |= SR_STAGE_COVARIANCE;
   uint32_t refreshed = n;
-  if (incremental)
-    SR_HIP(hipMemcpyAsync(&refreshed, o->refresh_n.p, sizeof(refreshed), hipMemcpyDeviceToHost, s));
-  SR_HIP(hipStreamSynchronize(s));
+  if (incremental) download(&refreshed, o->refresh_n.p, sizeof(refreshed), s);
+  else SR_HIP(hipStreamSynchronize(s));
   o->refreshed = refreshed;
   o->map_prep = std::move(fresh);
 }
@@ -2415,8 +2497,7 @@ static void odometry_register(const std::vector<sr_odometry*>& o, const std::vec
   batch_setup(src, tgt, cfgs, B, s, pose7.data());
   batch_run(B, s);
   std::vector<InstState> st(n);
-  SR_HIP(hipMemcpyAsync(st.data(), B.states.p, sizeof(InstState) * n, hipMemcpyDeviceToHost, s));
-  SR_HIP(hipStreamSynchronize(s));
+  download(st.data(), B.states.p, sizeof(InstState) * n, s);
   for (size_t i = 0; i < n; ++i) {
     std::string msg;
     const int32_t code = state_to_result(st[i], o[i]->gicp, sc[i]->src, o[i]->map_prep, &sc[i]->r, &msg);
@@ -2553,8 +2634,7 @@ int32_t sr_odometry_map_read(const sr_odometry* odo, float* xyz, uint64_t cap, u
     if (!xyz) fail(SR_ERR_INVALID_ARGUMENT, "xyz must not be null");
     std::vector<float4> host(m);
     hipStream_t s = thread_stream();
-    SR_HIP(hipMemcpyAsync(host.data(), odo->map.list.pts.p, sizeof(float4) * m, hipMemcpyDeviceToHost, s));
-    SR_HIP(hipStreamSynchronize(s));
+    download(host.data(), odo->map.list.pts.p, sizeof(float4) * m, s);
     for (uint64_t i = 0; i < m; ++i) {
       xyz[3 * i] = host[i].x;
       xyz[3 * i + 1] = host[i].y;

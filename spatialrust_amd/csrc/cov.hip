@@ -637,8 +637,7 @@ static void launch_knn_normals(const std::vector<KnnCloud>& host, int k, float b
   const uint32_t blocks = host.back().block_begin +
                           (host.back().g.npoints + kBlock - 1) / kBlock;
   DevBuf<KnnCloud> clouds(host.size());
-  SR_HIP(hipMemcpyAsync(clouds.p, host.data(), sizeof(KnnCloud) * host.size(),
-                        hipMemcpyHostToDevice, s));
+  upload(clouds.p, host.data(), sizeof(KnnCloud) * host.size(), s);
   DevBuf<unsigned long long> far_queue(std::max<uint64_t>(n, 1));
   DevBuf<uint32_t> far_count(1);
   fill_u32(far_count.p, 0u, 1, s);

@@ -21,8 +21,7 @@ constexpr uint64_t kMaxCells = 64000000ull;  // MAX_UNIFORM_GRID_CELLS, uniform_
 void host_bounds(const Cloud& c, uint32_t* host6, hipStream_t s) {
   DevBuf<uint32_t> b(8);
   cloud_bounds(c.pts.p, c.n, b.p, s);
-  SR_HIP(hipMemcpyAsync(host6, b.p, 6 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  SR_HIP(hipStreamSynchronize(s));
+  download(host6, b.p, 6 * sizeof(uint32_t), s);
 }
 
 static void grid_desc(const uint32_t* hb, float h, uint32_t npoints, GridDesc& d) {
@@ -471,14 +470,13 @@ static void sketch_edges(const float4* pts, const std::vector<uint32_t>& off,
   DevBuf<uint32_t> d_wg(m);
   DevBuf<uint8_t> regs(size_t(wgs) * kSketchEdges * kSketchRegs);
   DevBuf<float> est(size_t(m) * kSketchEdges);
-  SR_HIP(hipMemcpyAsync(d_segs.p, segs.data(), sizeof(SketchSeg) * m, hipMemcpyHostToDevice, s));
-  SR_HIP(hipMemcpyAsync(d_wg.p, wg_begin.data(), sizeof(uint32_t) * m, hipMemcpyHostToDevice, s));
+  upload(d_segs.p, segs.data(), sizeof(SketchSeg) * m, s);
+  upload(d_wg.p, wg_begin.data(), sizeof(uint32_t) * m, s);
   hipLaunchKernelGGL(k_edge_sketch, dim3(wgs), dim3(kBlock), 0, s, pts, d_segs.p, m, d_wg.p, regs.p);
   hipLaunchKernelGGL(k_edge_sketch_merge, dim3(m * kSketchEdges), dim3(kSketchRegs), 0, s, regs.p,
                      d_segs.p, est.p);
   std::vector<float> host(size_t(m) * kSketchEdges);
-  SR_HIP(hipMemcpyAsync(host.data(), est.p, sizeof(float) * host.size(), hipMemcpyDeviceToHost, s));
-  SR_HIP(hipStreamSynchronize(s));  // also keeps the tables alive until the kernels end
+  download(host.data(), est.p, sizeof(float) * host.size(), s);  // also keeps the tables alive until the kernels end
   for (uint32_t k = 0; k < m; ++k)
     h[ids[k]] = edge_from_sketch(segs[k].n, &hc[size_t(k) * kSketchEdges],
                                  &host[size_t(k) * kSketchEdges], occupancy);
@@ -699,7 +697,7 @@ template <typename T>
 static void upload_vec(DevBuf<T>& d, const std::vector<T>& h, hipStream_t s) {
   d.resize(std::max<size_t>(h.size(), 1));
   if (!h.empty())
-    SR_HIP(hipMemcpyAsync(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s));
+    upload(d.p, h.data(), sizeof(T) * h.size(), s);
 }
 
 static std::vector<uint32_t> blocks_of(const std::vector<uint32_t>& off) {

@@ -92,6 +92,12 @@ struct DevBuf {
 
 // Per-thread stream (default priority) of the calling host thread.
 hipStream_t thread_stream();
+// hipMemcpyAsync host -> device of a small host buffer (staged in pinned
+// memory, so the call does not wait); `src` may be reused once it returns.
+void upload(void* dst, const void* src, size_t bytes, hipStream_t s);
+// hipMemcpyAsync device -> host of a small buffer, then the stream is
+// synchronised (through a pinned buffer, so the copy is a direct DMA).
+void download(void* dst, const void* src, size_t bytes, hipStream_t s);
 // Ensures the HIP device is initialised; throws SR_STATUS_UNAVAILABLE if not.
 void ensure_device();
 int device_cu_count();

@@ -277,8 +277,7 @@ uint32_t VoxelMap::insert(const float4* scan_local, uint32_t n, const MapPose& p
     hipLaunchKernelGGL(k_map_flags, dim3(blocks_for(n)), dim3(kBlk), 0, s, slot_of.p, n, val.p,
                        claim.p, flags.p);
     exclusive_scan_u32(flags.p, pos.p, n, scal.p, s);
-    SR_HIP(hipMemcpyAsync(host, scal.p, sizeof(host), hipMemcpyDeviceToHost, s));
-    SR_HIP(hipStreamSynchronize(s));
+    download(host, scal.p, sizeof(host), s);
     if (host[1]) fail(SR_ERR_INVALID_ARGUMENT, "voxel map hash table overflow");
     if (uint64_t(list.n) + host[0] <= capacity) break;
     if (list.n == 0) fail(SR_ERR_INVALID_ARGUMENT, "scan has more new voxels than the map capacity");

@@ -647,8 +647,7 @@ void voxel_downsample(const Cloud& in, float leaf, const float* origin, bool wan
                      ocount.p, members.p, big_list.p, scal.p + 1, out.centroids.pts.p);
   hipLaunchKernelGGL(k_error_check, dim3(1), dim3(64), 0, s, params.p, scal.p + 2);
   uint32_t host[3];
-  SR_HIP(hipMemcpyAsync(host, scal.p, sizeof(host), hipMemcpyDeviceToHost, s));
-  SR_HIP(hipStreamSynchronize(s));
+  download(host, scal.p, sizeof(host), s);
   if (host[2]) fail(3, "voxel key range does not fit a 64-bit linear key; use a larger leaf_size");
   out.cells = host[0];
   out.centroids.n = host[0];
@@ -793,7 +792,7 @@ template <typename T>
 static void upload(DevBuf<T>& d, const std::vector<T>& h, hipStream_t s) {
   d.resize(std::max<size_t>(h.size(), 1));
   if (!h.empty())
-    SR_HIP(hipMemcpyAsync(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s));
+    upload(d.p, h.data(), sizeof(T) * h.size(), s);
 }
 
 void pack_many(const std::vector<SegSource>& srcs, Cloud& out, hipStream_t s) {
@@ -892,8 +891,7 @@ void voxel_downsample_many(const Cloud& in, const std::vector<uint32_t>& off, fl
   hipLaunchKernelGGL(k_seg_voxel_tail, dim3((nseg + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
                      rank.p, d_off.p, nseg, params.p, scal.p, tailv.p);
   std::vector<uint32_t> host(nseg + 2);
-  SR_HIP(hipMemcpyAsync(host.data(), tailv.p, sizeof(uint32_t) * host.size(), hipMemcpyDeviceToHost, s));
-  SR_HIP(hipStreamSynchronize(s));
+  download(host.data(), tailv.p, sizeof(uint32_t) * host.size(), s);
   if (host[nseg + 1]) fail(3, "voxel key range does not fit a 64-bit linear key; use a larger leaf_size");
   for (uint32_t g = 0; g <= nseg; ++g) cell_off[g] = host[g];
   out.cells = host[nseg];
