@@ -83,7 +83,8 @@ __global__ void k_grid_scatter(const uint32_t* __restrict__ cid, const uint32_t*
   order[start[c] + rnk[i]] = i;
 }
 
-constexpr uint32_t kSmallSeg = 48;
+constexpr uint32_t kSmallSeg = 16;   // per thread, in registers
+constexpr uint32_t kMediumSeg = 64;  // one wave (k_sort_medium_cells)
 constexpr uint32_t kBigSeg = 4096;
 
 // A cell of at most N members: loaded into registers (padded with ~0, which
@@ -99,35 +100,63 @@ __device__ __forceinline__ void sort_small_cell(uint32_t* seg, uint32_t cnt) {
     if (uint32_t(t) < cnt) seg[t] = v[t];
 }
 
+// Cells of up to kSmallSeg members are sorted here, in registers; larger ones
+// are listed (medium from the end of big_list, big from its start; one atomic
+// per wave and list) for k_sort_medium_cells / k_sort_big_cells.
 __global__ void k_sort_cells(const uint32_t* __restrict__ start, uint32_t ncells,
                              uint32_t* __restrict__ order, uint32_t* __restrict__ big_list,
-                             uint32_t* __restrict__ big_count) {
-  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncells) return;
-  uint32_t b = start[c], e = start[c + 1];
-  uint32_t cnt = e - b;
-  if (cnt < 2) return;
-  if (cnt > kSmallSeg) {
-    big_list[atomicAdd(big_count, 1u)] = c;
-    return;
+                             uint32_t* __restrict__ counts) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t b = c < ncells ? start[c] : 0u, cnt = c < ncells ? start[c + 1] - b : 0u;
+  const bool big = cnt > kMediumSeg, med = cnt > kSmallSeg && !big;
+  const int lane = int(threadIdx.x & 63u);
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const unsigned long long mb = __ballot(big), mm = __ballot(med);
+  if (mb) {
+    const int leader = __ffsll((long long)mb) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&counts[0], uint32_t(__popcll(mb)));
+    base = __shfl(base, leader);
+    if (big) big_list[base + uint32_t(__popcll(mb & below))] = c;
   }
+  if (mm) {
+    const int leader = __ffsll((long long)mm) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&counts[1], uint32_t(__popcll(mm)));
+    base = __shfl(base, leader);
+    if (med) big_list[ncells - 1u - (base + uint32_t(__popcll(mm & below)))] = c;
+  }
+  if (cnt < 2 || big || med) return;
   uint32_t* seg = order + b;
-  if (cnt <= 8) {
-    sort_small_cell<8>(seg, cnt);
-    return;
-  }
-  if (cnt <= 16) {
-    sort_small_cell<16>(seg, cnt);
-    return;
-  }
-  for (uint32_t a = 1; a < cnt; ++a) {
-    uint32_t v = seg[a];
-    uint32_t k = a;
-    while (k > 0 && seg[k - 1] > v) {
-      seg[k] = seg[k - 1];
-      --k;
+  if (cnt <= 4) sort_small_cell<4>(seg, cnt);
+  else if (cnt <= 8) sort_small_cell<8>(seg, cnt);
+  else sort_small_cell<16>(seg, cnt);
+}
+
+// One wave per listed medium cell: its members, one per lane, sorted by a
+// bitonic network across the lanes and written back.
+__global__ void __launch_bounds__(kBlock) k_sort_medium_cells(const uint32_t* __restrict__ start,
+                                                              uint32_t ncells,
+                                                              uint32_t* __restrict__ order,
+                                                              const uint32_t* __restrict__ list,
+                                                              const uint32_t* __restrict__ counts) {
+  const uint32_t nmed = counts[1];
+  const int lane = int(threadIdx.x & 63u);
+  const uint32_t waves = gridDim.x * (blockDim.x / 64u);
+  for (uint32_t w = blockIdx.x * (blockDim.x / 64u) + threadIdx.x / 64u; w < nmed; w += waves) {
+    const uint32_t c = list[ncells - 1u - w];
+    const uint32_t b = start[c], cnt = start[c + 1] - b;
+    uint32_t v = uint32_t(lane) < cnt ? order[b + uint32_t(lane)] : 0xFFFFFFFFu;
+#pragma unroll
+    for (int size = 2; size <= 64; size <<= 1) {
+#pragma unroll
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        const uint32_t o = uint32_t(__shfl_xor(int(v), stride));
+        const bool up = (lane & size) == 0, lower = (lane & stride) == 0;
+        v = (lower == up) ? min(v, o) : max(v, o);
+      }
     }
-    seg[k] = v;
+    if (uint32_t(lane) < cnt) order[b + uint32_t(lane)] = v;
   }
 }
 
@@ -200,25 +229,8 @@ __global__ void k_grid_bits(const uint32_t* __restrict__ cs, GridDesc g, uint32_
   if (wi < nwords) grid_bits_word(cs, g, wi, bits);
 }
 
-// Row bitmap: one thread per 32-row word of a z-slab; bit set when any cell of
-// the x-row is occupied.
-__device__ __forceinline__ void grid_rowbits_word(const uint32_t* __restrict__ bits,
-                                                  const GridDesc& g, uint32_t wi,
-                                                  uint32_t* __restrict__ rows) {
-  const uint32_t cz = wi / g.wps, wy = wi % g.wps;
-  uint32_t m = 0;
-  for (uint32_t b = 0; b < 32; ++b) {
-    const uint32_t cy = wy * 32 + b;
-    if (cy >= uint32_t(g.dy)) break;
-    const uint32_t* rb = bits + (size_t(cz) * uint32_t(g.dy) + cy) * g.wpr;
-    uint32_t any = 0;
-    for (uint32_t w = 0; w < g.wpr; ++w) any |= rb[w];
-    if (any) m |= 1u << b;
-  }
-  rows[wi] = m;
-}
-
-// The same words with one thread per row (32 per word): each lane ORs its
+// Row bitmap: bit cy of word (cz, cy / 32) is set when any cell of the x-row
+// (cy, cz) is occupied. One thread per row (32 per word): each lane ORs its
 // row's occupancy words and a ballot forms the word, so a grid of few z-slabs
 // still spreads over many threads.
 __global__ void k_grid_rowbits(const uint32_t* __restrict__ bits, GridDesc g, uint32_t nwords,
@@ -528,10 +540,12 @@ void build_grid(const Cloud& c, const uint32_t* hb, float h, Grid& g, hipStream_
   if (n == 0) return;
   hipLaunchKernelGGL(k_grid_scatter, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, cid.p,
                      rnk.p, n, g.cell_start.p, g.order.p);
-  fill_u32(scal.p, 0u, 1, s);
+  fill_u32(scal.p, 0u, 2, s);  // big and medium cell counts
   DevBuf<uint32_t> big(std::max<uint32_t>(nc, 1));
   hipLaunchKernelGGL(k_sort_cells, dim3((nc + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
                      g.cell_start.p, nc, g.order.p, big.p, scal.p);
+  hipLaunchKernelGGL(k_sort_medium_cells, dim3(512), dim3(kBlock), 0, s, g.cell_start.p, nc,
+                     g.order.p, big.p, scal.p);
   hipLaunchKernelGGL(k_sort_big_cells, dim3(128), dim3(kBlock), 0, s, g.cell_start.p, g.order.p,
                      big.p, scal.p);
   hipLaunchKernelGGL(k_grid_gather, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, c.pts.p,
@@ -650,14 +664,27 @@ __global__ void k_seg_grid_bits(const uint32_t* __restrict__ cs_local, const Seg
   grid_bits_word(cs_local + G.cs_off, G.d, t - G.word_off, occ + G.word_off);
 }
 
+// Row bitmaps of every segment, one thread per row as in k_grid_rowbits
+// (thread t serves bit t % 32 of word t / 32 of the flat row-bitmap array).
 __global__ void k_seg_grid_rowbits(const uint32_t* __restrict__ occ, const SegGrid* __restrict__ sg,
                                    uint32_t nseg, const uint32_t* __restrict__ row_begin,
                                    uint32_t total, uint32_t* __restrict__ rows) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= total) return;
-  const uint32_t s = seg_of(row_begin, nseg, t);
-  const SegGrid& G = sg[s];
-  grid_rowbits_word(occ + G.word_off, G.d, t - G.row_off, rows + G.row_off);
+  const uint32_t wi = t >> 5;
+  uint32_t any = 0;
+  if (wi < total) {
+    const SegGrid& G = sg[seg_of(row_begin, nseg, wi)];
+    const GridDesc& g = G.d;
+    const uint32_t w = wi - G.row_off;
+    const uint32_t cz = w / g.wps, cy = (w % g.wps) * 32u + (t & 31u);
+    if (cy < uint32_t(g.dy)) {
+      const uint32_t* rb = occ + G.word_off + (size_t(cz) * uint32_t(g.dy) + cy) * g.wpr;
+      for (uint32_t k = 0; k < g.wpr; ++k) any |= rb[k];
+    }
+  }
+  const unsigned long long m = __ballot(any != 0u);
+  const uint32_t lane = threadIdx.x & 63u;
+  if ((lane & 31u) == 0u && wi < total) rows[wi] = uint32_t(m >> lane);
 }
 
 __global__ void k_seg_grid_gather(const float4* __restrict__ pts, const uint32_t* __restrict__ order,
@@ -799,13 +826,15 @@ void build_grids_many(const Cloud& c, const std::vector<uint32_t>& off,
     hipLaunchKernelGGL(k_seg_grid_bits, grid1(words), dim3(kBlock), 0, s, out.cell_start.p, d_sg.p,
                        nseg, d_word.p, uint32_t(words), out.occ.p);
   if (rws)
-    hipLaunchKernelGGL(k_seg_grid_rowbits, grid1(rws), dim3(kBlock), 0, s, out.occ.p, d_sg.p, nseg,
+    hipLaunchKernelGGL(k_seg_grid_rowbits, grid1(32 * rws), dim3(kBlock), 0, s, out.occ.p, d_sg.p, nseg,
                        d_row.p, uint32_t(rws), out.rowocc.p);
   if (n) {
     hipLaunchKernelGGL(k_grid_scatter, grid1(n), dim3(kBlock), 0, s, cid.p, rnk.p, n, csg.p, order.p);
-    fill_u32(scal.p, 0u, 1, s);
+    fill_u32(scal.p, 0u, 2, s);  // big and medium cell counts
     DevBuf<uint32_t> big(std::max<uint32_t>(ncells, 1));
     hipLaunchKernelGGL(k_sort_cells, grid1(ncells), dim3(kBlock), 0, s, csg.p, ncells, order.p,
+                       big.p, scal.p);
+    hipLaunchKernelGGL(k_sort_medium_cells, dim3(512), dim3(kBlock), 0, s, csg.p, ncells, order.p,
                        big.p, scal.p);
     hipLaunchKernelGGL(k_sort_big_cells, dim3(128), dim3(kBlock), 0, s, csg.p, order.p, big.p, scal.p);
     hipLaunchKernelGGL(k_seg_grid_gather, grid1(n), dim3(kBlock), 0, s, c.pts.p, order.p, d_pt.p, nseg,
