@@ -1177,9 +1177,9 @@ __device__ __forceinline__ void hard_query(const IterInstance* __restrict__ inst
 // kSingle: the launch holds one registration, so every queued point belongs to
 // insts[0] and its grid descriptor and pointers are wave-uniform (scalar
 // registers) instead of per-lane copies (cfg1: 64 -> 50 VGPRs, no spill).
-// Static split, kWave/kNNGroup queries per wave and round: the grid holds a
-// wave per such slice of a fully queued launch (capped at what the chip runs
-// at once), so every wave takes few slices (no shared cursor to contend on).
+// Static split, kWave/kG queries per wave and round (a slice): the grid holds
+// a wave per slice of a fully queued launch up to a cap, so every wave takes
+// few slices (no shared cursor to contend on).
 // kG: lanes per queued point (a launch-tuning choice: any group size gives the
 // same matches, since every point of a ring is visited by exactly one lane and
 // the group takes the minimum packed key).
@@ -1193,7 +1193,12 @@ __global__ void __launch_bounds__(kBlock)
   constexpr uint32_t G = kWave / kG;
   constexpr uint32_t kWavesPerBlock = kBlock / kWave;
   const uint32_t waves = gridDim.x * kWavesPerBlock;
-  const uint32_t wid = xcd_block() * kWavesPerBlock + threadIdx.x / kWave;
+  // Slices in hardware dispatch order: consecutive slices go to different
+  // XCDs, and slices beyond the resident waves go to waves dispatched as
+  // earlier ones finish. The queue is in source order and hard queries cluster
+  // in space, so giving each XCD a contiguous share (as the other stages do
+  // for locality) left some XCDs with most of the expensive slices.
+  const uint32_t wid = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
   for (uint32_t base = wid * G; base < qlen; base += waves * G) {
     const uint32_t e = base + uint32_t(lane / kG);
     const bool active = e < qlen;  // the same on a group's lanes
@@ -1321,9 +1326,10 @@ __global__ void __launch_bounds__(kFinBlock) k_gicp_finalize(const IterInstance*
 }
 
 uint32_t hard_stage_blocks(uint64_t points, uint32_t cap) {
-  // a wave per kWave/kNNGroup queries of a fully queued launch, capped (by
-  // default at what the chip holds at once: 256 CUs x 8 waves per SIMD; larger
-  // queues take several rounds per wave), so an empty queue costs a small launch
+  // a wave per kWave/kNNGroup queries of a fully queued launch, capped so that
+  // an empty queue costs a small launch (the default cap, twice what the chip
+  // holds at once, lets the waves beyond the resident ones pick up slices as
+  // slots free; queues beyond the cap take several rounds per wave)
   const uint64_t per_block = uint64_t(kBlock / kWave) * (kWave / kNNGroup);
   const uint64_t want = (points + per_block - 1) / per_block;
   return uint32_t(std::min<uint64_t>(std::max<uint64_t>(want, 1), std::max<uint32_t>(cap, 1)));

@@ -389,17 +389,17 @@ struct PassScratch {
 
 // Launch choices of a pass that leave every result unchanged (the tuner picks
 // them per problem class): lanes per queued point in the ring search (4, 8 or
-// 16), the workgroup cap of that stage, and the launch size (blocks) from
+// 16), the workgroup cap of that stage (2048 to 8192), and the launch size (blocks) from
 // which the first stage runs as two kernels (settle + marked search).
 struct PassTuning {
   int group = 8;
-  uint32_t hard_cap = 2048;
+  uint32_t hard_cap = 4096;
   uint32_t split_blocks = 4096;
 };
 
 // Workgroups of the queue-draining stage for a launch over `points` source
 // points, at most `cap`.
-uint32_t hard_stage_blocks(uint64_t points, uint32_t cap = 2048);
+uint32_t hard_stage_blocks(uint64_t points, uint32_t cap = 4096);
 
 // Launches one Gauss-Newton pass over `count` instances (device array `insts`),
 // total_blocks blocks; block_inst maps block -> instance. Without `finalize`
