@@ -329,13 +329,13 @@ __device__ __forceinline__ void nn_phase1(const GridDesc& g, const uint32_t* __r
   if (empty && inside) {
     // Rings closer than the empty radius hold no point, so every point lies
     // outside the box of rings 0..e-1: its clearance bounds the distance.
-    e = empty[(uint32_t(hz) * uint32_t(g.dy) + uint32_t(hy)) * uint32_t(g.dx) + uint32_t(hx)];
+    e = ldg(empty + ((uint32_t(hz) * uint32_t(g.dy) + uint32_t(hy)) * uint32_t(g.dx) + uint32_t(hx)));
     if (e > 0 && ring_clearance(g, px, py, pz, hx, hy, hz, e - 1) > dmax) {
       if constexpr (kStats) stats_add(stats, 0, 0, 0, kRejected);
       return;
     }
   }
-  if (prev != kNone) nn_consider<kGap>(Q, px, py, pz, prev, tp[prev]);
+  if (prev != kNone) nn_consider<kGap>(Q, px, py, pz, prev, ldg(tp + prev));
   if (inside && e <= 1) {
     // Rings 0 and 1 as nine contiguous point ranges (streamed as one sequence,
     // kBlockW loads in flight), cut to the sphere of the seeded radius widened
@@ -489,8 +489,8 @@ __device__ __forceinline__ void group_search(const GridDesc& g, const uint32_t* 
       uint32_t lo[2 * kRowBatch], hi[2 * kRowBatch];
 #pragma unroll
       for (int m = 0; m < 2 * kRowBatch; ++m) {
-        lo[m] = cs[ia[m]];
-        hi[m] = cs[ib[m]];
+        lo[m] = ldg(cs + ia[m]);
+        hi[m] = ldg(cs + ib[m]);
       }
       auto visit = [&](uint32_t s, const float4& q) {
         if constexpr (kStats) ++W.pts;
@@ -799,6 +799,41 @@ __device__ __forceinline__ void enqueue_hard(const PassScratch& ps, bool hard, c
   budget[i] = 0.0f;
 }
 
+// enqueue_hard for kernels whose threads all call it together: one atomic
+// per workgroup instead of one per wave (the queue length is a single counter
+// that every workgroup of the launch bumps).
+__device__ __forceinline__ void enqueue_hard_block(const PassScratch& ps, bool hard,
+                                                   const NNQuery& Q, float px, float py, float pz,
+                                                   uint32_t inst_id, uint32_t i, float* budget) {
+  __shared__ uint32_t wcount[kBlock / kWave];
+  __shared__ uint32_t bbase;
+  const int lane = int(threadIdx.x & (kWave - 1)), wave = int(threadIdx.x / kWave);
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const unsigned long long hm = __ballot(hard);
+  if (lane == 0) wcount[wave] = uint32_t(__popcll(hm));
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / kWave; ++w) tot += wcount[w];
+    bbase = tot ? atomicAdd(&ps.ctl[0], tot) : 0u;
+  }
+  __syncthreads();
+  if (!hard) return;
+  uint32_t base = bbase;
+  for (int w = 0; w < wave; ++w) base += wcount[w];
+  HardQuery h;
+  h.px = px;
+  h.py = py;
+  h.pz = pz;
+  h.inst = inst_id | (Q.skip_block ? kHardBlockFlag : 0u);
+  h.i = i;
+  h.pos = Q.best_pos;
+  h.key = Q.best_key;
+  ps.queue[base + uint32_t(__popcll(hm & below))] = h;
+  budget[i] = 0.0f;
+}
+
 // First stage. One-point-per-thread blocks: each point whose match the last
 // search certified for the pose's movement so far keeps it (budget spent, or
 // the match dropped when beyond d_max); the others are marked in the block's
@@ -828,10 +863,10 @@ __global__ void __launch_bounds__(kBlock) k_gicp_settle(const IterInstance* __re
   load_pose(st, P);
   const uint32_t i = lb * kBlock + threadIdx.x;
   const bool valid = i < I.n_src;
-  const float4 p = I.src[valid ? i : 0u];
-  const float4 ns = I.src_n[valid ? i : 0u];
-  const uint32_t prev = valid ? I.match[i] : kNone;
-  const float bud = valid ? I.budget[i] : 0.0f;
+  const float4 p = ldg(I.src + (valid ? i : 0u));
+  const float4 ns = ldg(I.src_n + (valid ? i : 0u));
+  const uint32_t prev = valid ? ldg(I.match + (i)) : kNone;
+  const float bud = valid ? ldg(I.budget + (i)) : 0.0f;
   const float move =
       move_rot * (sqrtf(p.x * p.x + p.y * p.y + p.z * p.z) * 1.000001f) + move_const;
   const bool keep = valid && prev != kNone && bud > move;
@@ -844,8 +879,8 @@ __global__ void __launch_bounds__(kBlock) k_gicp_settle(const IterInstance* __re
   if (keep) {
     if constexpr (kStats) stats_add(stats, 0, 0, 0, kBlockDone);
     transform_point(P, p, px, py, pz);
-    q = I.tgt[prev];
-    nt = I.tgt_n[prev];
+    q = ldg(I.tgt + (prev));
+    nt = ldg(I.tgt_n + (prev));
     d2 = dist2(px, py, pz, q.x, q.y, q.z);
     if (d2 <= cfg.dmax2) {
       I.budget[i] = bud - move;
@@ -898,13 +933,13 @@ __device__ __forceinline__ void search_block(const IterInstance* __restrict__ in
   for (uint32_t k = 0; k < ppt; ++k) {
     const uint32_t i = (lb * ppt + k) * kBlock + threadIdx.x;
     const bool valid = i < I.n_src;
-    const float4 p = I.src[valid ? i : 0u];
+    const float4 p = ldg(I.src + (valid ? i : 0u));
     float px, py, pz;
     transform_point(P, p, px, py, pz);
-    const uint32_t prev = valid ? I.match[i] : kNone;
+    const uint32_t prev = valid ? ldg(I.match + (i)) : kNone;
     // The last search left this point a movement budget (half its gap to the
     // second-nearest point); while the pose moves it less, the match stands.
-    const float bud = valid ? I.budget[i] : 0.0f;
+    const float bud = valid ? ldg(I.budget + (i)) : 0.0f;
     const float move =
         move_rot * (sqrtf(p.x * p.x + p.y * p.y + p.z * p.z) * 1.000001f) + move_const;
     const bool keep = prev != kNone && bud > move;
@@ -919,11 +954,11 @@ __device__ __forceinline__ void search_block(const IterInstance* __restrict__ in
                                  cfg.dmax2, prev, 0.0f, Q, valid ? stats : nullptr);
     }
     const bool hard = valid && Q.hard;
-    enqueue_hard(ps, hard, Q, px, py, pz, inst_id, i, I.budget);
+    enqueue_hard_block(ps, hard, Q, px, py, pz, inst_id, i, I.budget);
     if (!valid || hard) continue;
     if (keep) {
       if constexpr (kStats) stats_add(stats, 0, 0, 0, kBlockDone);
-      const float4 q = I.tgt[prev];
+      const float4 q = ldg(I.tgt + (prev));
       if (dist2(px, py, pz, q.x, q.y, q.z) <= cfg.dmax2) {
         I.budget[i] = bud - move;
       } else {  // the nearest point is beyond d_max: no correspondence, search next time
@@ -972,8 +1007,8 @@ __global__ void __launch_bounds__(kBlock)
   NNQuery Q;
   Q.hard = false;
   if (mine) {
-    transform_point(P, I.src[i], px, py, pz);
-    const uint32_t prev = I.match[i];
+    transform_point(P, ldg(I.src + (i)), px, py, pz);
+    const uint32_t prev = ldg(I.match + (i));
     if (gap)
       nn_phase1<kStats, true>(g, I.cell_start, I.empty_radius, I.tgt, px, py, pz, cfg.dmax,
                               cfg.dmax2, prev, kGapMargin * g.h, Q, stats);
@@ -1028,10 +1063,10 @@ __global__ void __launch_bounds__(kBlock)
   if (ppt == 1) {
     const uint32_t i = lb * kBlock + threadIdx.x;
     const bool valid = i < I.n_src;
-    const float4 p = I.src[valid ? i : 0u];
-    const float4 ns = I.src_n[valid ? i : 0u];
-    const uint32_t prev = valid ? I.match[i] : kNone;
-    const float bud = valid ? I.budget[i] : 0.0f;
+    const float4 p = ldg(I.src + (valid ? i : 0u));
+    const float4 ns = ldg(I.src_n + (valid ? i : 0u));
+    const uint32_t prev = valid ? ldg(I.match + (i)) : kNone;
+    const float bud = valid ? ldg(I.budget + (i)) : 0.0f;
     const float move =
         move_rot * (sqrtf(p.x * p.x + p.y * p.y + p.z * p.z) * 1.000001f) + move_const;
     const bool keep = prev != kNone && bud > move;
@@ -1045,8 +1080,8 @@ __global__ void __launch_bounds__(kBlock)
         if constexpr (kStats) stats_add(stats, 0, 0, 0, kBlockDone);
         float px, py, pz;
         transform_point(P, p, px, py, pz);
-        const float4 q = I.tgt[prev];
-        const float4 nt = I.tgt_n[prev];
+        const float4 q = ldg(I.tgt + (prev));
+        const float4 nt = ldg(I.tgt_n + (prev));
         const float d2 = dist2(px, py, pz, q.x, q.y, q.z);
         if (d2 <= cfg.dmax2) {
           I.budget[i] = bud - move;
@@ -1068,13 +1103,13 @@ __global__ void __launch_bounds__(kBlock)
   for (uint32_t k = 0; k < ppt; ++k) {
     const uint32_t i = (lb * ppt + k) * kBlock + threadIdx.x;
     const bool valid = i < I.n_src;
-    const float4 p = I.src[valid ? i : 0u];
+    const float4 p = ldg(I.src + (valid ? i : 0u));
     float px, py, pz;
     transform_point(P, p, px, py, pz);
-    const uint32_t prev = valid ? I.match[i] : kNone;
+    const uint32_t prev = valid ? ldg(I.match + (i)) : kNone;
     // The last search left this point a movement budget (half its gap to the
     // second-nearest point); while the pose moves it less, the match stands.
-    const float bud = valid ? I.budget[i] : 0.0f;
+    const float bud = valid ? ldg(I.budget + (i)) : 0.0f;
     const float move =
         move_rot * (sqrtf(p.x * p.x + p.y * p.y + p.z * p.z) * 1.000001f) + move_const;
     const bool keep = prev != kNone && bud > move;
@@ -1090,11 +1125,11 @@ __global__ void __launch_bounds__(kBlock)
     }
     const bool hard = valid && Q.hard;
     queued = queued || __ballot(hard) != 0ull;
-    enqueue_hard(ps, hard, Q, px, py, pz, inst_id, i, I.budget);
+    enqueue_hard_block(ps, hard, Q, px, py, pz, inst_id, i, I.budget);
     if (!valid || hard) continue;
     if (keep) {
       if constexpr (kStats) stats_add(stats, 0, 0, 0, kBlockDone);
-      const float4 q = I.tgt[prev];
+      const float4 q = ldg(I.tgt + (prev));
       if (dist2(px, py, pz, q.x, q.y, q.z) <= cfg.dmax2) {
         I.budget[i] = bud - move;
       } else {  // the nearest point is beyond d_max: no correspondence, search next time
@@ -1119,13 +1154,13 @@ __global__ void __launch_bounds__(kBlock)
   for (uint32_t k = 0; k < ppt; ++k) {
     const uint32_t i = (lb * ppt + k) * kBlock + threadIdx.x;
     if (i >= I.n_src) continue;
-    const uint32_t j = I.match[i];
+    const uint32_t j = ldg(I.match + (i));
     if (j == kNone) continue;
     float px, py, pz;
-    transform_point(P, I.src[i], px, py, pz);
-    const float4 q = I.tgt[j];
+    transform_point(P, ldg(I.src + (i)), px, py, pz);
+    const float4 q = ldg(I.tgt + (j));
     const float d2 = dist2(px, py, pz, q.x, q.y, q.z);
-    add_terms(acc, P, beta, eps, px, py, pz, q, I.tgt_n[j], I.src_n[i], d2);
+    add_terms(acc, P, beta, eps, px, py, pz, q, ldg(I.tgt_n + (j)), ldg(I.src_n + (i)), d2);
     ++count;
   }
   write_partial(L, acc, count, I.partials + size_t(lb) * 64);
@@ -1202,7 +1237,7 @@ __global__ void __launch_bounds__(kBlock)
   for (uint32_t base = wid * G; base < qlen; base += waves * G) {
     const uint32_t e = base + uint32_t(lane / kG);
     const bool active = e < qlen;  // the same on a group's lanes
-    const HardQuery h = ps.queue[active ? e : base];
+    const HardQuery h = ldg(ps.queue + (active ? e : base));
     hard_query<kStats, kSingle, kG>(insts, h, active, stats);
   }
 }
@@ -1235,14 +1270,14 @@ __global__ void __launch_bounds__(kBlock) k_gicp_terms(const IterInstance* __res
     for (uint32_t k = 0; k < ppt; ++k) {
       const uint32_t i = (lb * ppt + k) * kBlock + threadIdx.x;
       if (i >= I.n_src) continue;
-      const uint32_t j = I.match[i];
+      const uint32_t j = ldg(I.match + (i));
       if (j == kNone) continue;
       float px, py, pz;
-      transform_point(P, I.src[i], px, py, pz);
-      const float4 q = I.tgt[j];
+      transform_point(P, ldg(I.src + (i)), px, py, pz);
+      const float4 q = ldg(I.tgt + (j));
       // the squared distance the search compared (same operations, same operands)
       const float d2 = dist2(px, py, pz, q.x, q.y, q.z);
-      add_terms(acc, P, beta, eps, px, py, pz, q, I.tgt_n[j], I.src_n[i], d2);
+      add_terms(acc, P, beta, eps, px, py, pz, q, ldg(I.tgt_n + (j)), ldg(I.src_n + (i)), d2);
       ++count;
     }
     write_partial(L, acc, count, I.partials + size_t(lb) * 64);

@@ -173,6 +173,37 @@ __device__ __forceinline__ float axis_gap(float v, float o, float h, float m, in
   return fmaxf(fmaxf(lo - v, v - (lo + h + 2.0f * m)), 0.0f);
 }
 
+// Loads through the global address space: pointers read from descriptors in
+// memory are generic, and generic (flat) loads also count against the LDS
+// counter, so every wait on them waits for LDS traffic too.
+// (float4 is read as a native vector: binding its class copy constructor
+// would turn the pointer generic again.)
+__device__ __forceinline__ uint32_t ldg(const uint32_t* p) {
+  return *(const __attribute__((address_space(1))) uint32_t*)p;
+}
+__device__ __forceinline__ uint8_t ldg(const uint8_t* p) {
+  return *(const __attribute__((address_space(1))) uint8_t*)p;
+}
+__device__ __forceinline__ float ldg(const float* p) {
+  return *(const __attribute__((address_space(1))) float*)p;
+}
+// plain structs (no user-defined copy constructor)
+template <typename T>
+__device__ __forceinline__ T ldg(const T* p) {
+#ifdef __HIP_DEVICE_COMPILE__
+  T v;
+  __builtin_memcpy(&v, (const __attribute__((address_space(1))) T*)p, sizeof(T));
+  return v;
+#else
+  return *p;
+#endif
+}
+__device__ __forceinline__ float4 ldg(const float4* p) {
+  typedef float v4f __attribute__((ext_vector_type(4)));
+  const v4f v = *(const __attribute__((address_space(1))) v4f*)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+
 // Visits the points of N contiguous ranges [lo[k], hi[k]) of a grid-ordered
 // array. W independent loads per step (indices clamped into the range, the
 // extra lanes of a step skipped) keep several requests in flight per lane;
@@ -184,14 +215,14 @@ __device__ __forceinline__ void stream_ranges(const uint32_t (&lo)[N], const uin
 #pragma unroll
   for (int k = 0; k < N; ++k) {
     if (W == 1) {
-      for (uint32_t s = lo[k]; s < hi[k]; ++s) visit(s, pts[s]);
+      for (uint32_t s = lo[k]; s < hi[k]; ++s) visit(s, ldg(pts + s));
       continue;
     }
     for (uint32_t s = lo[k]; s < hi[k]; s += W) {
       const uint32_t last = hi[k] - 1;
       float4 q[W];
 #pragma unroll
-      for (int w = 0; w < W; ++w) q[w] = pts[min(s + uint32_t(w), last)];
+      for (int w = 0; w < W; ++w) q[w] = ldg(pts + min(s + uint32_t(w), last));
       visit(s, q[0]);
 #pragma unroll
       for (int w = 1; w < W; ++w)
@@ -234,7 +265,7 @@ __device__ __forceinline__ void stream_concat(const uint32_t (&lo)[N], const uin
 #pragma unroll
     for (int w = 0; w < W; ++w) {
       pos[w] = min(c + uint32_t(w), e - 1u) + d;  // padding reloads the range's last point
-      q[w] = pts[pos[w]];
+      q[w] = ldg(pts + pos[w]);
     }
 #pragma unroll
     for (int w = 0; w < W; ++w)
@@ -290,8 +321,8 @@ __device__ __forceinline__ void scan_ring(const GridDesc& g, const uint32_t* __r
             x1 = min(x1, cell_coord_free(px + rx, g.ox, g.inv_h) + 1);
           }
           if (x0 > x1) continue;
-          lo[0] = cs[cbase + uint32_t(x0)];
-          hi[0] = cs[cbase + uint32_t(x1) + 1];
+          lo[0] = ldg(cs + (cbase + uint32_t(x0)));
+          hi[0] = ldg(cs + (cbase + uint32_t(x1) + 1));
         } else {
           // interior row of the ring: only its two end cells are new
           const int ends[2] = {hx - r, hx + r};
@@ -301,8 +332,8 @@ __device__ __forceinline__ void scan_ring(const GridDesc& g, const uint32_t* __r
             if (cx < 0 || cx >= g.dx) continue;
             const float gx = axis_gap(px, g.ox, g.h, g.margin, cx);
             if (gyz + gx * gx > thr) continue;
-            lo[e] = cs[cbase + uint32_t(cx)];
-            hi[e] = cs[cbase + uint32_t(cx) + 1];
+            lo[e] = ldg(cs + (cbase + uint32_t(cx)));
+            hi[e] = ldg(cs + (cbase + uint32_t(cx) + 1));
           }
         }
         ++rows;
@@ -340,8 +371,8 @@ __device__ __forceinline__ void scan_block27(const GridDesc& g, const uint32_t* 
     lo[k] = hi[k] = 0;
     if (cz >= 0 && cz < g.dz && cy >= 0 && cy < g.dy) {
       const uint32_t base = (uint32_t(cz) * uint32_t(g.dy) + uint32_t(cy)) * uint32_t(g.dx);
-      lo[k] = cs[base + uint32_t(x0)];
-      hi[k] = cs[base + uint32_t(x1) + 1];
+      lo[k] = ldg(cs + (base + uint32_t(x0)));
+      hi[k] = ldg(cs + (base + uint32_t(x1) + 1));
     }
   }
   if constexpr (kConcat)
@@ -380,8 +411,8 @@ __device__ __forceinline__ void scan_block27_within(const GridDesc& g, const uin
       const int xa = (hx > 0 && gyz + gxl * gxl <= thr) ? hx - 1 : hx;
       const int xb = (hx + 1 < g.dx && gyz + gxr * gxr <= thr) ? hx + 1 : hx;
       const uint32_t base = (uint32_t(cz) * uint32_t(g.dy) + uint32_t(cy)) * uint32_t(g.dx);
-      lo[k] = cs[base + uint32_t(xa)];
-      hi[k] = cs[base + uint32_t(xb) + 1];
+      lo[k] = ldg(cs + (base + uint32_t(xa)));
+      hi[k] = ldg(cs + (base + uint32_t(xb) + 1));
     }
   }
   if constexpr (kConcat)
