@@ -302,15 +302,33 @@ void upload(void* dst, const void* src, size_t bytes, hipStream_t s) {
 
 // A small device-to-host read that the host waits for: through a per-thread
 // pinned buffer (a direct DMA), then copied out; the stream is synchronised.
+// Buffers waiting for a download() on their stream (release_after_sync).
+static thread_local std::vector<std::pair<void*, hipStream_t>> t_after_sync;
+
+void release_after_sync(void* p, hipStream_t s) { t_after_sync.emplace_back(p, s); }
+
+// After a wait for stream s: release what was left to it.
+static void released_by_sync(hipStream_t s) {
+  auto& v = t_after_sync;
+  size_t k = 0;
+  for (auto& e : v) {
+    if (e.second == s) dev_free(e.first);
+    else v[k++] = e;
+  }
+  v.resize(k);
+}
+
 void download(void* dst, const void* src, size_t bytes, hipStream_t s) {
   constexpr size_t kPinned = size_t(256) << 10;
   if (bytes == 0) {
     SR_HIP(hipStreamSynchronize(s));
+    released_by_sync(s);
     return;
   }
   if (t_arena || bytes > kPinned) {
     SR_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
     SR_HIP(hipStreamSynchronize(s));
+    released_by_sync(s);
     return;
   }
   struct Pinned {
@@ -323,6 +341,7 @@ void download(void* dst, const void* src, size_t bytes, hipStream_t s) {
   if (!pb.p) SR_HIP(hipHostMalloc(&pb.p, kPinned, hipHostMallocDefault));
   SR_HIP(hipMemcpyAsync(pb.p, src, bytes, hipMemcpyDeviceToHost, s));
   SR_HIP(hipStreamSynchronize(s));
+  released_by_sync(s);
   std::memcpy(dst, pb.p, bytes);
 }
 

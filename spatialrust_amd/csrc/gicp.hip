@@ -39,10 +39,11 @@ __device__ __forceinline__ void q_to_mat64(const double q0[4], double R[3][3]) {
   double len = sqrt(q0[0] * q0[0] + q0[1] * q0[1] + q0[2] * q0[2] + q0[3] * q0[3]);
   double x = 0, y = 0, z = 0, w = 1;
   if (len != 0.0) {
-    x = q0[0] / len;
-    y = q0[1] / len;
-    z = q0[2] / len;
-    w = q0[3] / len;
+    const double il = 1.0 / len;  // one division, four products (within an ulp)
+    x = q0[0] * il;
+    y = q0[1] * il;
+    z = q0[2] * il;
+    w = q0[3] * il;
   }
   double xx = x * x, yy = y * y, zz = z * z, xy = x * y, xz = x * z, yz = y * z;
   double wx = w * x, wy = w * y, wz = w * z;
@@ -106,7 +107,7 @@ void init_states(InstState* states, const double* init7_dev, uint32_t count, hip
 // Every loop bound is a constant, so the factor stays in registers: this runs
 // on one thread of the finalize kernel, whose registers cost nothing else.
 __device__ __forceinline__ bool ldlt_solve(const double* Hp, const double* g, double* x) {
-  double L[6][6], D[6], y[6];
+  double L[6][6], D[6], iD[6], y[6];
 #pragma unroll
   for (int j = 0; j < 6; ++j) {
     double d = Hp[ut(j, j)];
@@ -114,12 +115,13 @@ __device__ __forceinline__ bool ldlt_solve(const double* Hp, const double* g, do
     for (int k = 0; k < j; ++k) d -= L[j][k] * L[j][k] * D[k];
     if (!(d >= 1e-12)) return false;
     D[j] = d;
+    iD[j] = 1.0 / d;  // one division per pivot; the column takes products
 #pragma unroll
     for (int i = j + 1; i < 6; ++i) {
       double v = Hp[ut(j, i)];
 #pragma unroll
       for (int k = 0; k < j; ++k) v -= L[i][k] * L[j][k] * D[k];
-      L[i][j] = v / d;
+      L[i][j] = v * iD[j];
     }
   }
 #pragma unroll
@@ -131,7 +133,7 @@ __device__ __forceinline__ bool ldlt_solve(const double* Hp, const double* g, do
   }
 #pragma unroll
   for (int i = 5; i >= 0; --i) {
-    double v = y[i] / D[i];
+    double v = y[i] * iD[i];
 #pragma unroll
     for (int k = i + 1; k < 6; ++k) v -= L[k][i] * x[k];
     x[i] = v;
@@ -178,11 +180,12 @@ __device__ __forceinline__ void finalize(InstState* st, const InstConfig& cfg, c
   const double angle = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
   double dq[4] = {0.0, 0.0, 0.0, 1.0};
   if (angle > 1e-12) {
-    double ax = x[0] / angle, ay = x[1] / angle, az = x[2] / angle;
-    double al = sqrt(ax * ax + ay * ay + az * az);
-    ax /= al;
-    ay /= al;
-    az /= al;
+    const double ia = 1.0 / angle;
+    double ax = x[0] * ia, ay = x[1] * ia, az = x[2] * ia;
+    const double ial = 1.0 / sqrt(ax * ax + ay * ay + az * az);
+    ax *= ial;
+    ay *= ial;
+    az *= ial;
     double sh, ch;
     const double h = 0.5 * angle;
     if (h < 0.0625) {
@@ -202,8 +205,8 @@ __device__ __forceinline__ void finalize(InstState* st, const InstConfig& cfg, c
     dq[1] = ay * sh;
     dq[2] = az * sh;
     dq[3] = ch;
-    double l = sqrt(dq[0] * dq[0] + dq[1] * dq[1] + dq[2] * dq[2] + dq[3] * dq[3]);
-    for (int a = 0; a < 4; ++a) dq[a] /= l;
+    const double il = 1.0 / sqrt(dq[0] * dq[0] + dq[1] * dq[1] + dq[2] * dq[2] + dq[3] * dq[3]);
+    for (int a = 0; a < 4; ++a) dq[a] *= il;
   }
   const double* q = st->q;
   double nq[4] = {dq[3] * q[0] + dq[0] * q[3] + dq[1] * q[2] - dq[2] * q[1],
@@ -225,7 +228,8 @@ __device__ __forceinline__ void finalize(InstState* st, const InstConfig& cfg, c
     st->move_rot = float(angle * (1.0 + 1e-6) + 1e-30);
     st->move_const = float((angle * tn + dn) * (1.0 + 1e-6) + 1e-30);
   }
-  for (int a = 0; a < 4; ++a) st->q[a] = nq[a] / l;
+  const double il = 1.0 / l;
+  for (int a = 0; a < 4; ++a) st->q[a] = nq[a] * il;
   for (int a = 0; a < 3; ++a) st->t[a] = nt[a];
   refresh_f32_pose(st);
   double upd = 0.0;  // update_magnitude (gicp.rs:326-328)

@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 #include "internal.hpp"
 
@@ -720,12 +721,23 @@ __global__ void k_seg_empty_pass(const uint8_t* __restrict__ in, uint8_t* __rest
   empty_pass_cell(in + G.cell_off, out + G.cell_off, G.d, axis, t - G.cell_off);
 }
 
-template <typename T>
-static void upload_vec(DevBuf<T>& d, const std::vector<T>& h, hipStream_t s) {
-  d.resize(std::max<size_t>(h.size(), 1));
-  if (!h.empty())
-    upload(d.p, h.data(), sizeof(T) * h.size(), s);
-}
+// Host arrays packed for one upload: add() returns each array's byte offset
+// (256-byte aligned), at() the device pointer at an offset of the uploaded copy.
+struct HostPack {
+  std::vector<uint8_t> bytes;
+  template <typename T>
+  size_t add(const std::vector<T>& v) {
+    const size_t o = (bytes.size() + 255) & ~size_t(255);
+    bytes.resize(o + sizeof(T) * std::max<size_t>(v.size(), 1));
+    if (!v.empty()) std::memcpy(bytes.data() + o, v.data(), sizeof(T) * v.size());
+    return o;
+  }
+  template <typename T>
+  static const T* at(const uint8_t* base, size_t o) {
+    return reinterpret_cast<const T*>(base + o);
+  }
+};
+
 
 static std::vector<uint32_t> blocks_of(const std::vector<uint32_t>& off) {
   std::vector<uint32_t> b(off.size());
@@ -803,31 +815,37 @@ void build_grids_many(const Cloud& c, const std::vector<uint32_t>& off,
     out.desc[g] = sg[g].d;
   }
   std::vector<uint32_t> blk = blocks_of(off);
-  DevBuf<SegGrid> d_sg;
-  DevBuf<uint32_t> d_blk, d_word, d_row, d_cs, d_cell, d_pt;
-  upload_vec(d_sg, sg, s);
-  upload_vec(d_blk, blk, s);
-  upload_vec(d_word, word_begin, s);
-  upload_vec(d_row, row_begin, s);
-  upload_vec(d_cs, cs_begin, s);
   std::vector<uint32_t> cell_begin(out.cell_off.begin(), out.cell_off.end() - 1);
-  upload_vec(d_cell, cell_begin, s);
-  upload_vec(d_pt, off, s);
+  // the per-segment tables go up in one copy (each small copy costs a few
+  // microseconds of its own on the stream)
+  HostPack pk;
+  const size_t o_sg = pk.add(sg), o_blk = pk.add(blk), o_word = pk.add(word_begin),
+               o_row = pk.add(row_begin), o_cs = pk.add(cs_begin), o_cell = pk.add(cell_begin),
+               o_pt = pk.add(off);
+  DevBuf<uint8_t> tables(pk.bytes.size());
+  upload(tables.p, pk.bytes.data(), pk.bytes.size(), s);
+  const SegGrid* d_sg = HostPack::at<SegGrid>(tables.p, o_sg);
+  const uint32_t *d_blk = HostPack::at<uint32_t>(tables.p, o_blk),
+                 *d_word = HostPack::at<uint32_t>(tables.p, o_word),
+                 *d_row = HostPack::at<uint32_t>(tables.p, o_row),
+                 *d_cs = HostPack::at<uint32_t>(tables.p, o_cs),
+                 *d_cell = HostPack::at<uint32_t>(tables.p, o_cell),
+                 *d_pt = HostPack::at<uint32_t>(tables.p, o_pt);
   DevBuf<uint32_t> counts(size_t(ncells) + 1), csg(size_t(ncells) + 1), cid(std::max<uint32_t>(n, 1)),
       rnk(std::max<uint32_t>(n, 1)), order(std::max<uint32_t>(n, 1)), scal(2);
   fill_u32(counts.p, 0u, size_t(ncells) + 1, s);
-  hipLaunchKernelGGL(k_seg_grid_count, dim3(blk.back()), dim3(kBlock), 0, s, c.pts.p, d_sg.p, nseg,
-                     d_blk.p, counts.p, cid.p, rnk.p);
+  hipLaunchKernelGGL(k_seg_grid_count, dim3(blk.back()), dim3(kBlock), 0, s, c.pts.p, d_sg, nseg,
+                     d_blk, counts.p, cid.p, rnk.p);
   exclusive_scan_u32(counts.p, csg.p, ncells + 1, nullptr, s);
   auto grid1 = [](uint64_t t) { return dim3(uint32_t(std::max<uint64_t>((t + kBlock - 1) / kBlock, 1))); };
-  hipLaunchKernelGGL(k_seg_local_cs, grid1(ncs), dim3(kBlock), 0, s, csg.p, d_sg.p, nseg, d_cs.p, ncs,
+  hipLaunchKernelGGL(k_seg_local_cs, grid1(ncs), dim3(kBlock), 0, s, csg.p, d_sg, nseg, d_cs, ncs,
                      out.cell_start.p);
   if (words)
-    hipLaunchKernelGGL(k_seg_grid_bits, grid1(words), dim3(kBlock), 0, s, out.cell_start.p, d_sg.p,
-                       nseg, d_word.p, uint32_t(words), out.occ.p);
+    hipLaunchKernelGGL(k_seg_grid_bits, grid1(words), dim3(kBlock), 0, s, out.cell_start.p, d_sg,
+                       nseg, d_word, uint32_t(words), out.occ.p);
   if (rws)
-    hipLaunchKernelGGL(k_seg_grid_rowbits, grid1(32 * rws), dim3(kBlock), 0, s, out.occ.p, d_sg.p, nseg,
-                       d_row.p, uint32_t(rws), out.rowocc.p);
+    hipLaunchKernelGGL(k_seg_grid_rowbits, grid1(32 * rws), dim3(kBlock), 0, s, out.occ.p, d_sg, nseg,
+                       d_row, uint32_t(rws), out.rowocc.p);
   if (n) {
     hipLaunchKernelGGL(k_grid_scatter, grid1(n), dim3(kBlock), 0, s, cid.p, rnk.p, n, csg.p, order.p);
     fill_u32(scal.p, 0u, 2, s);  // big and medium cell counts
@@ -837,7 +855,7 @@ void build_grids_many(const Cloud& c, const std::vector<uint32_t>& off,
     hipLaunchKernelGGL(k_sort_medium_cells, dim3(512), dim3(kBlock), 0, s, csg.p, ncells, order.p,
                        big.p, scal.p);
     hipLaunchKernelGGL(k_sort_big_cells, dim3(128), dim3(kBlock), 0, s, csg.p, order.p, big.p, scal.p);
-    hipLaunchKernelGGL(k_seg_grid_gather, grid1(n), dim3(kBlock), 0, s, c.pts.p, order.p, d_pt.p, nseg,
+    hipLaunchKernelGGL(k_seg_grid_gather, grid1(n), dim3(kBlock), 0, s, c.pts.p, order.p, d_pt, nseg,
                        n, out.pts.p);
   }
   empty_segs = std::min(empty_segs, nseg);
@@ -846,13 +864,13 @@ void build_grids_many(const Cloud& c, const std::vector<uint32_t>& off,
     // ping-pong between two buffers; the third pass lands in `tmp`, which
     // then becomes the result
     DevBuf<uint8_t> er(ecells), tmp(ecells);
-    hipLaunchKernelGGL(k_seg_empty_init, grid1(ecells), dim3(kBlock), 0, s, out.cell_start.p, d_sg.p,
-                       empty_segs, d_cell.p, ecells, er.p);
+    hipLaunchKernelGGL(k_seg_empty_init, grid1(ecells), dim3(kBlock), 0, s, out.cell_start.p, d_sg,
+                       empty_segs, d_cell, ecells, er.p);
     for (int axis = 0; axis < 3; ++axis) {
       const uint8_t* in = axis == 1 ? tmp.p : er.p;
       uint8_t* o = axis == 1 ? er.p : tmp.p;
-      hipLaunchKernelGGL(k_seg_empty_pass, grid1(ecells), dim3(kBlock), 0, s, in, o, d_sg.p,
-                         empty_segs, d_cell.p, ecells, axis);
+      hipLaunchKernelGGL(k_seg_empty_pass, grid1(ecells), dim3(kBlock), 0, s, in, o, d_sg,
+                         empty_segs, d_cell, ecells, axis);
     }
     out.empty_radius = std::move(tmp);
   }

@@ -38,6 +38,10 @@ void dev_free(void* p);
 // True while the calling thread runs the allocation-free entry point: its
 // buffers then come from the library's static scratch array.
 bool arena_active();
+// Hands device memory that kernels queued on stream s may still read to the
+// calling thread's next download() on s (which waits for the stream), which
+// releases it; lets a host function return without waiting for its kernels.
+void release_after_sync(void* p, hipStream_t s);
 
 // RAII buffer from the caching allocator.
 template <typename T>
@@ -80,6 +84,15 @@ struct DevBuf {
     release();
     p = static_cast<T*>(dev_alloc(count * sizeof(T) > 0 ? count * sizeof(T) : 16));
     n = count;
+  }
+  // Leaves the memory to release_after_sync (see there); false inside the
+  // allocation-free entry point, whose caller must then wait for s itself.
+  bool release_after(hipStream_t s) {
+    if (arena_active()) return false;
+    if (p && owned) release_after_sync(p, s);
+    p = nullptr;
+    n = 0;
+    return true;
   }
   void release() {
     if (p && owned) dev_free(p);
@@ -162,8 +175,8 @@ struct SegSource {
 };
 // Packs many device xyz columns into one float4 cloud, segment after segment.
 void pack_many(const std::vector<SegSource>& srcs, Cloud& out, hipStream_t s);
-// Bounds (6 ordered words per segment) of segments [off[s], off[s+1]) of pts;
-// synchronises the stream.
+// Bounds (6 ordered words per segment) of segments [off[s], off[s+1]) of pts
+// into dev_bounds (stream-ordered; the host does not wait).
 void seg_bounds(const float4* pts, const std::vector<uint32_t>& off, uint32_t* dev_bounds,
                 hipStream_t s);
 // voxel_downsample of every segment [off[s], off[s+1]) of `in` (origin = the

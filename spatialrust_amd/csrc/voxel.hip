@@ -808,7 +808,8 @@ void pack_many(const std::vector<SegSource>& srcs, Cloud& out, hipStream_t s) {
   upload(d_blk, blk, s);
   hipLaunchKernelGGL(k_seg_pack, dim3(blk.back()), dim3(kBlock), 0, s, d_src.p,
                      uint32_t(srcs.size()), d_blk.p, out.pts.p);
-  SR_HIP(hipStreamSynchronize(s));  // the tables above are released on return
+  // the tables outlive the kernel: released at the next wait on s
+  if (!d_src.release_after(s) || !d_blk.release_after(s)) SR_HIP(hipStreamSynchronize(s));
 }
 
 void seg_bounds(const float4* pts, const std::vector<uint32_t>& off, uint32_t* dev_bounds,
@@ -822,7 +823,7 @@ void seg_bounds(const float4* pts, const std::vector<uint32_t>& off, uint32_t* d
                      dev_bounds, nseg);
   hipLaunchKernelGGL(k_seg_bounds, dim3(blk.back()), dim3(kBlock), 0, s, pts, d_off.p, nseg, d_blk.p,
                      dev_bounds);
-  SR_HIP(hipStreamSynchronize(s));
+  if (!d_off.release_after(s) || !d_blk.release_after(s)) SR_HIP(hipStreamSynchronize(s));
 }
 
 void voxel_downsample_many(const Cloud& in, const std::vector<uint32_t>& off, float leaf,
