@@ -765,6 +765,8 @@ constexpr uint32_t kHardBlockFlag = 0x80000000u;  // HardQuery.inst: block scann
 #ifndef SR_HARD_WAVES
 #define SR_HARD_WAVES 8
 #endif
+// Consecutive queue slices per XCD chunk in the queue stage (see k_gicp_hard).
+constexpr uint32_t kHardChunk = 64;
 
 #ifndef SR_SEARCH_WAVES
 #define SR_SEARCH_WAVES 4
@@ -814,6 +816,7 @@ __device__ __forceinline__ void enqueue_hard_block(const PassScratch& ps, bool h
   const int lane = int(threadIdx.x & (kWave - 1)), wave = int(threadIdx.x / kWave);
   const unsigned long long below = (1ull << lane) - 1ull;
   const unsigned long long hm = __ballot(hard);
+  __syncthreads();  // a previous call's readers of wcount / bbase are done
   if (lane == 0) wcount[wave] = uint32_t(__popcll(hm));
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -1232,12 +1235,18 @@ __global__ void __launch_bounds__(kBlock)
   constexpr uint32_t G = kWave / kG;
   constexpr uint32_t kWavesPerBlock = kBlock / kWave;
   const uint32_t waves = gridDim.x * kWavesPerBlock;
-  // Slices in hardware dispatch order: consecutive slices go to different
-  // XCDs, and slices beyond the resident waves go to waves dispatched as
-  // earlier ones finish. The queue is in source order and hard queries cluster
-  // in space, so giving each XCD a contiguous share (as the other stages do
-  // for locality) left some XCDs with most of the expensive slices.
-  const uint32_t wid = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+  // Slice order. The queue is in source order and hard queries cluster in
+  // space, so giving each XCD one contiguous share of it (as the other stages
+  // do for locality) left some XCDs with most of the expensive slices. Here XCD
+  // x (workgroup id mod 8) takes chunks x, x + 8, x + 16, ... of kHardChunk
+  // consecutive slices, in dispatch order: every XCD samples the whole queue,
+  // a chunk's queries still share target cells in its L2, and slices beyond
+  // the resident waves go to waves dispatched as earlier ones finish. (A
+  // bijection on the wave ids: hard_stage_blocks makes the grid a multiple of
+  // 2 * kHardChunk workgroups.)
+  const uint32_t xcd = blockIdx.x & 7u;
+  const uint32_t j = (blockIdx.x >> 3) * kWavesPerBlock + threadIdx.x / kWave;
+  const uint32_t wid = ((j / kHardChunk) * 8u + xcd) * kHardChunk + (j % kHardChunk);
   for (uint32_t base = wid * G; base < qlen; base += waves * G) {
     const uint32_t e = base + uint32_t(lane / kG);
     const bool active = e < qlen;  // the same on a group's lanes
@@ -1371,7 +1380,10 @@ uint32_t hard_stage_blocks(uint64_t points, uint32_t cap) {
   // slots free; queues beyond the cap take several rounds per wave)
   const uint64_t per_block = uint64_t(kBlock / kWave) * (kWave / kNNGroup);
   const uint64_t want = (points + per_block - 1) / per_block;
-  return uint32_t(std::min<uint64_t>(std::max<uint64_t>(want, 1), std::max<uint32_t>(cap, 1)));
+  uint64_t b = std::min<uint64_t>(std::max<uint64_t>(want, 1), std::max<uint32_t>(cap, 1));
+  constexpr uint64_t kRound = 2 * kHardChunk;  // the queue stage's slice order needs it
+  b = (b + kRound - 1) / kRound * kRound;
+  return uint32_t(b);
 }
 
 // Workgroups of the terms stage, each taking a run of blocks: several rounds

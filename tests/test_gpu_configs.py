@@ -14,6 +14,9 @@ pass cannot show (gpu).
                           exactly zero row
   radius covariances      pose vs the FP64 oracle fed the oracle's radius
                           covariances: <= 1e-4 rad and <= 1 mm
+  two source points per thread (2.3M points)
+                          alone == inside a batch under every launch
+                          candidate, bit for bit; deterministic
   serving workload (the three smallest of its 32 maps, 0.50M-0.54M points, all
     their 24 sources of 2k-200k points; leaf 0.2, k 15, d_max 2.0, 40
     iterations or a pose update below 1e-6)
@@ -307,6 +310,33 @@ def test_autotune_candidates_bit_identical(hip, tune_restored):
         for c in srcs + tgts:
             c.close()
     print(sr.tune_describe(), flush=True)
+
+
+def test_two_points_per_thread_stages_agree(hip, tune_restored):
+    """Clouds above 2^21 points run two source points per thread, and the first
+    stage then appends to the queue once per point of a thread (the workgroup
+    reservation is reused within a launch). A 2.3M-point registration alone
+    (split first stage) and twice inside a batch under every batch candidate
+    (split and one-kernel first stage) give the same pose, fitness and
+    iterations, bit for bit, and the registration alone is deterministic."""
+    scene = synth.small_cube_scene(synth.SEED_BASE + 77)
+    tgt = synth.sample_scene(scene, 2_300_000, 0.01, 11)
+    q, t = synth.random_motion(78, 6.0, 0.3, per_axis=True)
+    src = synth.apply_motion(q, t, synth.sample_scene(scene, 2_300_000, 0.01, 12))
+    cfg = _fixed(max_iterations=6, k_neighbors=20, max_correspondence_distance=1.0, voxel_leaf=0.0)
+    s_dev, t_dev = devmem.DeviceCloud(src), devmem.DeviceCloud(tgt)
+    try:
+        reg = sr.GicpRegistration(cfg)
+        ref = reg.align(s_dev.view, t_dev.view)
+        assert ref.source_points >= 1 << 21
+        assert _same(reg.align(s_dev.view, t_dev.view), ref)
+        for c in range(sr.get_option("tune_candidates_batch")):
+            sr.set_option("tune_force", c)
+            got = sr.batch_align_pairs([s_dev.view, s_dev.view], [t_dev.view, t_dev.view], cfg)
+            assert _same(got[0], ref) and _same(got[1], ref), c
+    finally:
+        s_dev.close()
+        t_dev.close()
 
 
 def test_serving_workload_maps_served_equal_alone_and_fp64_golden(hip):
