@@ -385,6 +385,13 @@ constexpr int kRowBatch = SR_ROW_BATCH;  // rows per bound-loading batch in phas
 #define SR_STREAM_W 4
 #endif
 constexpr int kStreamW = SR_STREAM_W;  // point loads in flight per lane in phase 2
+#ifndef SR_RING_CONCAT
+#define SR_RING_CONCAT 4
+#endif
+// Rows per lane batched in the ring search of one-registration launches (their
+// grid descriptor sits in scalar registers; with per-lane descriptors, as in
+// batches, the batch's registers would spill).
+constexpr int kRingConcat = SR_RING_CONCAT;
 
 
 // Row t of Chebyshev ring r (r >= 1) of (y,z) rows around the query's row:
@@ -424,7 +431,7 @@ __device__ __forceinline__ void ring_row(int r, int t, int& dy, int& dz) {
 // best is known to lie at least min(sqrt(W.second), best + margin) away (see
 // hard_query for the movement budget this gives). The group's lanes scan
 // disjoint rows, so the other lanes' bests count as other points too.
-template <bool kStats, int kG = kNNGroup, bool kGap = false>
+template <bool kStats, int kG = kNNGroup, bool kGap = false, int kRB = 0>
 __device__ __forceinline__ void group_search(const GridDesc& g, const uint32_t* __restrict__ cs,
                                              const float4* __restrict__ tp, float qx, float qy,
                                              float qz, bool skip, float margin, NNQuery& W) {
@@ -449,58 +456,100 @@ __device__ __forceinline__ void group_search(const GridDesc& g, const uint32_t* 
       if (c > 0.0f && c * c > scan2(W.thr)) break;
     }
     const int nrows = r == 0 ? 1 : 8 * r;
-    // This lane's rows of the ring, kRowBatch at a time: the bounds of a
-    // batch's ranges are loaded together and then its points streamed, so a
-    // batch costs one bound round trip instead of one per row. A row whose
-    // (y,z) gap exceeds the radius contributes an empty range.
-    for (int t0 = sub; t0 < nrows; t0 += kG * kRowBatch) {
-      uint32_t ia[2 * kRowBatch], ib[2 * kRowBatch];
+    auto visit = [&](uint32_t s, const float4& q) {
+      if constexpr (kStats) ++W.pts;
+      nn_consider<kGap>(W, qx, qy, qz, s, q);
+    };
+    if (kRB > 0 && r >= 2) {
+      // kRB rows per lane at a time: their range bounds in one round of
+      // loads, then their points as one sequence (stream_concat), so a lane's
+      // rows of a ring cost about one bound round trip and one point round
+      // trip per kStreamW points instead of both per row. (Rings 0 and 1,
+      // whose rows skip the 3x3x3 block, take the per-row path.)
+      constexpr int kRB1 = kRB > 0 ? kRB : 1;
+      for (int t0 = sub; t0 < nrows; t0 += kG * kRB1) {
+        uint32_t lo[kRB1], hi[kRB1];
 #pragma unroll
-      for (int m = 0; m < kRowBatch; ++m) {
-        ia[2 * m] = ib[2 * m] = ia[2 * m + 1] = ib[2 * m + 1] = 0;
-        const int t = t0 + m * kG;
-        if (t >= nrows) continue;
-        int dy, dz;
-        ring_row(r, t, dy, dz);
-        const int cz = hz + dz, cy = hy + dy;
-        if (cz < 0 || cz >= g.dz || cy < 0 || cy >= g.dy) continue;
-        const float gz = axis_gap(qz, g.oz, g.h, g.margin, cz);
-        const float gy = axis_gap(qy, g.oy, g.h, g.margin, cy);
-        const float gyz = gz * gz + gy * gy;
-        const float R2 = scan2(W.thr);
-        if (gyz > R2) continue;
-        const float rx = __builtin_amdgcn_sqrtf(R2 - gyz) + g.margin;
-        const int x0 = max(cell_coord_free(qx - rx, g.ox, g.inv_h) - 1, 0);
-        const int x1 = min(cell_coord_free(qx + rx, g.ox, g.inv_h) + 1, g.dx - 1);
-        if (x0 > x1) continue;
-        const uint32_t cbase = (uint32_t(cz) * uint32_t(g.dy) + uint32_t(cy)) * uint32_t(g.dx);
-        if constexpr (kStats) ++W.rows;
-        if (skip && r <= 1) {
-          // left and right of the block's three cells
-          if (x0 <= hx - 2) {
-            ia[2 * m] = cbase + uint32_t(x0);
-            ib[2 * m] = cbase + uint32_t(min(x1, hx - 2)) + 1;
+        for (int m = 0; m < kRB1; ++m) {
+          uint32_t a = 0, b = 0;
+          const int t = t0 + m * kG;
+          if (t < nrows) {
+            int dy, dz;
+            ring_row(r, t, dy, dz);
+            const int cz = hz + dz, cy = hy + dy;
+            if (cz >= 0 && cz < g.dz && cy >= 0 && cy < g.dy) {
+              const float gz = axis_gap(qz, g.oz, g.h, g.margin, cz);
+              const float gy = axis_gap(qy, g.oy, g.h, g.margin, cy);
+              const float gyz = gz * gz + gy * gy;
+              const float R2 = scan2(W.thr);
+              if (gyz <= R2) {
+                const float rx = __builtin_amdgcn_sqrtf(R2 - gyz) + g.margin;
+                const int x0 = max(cell_coord_free(qx - rx, g.ox, g.inv_h) - 1, 0);
+                const int x1 = min(cell_coord_free(qx + rx, g.ox, g.inv_h) + 1, g.dx - 1);
+                if (x0 <= x1) {
+                  const uint32_t cbase = (uint32_t(cz) * uint32_t(g.dy) + uint32_t(cy)) * uint32_t(g.dx);
+                  if constexpr (kStats) ++W.rows;
+                  a = cbase + uint32_t(x0);
+                  b = cbase + uint32_t(x1) + 1;
+                }
+              }
+            }
           }
-          if (x1 >= hx + 2) {
-            ia[2 * m + 1] = cbase + uint32_t(max(x0, hx + 2));
-            ib[2 * m + 1] = cbase + uint32_t(x1) + 1;
-          }
-        } else {
-          ia[2 * m] = cbase + uint32_t(x0);
-          ib[2 * m] = cbase + uint32_t(x1) + 1;
+          lo[m] = ldg(cs + a);
+          hi[m] = ldg(cs + b);
         }
+        stream_concat<kStreamW>(lo, hi, tp, visit);
       }
-      uint32_t lo[2 * kRowBatch], hi[2 * kRowBatch];
+    } else {
+      // This lane's rows of the ring, kRowBatch at a time: the bounds of a
+      // batch's ranges are loaded together and then its points streamed, so a
+      // batch costs one bound round trip instead of one per row. A row whose
+      // (y,z) gap exceeds the radius contributes an empty range.
+      for (int t0 = sub; t0 < nrows; t0 += kG * kRowBatch) {
+        uint32_t ia[2 * kRowBatch], ib[2 * kRowBatch];
 #pragma unroll
-      for (int m = 0; m < 2 * kRowBatch; ++m) {
-        lo[m] = ldg(cs + ia[m]);
-        hi[m] = ldg(cs + ib[m]);
+        for (int m = 0; m < kRowBatch; ++m) {
+          ia[2 * m] = ib[2 * m] = ia[2 * m + 1] = ib[2 * m + 1] = 0;
+          const int t = t0 + m * kG;
+          if (t >= nrows) continue;
+          int dy, dz;
+          ring_row(r, t, dy, dz);
+          const int cz = hz + dz, cy = hy + dy;
+          if (cz < 0 || cz >= g.dz || cy < 0 || cy >= g.dy) continue;
+          const float gz = axis_gap(qz, g.oz, g.h, g.margin, cz);
+          const float gy = axis_gap(qy, g.oy, g.h, g.margin, cy);
+          const float gyz = gz * gz + gy * gy;
+          const float R2 = scan2(W.thr);
+          if (gyz > R2) continue;
+          const float rx = __builtin_amdgcn_sqrtf(R2 - gyz) + g.margin;
+          const int x0 = max(cell_coord_free(qx - rx, g.ox, g.inv_h) - 1, 0);
+          const int x1 = min(cell_coord_free(qx + rx, g.ox, g.inv_h) + 1, g.dx - 1);
+          if (x0 > x1) continue;
+          const uint32_t cbase = (uint32_t(cz) * uint32_t(g.dy) + uint32_t(cy)) * uint32_t(g.dx);
+          if constexpr (kStats) ++W.rows;
+          if (skip && r <= 1) {
+            // left and right of the block's three cells
+            if (x0 <= hx - 2) {
+              ia[2 * m] = cbase + uint32_t(x0);
+              ib[2 * m] = cbase + uint32_t(min(x1, hx - 2)) + 1;
+            }
+            if (x1 >= hx + 2) {
+              ia[2 * m + 1] = cbase + uint32_t(max(x0, hx + 2));
+              ib[2 * m + 1] = cbase + uint32_t(x1) + 1;
+            }
+          } else {
+            ia[2 * m] = cbase + uint32_t(x0);
+            ib[2 * m] = cbase + uint32_t(x1) + 1;
+          }
+        }
+        uint32_t lo[2 * kRowBatch], hi[2 * kRowBatch];
+#pragma unroll
+        for (int m = 0; m < 2 * kRowBatch; ++m) {
+          lo[m] = ldg(cs + ia[m]);
+          hi[m] = ldg(cs + ib[m]);
+        }
+        stream_ranges<kStreamW>(lo, hi, tp, visit);
       }
-      auto visit = [&](uint32_t s, const float4& q) {
-        if constexpr (kStats) ++W.pts;
-        nn_consider<kGap>(W, qx, qy, qz, s, q);
-      };
-      stream_ranges<kStreamW>(lo, hi, tp, visit);
     }
     // the group agrees on the smallest radius before the next ring
 #pragma unroll
@@ -1192,11 +1241,12 @@ __device__ __forceinline__ void hard_query(const IterInstance* __restrict__ inst
   const bool gap = (I.cfg.flags & kCfgMatchReuse) &&
                    st->move_rot * I.src_radius + st->move_const < kGapMove * g.h;
   const float margin = kGapMargin * g.h;
+  constexpr int kRB = kSingle ? kRingConcat : 0;
   if (gap)  // the 3x3x3 block is scanned again: its second-nearest counts
-    group_search<kStats, kG, true>(g, I.cell_start, I.tgt, h.px, h.py, h.pz, false, margin, W);
+    group_search<kStats, kG, true, kRB>(g, I.cell_start, I.tgt, h.px, h.py, h.pz, false, margin, W);
   else
-    group_search<kStats, kG, false>(g, I.cell_start, I.tgt, h.px, h.py, h.pz,
-                                    (h.inst & kHardBlockFlag) != 0, 0.0f, W);
+    group_search<kStats, kG, false, kRB>(g, I.cell_start, I.tgt, h.px, h.py, h.pz,
+                                         (h.inst & kHardBlockFlag) != 0, 0.0f, W);
   if (int(threadIdx.x & (kWave - 1)) % kG == 0) {
     I.match[h.i] = W.best_pos;
     if (gap && W.best_pos != kNone) {
