@@ -299,6 +299,14 @@ constexpr int kBlockW = SR_BLOCK_W;  // point loads in flight per lane in the ph
 constexpr float kGapMargin = 0.05f;
 constexpr float kGapMove = 0.01f;
 
+// Movement budget of a query with no point within dmax, when every point is
+// certified at least D away: while the query moves less than D - dmax (less a
+// rounding slack), it still has none.
+__device__ __forceinline__ float none_budget(float D, float dmax, float px, float py, float pz) {
+  const float slack = 1e-6f * (fabsf(px) + fabsf(py) + fabsf(pz) + D + 1.0f);
+  return fmaxf(D - dmax - slack, 0.0f);
+}
+
 template <bool kStats, bool kGap>
 __device__ __forceinline__ void nn_phase1(const GridDesc& g, const uint32_t* __restrict__ cs,
                                           const uint8_t* __restrict__ empty,
@@ -321,6 +329,8 @@ __device__ __forceinline__ void nn_phase1(const GridDesc& g, const uint32_t* __r
     float lb = bx * bx + by * by + bz * bz;
     if (lb > dmax2 + 2.0f * dmax * g.margin + g.margin * g.margin) {
       if constexpr (kStats) stats_add(stats, 0, 0, 0, kRejected);
+      // every point lies within the box widened by the filing margin
+      if constexpr (kGap) Q.budget = none_budget(sqrtf(lb) - g.margin, dmax, px, py, pz);
       return;
     }
   }
@@ -334,8 +344,10 @@ __device__ __forceinline__ void nn_phase1(const GridDesc& g, const uint32_t* __r
     // Rings closer than the empty radius hold no point, so every point lies
     // outside the box of rings 0..e-1: its clearance bounds the distance.
     e = ldg(empty + ((uint32_t(hz) * uint32_t(g.dy) + uint32_t(hy)) * uint32_t(g.dx) + uint32_t(hx)));
-    if (e > 0 && ring_clearance(g, px, py, pz, hx, hy, hz, e - 1) > dmax) {
+    const float ce = e > 0 ? ring_clearance(g, px, py, pz, hx, hy, hz, e - 1) : 0.0f;
+    if (ce > dmax) {
       if constexpr (kStats) stats_add(stats, 0, 0, 0, kRejected);
+      if constexpr (kGap) Q.budget = none_budget(ce, dmax, px, py, pz);
       return;
     }
   }
@@ -356,15 +368,20 @@ __device__ __forceinline__ void nn_phase1(const GridDesc& g, const uint32_t* __r
     const float cl = ring_clearance(g, px, py, pz, hx, hy, hz, 1);
     if (cl > 0.0f && Q.thr < cl * cl) {
       if constexpr (kStats) stats_add(stats, 1, 0, Q.pts, kBlockDone);
-      if (kGap && Q.best_pos != kNone) {
+      if (kGap) {
         // Every other point lies at least `other` away: scanned ones at
         // sqrt(second), unscanned cells of the block beyond rclip, cells
         // outside it beyond cl. While the point moves less than half the gap
-        // (less a rounding slack), its nearest point stays the same.
-        const float best = sqrtf(Q.thr);
+        // (less a rounding slack), its nearest point stays the same; with no
+        // point within dmax, every point is an other one.
         const float other = fminf(fminf(sqrtf(Q.second), rclip), cl);
-        const float slack = 1e-6f * (fabsf(px) + fabsf(py) + fabsf(pz) + best + 1.0f);
-        Q.budget = fmaxf(0.5f * (other - best) - slack, 0.0f);
+        if (Q.best_pos != kNone) {
+          const float best = sqrtf(Q.thr);
+          const float slack = 1e-6f * (fabsf(px) + fabsf(py) + fabsf(pz) + best + 1.0f);
+          Q.budget = fmaxf(0.5f * (other - best) - slack, 0.0f);
+        } else {
+          Q.budget = none_budget(other, dmax, px, py, pz);
+        }
       }
       return;
     }
@@ -925,7 +942,7 @@ __global__ void __launch_bounds__(kBlock) k_gicp_settle(const IterInstance* __re
   const float bud = valid ? ldg(I.budget + (i)) : 0.0f;
   const float move =
       move_rot * (sqrtf(p.x * p.x + p.y * p.y + p.z * p.z) * 1.000001f) + move_const;
-  const bool keep = valid && prev != kNone && bud > move;
+  const bool keep = valid && bud > move;  // a match, or none (kNone), certified
   const bool search = valid && !keep;
   const unsigned long long sm = __ballot(search);
   if ((threadIdx.x & (kWave - 1)) == 0) ps.pend[size_t(b) * (kBlock / kWave) + threadIdx.x / kWave] = sm;
@@ -934,16 +951,20 @@ __global__ void __launch_bounds__(kBlock) k_gicp_settle(const IterInstance* __re
   bool counted = false;
   if (keep) {
     if constexpr (kStats) stats_add(stats, 0, 0, 0, kBlockDone);
-    transform_point(P, p, px, py, pz);
-    q = ldg(I.tgt + (prev));
-    nt = ldg(I.tgt_n + (prev));
-    d2 = dist2(px, py, pz, q.x, q.y, q.z);
-    if (d2 <= cfg.dmax2) {
+    if (prev == kNone) {
       I.budget[i] = bud - move;
-      counted = true;
-    } else {  // the nearest point is beyond d_max: no correspondence, search next time
-      I.match[i] = kNone;
-      I.budget[i] = 0.0f;
+    } else {
+      transform_point(P, p, px, py, pz);
+      q = ldg(I.tgt + (prev));
+      nt = ldg(I.tgt_n + (prev));
+      d2 = dist2(px, py, pz, q.x, q.y, q.z);
+      if (d2 <= cfg.dmax2) {
+        I.budget[i] = bud - move;
+        counted = true;
+      } else {  // the nearest point is beyond d_max: no correspondence, search next time
+        I.match[i] = kNone;
+        I.budget[i] = 0.0f;
+      }
     }
   }
   const bool settled = __syncthreads_and(search ? 0 : 1);
@@ -998,7 +1019,7 @@ __device__ __forceinline__ void search_block(const IterInstance* __restrict__ in
     const float bud = valid ? ldg(I.budget + (i)) : 0.0f;
     const float move =
         move_rot * (sqrtf(p.x * p.x + p.y * p.y + p.z * p.z) * 1.000001f) + move_const;
-    const bool keep = prev != kNone && bud > move;
+    const bool keep = bud > move;  // a match, or none (kNone), certified
     NNQuery Q;
     Q.hard = false;
     if (!keep) {
@@ -1014,6 +1035,10 @@ __device__ __forceinline__ void search_block(const IterInstance* __restrict__ in
     if (!valid || hard) continue;
     if (keep) {
       if constexpr (kStats) stats_add(stats, 0, 0, 0, kBlockDone);
+      if (prev == kNone) {
+        I.budget[i] = bud - move;
+        continue;
+      }
       const float4 q = ldg(I.tgt + (prev));
       if (dist2(px, py, pz, q.x, q.y, q.z) <= cfg.dmax2) {
         I.budget[i] = bud - move;
@@ -1125,14 +1150,17 @@ __global__ void __launch_bounds__(kBlock)
     const float bud = valid ? ldg(I.budget + (i)) : 0.0f;
     const float move =
         move_rot * (sqrtf(p.x * p.x + p.y * p.y + p.z * p.z) * 1.000001f) + move_const;
-    const bool keep = prev != kNone && bud > move;
+    const bool keep = bud > move;  // a match, or none (kNone), certified
     if (__syncthreads_and(!valid || keep)) {
       if (threadIdx.x == 0) ps.done[b] = 1u;
       float acc[kTerms];
 #pragma unroll
       for (int t = 0; t < kTerms; ++t) acc[t] = 0.0f;
       uint32_t count = 0;
-      if (valid) {
+      if (valid && prev == kNone) {
+        if constexpr (kStats) stats_add(stats, 0, 0, 0, kBlockDone);
+        I.budget[i] = bud - move;
+      } else if (valid) {
         if constexpr (kStats) stats_add(stats, 0, 0, 0, kBlockDone);
         float px, py, pz;
         transform_point(P, p, px, py, pz);
@@ -1168,7 +1196,7 @@ __global__ void __launch_bounds__(kBlock)
     const float bud = valid ? ldg(I.budget + (i)) : 0.0f;
     const float move =
         move_rot * (sqrtf(p.x * p.x + p.y * p.y + p.z * p.z) * 1.000001f) + move_const;
-    const bool keep = prev != kNone && bud > move;
+    const bool keep = bud > move;  // a match, or none (kNone), certified
     NNQuery Q;
     Q.hard = false;
     if (!keep) {
@@ -1185,6 +1213,10 @@ __global__ void __launch_bounds__(kBlock)
     if (!valid || hard) continue;
     if (keep) {
       if constexpr (kStats) stats_add(stats, 0, 0, 0, kBlockDone);
+      if (prev == kNone) {
+        I.budget[i] = bud - move;
+        continue;
+      }
       const float4 q = ldg(I.tgt + (prev));
       if (dist2(px, py, pz, q.x, q.y, q.z) <= cfg.dmax2) {
         I.budget[i] = bud - move;
@@ -1249,17 +1281,22 @@ __device__ __forceinline__ void hard_query(const IterInstance* __restrict__ inst
                                          (h.inst & kHardBlockFlag) != 0, 0.0f, W);
   if (int(threadIdx.x & (kWave - 1)) % kG == 0) {
     I.match[h.i] = W.best_pos;
-    if (gap && W.best_pos != kNone) {
+    if (gap) {
       // Every other point lies at least `other` away: scanned ones at
       // sqrt(second), unscanned ones beyond the widened radius (rows cut to
       // it, rings stopped once their clearance exceeded it). While the point
       // moves less than half the gap (less a rounding slack), its nearest
       // point stays the same: the movement budget phase 1 gives its own
-      // certified points.
+      // certified points. With none within dmax (the radius stayed dmax),
+      // every point is an other one.
       const float best = sqrtf(W.thr);
       const float other = fminf(sqrtf(W.second), best + margin);
-      const float slack = 1e-6f * (fabsf(h.px) + fabsf(h.py) + fabsf(h.pz) + best + 1.0f);
-      I.budget[h.i] = fmaxf(0.5f * (other - best) - slack, 0.0f);
+      if (W.best_pos != kNone) {
+        const float slack = 1e-6f * (fabsf(h.px) + fabsf(h.py) + fabsf(h.pz) + best + 1.0f);
+        I.budget[h.i] = fmaxf(0.5f * (other - best) - slack, 0.0f);
+      } else {
+        I.budget[h.i] = none_budget(other, best, h.px, h.py, h.pz);
+      }
     }
     if constexpr (kStats)
       stats_add(stats, 2, W.rows, W.pts, W.best_pos == kNone ? kRingNone : kRingFound);
@@ -1387,7 +1424,10 @@ __global__ void __launch_bounds__(kFinBlock) k_gicp_finalize(const IterInstance*
   double acc = 0.0;
   unsigned long long cnt = 0;
   // kFinUnroll loads in flight per thread, then their adds in partial order
-  constexpr int kFinUnroll = 8;
+#ifndef SR_FIN_UNROLL
+#define SR_FIN_UNROLL 8
+#endif
+  constexpr int kFinUnroll = SR_FIN_UNROLL;
   for (uint32_t b0 = grp; b0 < nb; b0 += kFinGroups * kFinUnroll) {
     unsigned long long w[kFinUnroll];
 #pragma unroll

@@ -7,7 +7,9 @@ pass cannot show (gpu).
   256-pair batch (small-cube generator, sources log-uniform in [5k, 120k])
                           batched == one-at-a-time, bit for bit; eight golden
                           instances vs the FP64 oracle at <= 1e-4 rad, <= 1 mm
-  certified match reuse   on == off, bit for bit (poses, fitness, iterations),
+  certified match reuse   on == off, bit for bit (poses, fitness, iterations;
+                          on the batch, and on a partial overlap whose misses
+                          are kept under budgets too),
                           and on does less search work
   singular normal equations (gicp.rs:190-194, linalg.rs:25-27)
                           the reference's error on an input whose H has an
@@ -161,6 +163,29 @@ def test_match_reuse_off_is_bit_identical(hip, reuse_restored, batch256):
     assert all(_same(x, y) for x, y in zip(on[2], off[2]))
     print(f"points tested over 31 passes: reuse on {on[3]}, off {off[3]}", flush=True)
     assert on[3] < off[3]
+
+
+def test_match_reuse_off_is_bit_identical_partial_overlap(hip, reuse_restored):
+    """A target cropped to 60% of the cube in x leaves about a third of the
+    source with no target point within d_max on every pass (box and
+    empty-space rejects, fruitless ring searches): reuse keeps those misses
+    under a movement budget too, and the result must still be the searched
+    one, bit for bit."""
+    w = synth.small_cube_pair()
+    tgt = w.target[w.target[:, 0] < np.quantile(w.target[:, 0], 0.6)]
+    runs = {}
+    for reuse in (1, 0):
+        sr.set_option("match_reuse", reuse)
+        reg = sr.GicpRegistration(_fixed(max_iterations=30, voxel_leaf=0.1))
+        runs[reuse] = _search_points(lambda: reg.align(w.source, tgt))
+    (on, pts_on), (off, pts_off) = runs[1], runs[0]
+    assert _same(on, off)
+    assert on.correspondences == off.correspondences
+    misses = on.source_points - on.correspondences
+    print(f"partial overlap: {misses} of {on.source_points} source points without a match; "
+          f"points tested over 31 passes: reuse on {pts_on}, off {pts_off}", flush=True)
+    assert misses > on.source_points // 5
+    assert pts_on < pts_off
 
 
 def test_singular_normal_equations(hip, orc):
