@@ -796,6 +796,7 @@ struct Batch {
   DevBuf<double> init;
   DevBuf<uint32_t> match;
   DevBuf<float> budget;
+  DevBuf<float4> seen;
   uint32_t total_blocks = 0, count = 0, max_passes = 0;
   uint64_t total_pts = 0;
   PassScratch scratch{};
@@ -812,7 +813,8 @@ static uint32_t instance_blocks(uint32_t n_src) {
 // state, partials (instance_blocks x 64 floats) and match (n_src words) it owns.
 static IterInstance describe_instance(const Prepared& src, const Prepared& tgt,
                                       const sr_gicp_config& cfg, InstState* state, float* partials,
-                                      uint32_t* match, float* budget, uint32_t block_begin) {
+                                      uint32_t* match, float* budget, float4* seen,
+                                      uint32_t block_begin) {
   IterInstance I{};
   I.src = src.grid.pts.p;
   I.src_n = src.normals.p;
@@ -822,6 +824,7 @@ static IterInstance describe_instance(const Prepared& src, const Prepared& tgt,
   I.empty_radius = tgt.grid.empty_radius.p;
   I.match = match;
   I.budget = budget;
+  I.seen = seen;
   {
     // the source grid's box holds every source point
     const GridDesc& d = src.grid.desc;
@@ -868,12 +871,14 @@ static void batch_setup(const std::vector<const Prepared*>& src, const std::vect
   fill_u32(B.match.p, kNone, total_pts, s);
   B.budget.resize(std::max<uint64_t>(total_pts, 1));
   fill_u32(reinterpret_cast<uint32_t*>(B.budget.p), 0u, total_pts, s);
+  B.seen.resize(std::max<uint64_t>(total_pts, 1));
+  fill_u32(reinterpret_cast<uint32_t*>(B.seen.p), 0xFFFFFFFFu, 4 * total_pts, s);  // NaN
   uint32_t begin = 0;
   uint64_t src_begin = 0;
   for (uint32_t i = 0; i < count; ++i) {
     hi[i] = describe_instance(*src[i], *tgt[i], cfgs[i], B.states.p + i,
                               B.partials.p + size_t(begin) * 64, B.match.p + src_begin,
-                              B.budget.p + src_begin, begin);
+                              B.budget.p + src_begin, B.seen.p + src_begin, begin);
     src_begin += src[i]->n;
     for (uint32_t b = 0; b < nbl[i]; ++b) hbi.push_back(i);
     begin += nbl[i];
@@ -990,6 +995,7 @@ static void reset_batch(Batch& B, hipStream_t s) {
   init_states(B.states.p, B.init.p, B.count, s);
   fill_u32(B.match.p, kNone, B.total_pts, s);
   fill_u32(reinterpret_cast<uint32_t*>(B.budget.p), 0u, B.total_pts, s);
+  fill_u32(reinterpret_cast<uint32_t*>(B.seen.p), 0xFFFFFFFFu, 4 * B.total_pts, s);  // NaN
 }
 
 // Times every candidate on this batch (eager launches) and returns the
@@ -1337,6 +1343,7 @@ struct ServerRequest {
   DevBuf<float> partials;
   DevBuf<uint32_t> match;
   DevBuf<float> budget;
+  DevBuf<float4> seen;
 };
 
 struct ServerDone {
@@ -1397,14 +1404,14 @@ static sr_gicp_config request_config(const sr_server_config& sc, const sr_gicp_c
 }
 
 // Device bytes a prepared request holds while active: its grid (cell starts,
-// points, order, bitmaps), normals, match and budget words, block partials, and
-// its share of the pass queue.
+// points, order, bitmaps), normals, match, budget and last-search point words,
+// block partials, and its share of the pass queue.
 static uint64_t resident_bytes(const Prepared& p) {
   const uint64_t n = p.n;
   const GridDesc& d = p.grid.desc;
   const uint64_t words = uint64_t(d.wpr) * uint64_t(d.dy) * uint64_t(d.dz) +
                          uint64_t(d.wps) * uint64_t(d.dz);
-  return 4 * (uint64_t(d.ncells) + 1) + n * (16 + 4 + 16 + 4 + 4 + sizeof(HardQuery)) + 4 * words +
+  return 4 * (uint64_t(d.ncells) + 1) + n * (16 + 4 + 16 + 4 + 4 + 16 + sizeof(HardQuery)) + 4 * words +
          uint64_t(instance_blocks(p.n)) * 64 * sizeof(float);
 }
 
@@ -1521,7 +1528,7 @@ static void server_step(sr_server& S) {
     for (uint32_t i = 0; i < count; ++i) {
       ServerRequest& r = *S.active[i];
       hi[i] = describe_instance(r.src, r.map->tgt, S.gicp, S.states.p + r.slot, r.partials.p,
-                                r.match.p, r.budget.p, begin);
+                                r.match.p, r.budget.p, r.seen.p, begin);
       for (uint32_t b = 0; b < hi[i].nblocks; ++b) hbi.push_back(i);
       begin += hi[i].nblocks;
       points += r.src.n;
@@ -1573,6 +1580,8 @@ static void server_step(sr_server& S) {
       fill_u32(r.match.p, kNone, r.src.n, s);
       r.budget.resize(std::max<uint32_t>(r.src.n, 1));
       fill_u32(reinterpret_cast<uint32_t*>(r.budget.p), 0u, r.src.n, s);
+      r.seen.resize(std::max<uint32_t>(r.src.n, 1));
+      fill_u32(reinterpret_cast<uint32_t*>(r.seen.p), 0xFFFFFFFFu, 4 * uint64_t(r.src.n), s);  // NaN
       for (int a = 0; a < 4; ++a) init[7 * i + a] = double(S.gicp.init_q[a]);
       for (int a = 0; a < 3; ++a) init[7 * i + 4 + a] = double(S.gicp.init_t[a]);
     }

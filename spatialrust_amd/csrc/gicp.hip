@@ -672,6 +672,20 @@ __device__ __forceinline__ uint32_t xcd_block() {
   return xcd * q8 + min(xcd, rem) + (hw >> 3);
 }
 
+// The search's result is a function of the transformed point's f32
+// coordinates alone (the exact nearest point, ties to the lower index), so a
+// point back at the very coordinates of the search that gave its match keeps
+// that match whatever its budget: in converged passes the pose moves points
+// far less than an f32 ulp, and this keeps the points whose budget is nil (a
+// second-nearest point closer to the nearest than the budget's rounding slack)
+// from being searched on every pass. Only a point without a budget reads it,
+// and only with match reuse on (off, every point is searched on every pass).
+__device__ __forceinline__ bool same_spot(const float4* seen, uint32_t i, float px, float py,
+                                          float pz) {
+  const float4 s = ldg(seen + i);
+  return s.x == px && s.y == py && s.z == pz;  // NaN (never searched) compares unequal
+}
+
 // The f32 pose of a pass: high parts R, T and low parts Rl, Tl.
 struct PassPose {
   float R[9], T[3], Rl[9], Tl[3];
@@ -942,28 +956,32 @@ __global__ void __launch_bounds__(kBlock) k_gicp_settle(const IterInstance* __re
   const float bud = valid ? ldg(I.budget + (i)) : 0.0f;
   const float move =
       move_rot * (sqrtf(p.x * p.x + p.y * p.y + p.z * p.z) * 1.000001f) + move_const;
-  const bool keep = valid && bud > move;  // a match, or none (kNone), certified
+  float px = 0.0f, py = 0.0f, pz = 0.0f, d2 = 0.0f;
+  transform_point(P, p, px, py, pz);
+  // a match, or none (kNone), certified for the movement or searched right here
+  const bool cert = valid && bud > move;
+  const bool keep =
+      cert || (valid && (cfg.flags & kCfgMatchReuse) && same_spot(I.seen, i, px, py, pz));
   const bool search = valid && !keep;
   const unsigned long long sm = __ballot(search);
   if ((threadIdx.x & (kWave - 1)) == 0) ps.pend[size_t(b) * (kBlock / kWave) + threadIdx.x / kWave] = sm;
-  float px = 0.0f, py = 0.0f, pz = 0.0f, d2 = 0.0f;
   float4 q = p, nt = p;
   bool counted = false;
   if (keep) {
     if constexpr (kStats) stats_add(stats, 0, 0, 0, kBlockDone);
     if (prev == kNone) {
-      I.budget[i] = bud - move;
+      if (cert) I.budget[i] = bud - move;
     } else {
-      transform_point(P, p, px, py, pz);
       q = ldg(I.tgt + (prev));
       nt = ldg(I.tgt_n + (prev));
       d2 = dist2(px, py, pz, q.x, q.y, q.z);
       if (d2 <= cfg.dmax2) {
-        I.budget[i] = bud - move;
+        if (cert) I.budget[i] = bud - move;
         counted = true;
-      } else {  // the nearest point is beyond d_max: no correspondence, search next time
+      } else {  // the nearest point is beyond d_max: none here, search next time
         I.match[i] = kNone;
         I.budget[i] = 0.0f;
+        I.seen[i] = make_float4(px, py, pz, 0.0f);
       }
     }
   }
@@ -1019,7 +1037,10 @@ __device__ __forceinline__ void search_block(const IterInstance* __restrict__ in
     const float bud = valid ? ldg(I.budget + (i)) : 0.0f;
     const float move =
         move_rot * (sqrtf(p.x * p.x + p.y * p.y + p.z * p.z) * 1.000001f) + move_const;
-    const bool keep = bud > move;  // a match, or none (kNone), certified
+    // a match, or none (kNone), certified for the movement or searched right here
+    const bool cert = bud > move;
+    const bool keep =
+        cert || (valid && (cfg.flags & kCfgMatchReuse) && same_spot(I.seen, i, px, py, pz));
     NNQuery Q;
     Q.hard = false;
     if (!keep) {
@@ -1036,20 +1057,22 @@ __device__ __forceinline__ void search_block(const IterInstance* __restrict__ in
     if (keep) {
       if constexpr (kStats) stats_add(stats, 0, 0, 0, kBlockDone);
       if (prev == kNone) {
-        I.budget[i] = bud - move;
+        if (cert) I.budget[i] = bud - move;
         continue;
       }
       const float4 q = ldg(I.tgt + (prev));
       if (dist2(px, py, pz, q.x, q.y, q.z) <= cfg.dmax2) {
-        I.budget[i] = bud - move;
-      } else {  // the nearest point is beyond d_max: no correspondence, search next time
+        if (cert) I.budget[i] = bud - move;
+      } else {  // the nearest point is beyond d_max: none here, search next time
         I.match[i] = kNone;
         I.budget[i] = 0.0f;
+        I.seen[i] = make_float4(px, py, pz, 0.0f);
       }
       continue;
     }
     I.budget[i] = Q.budget;
     I.match[i] = Q.best_pos;
+    I.seen[i] = make_float4(px, py, pz, 0.0f);
   }
 }
 
@@ -1102,6 +1125,7 @@ __global__ void __launch_bounds__(kBlock)
   if (mine && !hard) {
     I.budget[i] = Q.budget;
     I.match[i] = Q.best_pos;
+    I.seen[i] = make_float4(px, py, pz, 0.0f);
   }
 }
 
@@ -1150,7 +1174,12 @@ __global__ void __launch_bounds__(kBlock)
     const float bud = valid ? ldg(I.budget + (i)) : 0.0f;
     const float move =
         move_rot * (sqrtf(p.x * p.x + p.y * p.y + p.z * p.z) * 1.000001f) + move_const;
-    const bool keep = bud > move;  // a match, or none (kNone), certified
+    float px, py, pz;
+    transform_point(P, p, px, py, pz);
+    // a match, or none (kNone), certified for the movement or searched right here
+    const bool cert = bud > move;
+    const bool keep =
+        cert || (valid && (cfg.flags & kCfgMatchReuse) && same_spot(I.seen, i, px, py, pz));
     if (__syncthreads_and(!valid || keep)) {
       if (threadIdx.x == 0) ps.done[b] = 1u;
       float acc[kTerms];
@@ -1159,21 +1188,20 @@ __global__ void __launch_bounds__(kBlock)
       uint32_t count = 0;
       if (valid && prev == kNone) {
         if constexpr (kStats) stats_add(stats, 0, 0, 0, kBlockDone);
-        I.budget[i] = bud - move;
+        if (cert) I.budget[i] = bud - move;
       } else if (valid) {
         if constexpr (kStats) stats_add(stats, 0, 0, 0, kBlockDone);
-        float px, py, pz;
-        transform_point(P, p, px, py, pz);
         const float4 q = ldg(I.tgt + (prev));
         const float4 nt = ldg(I.tgt_n + (prev));
         const float d2 = dist2(px, py, pz, q.x, q.y, q.z);
         if (d2 <= cfg.dmax2) {
-          I.budget[i] = bud - move;
+          if (cert) I.budget[i] = bud - move;
           add_terms(acc, P, beta, eps, px, py, pz, q, nt, ns, d2);
           ++count;
-        } else {  // the nearest point is beyond d_max: no correspondence
+        } else {  // the nearest point is beyond d_max: none here
           I.match[i] = kNone;
           I.budget[i] = 0.0f;
+          I.seen[i] = make_float4(px, py, pz, 0.0f);
         }
       }
       write_partial(L, acc, count, I.partials + size_t(lb) * 64);
@@ -1196,7 +1224,10 @@ __global__ void __launch_bounds__(kBlock)
     const float bud = valid ? ldg(I.budget + (i)) : 0.0f;
     const float move =
         move_rot * (sqrtf(p.x * p.x + p.y * p.y + p.z * p.z) * 1.000001f) + move_const;
-    const bool keep = bud > move;  // a match, or none (kNone), certified
+    // a match, or none (kNone), certified for the movement or searched right here
+    const bool cert = bud > move;
+    const bool keep =
+        cert || (valid && (cfg.flags & kCfgMatchReuse) && same_spot(I.seen, i, px, py, pz));
     NNQuery Q;
     Q.hard = false;
     if (!keep) {
@@ -1214,20 +1245,22 @@ __global__ void __launch_bounds__(kBlock)
     if (keep) {
       if constexpr (kStats) stats_add(stats, 0, 0, 0, kBlockDone);
       if (prev == kNone) {
-        I.budget[i] = bud - move;
+        if (cert) I.budget[i] = bud - move;
         continue;
       }
       const float4 q = ldg(I.tgt + (prev));
       if (dist2(px, py, pz, q.x, q.y, q.z) <= cfg.dmax2) {
-        I.budget[i] = bud - move;
-      } else {  // the nearest point is beyond d_max: no correspondence, search next time
+        if (cert) I.budget[i] = bud - move;
+      } else {  // the nearest point is beyond d_max: none here, search next time
         I.match[i] = kNone;
         I.budget[i] = 0.0f;
+        I.seen[i] = make_float4(px, py, pz, 0.0f);
       }
       continue;
     }
     I.budget[i] = Q.budget;
     I.match[i] = Q.best_pos;
+    I.seen[i] = make_float4(px, py, pz, 0.0f);
   }
   // A block with queued points gets its partial from k_gicp_terms.
   const int any = __syncthreads_or(queued ? 1 : 0);
@@ -1281,6 +1314,7 @@ __device__ __forceinline__ void hard_query(const IterInstance* __restrict__ inst
                                          (h.inst & kHardBlockFlag) != 0, 0.0f, W);
   if (int(threadIdx.x & (kWave - 1)) % kG == 0) {
     I.match[h.i] = W.best_pos;
+    I.seen[h.i] = make_float4(h.px, h.py, h.pz, 0.0f);
     if (gap) {
       // Every other point lies at least `other` away: scanned ones at
       // sqrt(second), unscanned ones beyond the widened radius (rows cut to

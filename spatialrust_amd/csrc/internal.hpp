@@ -348,6 +348,8 @@ struct IterInstance {
                           // previous pass (kNone: none); seeds the next search
   float* budget;          // per source point: how far it may still move before its
                           // match can change (0: search again)
+  float4* seen;           // per source point: the transformed point its match was
+                          // searched for (x, y, z; NaN before any search)
   float src_radius;       // bound on |s| over the source points
   GridDesc grid;
   InstConfig cfg;

@@ -8,8 +8,8 @@ pass cannot show (gpu).
                           batched == one-at-a-time, bit for bit; eight golden
                           instances vs the FP64 oracle at <= 1e-4 rad, <= 1 mm
   certified match reuse   on == off, bit for bit (poses, fitness, iterations;
-                          on the batch, and on a partial overlap whose misses
-                          are kept under budgets too),
+                          on the batch, on the large block, and on a partial
+                          overlap whose misses are kept under budgets too),
                           and on does less search work
   singular normal equations (gicp.rs:190-194, linalg.rs:25-27)
                           the reference's error on an input whose H has an
@@ -185,6 +185,33 @@ def test_match_reuse_off_is_bit_identical_partial_overlap(hip, reuse_restored):
     print(f"partial overlap: {misses} of {on.source_points} source points without a match; "
           f"points tested over 31 passes: reuse on {pts_on}, off {pts_off}", flush=True)
     assert misses > on.source_points // 5
+    assert pts_on < pts_off
+
+
+def test_match_reuse_off_is_bit_identical_large_block(hip, reuse_restored):
+    """Over the large block's 32 converged passes the pose moves points by far
+    less than an f32 ulp: reuse keeps a match whose point is back at the very
+    coordinates of its search even when its movement budget is nil, and the
+    result must still be the searched one, bit for bit."""
+    w = synth.large_block_pair()
+    src, tgt = devmem.DeviceCloud(w.source), devmem.DeviceCloud(w.target)
+    runs = {}
+    try:
+        cfg = _fixed(max_iterations=w.max_iterations, k_neighbors=w.k_neighbors,
+                     max_correspondence_distance=w.max_correspondence_distance,
+                     voxel_leaf=w.voxel_leaf)
+        for reuse in (1, 0):
+            sr.set_option("match_reuse", reuse)
+            reg = sr.GicpRegistration(cfg)
+            runs[reuse] = _search_points(lambda: reg.align(src.view, tgt.view))
+    finally:
+        src.close()
+        tgt.close()
+    (on, pts_on), (off, pts_off) = runs[1], runs[0]
+    assert _same(on, off)
+    assert on.correspondences == off.correspondences
+    print(f"large block: points tested over 51 passes: reuse on {pts_on}, off {pts_off}",
+          flush=True)
     assert pts_on < pts_off
 
 
