@@ -10,6 +10,12 @@ into spatialrust_amd/lib/libsr_gicp.so. Flags:
                       operations explicitly where it wants them.
   -Xclang -target-feature -Xclang -packed-fp32-ops
                       keep the compiler from emitting packed FP32 VALU ops.
+  -mllvm -amdgpu-kernarg-preload-count=16
+                      the first kernel arguments arrive in SGPRs at wave
+                      launch instead of through a load from the argument
+                      segment: every kernel's chain of dependent loads starts
+                      one round trip earlier (the pass stages are launched
+                      some 200 times per registration).
 """
 import concurrent.futures
 import glob
@@ -26,6 +32,7 @@ ARCH = "gfx950"
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-mllvm", "-amdgpu-kernarg-preload-count=16",
           "-Wall", "-Wno-unused-function", "-Wno-unused-variable"]
 
 

@@ -35,11 +35,34 @@
 
 namespace sr {
 
+// 1/x and 1/sqrt(x) on the finalize's serial FP64 path: the hardware estimate
+// refined by two Newton steps (each at least doubles the correct bits, so the
+// result is within an ulp or two), in place of the IEEE-exact sequences, whose
+// operand scaling and fix-up steps the well-scaled operands here (pivots
+// >= 1e-12, lengths near 1) do not need.
+__device__ __forceinline__ double rcp64(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+__device__ __forceinline__ double rsq64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  double t = fma(-h * y, y, 0.5);  // y <- y (3 - x y^2) / 2
+  y = fma(y, t, y);
+  t = fma(-h * y, y, 0.5);
+  return fma(y, t, y);
+}
+
 __device__ __forceinline__ void q_to_mat64(const double q0[4], double R[3][3]) {
-  double len = sqrt(q0[0] * q0[0] + q0[1] * q0[1] + q0[2] * q0[2] + q0[3] * q0[3]);
+  const double n2 =
+      fma(q0[0], q0[0], fma(q0[1], q0[1], fma(q0[2], q0[2], q0[3] * q0[3])));
   double x = 0, y = 0, z = 0, w = 1;
-  if (len != 0.0) {
-    const double il = 1.0 / len;  // one division, four products (within an ulp)
+  if (n2 != 0.0) {
+    // one reciprocal square root, four products
+    const double il = n2 >= 1e-280 ? rsq64(n2) : 1.0 / sqrt(n2);
     x = q0[0] * il;
     y = q0[1] * il;
     z = q0[2] * il;
@@ -107,20 +130,21 @@ void init_states(InstState* states, const double* init7_dev, uint32_t count, hip
 // Every loop bound is a constant, so the factor stays in registers: this runs
 // on one thread of the finalize kernel, whose registers cost nothing else.
 __device__ __forceinline__ bool ldlt_solve(const double* Hp, const double* g, double* x) {
-  double L[6][6], D[6], iD[6], y[6];
+  // LD[j][k] = L[j][k] D[k]: each elimination term is one fused product
+  double L[6][6], LD[6][6], iD[6], y[6];
 #pragma unroll
   for (int j = 0; j < 6; ++j) {
     double d = Hp[ut(j, j)];
 #pragma unroll
-    for (int k = 0; k < j; ++k) d -= L[j][k] * L[j][k] * D[k];
+    for (int k = 0; k < j; ++k) d = fma(-L[j][k], LD[j][k], d);
     if (!(d >= 1e-12)) return false;
-    D[j] = d;
-    iD[j] = 1.0 / d;  // one division per pivot; the column takes products
+    iD[j] = rcp64(d);  // one reciprocal per pivot; the column takes products
 #pragma unroll
     for (int i = j + 1; i < 6; ++i) {
       double v = Hp[ut(j, i)];
 #pragma unroll
-      for (int k = 0; k < j; ++k) v -= L[i][k] * L[j][k] * D[k];
+      for (int k = 0; k < j; ++k) v = fma(-L[i][k], LD[j][k], v);
+      LD[i][j] = v;
       L[i][j] = v * iD[j];
     }
   }
@@ -128,14 +152,14 @@ __device__ __forceinline__ bool ldlt_solve(const double* Hp, const double* g, do
   for (int i = 0; i < 6; ++i) {
     double v = -g[i];
 #pragma unroll
-    for (int k = 0; k < i; ++k) v -= L[i][k] * y[k];
+    for (int k = 0; k < i; ++k) v = fma(-L[i][k], y[k], v);
     y[i] = v;
   }
 #pragma unroll
   for (int i = 5; i >= 0; --i) {
     double v = y[i] * iD[i];
 #pragma unroll
-    for (int k = i + 1; k < 6; ++k) v -= L[k][i] * x[k];
+    for (int k = i + 1; k < 6; ++k) v = fma(-L[k][i], x[k], v);
     x[i] = v;
   }
   return true;
@@ -177,12 +201,12 @@ __device__ __forceinline__ void finalize(InstState* st, const InstConfig& cfg, c
     return;
   }
   // delta_from_solution (gicp.rs:314-324) and Isometry3::compose (transform.rs:104-108), FP64.
-  const double angle = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+  const double angle = sqrt(fma(x[0], x[0], fma(x[1], x[1], x[2] * x[2])));
   double dq[4] = {0.0, 0.0, 0.0, 1.0};
   if (angle > 1e-12) {
-    const double ia = 1.0 / angle;
+    const double ia = rcp64(angle);
     double ax = x[0] * ia, ay = x[1] * ia, az = x[2] * ia;
-    const double ial = 1.0 / sqrt(ax * ax + ay * ay + az * az);
+    const double ial = rsq64(fma(ax, ax, fma(ay, ay, az * az)));
     ax *= ial;
     ay *= ial;
     az *= ial;
@@ -191,13 +215,27 @@ __device__ __forceinline__ void finalize(InstState* st, const InstConfig& cfg, c
     if (h < 0.0625) {
       // Taylor series to x^17 / x^16: below 1/16 the first omitted terms are
       // under 2^-100 relative, so this is sin / cos to FP64 rounding without
-      // the general argument reduction.
+      // the general argument reduction (Horner form, one fused step a term).
       const double h2 = h * h;
-      sh = h * (1.0 + h2 * (-1.0 / 6 + h2 * (1.0 / 120 + h2 * (-1.0 / 5040 + h2 * (1.0 / 362880 +
-           h2 * (-1.0 / 39916800 + h2 * (1.0 / 6227020800.0 + h2 * (-1.0 / 1307674368000.0 +
-           h2 * (1.0 / 355687428096000.0)))))))));
-      ch = 1.0 + h2 * (-0.5 + h2 * (1.0 / 24 + h2 * (-1.0 / 720 + h2 * (1.0 / 40320 + h2 * (-1.0 / 3628800 +
-           h2 * (1.0 / 479001600 + h2 * (-1.0 / 87178291200.0 + h2 * (1.0 / 20922789888000.0))))))));
+      double ps = 1.0 / 355687428096000.0;
+      ps = fma(h2, ps, -1.0 / 1307674368000.0);
+      ps = fma(h2, ps, 1.0 / 6227020800.0);
+      ps = fma(h2, ps, -1.0 / 39916800);
+      ps = fma(h2, ps, 1.0 / 362880);
+      ps = fma(h2, ps, -1.0 / 5040);
+      ps = fma(h2, ps, 1.0 / 120);
+      ps = fma(h2, ps, -1.0 / 6);
+      ps = fma(h2, ps, 1.0);
+      sh = h * ps;
+      double pc = 1.0 / 20922789888000.0;
+      pc = fma(h2, pc, -1.0 / 87178291200.0);
+      pc = fma(h2, pc, 1.0 / 479001600);
+      pc = fma(h2, pc, -1.0 / 3628800);
+      pc = fma(h2, pc, 1.0 / 40320);
+      pc = fma(h2, pc, -1.0 / 720);
+      pc = fma(h2, pc, 1.0 / 24);
+      pc = fma(h2, pc, -0.5);
+      ch = fma(h2, pc, 1.0);
     } else {
       sincos(h, &sh, &ch);
     }
@@ -205,20 +243,20 @@ __device__ __forceinline__ void finalize(InstState* st, const InstConfig& cfg, c
     dq[1] = ay * sh;
     dq[2] = az * sh;
     dq[3] = ch;
-    const double il = 1.0 / sqrt(dq[0] * dq[0] + dq[1] * dq[1] + dq[2] * dq[2] + dq[3] * dq[3]);
+    const double il = rsq64(fma(dq[0], dq[0], fma(dq[1], dq[1], fma(dq[2], dq[2], dq[3] * dq[3]))));
     for (int a = 0; a < 4; ++a) dq[a] *= il;
   }
   const double* q = st->q;
-  double nq[4] = {dq[3] * q[0] + dq[0] * q[3] + dq[1] * q[2] - dq[2] * q[1],
-                  dq[3] * q[1] - dq[0] * q[2] + dq[1] * q[3] + dq[2] * q[0],
-                  dq[3] * q[2] + dq[0] * q[1] - dq[1] * q[0] + dq[2] * q[3],
-                  dq[3] * q[3] - dq[0] * q[0] - dq[1] * q[1] - dq[2] * q[2]};
-  double l = sqrt(nq[0] * nq[0] + nq[1] * nq[1] + nq[2] * nq[2] + nq[3] * nq[3]);
+  double nq[4] = {fma(dq[3], q[0], fma(dq[0], q[3], fma(dq[1], q[2], -dq[2] * q[1]))),
+                  fma(dq[3], q[1], fma(-dq[0], q[2], fma(dq[1], q[3], dq[2] * q[0]))),
+                  fma(dq[3], q[2], fma(dq[0], q[1], fma(-dq[1], q[0], dq[2] * q[3]))),
+                  fma(dq[3], q[3], fma(-dq[0], q[0], fma(-dq[1], q[1], -dq[2] * q[2])))};
+  const double l2 = fma(nq[0], nq[0], fma(nq[1], nq[1], fma(nq[2], nq[2], nq[3] * nq[3])));
   double Rd[3][3];
   q_to_mat64(dq, Rd);
   double nt[3];
   for (int a = 0; a < 3; ++a)
-    nt[a] = Rd[a][0] * st->t[0] + Rd[a][1] * st->t[1] + Rd[a][2] * st->t[2] + x[3 + a];
+    nt[a] = fma(Rd[a][0], st->t[0], fma(Rd[a][1], st->t[1], fma(Rd[a][2], st->t[2], x[3 + a])));
   // Movement bound of the next pass: p'_new - p'_old = (dR - I)(R s + t) + dt,
   // and |dR - I| = 2 sin(angle/2) <= angle, so |dp'| <= angle (|s| + |t|) + |dt|
   // (rounded up; the f32 pose's own error is covered by the search's slack).
@@ -228,7 +266,7 @@ __device__ __forceinline__ void finalize(InstState* st, const InstConfig& cfg, c
     st->move_rot = float(angle * (1.0 + 1e-6) + 1e-30);
     st->move_const = float((angle * tn + dn) * (1.0 + 1e-6) + 1e-30);
   }
-  const double il = 1.0 / l;
+  const double il = rsq64(l2);
   for (int a = 0; a < 4; ++a) st->q[a] = nq[a] * il;
   for (int a = 0; a < 3; ++a) st->t[a] = nt[a];
   refresh_f32_pose(st);
@@ -1434,16 +1472,17 @@ __global__ void __launch_bounds__(kFinBlock) k_gicp_finalize(const IterInstance*
   if (inst_id >= count) return;
   const IterInstance& I = insts[inst_id];
   InstState* st = I.state;
-  if (st->phase == 2) return;
   const InstConfig cfg = I.cfg;
   // The state is staged in LDS: the serial bookkeeping below reads and writes
   // it field by field, which from global memory would chain many round trips.
+  // Its loads and the partials' go out together; a finished registration (its
+  // partials stale but readable) leaves after the first barrier.
   static_assert(sizeof(InstState) % 8 == 0 && sizeof(InstState) / 8 <= kFinBlock, "state staging");
   constexpr int kStateWords = int(sizeof(InstState) / 8);
   __shared__ InstState sst;
   if (threadIdx.x < kStateWords)
     reinterpret_cast<unsigned long long*>(&sst)[threadIdx.x] =
-        reinterpret_cast<const unsigned long long*>(st)[threadIdx.x];
+        ldg(reinterpret_cast<const unsigned long long*>(st) + threadIdx.x);
   // A block partial is 32 u64 slots: slot t < kTerms holds term t as an f32
   // (hi, lo) pair, slot 30 the correspondence count. Thread (group, slot)
   // sums one slot over partials group, group + kFinGroups, ... in FP64, so a
@@ -1458,16 +1497,13 @@ __global__ void __launch_bounds__(kFinBlock) k_gicp_finalize(const IterInstance*
   double acc = 0.0;
   unsigned long long cnt = 0;
   // kFinUnroll loads in flight per thread, then their adds in partial order
-#ifndef SR_FIN_UNROLL
-#define SR_FIN_UNROLL 8
-#endif
-  constexpr int kFinUnroll = SR_FIN_UNROLL;
+  constexpr int kFinUnroll = 8;  // (24, one round for 768 partials, measured slower)
   for (uint32_t b0 = grp; b0 < nb; b0 += kFinGroups * kFinUnroll) {
     unsigned long long w[kFinUnroll];
 #pragma unroll
     for (int u = 0; u < kFinUnroll; ++u) {
       const uint32_t bb = b0 + uint32_t(u) * kFinGroups;
-      w[u] = bb < nb ? pp[size_t(bb) * 32 + slot] : 0ull;
+      w[u] = bb < nb ? ldg(pp + (size_t(bb) * 32 + slot)) : 0ull;
     }
 #pragma unroll
     for (int u = 0; u < kFinUnroll; ++u) {
@@ -1480,6 +1516,7 @@ __global__ void __launch_bounds__(kFinBlock) k_gicp_finalize(const IterInstance*
   if (slot < kTerms) gsum[grp][slot] = acc;
   if (slot == 30) gcnt[grp] = cnt;
   __syncthreads();
+  if (sst.phase == 2) return;  // the same on every thread
   if (threadIdx.x < kTerms) {
     double sacc = 0.0;
     for (int w = 0; w < kFinGroups; ++w) sacc += gsum[w][threadIdx.x];
