@@ -168,6 +168,36 @@ def _batched_vs_plain(count):
     t_batch = time.perf_counter() - t0
     identical = all(np.array_equal(a.rotation64, b.rotation64) and
                     np.array_equal(a.translation64, b.translation64) for a, b in zip(plain, batched))
+    # The same batch once more with the library's profiler on (eager launches,
+    # HIP events around each stage on the library's stream): per-pass and
+    # preparation stage times for the batched roofline.
+    import ctypes
+    from spatialrust_amd import native
+    lib = native.load()
+    lib.sr_profile_reset()
+    lib.sr_profile_enable(1)
+    again = sr.batch_align_pairs(sv, tv, cfg)
+    devmem.synchronize()
+    lib.sr_profile_enable(0)
+    stage = {}
+    for name in ("gicp_pass", "gicp_finalize", "knn_normals", "voxel_insert"):
+        ms, cnt = ctypes.c_double(0.0), ctypes.c_uint64(0)
+        lib.sr_profile_read(name.encode(), ctypes.byref(ms), ctypes.byref(cnt))
+        stage[name] = (ms.value, cnt.value)
+    lib.sr_profile_reset()
+    identical = identical and all(np.array_equal(a.rotation64, b.rotation64) for a, b in zip(batched, again))
+    n_src = sum(b.source_points for b in batched)
+    n_tgt = sum(b.target_points for b in batched)
+    n_raw = sum(len(p.source) + len(p.target) for p in pairs)
+    pass_ms, pass_n = stage["gicp_pass"]
+    avg_pass_s = pass_ms / max(pass_n, 1) * 1e-3
+    pass_bytes = 72 * n_src
+    pass_gbs = pass_bytes / avg_pass_s / 1e9 if avg_pass_s > 0 else 0.0
+    # whole call, the compulsory bytes of every stage: voxel 12 N + 28 C (both
+    # clouds), target grid 20 N_t, covariances 36 N (both clouds), passes 72 N_s
+    call_bytes = (12 * n_raw + 28 * (n_src + n_tgt) + 20 * n_tgt + 36 * (n_src + n_tgt)
+                  + pass_n * pass_bytes)
+    call_gbs = call_bytes / t_batch / 1e9
     for c in srcs + tgts:
         c.close()
     fp64 = None
@@ -190,6 +220,17 @@ def _batched_vs_plain(count):
             "batched_registrations_per_s": count / t_batch,
             "batched_over_plain": t_plain / t_batch,
             "identical_poses": bool(identical),
+            "after_voxel": {"source": int(n_src), "target": int(n_tgt)},
+            "stage_ms_per_call": {k: v[0] for k, v in stage.items()},
+            "roofline": {"kernel": "gicp pass over all pairs (k_gicp_settle + k_gicp_search + "
+                                   "k_gicp_hard + k_gicp_terms)",
+                         "bound": "hbm", "achieved": pass_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": pass_gbs / HBM_PEAK_GBS, "traffic": _traffic("gicp_pass", f"batch_{count}_small_cube_pairs"),
+                         "algorithmic_bytes_per_launch": pass_bytes,
+                         "avg_launch_us": avg_pass_s * 1e6, "launches": pass_n},
+            "call_roofline": {"bytes": call_bytes, "achieved": call_gbs, "peak": HBM_PEAK_GBS,
+                              "unit": "GB/s", "frac": call_gbs / HBM_PEAK_GBS,
+                              "formula": "12 N_raw + 28 C + 20 N_t + 36 (N_s + N_t) + passes x 72 N_s"},
             "max_rot_err_vs_ground_truth_rad": float(max(
                 synth.rotation_angle_between(b.rotation64, p.gt_q) for b, p in zip(batched, pairs)))}
 
@@ -564,6 +605,9 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (a plain copy "
                          "kernel of known size for calibrating hardware byte counters)")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="keep every pass loop on eager launches (pass_graphs option 0), for "
+                         "runs under an external profiler")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (independent "
@@ -578,6 +622,8 @@ def main():
         print("no gfx950 HIP device: " + native.last_error(), file=sys.stderr)
         sys.exit(1)
     lib = native.load()
+    if args.no_graphs:
+        sr.set_option("pass_graphs", 0)
     if args.copy_mib > 0:
         devmem.copy_calibration(args.copy_mib << 20)
     w = _workload(args.workload)

@@ -158,6 +158,10 @@ void sr_gicp_config_default(sr_gicp_config* out);
  *     whose k-th nearest neighbour is not provably closer than every new voxel
  *     (from their grid cells' distance to the new voxels' cells); 0 recomputes
  *     every normal. Poses are bit-identical either way.
+ *   "pass_graphs" (default 1): a pass loop whose launch shape repeats is
+ *     captured as a HIP graph on its second run and replayed after that; 0
+ *     keeps every loop on eager launches (for external profilers). Results
+ *     are bit-identical either way.
  * Unknown names and values out of range are InvalidArgument. */
 int32_t sr_option_set(const char* name, int64_t value);
 int32_t sr_option_get(const char* name, int64_t* value);

@@ -423,7 +423,7 @@ Options& options() {
 
 struct Prepared {
   Grid grid;               // points (after the voxel filter) in grid order
-  DevBuf<float4> normals;  // grid order: float4(n, a)
+  DevBuf<float4> rec;  // grid-order point records (kRec float4 a point: point, normal)
   uint32_t n = 0;
 };
 
@@ -482,16 +482,16 @@ static void prepare_cloud(const Cloud& c, const sr_gicp_config& cfg, Prepared& o
   build_grid(*cloud, hb, h, out.grid, s);
   if (is_target) build_empty_radius(out.grid, s);
   rc.stages |= SR_STAGE_GRID;
-  out.normals.resize(std::max<uint32_t>(out.n, 1));
+  out.rec.resize(kRec * std::max<uint32_t>(out.n, 1));
   if (cfg.covariance_radius > 0.0f) {
     Grid rg;
     build_grid(*cloud, hb, cfg.covariance_radius, rg, s);
     DevBuf<float4> orig(std::max<uint32_t>(out.n, 1));
     radius_normals(rg, cfg.covariance_radius, float(cfg.epsilon), orig.p, nullptr, nullptr, s);
-    gather_to_grid_order(out.grid, orig.p, out.normals.p, s);
+    gather_records(out.grid, orig.p, out.rec.p, s);
     SR_HIP(hipStreamSynchronize(s));
   } else if (!defer_knn) {
-    knn_normals(out.grid, cfg.k_neighbors, cfg.epsilon, out.normals.p, nullptr, nullptr, s);
+    knn_normals(out.grid, cfg.k_neighbors, cfg.epsilon, out.rec.p, nullptr, nullptr, s);
   }
   rc.stages |= SR_STAGE_COVARIANCE;
 }
@@ -621,7 +621,7 @@ static void prepare_many(const std::vector<const sr_xyz_view*>& views,
 // Flat buffers the prepared clouds of a segmented preparation view into.
 struct SegHolder {
   GridSet set;
-  DevBuf<float4> normals;
+  DevBuf<float4> rec;
 };
 
 // Preparation of many device-resident clouds of one configuration (voxel leaf
@@ -675,7 +675,7 @@ static std::shared_ptr<SegHolder> prepare_segmented(const std::vector<const sr_x
   build_grids_many(vox.centroids, voff, hb, h, empty_segs, hold->set, s);
   rc.stages |= SR_STAGE_GRID;
   const GridSet& G = hold->set;
-  hold->normals.resize(std::max<uint32_t>(voff.back(), 1));
+  hold->rec.resize(kRec * std::max<uint32_t>(voff.back(), 1));
   std::vector<KnnCloud> kc(nseg);
   for (uint32_t c = 0; c < nseg; ++c) {
     const uint32_t n = voff[c + 1] - voff[c];
@@ -686,8 +686,8 @@ static std::shared_ptr<SegHolder> prepare_segmented(const std::vector<const sr_x
     P.grid.pts = DevBuf<float4>::view(G.pts.p + voff[c], n);
     if (is_target[c])
       P.grid.empty_radius = DevBuf<uint8_t>::view(G.empty_radius.p + G.cell_off[c], G.desc[c].ncells);
-    P.normals = DevBuf<float4>::view(hold->normals.p + voff[c], n);
-    kc[c] = knn_cloud(P.grid, P.normals.p, nullptr, nullptr);
+    P.rec = DevBuf<float4>::view(hold->rec.p + kRec * size_t(voff[c]), kRec * size_t(n));
+    kc[c] = knn_cloud(P.grid, P.rec.p, nullptr, nullptr);
   }
   knn_normals_many(kc, cfg.k_neighbors, cfg.epsilon, s);
   rc.stages |= SR_STAGE_COVARIANCE;
@@ -741,8 +741,8 @@ static std::shared_ptr<SegHolder> prepare_pairs(const std::vector<const sr_xyz_v
     auto& e = by_k[cfgs[i].k_neighbors];
     if (e.second.empty()) e.first = cfgs[i].epsilon;
     if (e.first != cfgs[i].epsilon) fail(SR_ERR_INVALID_ARGUMENT, "pairs of one batch must share epsilon");
-    e.second.push_back(knn_cloud(tout[i].grid, tout[i].normals.p, nullptr, nullptr));
-    e.second.push_back(knn_cloud(sout[i].grid, sout[i].normals.p, nullptr, nullptr));
+    e.second.push_back(knn_cloud(tout[i].grid, tout[i].rec.p, nullptr, nullptr));
+    e.second.push_back(knn_cloud(sout[i].grid, sout[i].rec.p, nullptr, nullptr));
   }
   hipStream_t s = thread_stream();
   for (auto& kv : by_k) knn_normals_many(kv.second.second, kv.first, kv.second.first, s);
@@ -816,10 +816,9 @@ static IterInstance describe_instance(const Prepared& src, const Prepared& tgt,
                                       uint32_t* match, float* budget, float4* seen,
                                       uint32_t block_begin) {
   IterInstance I{};
-  I.src = src.grid.pts.p;
-  I.src_n = src.normals.p;
+  I.src_rec = src.rec.p;
   I.tgt = tgt.grid.pts.p;
-  I.tgt_n = tgt.normals.p;
+  I.tgt_rec = tgt.rec.p;
   I.cell_start = tgt.grid.cell_start.p;
   I.empty_radius = tgt.grid.empty_radius.p;
   I.match = match;
@@ -1078,9 +1077,10 @@ struct GraphEntry {
 };
 
 static void batch_run(Batch& B, hipStream_t s) {
-  // eager when profiling (so each launch can be timed) and inside the
-  // allocation-free entry point (instantiating a graph allocates)
-  const bool eager = profiler().on.load() != 0 || arena_active();
+  // eager when profiling (so each launch can be timed), inside the
+  // allocation-free entry point (instantiating a graph allocates) and with
+  // the pass_graphs option off
+  const bool eager = profiler().on.load() != 0 || arena_active() || options().pass_graphs.load() == 0;
   bool tuned = false;
   const PassTuning tune = choose_tuning(B, s, !eager, tuned);
   if (tuned) return;  // the tuning runs finished the loop
@@ -1782,6 +1782,7 @@ static std::atomic<int64_t>* switch_option(const std::string& n) {
   if (n == "radius_tiles") return &o.radius_tiles;
   if (n == "autotune") return &o.autotune;
   if (n == "odometry_incremental") return &o.odometry_incremental;
+  if (n == "pass_graphs") return &o.pass_graphs;
   return nullptr;
 }
 
@@ -2070,8 +2071,8 @@ int32_t sr_plane_covariances(const sr_xyz_view* cloud, uint32_t k_neighbors, flo
     } else {
       Grid g;
       build_grid(c, hb, choose_cell_edge(c, hb, knn_occupancy(k_neighbors)), g, s);
-      DevBuf<float4> nrm(n);
-      knn_normals(g, k_neighbors, epsilon, nrm.p, cov6.p, nrm3.p, s);
+      DevBuf<float4> rec(size_t(kRec) * n);
+      knn_normals(g, k_neighbors, epsilon, rec.p, cov6.p, nrm3.p, s);
     }
     if (out_cov6) SR_HIP(hipMemcpyAsync(out_cov6, cov6.p, sizeof(float) * 6 * n, hipMemcpyDeviceToHost, s));
     if (out_normals) SR_HIP(hipMemcpyAsync(out_normals, nrm3.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s));
@@ -2169,7 +2170,7 @@ uint64_t sr_map_points(const sr_map* map) { return map ? map->tgt.n : 0; }
 uint64_t sr_map_device_bytes(const sr_map* map) {
   if (!map) return 0;
   const Prepared& p = map->tgt;
-  return p.grid.cell_start.bytes() + p.grid.pts.bytes() + p.grid.order.bytes() + p.normals.bytes();
+  return p.grid.cell_start.bytes() + p.grid.pts.bytes() + p.grid.order.bytes() + p.rec.bytes();
 }
 
 int32_t sr_map_align(const sr_map* map, const sr_xyz_view* source, const sr_gicp_config* config,
@@ -2468,8 +2469,8 @@ static void odometry_update_map(sr_odometry* o, uint32_t n_old, Receipt& rc, hip
   build_grid(list, o->geo_hb, o->geo_h, fresh.grid, s);
   build_empty_radius(fresh.grid, s);
   rc.stages |= SR_STAGE_GRID;
-  fresh.normals.resize(std::max<uint32_t>(n, 1));
-  KnnCloud kc = knn_cloud(fresh.grid, fresh.normals.p, nullptr, nullptr);
+  fresh.rec.resize(kRec * std::max<uint32_t>(n, 1));
+  KnnCloud kc = knn_cloud(fresh.grid, fresh.rec.p, nullptr, nullptr);
   kc.nrm_orig = o->lnrm.p;
   kc.tau_orig = o->ltau.p;
   const bool incremental = !full && options().odometry_incremental.load() != 0;
@@ -2479,7 +2480,7 @@ static void odometry_update_map(sr_odometry* o, uint32_t n_old, Receipt& rc, hip
     kc.subset_n = o->refresh_n.p;
   }
   knn_normals_many({kc}, o->gicp.k_neighbors, o->gicp.epsilon, s);
-  if (incremental) gather_list_normals(fresh.grid, o->lnrm.p, fresh.normals.p, s);
+  if (incremental) gather_list_normals(fresh.grid, o->lnrm.p, fresh.rec.p, s);
   rc.stages |= SR_STAGE_COVARIANCE;
   uint32_t refreshed = n;
   if (incremental) download(&refreshed, o->refresh_n.p, sizeof(refreshed), s);

@@ -282,16 +282,18 @@ __device__ __forceinline__ void knn_candidates(const GridDesc& g, const uint32_t
 // k_knn_normals_far instead, so one such point does not hold its whole wave.
 constexpr int kKnnLaneRings = 4;
 
-// Output of one point: normal in grid order; optionally the regularized
-// covariance and the normal in original order.
-__device__ __forceinline__ void write_knn_normal(uint32_t i, uint32_t orig, const float n[3],
-                                                 float b, float4* __restrict__ nrm_grid,
+// Output of one point: its grid-order record (point and normal); optionally
+// the regularized covariance and the normal in original order.
+__device__ __forceinline__ void write_knn_normal(uint32_t i, const float4& p, const float n[3],
+                                                 float b, float4* __restrict__ rec,
                                                  float* __restrict__ cov6,
                                                  float* __restrict__ normal3,
                                                  float4* __restrict__ nrm_orig = nullptr,
                                                  float* __restrict__ tau_orig = nullptr,
                                                  float tau = 0.0f) {
-  nrm_grid[i] = make_float4(n[0], n[1], n[2], 1.0f);
+  const uint32_t orig = __float_as_uint(p.w);
+  rec[kRec * size_t(i)] = p;
+  rec[kRec * size_t(i) + 1] = make_float4(n[0], n[1], n[2], 1.0f);
   if (nrm_orig) nrm_orig[orig] = make_float4(n[0], n[1], n[2], 1.0f);
   if (tau_orig) tau_orig[orig] = tau;
   if (cov6) {
@@ -344,7 +346,7 @@ __global__ void __launch_bounds__(kBlock)
   const GridDesc g = C.g;
   const uint32_t* __restrict__ cs = C.cs;
   const float4* __restrict__ gp = C.gp;
-  float4* __restrict__ nrm_grid = C.nrm_grid;
+  float4* __restrict__ rec = C.rec;
   float* __restrict__ cov6 = C.cov6;
   float* __restrict__ normal3 = C.normal3;
   uint32_t i = (blockIdx.x - C.block_begin) * blockDim.x + threadIdx.x;
@@ -440,7 +442,7 @@ __global__ void __launch_bounds__(kBlock)
   }
 
   if (found >= 2) smallest_eigvec(c00, c11, c22, c01, c02, c12, n);
-  write_knn_normal(i, __float_as_uint(p.w), n, one_minus_eps, nrm_grid, cov6, normal3, C.nrm_orig,
+  write_knn_normal(i, p, n, one_minus_eps, rec, cov6, normal3, C.nrm_orig,
                    C.tau_orig, tau);
 }
 
@@ -625,7 +627,7 @@ __global__ void __launch_bounds__(kBlock) k_knn_normals_far(const KnnCloud* __re
       smallest_eigvec(c00, c11, c22, c01, c02, c12, nv);
     }
     if (lane == 0)
-      write_knn_normal(i, __float_as_uint(p.w), nv, one_minus_eps, C.nrm_grid, C.cov6, C.normal3,
+      write_knn_normal(i, p, nv, one_minus_eps, C.rec, C.cov6, C.normal3,
                        C.nrm_orig, C.tau_orig, tau);
   }
 }
@@ -675,21 +677,21 @@ void knn_normals_many(const std::vector<KnnCloud>& clouds_in, uint32_t k, double
   else fail(3, "k_neighbors above 64 is not supported by the HIP covariance kernel");
 }
 
-KnnCloud knn_cloud(const Grid& g, float4* nrm, float* cov6, float* normal3) {
+KnnCloud knn_cloud(const Grid& g, float4* rec, float* cov6, float* normal3) {
   KnnCloud c{};
   c.g = g.desc;
   c.cs = g.cell_start.p;
   c.gp = g.pts.p;
-  c.nrm_grid = nrm;
+  c.rec = rec;
   c.cov6 = cov6;
   c.normal3 = normal3;
   return c;
 }
 
-void knn_normals(const Grid& g, uint32_t k, double eps, float4* nrm, float* cov6, float* normal3,
+void knn_normals(const Grid& g, uint32_t k, double eps, float4* rec, float* cov6, float* normal3,
                  hipStream_t s) {
   if (g.desc.npoints == 0) return;
-  knn_normals_many({knn_cloud(g, nrm, cov6, normal3)}, k, eps, s);
+  knn_normals_many({knn_cloud(g, rec, cov6, normal3)}, k, eps, s);
 }
 
 // ------------------------------------------------------------ k-NN query ----

@@ -988,8 +988,8 @@ __global__ void __launch_bounds__(kBlock) k_gicp_settle(const IterInstance* __re
   load_pose(st, P);
   const uint32_t i = lb * kBlock + threadIdx.x;
   const bool valid = i < I.n_src;
-  const float4 p = ldg(I.src + (valid ? i : 0u));
-  const float4 ns = ldg(I.src_n + (valid ? i : 0u));
+  const float4 p = ldg(I.src_rec + kRec * (valid ? i : 0u));
+  const float4 ns = ldg(I.src_rec + kRec * (valid ? i : 0u) + 1);
   const uint32_t prev = valid ? ldg(I.match + (i)) : kNone;
   const float bud = valid ? ldg(I.budget + (i)) : 0.0f;
   const float move =
@@ -1010,8 +1010,8 @@ __global__ void __launch_bounds__(kBlock) k_gicp_settle(const IterInstance* __re
     if (prev == kNone) {
       if (cert) I.budget[i] = bud - move;
     } else {
-      q = ldg(I.tgt + (prev));
-      nt = ldg(I.tgt_n + (prev));
+      q = ldg(I.tgt_rec + kRec * prev);
+      nt = ldg(I.tgt_rec + kRec * prev + 1);
       d2 = dist2(px, py, pz, q.x, q.y, q.z);
       if (d2 <= cfg.dmax2) {
         if (cert) I.budget[i] = bud - move;
@@ -1066,7 +1066,7 @@ __device__ __forceinline__ void search_block(const IterInstance* __restrict__ in
   for (uint32_t k = 0; k < ppt; ++k) {
     const uint32_t i = (lb * ppt + k) * kBlock + threadIdx.x;
     const bool valid = i < I.n_src;
-    const float4 p = ldg(I.src + (valid ? i : 0u));
+    const float4 p = ldg(I.src_rec + kRec * (valid ? i : 0u));
     float px, py, pz;
     transform_point(P, p, px, py, pz);
     const uint32_t prev = valid ? ldg(I.match + (i)) : kNone;
@@ -1098,7 +1098,7 @@ __device__ __forceinline__ void search_block(const IterInstance* __restrict__ in
         if (cert) I.budget[i] = bud - move;
         continue;
       }
-      const float4 q = ldg(I.tgt + (prev));
+      const float4 q = ldg(I.tgt_rec + kRec * prev);
       if (dist2(px, py, pz, q.x, q.y, q.z) <= cfg.dmax2) {
         if (cert) I.budget[i] = bud - move;
       } else {  // the nearest point is beyond d_max: none here, search next time
@@ -1149,7 +1149,7 @@ __global__ void __launch_bounds__(kBlock)
   NNQuery Q;
   Q.hard = false;
   if (mine) {
-    transform_point(P, ldg(I.src + (i)), px, py, pz);
+    transform_point(P, ldg(I.src_rec + kRec * i), px, py, pz);
     const uint32_t prev = ldg(I.match + (i));
     if (gap)
       nn_phase1<kStats, true>(g, I.cell_start, I.empty_radius, I.tgt, px, py, pz, cfg.dmax,
@@ -1206,8 +1206,8 @@ __global__ void __launch_bounds__(kBlock)
   if (ppt == 1) {
     const uint32_t i = lb * kBlock + threadIdx.x;
     const bool valid = i < I.n_src;
-    const float4 p = ldg(I.src + (valid ? i : 0u));
-    const float4 ns = ldg(I.src_n + (valid ? i : 0u));
+    const float4 p = ldg(I.src_rec + kRec * (valid ? i : 0u));
+    const float4 ns = ldg(I.src_rec + kRec * (valid ? i : 0u) + 1);
     const uint32_t prev = valid ? ldg(I.match + (i)) : kNone;
     const float bud = valid ? ldg(I.budget + (i)) : 0.0f;
     const float move =
@@ -1229,8 +1229,8 @@ __global__ void __launch_bounds__(kBlock)
         if (cert) I.budget[i] = bud - move;
       } else if (valid) {
         if constexpr (kStats) stats_add(stats, 0, 0, 0, kBlockDone);
-        const float4 q = ldg(I.tgt + (prev));
-        const float4 nt = ldg(I.tgt_n + (prev));
+        const float4 q = ldg(I.tgt_rec + kRec * prev);
+        const float4 nt = ldg(I.tgt_rec + kRec * prev + 1);
         const float d2 = dist2(px, py, pz, q.x, q.y, q.z);
         if (d2 <= cfg.dmax2) {
           if (cert) I.budget[i] = bud - move;
@@ -1253,7 +1253,7 @@ __global__ void __launch_bounds__(kBlock)
   for (uint32_t k = 0; k < ppt; ++k) {
     const uint32_t i = (lb * ppt + k) * kBlock + threadIdx.x;
     const bool valid = i < I.n_src;
-    const float4 p = ldg(I.src + (valid ? i : 0u));
+    const float4 p = ldg(I.src_rec + kRec * (valid ? i : 0u));
     float px, py, pz;
     transform_point(P, p, px, py, pz);
     const uint32_t prev = valid ? ldg(I.match + (i)) : kNone;
@@ -1286,7 +1286,7 @@ __global__ void __launch_bounds__(kBlock)
         if (cert) I.budget[i] = bud - move;
         continue;
       }
-      const float4 q = ldg(I.tgt + (prev));
+      const float4 q = ldg(I.tgt_rec + kRec * prev);
       if (dist2(px, py, pz, q.x, q.y, q.z) <= cfg.dmax2) {
         if (cert) I.budget[i] = bud - move;
       } else {  // the nearest point is beyond d_max: none here, search next time
@@ -1316,10 +1316,11 @@ __global__ void __launch_bounds__(kBlock)
     const uint32_t j = ldg(I.match + (i));
     if (j == kNone) continue;
     float px, py, pz;
-    transform_point(P, ldg(I.src + (i)), px, py, pz);
-    const float4 q = ldg(I.tgt + (j));
+    transform_point(P, ldg(I.src_rec + kRec * i), px, py, pz);
+    const float4 q = ldg(I.tgt_rec + kRec * j);
     const float d2 = dist2(px, py, pz, q.x, q.y, q.z);
-    add_terms(acc, P, beta, eps, px, py, pz, q, ldg(I.tgt_n + (j)), ldg(I.src_n + (i)), d2);
+    add_terms(acc, P, beta, eps, px, py, pz, q, ldg(I.tgt_rec + kRec * j + 1),
+              ldg(I.src_rec + kRec * i + 1), d2);
     ++count;
   }
   write_partial(L, acc, count, I.partials + size_t(lb) * 64);
@@ -1445,11 +1446,12 @@ __global__ void __launch_bounds__(kBlock) k_gicp_terms(const IterInstance* __res
       const uint32_t j = ldg(I.match + (i));
       if (j == kNone) continue;
       float px, py, pz;
-      transform_point(P, ldg(I.src + (i)), px, py, pz);
-      const float4 q = ldg(I.tgt + (j));
+      transform_point(P, ldg(I.src_rec + kRec * i), px, py, pz);
+      const float4 q = ldg(I.tgt_rec + kRec * j);
       // the squared distance the search compared (same operations, same operands)
       const float d2 = dist2(px, py, pz, q.x, q.y, q.z);
-      add_terms(acc, P, beta, eps, px, py, pz, q, ldg(I.tgt_n + (j)), ldg(I.src_n + (i)), d2);
+      add_terms(acc, P, beta, eps, px, py, pz, q, ldg(I.tgt_rec + kRec * j + 1),
+                ldg(I.src_rec + kRec * i + 1), d2);
       ++count;
     }
     write_partial(L, acc, count, I.partials + size_t(lb) * 64);

@@ -260,17 +260,20 @@ __global__ void k_grid_gather(const float4* __restrict__ pts, const uint32_t* __
   out[i] = make_float4(p.x, p.y, p.z, __uint_as_float(o));
 }
 
-__global__ void k_gather_to_grid(const float4* __restrict__ orig, const uint32_t* __restrict__ order,
-                                 uint32_t n, float4* __restrict__ out) {
+__global__ void k_gather_records(const float4* __restrict__ gp, const float4* __restrict__ orig,
+                                 const uint32_t* __restrict__ order, uint32_t n,
+                                 float4* __restrict__ rec) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = orig[order[i]];
+  if (i >= n) return;
+  rec[kRec * size_t(i)] = gp[i];
+  rec[kRec * size_t(i) + 1] = orig[order[i]];
 }
 
-void gather_to_grid_order(const Grid& g, const float4* orig, float4* out, hipStream_t s) {
+void gather_records(const Grid& g, const float4* orig, float4* rec, hipStream_t s) {
   uint32_t n = g.desc.npoints;
   if (n == 0) return;
-  hipLaunchKernelGGL(k_gather_to_grid, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, orig,
-                     g.order.p, n, out);
+  hipLaunchKernelGGL(k_gather_records, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                     g.pts.p, orig, g.order.p, n, rec);
 }
 
 // The cell edge is picked from the measured mean occupancy of non-empty cells

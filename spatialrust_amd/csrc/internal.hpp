@@ -254,17 +254,24 @@ void host_bounds(const Cloud& c, uint32_t* host6, hipStream_t s);
 // Largest k of the k-NN kernels (their per-lane sorting networks).
 constexpr uint32_t kMaxKnnK = 64;
 
-// Exact k-NN normals of the grid's own points, in grid order: float4(n, a) with
-// C = a*I - (1-eps) n n^T (a = 1). If cov6 is non-null also writes the 6
+// Grid-order point records (2 float4 per grid position i, 32 bytes, one
+// aligned sector): [2i] = the grid point (x, y, z, original index bits),
+// [2i+1] = its plane normal float4(n, a) with C = a*I - (1-eps) n n^T. The
+// Gauss-Newton passes gather a matched target point and its normal from one
+// record instead of two arrays.
+constexpr uint32_t kRec = 2;
+
+// Exact k-NN normals of the grid's own points as grid-order point records
+// (a = 1; 2 * npoints float4). If cov6 is non-null also writes the 6
 // regularized covariance entries per point in ORIGINAL index order.
-void knn_normals(const Grid& g, uint32_t k, double eps, float4* normals_grid_order, float* cov6,
+void knn_normals(const Grid& g, uint32_t k, double eps, float4* records, float* cov6,
                  float* normal3, hipStream_t s);
 // One cloud of a k-NN normals launch over many clouds (outputs as knn_normals).
 struct KnnCloud {
   GridDesc g;
   const uint32_t* cs;
   const float4* gp;
-  float4* nrm_grid;
+  float4* rec;  // grid-order point records (kRec float4 a point)
   float* cov6;
   float* normal3;
   uint32_t block_begin;  // first workgroup of this cloud in the launch (set by the launcher)
@@ -278,7 +285,7 @@ struct KnnCloud {
   float4* nrm_orig = nullptr;
   float* tau_orig = nullptr;
 };
-KnnCloud knn_cloud(const Grid& g, float4* normals_grid_order, float* cov6, float* normal3);
+KnnCloud knn_cloud(const Grid& g, float4* records, float* cov6, float* normal3);
 // knn_normals over many clouds in one launch (one K for all): a cloud's
 // results do not depend on the others.
 void knn_normals_many(const std::vector<KnnCloud>& clouds, uint32_t k, double eps, hipStream_t s);
@@ -291,8 +298,9 @@ void radius_normals(const Grid& rg, float radius, float eps, float4* normals_ori
 // original point order as float4(n, curvature).
 void grid_normals(const Grid& rg, float radius, const float* viewpoint, float4* out_orig,
                   hipStream_t s);
-// Permutes per-point float4 values from original order to grid order.
-void gather_to_grid_order(const Grid& g, const float4* orig, float4* out, hipStream_t s);
+// Grid-order point records whose normals are per-point float4 values given in
+// original order.
+void gather_records(const Grid& g, const float4* orig, float4* records, hipStream_t s);
 
 // Exact k-NN of queries against a grid (sr_knn).
 void knn_query(const Grid& g, const float4* queries, uint32_t nq, uint32_t k, int32_t* out_idx,
@@ -338,10 +346,9 @@ struct InstConfig {
 
 // One registration instance of a pass launch.
 struct IterInstance {
-  const float4* src;      // source points (any order; grid order for locality)
-  const float4* src_n;    // source normals float4(n, a)
+  const float4* src_rec;  // source point records (grid order for locality; kRec float4 a point)
   const float4* tgt;      // target points in grid order
-  const float4* tgt_n;    // target normals in grid order
+  const float4* tgt_rec;  // target point records (grid order, kRec float4 a point)
   const uint32_t* cell_start;
   const uint8_t* empty_radius;  // target grid's empty-radius field (may be null)
   uint32_t* match;        // per source point: target grid position matched by the
@@ -371,6 +378,7 @@ struct Options {
   std::atomic<int64_t> autotune{1};      // pick pass launch choices per problem class by timing
   std::atomic<int64_t> tune_force{-1};   // >= 0: use that candidate for every pass loop
   std::atomic<int64_t> odometry_incremental{1};  // odometry map: recompute affected normals only
+  std::atomic<int64_t> pass_graphs{1};   // replay repeated pass loops as captured HIP graphs
 };
 Options& options();
 
@@ -462,7 +470,7 @@ struct VoxelMap {
 // geometry as before. Writes subset[0 .. *count) (device).
 void map_refresh_set(const Grid& g, const float4* list, uint32_t n_old, const float* ltau,
                      uint32_t* subset, uint32_t* count, hipStream_t s);
-// nrm_grid[i] = lnrm[g.order[i]] for every grid position.
-void gather_list_normals(const Grid& g, const float4* lnrm, float4* nrm_grid, hipStream_t s);
+// Grid-order point records with normal lnrm[g.order[i]] at grid position i.
+void gather_list_normals(const Grid& g, const float4* lnrm, float4* records, hipStream_t s);
 
 }  // namespace sr

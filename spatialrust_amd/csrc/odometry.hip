@@ -184,11 +184,14 @@ __global__ void k_map_affected(const float4* __restrict__ gp, uint32_t n, GridDe
   if (aff) out[base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)))] = i;
 }
 
-// nrm_grid[i] = lnrm[order[i]]: list-order normals into grid order.
-__global__ void k_gather_normals(const uint32_t* __restrict__ order, const float4* __restrict__ lnrm,
-                                 uint32_t n, float4* __restrict__ nrm_grid) {
+// Grid-order point records from the grid points and the list-order normals.
+__global__ void k_gather_normals(const float4* __restrict__ gp, const uint32_t* __restrict__ order,
+                                 const float4* __restrict__ lnrm, uint32_t n,
+                                 float4* __restrict__ rec) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) nrm_grid[i] = lnrm[order[i]];
+  if (i >= n) return;
+  rec[kRec * size_t(i)] = gp[i];
+  rec[kRec * size_t(i) + 1] = lnrm[order[i]];
 }
 
 }  // namespace
@@ -213,11 +216,11 @@ void map_refresh_set(const Grid& g, const float4* list, uint32_t n_old, const fl
   // the field returns to the cache; ordering on s keeps its reuse safe
 }
 
-void gather_list_normals(const Grid& g, const float4* lnrm, float4* nrm_grid, hipStream_t s) {
+void gather_list_normals(const Grid& g, const float4* lnrm, float4* rec, hipStream_t s) {
   const uint32_t n = g.desc.npoints;
   if (n)
-    hipLaunchKernelGGL(k_gather_normals, dim3(blocks_for(n)), dim3(kBlk), 0, s, g.order.p, lnrm, n,
-                       nrm_grid);
+    hipLaunchKernelGGL(k_gather_normals, dim3(blocks_for(n)), dim3(kBlk), 0, s, g.pts.p, g.order.p,
+                       lnrm, n, rec);
 }
 
 void VoxelMap::init(float leaf_, uint32_t capacity_, hipStream_t s) {

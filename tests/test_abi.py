@@ -145,7 +145,8 @@ def test_runtime_options_round_trip_and_errors():
         sr.set_option("no_such_option", 1)
     with pytest.raises(sr.InvalidArgument, match="0 or 1"):
         sr.set_option("match_reuse", 2)
-    for name, default in (("radius_tiles", 0), ("autotune", 1), ("odometry_incremental", 1)):
+    for name, default in (("radius_tiles", 0), ("autotune", 1), ("odometry_incremental", 1),
+                          ("pass_graphs", 1)):
         assert sr.get_option(name) == default
         sr.set_option(name, 1 - default)
         try:
