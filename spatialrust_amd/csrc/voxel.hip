@@ -339,12 +339,19 @@ __global__ void k_voxel_insert(const float4* __restrict__ pts, uint32_t n,
   const unsigned long long lin = linear_voxel_key(P, p);
   uint32_t h = hash64(lin, mask);
   while (true) {
-    unsigned long long prev = atomicCAS(&tkeys[h], kEmptyKey, lin);
-    if (prev == kEmptyKey) {
-      tfirst[h] = i;  // a first guess at the slot's lowest index (k_voxel_first_fix)
-      break;
+    // A slot's key never changes once set, so a plain read that finds this
+    // key (or another one) decides without the atomic; a stale empty read
+    // only falls through to it.
+    const unsigned long long cur = tkeys[h];
+    if (cur == lin) break;
+    if (cur == kEmptyKey) {
+      const unsigned long long prev = atomicCAS(&tkeys[h], kEmptyKey, lin);
+      if (prev == kEmptyKey) {
+        tfirst[h] = i;  // a first guess at the slot's lowest index (k_voxel_first_fix)
+        break;
+      }
+      if (prev == lin) break;
     }
-    if (prev == lin) break;
     h = (h + 1) & mask;
   }
   slot_of[i] = h;
@@ -362,11 +369,15 @@ __global__ void k_voxel_first_fix(const uint32_t* __restrict__ slot_of, uint32_t
   if (*f > i) atomicMin(f, i);
 }
 
+// First-occurrence flags; also clears the per-cell counters (cells <= points).
 __global__ void k_voxel_flags(const uint32_t* __restrict__ slot_of,
                               const uint32_t* __restrict__ tfirst, uint32_t n,
-                              uint32_t* __restrict__ flag) {
+                              uint32_t* __restrict__ flag, uint32_t* __restrict__ ocount) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) flag[i] = tfirst[slot_of[i]] == i ? 1u : 0u;
+  if (i < n) {
+    flag[i] = tfirst[slot_of[i]] == i ? 1u : 0u;
+    ocount[i] = 0u;
+  }
 }
 
 __global__ void k_voxel_cellinfo(const float4* __restrict__ pts, const uint32_t* __restrict__ slot_of,
@@ -390,23 +401,29 @@ __global__ void k_voxel_cellinfo(const float4* __restrict__ pts, const uint32_t*
 // Points per output cell. The cells are numbered densely in first-occurrence
 // order, so these counters are a compact array (the hash table's slots are
 // spread over twice the points and their atomics cost several times more).
+// Each point keeps its cell and its arrival rank in the cell (the counter's
+// old value), so the scatter below needs no second round of atomics.
 __global__ void k_voxel_count(const uint32_t* __restrict__ slot_of,
                               const uint32_t* __restrict__ cell_of_slot, uint32_t n,
-                              uint32_t* __restrict__ ocount) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  // (the clamp only matters when a key-range error has left slots unnumbered)
-  if (i < n) atomicAdd(&ocount[min(cell_of_slot[slot_of[i]], n - 1)], 1u);
-}
-
-__global__ void k_voxel_scatter(const uint32_t* __restrict__ slot_of,
-                                const uint32_t* __restrict__ cell_of_slot,
-                                const uint32_t* __restrict__ cstart, uint32_t* __restrict__ cursor,
-                                uint32_t n, uint32_t* __restrict__ members) {
+                              uint32_t* __restrict__ ocount, uint32_t* __restrict__ cell_of,
+                              uint32_t* __restrict__ rank_in_cell) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  uint32_t c = min(cell_of_slot[slot_of[i]], n - 1);  // see k_voxel_count
-  uint32_t pos = min(cstart[c] + atomicAdd(&cursor[c], 1u), n - 1);
-  members[pos] = i;
+  // (the clamp only matters when a key-range error has left slots unnumbered)
+  const uint32_t c = min(cell_of_slot[slot_of[i]], n - 1);
+  cell_of[i] = c;
+  rank_in_cell[i] = atomicAdd(&ocount[c], 1u);
+}
+
+// Members of each cell in arrival order (the centroid kernels put them in
+// index order).
+__global__ void k_voxel_scatter(const uint32_t* __restrict__ cell_of,
+                                const uint32_t* __restrict__ rank_in_cell,
+                                const uint32_t* __restrict__ cstart, uint32_t n,
+                                uint32_t* __restrict__ members) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  members[min(cstart[cell_of[i]] + rank_in_cell[i], n - 1)] = i;
 }
 
 constexpr uint32_t kSmallCell = 64;
@@ -622,18 +639,18 @@ void voxel_downsample(const Cloud& in, float leaf, const float* origin, bool wan
   hipLaunchKernelGGL(k_voxel_insert, dim3(blocks), dim3(kBlock), 0, s, in.pts.p, n, params.p,
                      tkeys.p, tfirst.p, cap - 1, slot_of.p);
   hipLaunchKernelGGL(k_voxel_first_fix, dim3(blocks), dim3(kBlock), 0, s, slot_of.p, n, tfirst.p);
-  hipLaunchKernelGGL(k_voxel_flags, dim3(blocks), dim3(kBlock), 0, s, slot_of.p, tfirst.p, n, flag.p);
+  hipLaunchKernelGGL(k_voxel_flags, dim3(blocks), dim3(kBlock), 0, s, slot_of.p, tfirst.p, n, flag.p,
+                     ocount.p);
   exclusive_scan_u32(flag.p, rank.p, n, scal.p, s);  // scal[0] = cell count
-  fill_u32(ocount.p, 0u, n, s);
-  fill_u32(cursor.p, 0u, n, s);
   if (want_keys) out.keys.resize(size_t(3) * n);
   hipLaunchKernelGGL(k_voxel_cellinfo, dim3(blocks), dim3(kBlock), 0, s, in.pts.p, slot_of.p,
                      flag.p, rank.p, n, params.p, cell_of_slot.p, want_keys ? out.keys.p : nullptr);
+  // flag[] is free after k_voxel_cellinfo: it takes each point's cell
   hipLaunchKernelGGL(k_voxel_count, dim3(blocks), dim3(kBlock), 0, s, slot_of.p, cell_of_slot.p, n,
-                     ocount.p);
+                     ocount.p, flag.p, cursor.p);
   exclusive_scan_u32(ocount.p, cstart.p, n, nullptr, s);
-  hipLaunchKernelGGL(k_voxel_scatter, dim3(blocks), dim3(kBlock), 0, s, slot_of.p, cell_of_slot.p,
-                     cstart.p, cursor.p, n, members.p);
+  hipLaunchKernelGGL(k_voxel_scatter, dim3(blocks), dim3(kBlock), 0, s, flag.p, cursor.p, cstart.p, n,
+                     members.p);
   out.centroids.pts.resize(n);
   fill_u32(scal.p + 1, 0u, 1, s);
   fill_u32(scal.p + 4, 0u, 1, s);  // medium-cell count
@@ -755,12 +772,19 @@ __global__ void k_seg_voxel_insert(const float4* __restrict__ pts, const uint32_
   const unsigned long long lin = linear_voxel_key(P, pts[i]);
   uint32_t h = hash64(lin, T.mask);
   while (true) {
-    unsigned long long prev = atomicCAS(&tkeys[T.base + h], kEmptyKey, lin);
-    if (prev == kEmptyKey) {
-      tfirst[T.base + h] = i;  // a first guess at the slot's lowest index (k_voxel_first_fix)
-      break;
+    // A slot's key never changes once set, so a plain read that finds this
+    // key (or another one) decides without the atomic; a stale empty read
+    // only falls through to it.
+    const unsigned long long cur = tkeys[T.base + h];
+    if (cur == lin) break;
+    if (cur == kEmptyKey) {
+      const unsigned long long prev = atomicCAS(&tkeys[T.base + h], kEmptyKey, lin);
+      if (prev == kEmptyKey) {
+        tfirst[T.base + h] = i;  // a first guess at the slot's lowest index (k_voxel_first_fix)
+        break;
+      }
+      if (prev == lin) break;
     }
-    if (prev == lin) break;
     h = (h + 1) & T.mask;
   }
   slot_of[i] = T.base + h;
@@ -866,17 +890,17 @@ void voxel_downsample_many(const Cloud& in, const std::vector<uint32_t>& off, fl
   hipLaunchKernelGGL(k_seg_voxel_insert, dim3(blk.back()), dim3(kBlock), 0, s, in.pts.p, d_off.p,
                      nseg, d_blk.p, params.p, d_tab.p, tkeys.p, tfirst.p, slot_of.p);
   hipLaunchKernelGGL(k_voxel_first_fix, dim3(blocks), dim3(kBlock), 0, s, slot_of.p, n, tfirst.p);
-  hipLaunchKernelGGL(k_voxel_flags, dim3(blocks), dim3(kBlock), 0, s, slot_of.p, tfirst.p, n, flag.p);
+  hipLaunchKernelGGL(k_voxel_flags, dim3(blocks), dim3(kBlock), 0, s, slot_of.p, tfirst.p, n, flag.p,
+                     ocount.p);
   exclusive_scan_u32(flag.p, rank.p, n, scal.p, s);  // scal[0] = cell count
-  fill_u32(ocount.p, 0u, n, s);
-  fill_u32(cursor.p, 0u, n, s);
   hipLaunchKernelGGL(k_voxel_cellinfo, dim3(blocks), dim3(kBlock), 0, s, in.pts.p, slot_of.p,
                      flag.p, rank.p, n, params.p, cell_of_slot.p, (int64_t*)nullptr);
+  // flag[] is free after k_voxel_cellinfo: it takes each point's cell
   hipLaunchKernelGGL(k_voxel_count, dim3(blocks), dim3(kBlock), 0, s, slot_of.p, cell_of_slot.p, n,
-                     ocount.p);
+                     ocount.p, flag.p, cursor.p);
   exclusive_scan_u32(ocount.p, cstart.p, n, nullptr, s);
-  hipLaunchKernelGGL(k_voxel_scatter, dim3(blocks), dim3(kBlock), 0, s, slot_of.p, cell_of_slot.p,
-                     cstart.p, cursor.p, n, members.p);
+  hipLaunchKernelGGL(k_voxel_scatter, dim3(blocks), dim3(kBlock), 0, s, flag.p, cursor.p, cstart.p, n,
+                     members.p);
   out.centroids.pts.resize(n);
   fill_u32(scal.p + 1, 0u, 1, s);
   fill_u32(scal.p + 4, 0u, 1, s);  // medium-cell count
