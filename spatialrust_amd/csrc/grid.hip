@@ -482,15 +482,17 @@ static void sketch_edges(const float4* pts, const std::vector<uint32_t>& off,
     wg_begin[k] = wgs;
     wgs += S.nwg;
   }
-  DevBuf<SketchSeg> d_segs(m);
-  DevBuf<uint32_t> d_wg(m);
+  HostPack pk;
+  const size_t o_segs = pk.add(segs), o_wg = pk.add(wg_begin);
+  DevBuf<uint8_t> tables(pk.bytes.size());
   DevBuf<uint8_t> regs(size_t(wgs) * kSketchEdges * kSketchRegs);
   DevBuf<float> est(size_t(m) * kSketchEdges);
-  upload(d_segs.p, segs.data(), sizeof(SketchSeg) * m, s);
-  upload(d_wg.p, wg_begin.data(), sizeof(uint32_t) * m, s);
-  hipLaunchKernelGGL(k_edge_sketch, dim3(wgs), dim3(kBlock), 0, s, pts, d_segs.p, m, d_wg.p, regs.p);
+  upload(tables.p, pk.bytes.data(), pk.bytes.size(), s);  // one copy for both tables
+  const SketchSeg* d_segs = HostPack::at<SketchSeg>(tables.p, o_segs);
+  hipLaunchKernelGGL(k_edge_sketch, dim3(wgs), dim3(kBlock), 0, s, pts, d_segs, m,
+                     HostPack::at<uint32_t>(tables.p, o_wg), regs.p);
   hipLaunchKernelGGL(k_edge_sketch_merge, dim3(m * kSketchEdges), dim3(kSketchRegs), 0, s, regs.p,
-                     d_segs.p, est.p);
+                     d_segs, est.p);
   std::vector<float> host(size_t(m) * kSketchEdges);
   download(host.data(), est.p, sizeof(float) * host.size(), s);  // also keeps the tables alive until the kernels end
   for (uint32_t k = 0; k < m; ++k)
@@ -724,22 +726,6 @@ __global__ void k_seg_empty_pass(const uint8_t* __restrict__ in, uint8_t* __rest
   empty_pass_cell(in + G.cell_off, out + G.cell_off, G.d, axis, t - G.cell_off);
 }
 
-// Host arrays packed for one upload: add() returns each array's byte offset
-// (256-byte aligned), at() the device pointer at an offset of the uploaded copy.
-struct HostPack {
-  std::vector<uint8_t> bytes;
-  template <typename T>
-  size_t add(const std::vector<T>& v) {
-    const size_t o = (bytes.size() + 255) & ~size_t(255);
-    bytes.resize(o + sizeof(T) * std::max<size_t>(v.size(), 1));
-    if (!v.empty()) std::memcpy(bytes.data() + o, v.data(), sizeof(T) * v.size());
-    return o;
-  }
-  template <typename T>
-  static const T* at(const uint8_t* base, size_t o) {
-    return reinterpret_cast<const T*>(base + o);
-  }
-};
 
 
 static std::vector<uint32_t> blocks_of(const std::vector<uint32_t>& off) {
@@ -836,6 +822,10 @@ void build_grids_many(const Cloud& c, const std::vector<uint32_t>& off,
                  *d_pt = HostPack::at<uint32_t>(tables.p, o_pt);
   DevBuf<uint32_t> counts(size_t(ncells) + 1), csg(size_t(ncells) + 1), cid(std::max<uint32_t>(n, 1)),
       rnk(std::max<uint32_t>(n, 1)), order(std::max<uint32_t>(n, 1)), scal(2);
+  // The scratch below is released at the stream's next wait instead of
+  // waiting here (inside the allocation-free entry point, where that is not
+  // available, the stream is waited for at the end).
+  bool deferred = true;
   fill_u32(counts.p, 0u, size_t(ncells) + 1, s);
   hipLaunchKernelGGL(k_seg_grid_count, dim3(blk.back()), dim3(kBlock), 0, s, c.pts.p, d_sg, nseg,
                      d_blk, counts.p, cid.p, rnk.p);
@@ -860,6 +850,7 @@ void build_grids_many(const Cloud& c, const std::vector<uint32_t>& off,
     hipLaunchKernelGGL(k_sort_big_cells, dim3(128), dim3(kBlock), 0, s, csg.p, order.p, big.p, scal.p);
     hipLaunchKernelGGL(k_seg_grid_gather, grid1(n), dim3(kBlock), 0, s, c.pts.p, order.p, d_pt, nseg,
                        n, out.pts.p);
+    deferred = big.release_after(s) && deferred;
   }
   empty_segs = std::min(empty_segs, nseg);
   const uint32_t ecells = empty_segs ? out.cell_off[empty_segs] : 0u;
@@ -876,8 +867,12 @@ void build_grids_many(const Cloud& c, const std::vector<uint32_t>& off,
                          empty_segs, d_cell, ecells, axis);
     }
     out.empty_radius = std::move(tmp);
+    deferred = er.release_after(s) && deferred;
   }
-  SR_HIP(hipStreamSynchronize(s));  // the tables above are released on return
+  for (DevBuf<uint32_t>* b : {&counts, &csg, &cid, &rnk, &order, &scal})
+    deferred = b->release_after(s) && deferred;
+  deferred = tables.release_after(s) && deferred;
+  if (!deferred) SR_HIP(hipStreamSynchronize(s));
 }
 
 }  // namespace sr

@@ -4,8 +4,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cstddef>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -105,6 +107,23 @@ struct DevBuf {
 
 // Per-thread stream (default priority) of the calling host thread.
 hipStream_t thread_stream();
+
+// Host arrays packed for one upload: add() returns each array's byte offset
+// (256-byte aligned), at() the device pointer at an offset of the uploaded copy.
+struct HostPack {
+  std::vector<uint8_t> bytes;
+  template <typename T>
+  size_t add(const std::vector<T>& v) {
+    const size_t o = (bytes.size() + 255) & ~size_t(255);
+    bytes.resize(o + sizeof(T) * std::max<size_t>(v.size(), 1));
+    if (!v.empty()) std::memcpy(bytes.data() + o, v.data(), sizeof(T) * v.size());
+    return o;
+  }
+  template <typename T>
+  static const T* at(const uint8_t* base, size_t o) {
+    return reinterpret_cast<const T*>(base + o);
+  }
+};
 // hipMemcpyAsync host -> device of a small host buffer (staged in pinned
 // memory, so the call does not wait); `src` may be reused once it returns.
 void upload(void* dst, const void* src, size_t bytes, hipStream_t s);

@@ -812,13 +812,6 @@ static std::vector<uint32_t> seg_blocks(const std::vector<uint32_t>& off, uint32
   return b;
 }
 
-template <typename T>
-static void upload(DevBuf<T>& d, const std::vector<T>& h, hipStream_t s) {
-  d.resize(std::max<size_t>(h.size(), 1));
-  if (!h.empty())
-    upload(d.p, h.data(), sizeof(T) * h.size(), s);
-}
-
 void pack_many(const std::vector<SegSource>& srcs, Cloud& out, hipStream_t s) {
   std::vector<uint32_t> off(srcs.size() + 1, 0);
   for (size_t i = 0; i < srcs.size(); ++i) off[i + 1] = off[i] + srcs[i].n;
@@ -826,28 +819,31 @@ void pack_many(const std::vector<SegSource>& srcs, Cloud& out, hipStream_t s) {
   out.pts.resize(std::max<uint32_t>(out.n, 1));
   if (srcs.empty() || out.n == 0) return;
   std::vector<uint32_t> blk = seg_blocks(off, kBlock);
-  DevBuf<SegSource> d_src;
-  DevBuf<uint32_t> d_blk;
-  upload(d_src, srcs, s);
-  upload(d_blk, blk, s);
-  hipLaunchKernelGGL(k_seg_pack, dim3(blk.back()), dim3(kBlock), 0, s, d_src.p,
-                     uint32_t(srcs.size()), d_blk.p, out.pts.p);
+  HostPack pk;
+  const size_t o_src = pk.add(srcs), o_blk = pk.add(blk);
+  DevBuf<uint8_t> tables(pk.bytes.size());
+  upload(tables.p, pk.bytes.data(), pk.bytes.size(), s);  // one copy for both tables
+  hipLaunchKernelGGL(k_seg_pack, dim3(blk.back()), dim3(kBlock), 0, s,
+                     HostPack::at<SegSource>(tables.p, o_src), uint32_t(srcs.size()),
+                     HostPack::at<uint32_t>(tables.p, o_blk), out.pts.p);
   // the tables outlive the kernel: released at the next wait on s
-  if (!d_src.release_after(s) || !d_blk.release_after(s)) SR_HIP(hipStreamSynchronize(s));
+  if (!tables.release_after(s)) SR_HIP(hipStreamSynchronize(s));
 }
 
 void seg_bounds(const float4* pts, const std::vector<uint32_t>& off, uint32_t* dev_bounds,
                 hipStream_t s) {
   const uint32_t nseg = uint32_t(off.size() - 1);
   std::vector<uint32_t> blk = seg_blocks(off, kSegBoundsChunk);
-  DevBuf<uint32_t> d_off, d_blk;
-  upload(d_off, off, s);
-  upload(d_blk, blk, s);
+  HostPack pk;
+  const size_t o_off = pk.add(off), o_blk = pk.add(blk);
+  DevBuf<uint8_t> tables(pk.bytes.size());
+  upload(tables.p, pk.bytes.data(), pk.bytes.size(), s);
   hipLaunchKernelGGL(k_seg_bounds_init, dim3((6 * nseg + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
                      dev_bounds, nseg);
-  hipLaunchKernelGGL(k_seg_bounds, dim3(blk.back()), dim3(kBlock), 0, s, pts, d_off.p, nseg, d_blk.p,
-                     dev_bounds);
-  if (!d_off.release_after(s) || !d_blk.release_after(s)) SR_HIP(hipStreamSynchronize(s));
+  hipLaunchKernelGGL(k_seg_bounds, dim3(blk.back()), dim3(kBlock), 0, s, pts,
+                     HostPack::at<uint32_t>(tables.p, o_off), nseg,
+                     HostPack::at<uint32_t>(tables.p, o_blk), dev_bounds);
+  if (!tables.release_after(s)) SR_HIP(hipStreamSynchronize(s));
 }
 
 void voxel_downsample_many(const Cloud& in, const std::vector<uint32_t>& off, float leaf,
@@ -874,11 +870,13 @@ void voxel_downsample_many(const Cloud& in, const std::vector<uint32_t>& off, fl
   if (slots > 0xFFFFFFFFull) fail(3, "too many points for one batch");
   const uint32_t cap = uint32_t(slots);
   std::vector<uint32_t> blk = seg_blocks(off, kBlock);
-  DevBuf<SegTable> d_tab;
-  DevBuf<uint32_t> d_off, d_blk;
-  upload(d_tab, tab, s);
-  upload(d_off, off, s);
-  upload(d_blk, blk, s);
+  HostPack pk;
+  const size_t o_tab = pk.add(tab), o_off = pk.add(off), o_blk = pk.add(blk);
+  DevBuf<uint8_t> tables(pk.bytes.size());
+  upload(tables.p, pk.bytes.data(), pk.bytes.size(), s);  // one copy for the three tables
+  const SegTable* d_tab = HostPack::at<SegTable>(tables.p, o_tab);
+  const uint32_t* d_off = HostPack::at<uint32_t>(tables.p, o_off);
+  const uint32_t* d_blk = HostPack::at<uint32_t>(tables.p, o_blk);
   DevBuf<unsigned long long> tkeys(cap);
   DevBuf<uint32_t> tfirst(cap), cell_of_slot(cap);
   hipLaunchKernelGGL(k_fill_u64, dim3(fill_blocks(cap)), dim3(kBlock), 0, s, tkeys.p, kEmptyKey,
@@ -887,8 +885,8 @@ void voxel_downsample_many(const Cloud& in, const std::vector<uint32_t>& off, fl
       scal(5), tailv(nseg + 2);
   const unsigned blocks = (n + kBlock - 1) / kBlock;
   ProfScope prof_insert("voxel_insert", s);
-  hipLaunchKernelGGL(k_seg_voxel_insert, dim3(blk.back()), dim3(kBlock), 0, s, in.pts.p, d_off.p,
-                     nseg, d_blk.p, params.p, d_tab.p, tkeys.p, tfirst.p, slot_of.p);
+  hipLaunchKernelGGL(k_seg_voxel_insert, dim3(blk.back()), dim3(kBlock), 0, s, in.pts.p, d_off,
+                     nseg, d_blk, params.p, d_tab, tkeys.p, tfirst.p, slot_of.p);
   hipLaunchKernelGGL(k_voxel_first_fix, dim3(blocks), dim3(kBlock), 0, s, slot_of.p, n, tfirst.p);
   hipLaunchKernelGGL(k_voxel_flags, dim3(blocks), dim3(kBlock), 0, s, slot_of.p, tfirst.p, n, flag.p,
                      ocount.p);
@@ -914,7 +912,7 @@ void voxel_downsample_many(const Cloud& in, const std::vector<uint32_t>& off, fl
                      ocount.p, members.p, big_list.p, scal.p + 1, out.centroids.pts.p);
   fill_u32(tailv.p + nseg + 1, 0u, 1, s);
   hipLaunchKernelGGL(k_seg_voxel_tail, dim3((nseg + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
-                     rank.p, d_off.p, nseg, params.p, scal.p, tailv.p);
+                     rank.p, d_off, nseg, params.p, scal.p, tailv.p);
   std::vector<uint32_t> host(nseg + 2);
   download(host.data(), tailv.p, sizeof(uint32_t) * host.size(), s);
   if (host[nseg + 1]) fail(3, "voxel key range does not fit a 64-bit linear key; use a larger leaf_size");
