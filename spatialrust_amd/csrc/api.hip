@@ -650,10 +650,12 @@ static std::shared_ptr<SegHolder> prepare_segmented(const std::vector<const sr_x
     off[c + 1] = off[c] + uint32_t(views[c]->n);
   }
   Cloud raw;
-  pack_many(src, raw, s);
+  DevBuf<uint8_t> pack_tables;
+  const uint32_t* raw_bounds = nullptr;
+  pack_many(src, raw, s, &pack_tables, &raw_bounds);  // also the segments' bounds
   VoxelOut vox;
   std::vector<uint32_t> voff;
-  voxel_downsample_many(raw, off, cfg.voxel_leaf, vox, voff, s);
+  voxel_downsample_many(raw, off, cfg.voxel_leaf, vox, voff, s, raw_bounds);
   rc.stages |= SR_STAGE_VOXEL;
   raw.pts.release();
   std::vector<uint32_t> hb(6 * size_t(nseg));
@@ -867,11 +869,8 @@ static void batch_setup(const std::vector<const Prepared*>& src, const std::vect
   B.states.resize(count);
   B.partials.resize(size_t(blocks) * 64);
   B.match.resize(std::max<uint64_t>(total_pts, 1));
-  fill_u32(B.match.p, kNone, total_pts, s);
   B.budget.resize(std::max<uint64_t>(total_pts, 1));
-  fill_u32(reinterpret_cast<uint32_t*>(B.budget.p), 0u, total_pts, s);
   B.seen.resize(std::max<uint64_t>(total_pts, 1));
-  fill_u32(reinterpret_cast<uint32_t*>(B.seen.p), 0xFFFFFFFFu, 4 * total_pts, s);  // NaN
   uint32_t begin = 0;
   uint64_t src_begin = 0;
   for (uint32_t i = 0; i < count; ++i) {
@@ -900,7 +899,7 @@ static void batch_setup(const std::vector<const Prepared*>& src, const std::vect
   upload(B.insts.p, hi.data(), sizeof(IterInstance) * count, s);
   upload(B.block_inst.p, hbi.data(), sizeof(uint32_t) * blocks, s);
   upload(B.init.p, init.data(), sizeof(double) * init.size(), s);
-  init_states(B.states.p, B.init.p, count, s);
+  reset_batch_state(B.states.p, B.init.p, count, B.match.p, B.budget.p, B.seen.p, total_pts, s);
 }
 
 // ---- launch tuner -------------------------------------------------------------
@@ -991,10 +990,7 @@ static void run_passes(Batch& B, hipStream_t s, const PassTuning& t) {
 
 // The registrations' starting state (as batch_setup leaves it).
 static void reset_batch(Batch& B, hipStream_t s) {
-  init_states(B.states.p, B.init.p, B.count, s);
-  fill_u32(B.match.p, kNone, B.total_pts, s);
-  fill_u32(reinterpret_cast<uint32_t*>(B.budget.p), 0u, B.total_pts, s);
-  fill_u32(reinterpret_cast<uint32_t*>(B.seen.p), 0xFFFFFFFFu, 4 * B.total_pts, s);  // NaN
+  reset_batch_state(B.states.p, B.init.p, B.count, B.match.p, B.budget.p, B.seen.p, B.total_pts, s);
 }
 
 // Times every candidate on this batch (eager launches) and returns the

@@ -98,9 +98,8 @@ __device__ __forceinline__ void refresh_f32_pose(InstState* st) {
   }
 }
 
-__global__ void k_init_states(InstState* states, const double* __restrict__ init7, uint32_t count) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
+__device__ __forceinline__ void init_state(InstState* states, const double* __restrict__ init7,
+                                           uint32_t i) {
   InstState s;
   for (int a = 0; a < 4; ++a) s.q[a] = init7[7 * i + a];
   for (int a = 0; a < 3; ++a) s.t[a] = init7[7 * i + 4 + a];
@@ -119,10 +118,39 @@ __global__ void k_init_states(InstState* states, const double* __restrict__ init
   states[i] = s;
 }
 
+__global__ void k_init_states(InstState* states, const double* __restrict__ init7, uint32_t count) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) init_state(states, init7, i);
+}
+
 void init_states(InstState* states, const double* init7_dev, uint32_t count, hipStream_t s) {
   if (count == 0) return;
   hipLaunchKernelGGL(k_init_states, dim3((count + 127) / 128), dim3(128), 0, s, states, init7_dev,
                      count);
+}
+
+// A batch's starting state in one launch: each registration's state from its
+// initial pose, and each source point without a match (kNone), budget (0) or
+// searched position (NaN).
+__global__ void k_reset_batch(InstState* states, const double* __restrict__ init7, uint32_t count,
+                              uint32_t* __restrict__ match, float* __restrict__ budget,
+                              float4* __restrict__ seen, uint64_t n) {
+  const uint64_t t0 = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t0 < count) init_state(states, init7, uint32_t(t0));
+  const float nan = __uint_as_float(0xFFFFFFFFu);
+  for (uint64_t i = t0; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
+    match[i] = kNone;
+    budget[i] = 0.0f;
+    seen[i] = make_float4(nan, nan, nan, nan);
+  }
+}
+
+void reset_batch_state(InstState* states, const double* init7_dev, uint32_t count, uint32_t* match,
+                       float* budget, float4* seen, uint64_t n, hipStream_t s) {
+  const uint64_t want = (std::max<uint64_t>(n, count) + kBlock - 1) / kBlock;
+  const uint32_t blocks = uint32_t(std::min<uint64_t>(std::max<uint64_t>(want, 1), 4096));
+  hipLaunchKernelGGL(k_reset_batch, dim3(blocks), dim3(kBlock), 0, s, states, init7_dev, count, match,
+                     budget, seen, n);
 }
 
 // FP64 LDL^T solve of H x = -g (H symmetric, packed upper). false if a pivot

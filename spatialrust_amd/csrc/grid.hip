@@ -74,11 +74,13 @@ __global__ void k_grid_count(const float4* __restrict__ pts, uint32_t n, GridDes
 }
 
 // Point i goes to its cell's range at the slot its count increment returned
-// (arrival order; k_sort_cells restores index order within each cell).
+// (arrival order; k_sort_cells restores index order within each cell). Also
+// clears the two list counters clear2[0..1] of the cell sorts.
 __global__ void k_grid_scatter(const uint32_t* __restrict__ cid, const uint32_t* __restrict__ rnk,
                                uint32_t n, const uint32_t* __restrict__ start,
-                               uint32_t* __restrict__ order) {
+                               uint32_t* __restrict__ order, uint32_t* __restrict__ clear2) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 2) clear2[i] = 0u;
   if (i >= n) return;
   uint32_t c = cid[i];
   order[start[c] + rnk[i]] = i;
@@ -545,8 +547,7 @@ void build_grid(const Cloud& c, const uint32_t* hb, float h, Grid& g, hipStream_
   }
   if (n == 0) return;
   hipLaunchKernelGGL(k_grid_scatter, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, cid.p,
-                     rnk.p, n, g.cell_start.p, g.order.p);
-  fill_u32(scal.p, 0u, 2, s);  // big and medium cell counts
+                     rnk.p, n, g.cell_start.p, g.order.p, scal.p);  // (scal: big and medium cell counts)
   DevBuf<uint32_t> big(std::max<uint32_t>(nc, 1));
   hipLaunchKernelGGL(k_sort_cells, dim3((nc + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
                      g.cell_start.p, nc, g.order.p, big.p, scal.p);
@@ -840,8 +841,8 @@ void build_grids_many(const Cloud& c, const std::vector<uint32_t>& off,
     hipLaunchKernelGGL(k_seg_grid_rowbits, grid1(32 * rws), dim3(kBlock), 0, s, out.occ.p, d_sg, nseg,
                        d_row, uint32_t(rws), out.rowocc.p);
   if (n) {
-    hipLaunchKernelGGL(k_grid_scatter, grid1(n), dim3(kBlock), 0, s, cid.p, rnk.p, n, csg.p, order.p);
-    fill_u32(scal.p, 0u, 2, s);  // big and medium cell counts
+    hipLaunchKernelGGL(k_grid_scatter, grid1(n), dim3(kBlock), 0, s, cid.p, rnk.p, n, csg.p, order.p,
+                       scal.p);  // (scal: big and medium cell counts)
     DevBuf<uint32_t> big(std::max<uint32_t>(ncells, 1));
     hipLaunchKernelGGL(k_sort_cells, grid1(ncells), dim3(kBlock), 0, s, csg.p, ncells, order.p,
                        big.p, scal.p);

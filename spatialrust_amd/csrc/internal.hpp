@@ -193,7 +193,11 @@ struct SegSource {
   uint32_t offset;  // first point in the packed cloud
 };
 // Packs many device xyz columns into one float4 cloud, segment after segment.
-void pack_many(const std::vector<SegSource>& srcs, Cloud& out, hipStream_t s);
+// With `hold` set, the same sweep also reduces each segment's bounds (6
+// ordered words, as seg_bounds) into *dev_bounds, which lives in the buffer
+// handed to *hold.
+void pack_many(const std::vector<SegSource>& srcs, Cloud& out, hipStream_t s,
+               DevBuf<uint8_t>* hold = nullptr, const uint32_t** dev_bounds = nullptr);
 // Bounds (6 ordered words per segment) of segments [off[s], off[s+1]) of pts
 // into dev_bounds (stream-ordered; the host does not wait).
 void seg_bounds(const float4* pts, const std::vector<uint32_t>& off, uint32_t* dev_bounds,
@@ -201,9 +205,11 @@ void seg_bounds(const float4* pts, const std::vector<uint32_t>& off, uint32_t* d
 // voxel_downsample of every segment [off[s], off[s+1]) of `in` (origin = the
 // segment's minimum), in one set of launches. The segments' centroids are
 // out.centroids[cell_off[s], cell_off[s+1]), each exactly what voxel_downsample
-// returns for the segment alone. Synchronises the stream.
+// returns for the segment alone. Synchronises the stream. dev_bounds: the
+// segments' bounds when already reduced (pack_many), else computed here.
 void voxel_downsample_many(const Cloud& in, const std::vector<uint32_t>& off, float leaf,
-                           VoxelOut& out, std::vector<uint32_t>& cell_off, hipStream_t s);
+                           VoxelOut& out, std::vector<uint32_t>& cell_off, hipStream_t s,
+                           const uint32_t* dev_bounds = nullptr);
 
 // Centroid voxel filter of a device cloud (voxel.rs:550-636 semantics).
 void voxel_downsample(const Cloud& in, float leaf, const float* origin, bool want_keys,
@@ -452,6 +458,10 @@ void gicp_pass(const IterInstance* insts, const uint32_t* block_inst, uint32_t t
 // Initialises instance states from poses (device array of 7 doubles per
 // instance: quaternion x y z w, translation).
 void init_states(InstState* states, const double* init7, uint32_t count, hipStream_t s);
+// init_states, plus every source point without a match (kNone), budget (0) or
+// searched position (NaN): a batch's starting state in one launch.
+void reset_batch_state(InstState* states, const double* init7_dev, uint32_t count, uint32_t* match,
+                       float* budget, float4* seen, uint64_t n, hipStream_t s);
 
 // ---- rolling voxel map (odometry) ----------------------------------------
 

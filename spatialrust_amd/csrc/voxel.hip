@@ -369,11 +369,14 @@ __global__ void k_voxel_first_fix(const uint32_t* __restrict__ slot_of, uint32_t
   if (*f > i) atomicMin(f, i);
 }
 
-// First-occurrence flags; also clears the per-cell counters (cells <= points).
+// First-occurrence flags; also clears the per-cell counters (cells <= points)
+// and the four list counters clear4[0..3] of the centroid kernels.
 __global__ void k_voxel_flags(const uint32_t* __restrict__ slot_of,
                               const uint32_t* __restrict__ tfirst, uint32_t n,
-                              uint32_t* __restrict__ flag, uint32_t* __restrict__ ocount) {
+                              uint32_t* __restrict__ flag, uint32_t* __restrict__ ocount,
+                              uint32_t* __restrict__ clear4) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 4) clear4[i] = 0u;
   if (i < n) {
     flag[i] = tfirst[slot_of[i]] == i ? 1u : 0u;
     ocount[i] = 0u;
@@ -640,7 +643,7 @@ void voxel_downsample(const Cloud& in, float leaf, const float* origin, bool wan
                      tkeys.p, tfirst.p, cap - 1, slot_of.p);
   hipLaunchKernelGGL(k_voxel_first_fix, dim3(blocks), dim3(kBlock), 0, s, slot_of.p, n, tfirst.p);
   hipLaunchKernelGGL(k_voxel_flags, dim3(blocks), dim3(kBlock), 0, s, slot_of.p, tfirst.p, n, flag.p,
-                     ocount.p);
+                     ocount.p, scal.p + 1);  // (scal[1..4]: list counters and error word)
   exclusive_scan_u32(flag.p, rank.p, n, scal.p, s);  // scal[0] = cell count
   if (want_keys) out.keys.resize(size_t(3) * n);
   hipLaunchKernelGGL(k_voxel_cellinfo, dim3(blocks), dim3(kBlock), 0, s, in.pts.p, slot_of.p,
@@ -652,8 +655,6 @@ void voxel_downsample(const Cloud& in, float leaf, const float* origin, bool wan
   hipLaunchKernelGGL(k_voxel_scatter, dim3(blocks), dim3(kBlock), 0, s, flag.p, cursor.p, cstart.p, n,
                      members.p);
   out.centroids.pts.resize(n);
-  fill_u32(scal.p + 1, 0u, 1, s);
-  fill_u32(scal.p + 4, 0u, 1, s);  // medium-cell count
   DevBuf<uint32_t> big_list(n);
   hipLaunchKernelGGL(k_voxel_centroids, dim3(blocks), dim3(kBlock), 0, s, in.pts.p, scal.p,
                      cstart.p, ocount.p, members.p, out.centroids.pts.p, big_list.p, scal.p + 1,
@@ -682,12 +683,61 @@ void voxel_downsample(const Cloud& in, float leaf, const float* origin, bool wan
 
 constexpr uint32_t kSegBoundsChunk = kBlock * 8;  // points per workgroup of k_seg_bounds
 
+// Workgroup min/max of per-thread partial bounds, merged into the segment's
+// six ordered words (one atomic per word per workgroup).
+__device__ __forceinline__ void merge_block_bounds(float (&mn)[3], float (&mx)[3],
+                                                   uint32_t* __restrict__ b6) {
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    for (int a = 0; a < 3; ++a) {
+      mn[a] = fminf(mn[a], __shfl_xor(mn[a], o));
+      mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], o));
+    }
+  }
+  __shared__ float smn[kBlock / kWave][3], smx[kBlock / kWave][3];
+  const int wid = threadIdx.x / kWave;
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    for (int a = 0; a < 3; ++a) {
+      smn[wid][a] = mn[a];
+      smx[wid][a] = mx[a];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int a = threadIdx.x;
+    float l = smn[0][a], h = smx[0][a];
+    for (int w = 1; w < kBlock / kWave; ++w) {
+      l = fminf(l, smn[w][a]);
+      h = fmaxf(h, smx[w][a]);
+    }
+    if (l <= h) {
+      atomicMin(&b6[a], f2ord(l));
+      atomicMax(&b6[3 + a], f2ord(h));
+    }
+  }
+}
+
+// Workgroup w of segment s packs points [w*kSegBoundsChunk, ...) of it and,
+// when b is set, merges their bounds into the segment's words (b holds
+// f2ord(+inf) / f2ord(-inf) on entry).
 __global__ void k_seg_pack(const SegSource* __restrict__ src, uint32_t nseg,
-                           const uint32_t* __restrict__ blk_begin, float4* __restrict__ out) {
+                           const uint32_t* __restrict__ blk_begin, float4* __restrict__ out,
+                           uint32_t* __restrict__ b) {
   const uint32_t s = seg_of(blk_begin, nseg, blockIdx.x);
   const SegSource S = src[s];
-  const uint32_t i = (blockIdx.x - blk_begin[s]) * blockDim.x + threadIdx.x;
-  if (i < S.n) out[S.offset + i] = make_float4(S.x[i], S.y[i], S.z[i], 0.0f);
+  const uint32_t lo = (blockIdx.x - blk_begin[s]) * kSegBoundsChunk;
+  const uint32_t hi = min(S.n, lo + kSegBoundsChunk);
+  float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    const float x = S.x[i], y = S.y[i], z = S.z[i];
+    out[S.offset + i] = make_float4(x, y, z, 0.0f);
+    mn[0] = fminf(mn[0], x);
+    mn[1] = fminf(mn[1], y);
+    mn[2] = fminf(mn[2], z);
+    mx[0] = fmaxf(mx[0], x);
+    mx[1] = fmaxf(mx[1], y);
+    mx[2] = fmaxf(mx[2], z);
+  }
+  if (b) merge_block_bounds(mn, mx, b + 6 * s);
 }
 
 __global__ void k_seg_bounds_init(uint32_t* __restrict__ b, uint32_t nseg) {
@@ -713,38 +763,13 @@ __global__ void k_seg_bounds(const float4* __restrict__ pts, const uint32_t* __r
     mx[1] = fmaxf(mx[1], p.y);
     mx[2] = fmaxf(mx[2], p.z);
   }
-  for (int o = kWave / 2; o > 0; o >>= 1) {
-    for (int a = 0; a < 3; ++a) {
-      mn[a] = fminf(mn[a], __shfl_xor(mn[a], o));
-      mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], o));
-    }
-  }
-  __shared__ float smn[kBlock / kWave][3], smx[kBlock / kWave][3];
-  const int wid = threadIdx.x / kWave;
-  if ((threadIdx.x & (kWave - 1)) == 0) {
-    for (int a = 0; a < 3; ++a) {
-      smn[wid][a] = mn[a];
-      smx[wid][a] = mx[a];
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x < 3) {
-    const int a = threadIdx.x;
-    float l = smn[0][a], h = smx[0][a];
-    for (int w = 1; w < kBlock / kWave; ++w) {
-      l = fminf(l, smn[w][a]);
-      h = fmaxf(h, smx[w][a]);
-    }
-    if (l <= h) {
-      atomicMin(&b[6 * s + a], f2ord(l));
-      atomicMax(&b[6 * s + 3 + a], f2ord(h));
-    }
-  }
+  merge_block_bounds(mn, mx, b + 6 * s);
 }
 
 __global__ void k_seg_voxel_params(const uint32_t* __restrict__ b, uint32_t nseg, float leaf_inv,
-                                   VoxelParams* __restrict__ P) {
+                                   VoxelParams* __restrict__ P, uint32_t* __restrict__ err_word) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s == 0) *err_word = 0u;  // the key-range error flag of k_seg_voxel_tail
   if (s < nseg) voxel_params_from_bounds(b + 6 * s, leaf_inv, 0, 0.0f, 0.0f, 0.0f, P + s);
 }
 
@@ -812,20 +837,31 @@ static std::vector<uint32_t> seg_blocks(const std::vector<uint32_t>& off, uint32
   return b;
 }
 
-void pack_many(const std::vector<SegSource>& srcs, Cloud& out, hipStream_t s) {
+void pack_many(const std::vector<SegSource>& srcs, Cloud& out, hipStream_t s,
+               DevBuf<uint8_t>* hold, const uint32_t** dev_bounds) {
   std::vector<uint32_t> off(srcs.size() + 1, 0);
   for (size_t i = 0; i < srcs.size(); ++i) off[i + 1] = off[i] + srcs[i].n;
   out.n = off.back();
   out.pts.resize(std::max<uint32_t>(out.n, 1));
   if (srcs.empty() || out.n == 0) return;
-  std::vector<uint32_t> blk = seg_blocks(off, kBlock);
+  std::vector<uint32_t> blk = seg_blocks(off, kSegBoundsChunk);
+  // the bounds words start as f2ord(+inf) / f2ord(-inf) and go up with the
+  // tables (one copy for all three)
+  std::vector<uint32_t> binit(hold ? 6 * srcs.size() : 0);
+  for (size_t t = 0; t < binit.size(); ++t) binit[t] = (t % 6) < 3 ? 0xFF800000u : 0x007FFFFFu;
   HostPack pk;
-  const size_t o_src = pk.add(srcs), o_blk = pk.add(blk);
+  const size_t o_src = pk.add(srcs), o_blk = pk.add(blk), o_b = pk.add(binit);
   DevBuf<uint8_t> tables(pk.bytes.size());
-  upload(tables.p, pk.bytes.data(), pk.bytes.size(), s);  // one copy for both tables
+  upload(tables.p, pk.bytes.data(), pk.bytes.size(), s);
+  uint32_t* b = hold ? reinterpret_cast<uint32_t*>(tables.p + o_b) : nullptr;
   hipLaunchKernelGGL(k_seg_pack, dim3(blk.back()), dim3(kBlock), 0, s,
                      HostPack::at<SegSource>(tables.p, o_src), uint32_t(srcs.size()),
-                     HostPack::at<uint32_t>(tables.p, o_blk), out.pts.p);
+                     HostPack::at<uint32_t>(tables.p, o_blk), out.pts.p, b);
+  if (hold) {  // the caller keeps the tables (and the bounds in them)
+    *dev_bounds = b;
+    *hold = std::move(tables);
+    return;
+  }
   // the tables outlive the kernel: released at the next wait on s
   if (!tables.release_after(s)) SR_HIP(hipStreamSynchronize(s));
 }
@@ -847,17 +883,23 @@ void seg_bounds(const float4* pts, const std::vector<uint32_t>& off, uint32_t* d
 }
 
 void voxel_downsample_many(const Cloud& in, const std::vector<uint32_t>& off, float leaf,
-                           VoxelOut& out, std::vector<uint32_t>& cell_off, hipStream_t s) {
+                           VoxelOut& out, std::vector<uint32_t>& cell_off, hipStream_t s,
+                           const uint32_t* dev_bounds) {
   const uint32_t n = in.n, nseg = uint32_t(off.size() - 1);
   out.cells = 0;
   cell_off.assign(nseg + 1, 0);
   if (n == 0 || nseg == 0) return;
   const float inv_leaf = 1.0f / leaf;  // voxel.rs:267, f32 on the host
-  DevBuf<uint32_t> bounds(6 * size_t(nseg));
-  seg_bounds(in.pts.p, off, bounds.p, s);
+  DevBuf<uint32_t> bounds;
+  if (!dev_bounds) {
+    bounds.resize(6 * size_t(nseg));
+    seg_bounds(in.pts.p, off, bounds.p, s);
+    dev_bounds = bounds.p;
+  }
   DevBuf<VoxelParams> params(nseg);
+  DevBuf<uint32_t> tailv(nseg + 2);  // per-segment first cells, the total, the error flag
   hipLaunchKernelGGL(k_seg_voxel_params, dim3((nseg + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
-                     bounds.p, nseg, inv_leaf, params.p);
+                     dev_bounds, nseg, inv_leaf, params.p, tailv.p + nseg + 1);
   // one hash-table region per segment, sized as voxel_downsample sizes its table
   std::vector<SegTable> tab(nseg);
   uint64_t slots = 0;
@@ -882,14 +924,14 @@ void voxel_downsample_many(const Cloud& in, const std::vector<uint32_t>& off, fl
   hipLaunchKernelGGL(k_fill_u64, dim3(fill_blocks(cap)), dim3(kBlock), 0, s, tkeys.p, kEmptyKey,
                      size_t(cap));
   DevBuf<uint32_t> slot_of(n), flag(n), rank(n), ocount(n), cstart(n), cursor(n), members(n),
-      scal(5), tailv(nseg + 2);
+      scal(5);
   const unsigned blocks = (n + kBlock - 1) / kBlock;
   ProfScope prof_insert("voxel_insert", s);
   hipLaunchKernelGGL(k_seg_voxel_insert, dim3(blk.back()), dim3(kBlock), 0, s, in.pts.p, d_off,
                      nseg, d_blk, params.p, d_tab, tkeys.p, tfirst.p, slot_of.p);
   hipLaunchKernelGGL(k_voxel_first_fix, dim3(blocks), dim3(kBlock), 0, s, slot_of.p, n, tfirst.p);
   hipLaunchKernelGGL(k_voxel_flags, dim3(blocks), dim3(kBlock), 0, s, slot_of.p, tfirst.p, n, flag.p,
-                     ocount.p);
+                     ocount.p, scal.p + 1);  // (scal[1..4]: list counters and error word)
   exclusive_scan_u32(flag.p, rank.p, n, scal.p, s);  // scal[0] = cell count
   hipLaunchKernelGGL(k_voxel_cellinfo, dim3(blocks), dim3(kBlock), 0, s, in.pts.p, slot_of.p,
                      flag.p, rank.p, n, params.p, cell_of_slot.p, (int64_t*)nullptr);
@@ -900,8 +942,6 @@ void voxel_downsample_many(const Cloud& in, const std::vector<uint32_t>& off, fl
   hipLaunchKernelGGL(k_voxel_scatter, dim3(blocks), dim3(kBlock), 0, s, flag.p, cursor.p, cstart.p, n,
                      members.p);
   out.centroids.pts.resize(n);
-  fill_u32(scal.p + 1, 0u, 1, s);
-  fill_u32(scal.p + 4, 0u, 1, s);  // medium-cell count
   DevBuf<uint32_t> big_list(n);
   hipLaunchKernelGGL(k_voxel_centroids, dim3(blocks), dim3(kBlock), 0, s, in.pts.p, scal.p,
                      cstart.p, ocount.p, members.p, out.centroids.pts.p, big_list.p, scal.p + 1,
@@ -910,7 +950,6 @@ void voxel_downsample_many(const Cloud& in, const std::vector<uint32_t>& off, fl
                      cstart.p, ocount.p, members.p, big_list.p, scal.p + 4, out.centroids.pts.p);
   hipLaunchKernelGGL(k_voxel_big_cells, dim3(256), dim3(kBlock), 0, s, in.pts.p, cstart.p,
                      ocount.p, members.p, big_list.p, scal.p + 1, out.centroids.pts.p);
-  fill_u32(tailv.p + nseg + 1, 0u, 1, s);
   hipLaunchKernelGGL(k_seg_voxel_tail, dim3((nseg + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
                      rank.p, d_off, nseg, params.p, scal.p, tailv.p);
   std::vector<uint32_t> host(nseg + 2);
