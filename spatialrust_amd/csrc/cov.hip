@@ -851,7 +851,121 @@ __device__ __forceinline__ float radius_eigen_with(const GridDesc& g, const floa
   return count;
 }
 
-// The neighbourhood read straight from the grid in device memory.
+// The same evaluation with the 27-cell neighbourhood read as nine row
+// ranges: in grid order the cells x-1 .. x+1 of one (y, z) row are one
+// contiguous run of points, visited in the per-cell order (dx ascending, then
+// the points ascending), so the f32 sums, and the results, are those of
+// radius_eigen_with. The nine ranges' cell starts are loaded together once
+// (18 words) and serve both moment passes, and each range is streamed four
+// points per round, so a lane waits on a few rounds of loads instead of a
+// chain of per-cell and per-point round trips.
+constexpr int kRadiusStream = 4;
+
+template <typename Point, typename Pass>
+__device__ __forceinline__ void radius_rows_visit(Point&& point, const uint32_t (&lo)[9],
+                                                  const uint32_t (&hi)[9], Pass&& visit) {
+#pragma unroll 1
+  for (int r = 0; r < 9; ++r) {
+    const uint32_t e = hi[r];
+#pragma unroll 1
+    for (uint32_t s = lo[r]; s < e; s += kRadiusStream) {
+      float4 q[kRadiusStream];
+#pragma unroll
+      for (int u = 0; u < kRadiusStream; ++u) q[u] = point(min(s + uint32_t(u), e - 1));
+#pragma unroll
+      for (int u = 0; u < kRadiusStream; ++u)
+        if (s + uint32_t(u) < e) visit(q[u]);
+    }
+  }
+}
+
+// The point's cell and, for each of the nine (y, z) rows around it (r = (dz +
+// 1) * 3 + (dy + 1)), its x-cell range [x0, x1); rows outside the grid get
+// an empty range (in = false).
+__device__ __forceinline__ void radius_rows(const GridDesc& g, const float4& p, int& cy, int& cz,
+                                            uint32_t& x0, uint32_t& x1) {
+  const int cx = cell_coord(p.x, g.ox, g.inv_h, g.dx);
+  cy = cell_coord(p.y, g.oy, g.inv_h, g.dy);
+  cz = cell_coord(p.z, g.oz, g.inv_h, g.dz);
+  x0 = uint32_t(max(cx - 1, 0));
+  x1 = uint32_t(min(cx + 1, g.dx - 1)) + 1u;
+}
+
+template <typename Point>
+__device__ __forceinline__ float radius_eigen_ranges(const float4& p, float r2,
+                                                     const uint32_t (&lo)[9],
+                                                     const uint32_t (&hi)[9], Point&& point,
+                                                     float (&a)[3][3], float (&v)[3][3]);
+
+__device__ __forceinline__ float radius_eigen_rows(const GridDesc& g,
+                                                   const uint32_t* __restrict__ cs,
+                                                   const float4* __restrict__ gp, const float4& p,
+                                                   float r2, float (&a)[3][3], float (&v)[3][3]) {
+  int cy, cz;
+  uint32_t x0, x1;
+  radius_rows(g, p, cy, cz, x0, x1);
+  uint32_t lo[9], hi[9];
+#pragma unroll
+  for (int r = 0; r < 9; ++r) {
+    const int ny = cy + r % 3 - 1, nz = cz + r / 3 - 1;
+    const bool in = ny >= 0 && ny < g.dy && nz >= 0 && nz < g.dz;
+    const uint32_t base = in ? (uint32_t(nz) * uint32_t(g.dy) + uint32_t(ny)) * uint32_t(g.dx) : 0u;
+    lo[r] = in ? cs[base + x0] : 0u;
+    hi[r] = in ? cs[base + x1] : 0u;
+  }
+  return radius_eigen_ranges(p, r2, lo, hi, [&](uint32_t s) { return gp[s]; }, a, v);
+}
+
+// The two moment passes and the Jacobi sweeps over the nine row ranges.
+template <typename Point>
+__device__ __forceinline__ float radius_eigen_ranges(const float4& p, float r2,
+                                                     const uint32_t (&lo)[9],
+                                                     const uint32_t (&hi)[9], Point&& point,
+                                                     float (&a)[3][3], float (&v)[3][3]) {
+  float mx = 0.0f, my = 0.0f, mz = 0.0f, count = 0.0f;
+  radius_rows_visit(point, lo, hi, [&](const float4& q) {
+    float ddx = __fsub_rn(q.x, p.x), ddy = __fsub_rn(q.y, p.y), ddz = __fsub_rn(q.z, p.z);
+    float d2 = __fadd_rn(__fadd_rn(__fmul_rn(ddx, ddx), __fmul_rn(ddy, ddy)), __fmul_rn(ddz, ddz));
+    if (d2 <= r2) {
+      mx = __fadd_rn(mx, q.x);
+      my = __fadd_rn(my, q.y);
+      mz = __fadd_rn(mz, q.z);
+      count = __fadd_rn(count, 1.0f);
+    }
+  });
+  if (count < 3.0f) return count;
+  mx = __fdiv_rn(mx, count);
+  my = __fdiv_rn(my, count);
+  mz = __fdiv_rn(mz, count);
+  float c00 = 0, c11 = 0, c22 = 0, c01 = 0, c02 = 0, c12 = 0;
+  radius_rows_visit(point, lo, hi, [&](const float4& q) {
+    float rx = __fsub_rn(q.x, p.x), ry = __fsub_rn(q.y, p.y), rz = __fsub_rn(q.z, p.z);
+    float d2 = __fadd_rn(__fadd_rn(__fmul_rn(rx, rx), __fmul_rn(ry, ry)), __fmul_rn(rz, rz));
+    if (d2 <= r2) {
+      float ex = __fsub_rn(q.x, mx), ey = __fsub_rn(q.y, my), ez = __fsub_rn(q.z, mz);
+      c00 = __fadd_rn(c00, __fmul_rn(ex, ex));
+      c11 = __fadd_rn(c11, __fmul_rn(ey, ey));
+      c22 = __fadd_rn(c22, __fmul_rn(ez, ez));
+      c01 = __fadd_rn(c01, __fmul_rn(ex, ey));
+      c02 = __fadd_rn(c02, __fmul_rn(ex, ez));
+      c12 = __fadd_rn(c12, __fmul_rn(ey, ez));
+    }
+  });
+  a[0][0] = c00; a[0][1] = c01; a[0][2] = c02;
+  a[1][0] = c01; a[1][1] = c11; a[1][2] = c12;
+  a[2][0] = c02; a[2][1] = c12; a[2][2] = c22;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) v[r][c] = r == c ? 1.0f : 0.0f;
+  for (int sweep = 0; sweep < 16; ++sweep) {
+    wgsl_rotate(a, v, 0, 1);
+    wgsl_rotate(a, v, 0, 2);
+    wgsl_rotate(a, v, 1, 2);
+  }
+  return count;
+}
+
+// The neighbourhood read straight from the grid in device memory, cell by
+// cell (the tiled form's fallback for tiles whose halo exceeds the LDS).
 __device__ __forceinline__ float radius_eigen(const GridDesc& g, const uint32_t* __restrict__ cs,
                                               const float4* __restrict__ gp, const float4& p,
                                               float r2, float (&a)[3][3], float (&v)[3][3]) {
@@ -949,7 +1063,8 @@ __device__ __forceinline__ void grid_normal_out(const float4& p, float count, co
 
 // Radius covariances, COV_GRID_WGSL semantics: eigenvalues (eps, 1, 1) with the
 // smallest mapped to eps; eps*I below 3 neighbours. One thread per point, the
-// neighbourhood read from device memory (the form without LDS tiles).
+// neighbourhood read from device memory as nine row ranges (the form without
+// LDS tiles).
 __global__ void __launch_bounds__(kBlock) k_radius_cov(GridDesc g, const uint32_t* __restrict__ cs,
                                                        const float4* __restrict__ gp, float r2,
                                                        float eps, float4* __restrict__ nrm_orig,
@@ -959,7 +1074,7 @@ __global__ void __launch_bounds__(kBlock) k_radius_cov(GridDesc g, const uint32_
   if (i >= g.npoints) return;
   const float4 p = gp[i];
   float a[3][3], v[3][3];
-  const float count = radius_eigen(g, cs, gp, p, r2, a, v);
+  const float count = radius_eigen_rows(g, cs, gp, p, r2, a, v);
   radius_cov_out(p, count, a, v, eps, nrm_orig, cov6, normal3);
 }
 
@@ -976,7 +1091,7 @@ __global__ void __launch_bounds__(kBlock) k_grid_normals(GridDesc g, const uint3
   if (i >= g.npoints) return;
   const float4 p = gp[i];
   float a[3][3], v[3][3];
-  const float count = radius_eigen(g, cs, gp, p, r2, a, v);
+  const float count = radius_eigen_rows(g, cs, gp, p, r2, a, v);
   grid_normal_out(p, count, a, v, has_vp, vx, vy, vz, out4);
 }
 
@@ -992,12 +1107,14 @@ __global__ void __launch_bounds__(kBlock) k_grid_normals(GridDesc g, const uint3
 // bit-identical to it. A tile whose neighbourhood exceeds kHaloCap points is
 // read from device memory instead (same arithmetic).
 //
-// Measured on 898k points (radius 0.4 m): 0.63 ms against 0.28 ms for the
-// per-point form, which reads the same neighbourhoods at L2 speed with far
-// more waves in flight; tiles of 128 or 256 threads, or larger halos, are
-// slower still (the LDS per workgroup caps the waves per CU, and many rows
-// crossing a wall make tiles of a few points). The per-point form is the
-// default ("radius_tiles" 0); this one is kept selectable and tested equal.
+// Both forms read a point's neighbourhood as nine row ranges, four points per
+// round. Measured on 898k points (radius 0.4 m): 0.46 ms against 0.19 ms for
+// the per-point form, which reads the same neighbourhoods at L2 speed with far
+// more waves in flight (cell by cell, the two took 0.63 and 0.29 ms); tiles of
+// 128 or 256 threads, or larger halos, are slower still (the LDS per workgroup
+// caps the waves per CU, and many rows crossing a wall make tiles of a few
+// points). The per-point form is the default ("radius_tiles" 0); this one is
+// kept selectable and tested equal.
 
 #ifndef SR_TILE_THREADS
 #define SR_TILE_THREADS 64
@@ -1106,14 +1223,19 @@ __global__ void __launch_bounds__(kTileThreads) k_radius_tiled(
       float a[3][3], v[3][3];
       float count;
       if (staged) {
-        count = radius_eigen_with(
-            g, p, r2,
-            [&](int nx, int ny, int nz, uint32_t& lo, uint32_t& hi) {
-              const int rr = (nz - cz + 1) * 3 + (ny - cy + 1), j = nx - xa;
-              lo = cst[rr][j];
-              hi = cst[rr][j + 1];
-            },
-            [&](uint32_t s) { return halo[s]; }, a, v);
+        // the point's nine row ranges in the halo (its rows lie within the
+        // tile's 3 x 3 rows, its cells within xa .. xb)
+        int py_, pz_;
+        uint32_t x0, x1;
+        radius_rows(g, p, py_, pz_, x0, x1);
+        uint32_t lo[9], hi[9];
+#pragma unroll
+        for (int r = 0; r < 9; ++r) {
+          const int rr = (pz_ + r / 3 - 1 - cz + 1) * 3 + (py_ + r % 3 - 1 - cy + 1);
+          lo[r] = cst[rr][x0 - uint32_t(xa)];
+          hi[r] = cst[rr][x1 - uint32_t(xa)];
+        }
+        count = radius_eigen_ranges(p, r2, lo, hi, [&](uint32_t s) { return halo[s]; }, a, v);
       } else {
         count = radius_eigen(g, cs, gp, p, r2, a, v);
       }
