@@ -11,6 +11,7 @@ pass cannot show (gpu).
                           on the batch, on the large block, and on a partial
                           overlap whose misses are kept under budgets too),
                           and on does less search work
+  pass loops as HIP graphs (eager, captured, replayed) == eager, bit for bit
   singular normal equations (gicp.rs:190-194, linalg.rs:25-27)
                           the reference's error on an input whose H has an
                           exactly zero row
@@ -163,6 +164,41 @@ def test_match_reuse_off_is_bit_identical(hip, reuse_restored, batch256):
     assert all(_same(x, y) for x, y in zip(on[2], off[2]))
     print(f"points tested over 31 passes: reuse on {on[3]}, off {off[3]}", flush=True)
     assert on[3] < off[3]
+
+
+@pytest.fixture
+def graphs_restored():
+    yield
+    sr.set_option("pass_graphs", 1)
+
+
+def test_pass_graphs_off_is_bit_identical(hip, graphs_restored, batch256):
+    """A repeated launch shape runs eagerly, is captured as a HIP graph on its
+    second run and replayed after that; with pass_graphs 0 every loop runs
+    eagerly. The same registrations (one pair alone, and a 6-pair batch) give
+    the same bits every time either way."""
+    assert sr.get_option("pass_graphs") == 1
+    w = synth.small_cube_pair()
+    src, tgt = devmem.DeviceCloud(w.source), devmem.DeviceCloud(w.target)
+    pairs, srcs, tgts = batch256
+    sub = [3, 17, 40, 88, 150, 201]
+    reg = sr.GicpRegistration(_fixed(max_iterations=30, voxel_leaf=0.1))
+    runs = {}
+    try:
+        for graphs in (1, 0):
+            sr.set_option("pass_graphs", graphs)
+            one = [reg.align(src.view, tgt.view) for _ in range(3)]
+            many = [sr.batch_align_pairs([srcs[i].view for i in sub], [tgts[i].view for i in sub],
+                                         _fixed(**BATCH_CFG)) for _ in range(3)]
+            runs[graphs] = (one, many)
+    finally:
+        src.close()
+        tgt.close()
+    ref = runs[1][0][0]
+    for graphs in (1, 0):
+        one, many = runs[graphs]
+        assert all(_same(ref, r) for r in one), graphs
+        assert all(_same(a, b) for m in many for a, b in zip(runs[1][1][0], m)), graphs
 
 
 def test_match_reuse_off_is_bit_identical_partial_overlap(hip, reuse_restored):
