@@ -730,8 +730,6 @@ __global__ void k_seg_empty_pass(const uint8_t* __restrict__ in, uint8_t* __rest
   empty_pass_cell(in + G.cell_off, out + G.cell_off, G.d, axis, t - G.cell_off);
 }
 
-
-
 static std::vector<uint32_t> blocks_of(const std::vector<uint32_t>& off) {
   std::vector<uint32_t> b(off.size());
   b[0] = 0;
