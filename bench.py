@@ -20,6 +20,12 @@ first into k_gicp_settle and k_gicp_search), timed with HIP events on the stream
 on; `cpu_baseline` is the
 CPU oracle (f32-faithful restatement of the reference, one thread) on a bounded
 sample of the same workload.
+
+The batched leg (`batched_vs_plain`: 256 small-cube pairs, one call against one
+registration per call) carries its own `roofline` (72 B per source point per
+pass over the library profiler's pass time) and `call_roofline` (the
+compulsory bytes of every stage over the timed call). `--no-graphs` keeps
+every pass loop on eager launches, for runs under an external profiler.
 """
 import argparse
 import json
