@@ -1134,27 +1134,57 @@ struct RadiusTile {
   uint32_t pad_;
 };
 
-// One thread per grid row: the row's tiles, appended to `tiles`.
+// One wave per grid row: the row's tiles, appended to `tiles`. The wave loads
+// the point counts of 64 cells at a time (a lane a cell) and walks them in
+// order, every lane keeping the same tile state: from the first occupied
+// cell, cells (empty ones included) join the open tile until it spans
+// kTileCells cells or the next cell would take an occupied tile past
+// kTileTarget points; the tile then closes and the walk goes on from that
+// cell. The walk runs twice: once to count the row's tiles, whose slots one
+// atomic then reserves, once to write them. (A thread walking its row with two
+// dependent loads a cell, and an atomic on the one counter per tile, took
+// 0.70 ms for the 898k-point block at r = 0.4 m; this form 0.16 ms.)
 __global__ void k_radius_tiles(GridDesc g, const uint32_t* __restrict__ cs,
                                RadiusTile* __restrict__ tiles, uint32_t* __restrict__ count) {
-  const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint32_t row = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;  // the same on a wave's lanes
   if (row >= uint32_t(g.dy) * uint32_t(g.dz)) return;
   const uint32_t base = row * uint32_t(g.dx), dx = uint32_t(g.dx);
   if (cs[base] == cs[base + dx]) return;
-  uint32_t x = 0;
-  while (x < dx) {
-    while (x < dx && cs[base + x] == cs[base + x + 1]) ++x;  // empty cells
-    if (x >= dx) break;
-    const uint32_t x0 = x;
-    uint32_t pts = 0;
-    while (x < dx && x - x0 < uint32_t(kTileCells)) {
-      const uint32_t c = cs[base + x + 1] - cs[base + x];
-      if (pts > 0 && pts + c > kTileTarget) break;
-      pts += c;
-      ++x;
+  auto walk = [&](auto&& emit) {
+    bool open = false;
+    uint32_t x0 = 0, pts = 0;
+    for (uint32_t c0 = 0; c0 < dx; c0 += kWave) {
+      const uint32_t x = c0 + lane;
+      const uint32_t cnt = x < dx ? cs[base + x + 1] - cs[base + x] : 0u;
+      const uint32_t m = min(dx - c0, uint32_t(kWave));
+      for (uint32_t k = 0; k < m; ++k) {
+        // (k is the same on every lane: a lane read, not a shuffle)
+        const uint32_t xc = c0 + k, c = uint32_t(__builtin_amdgcn_readlane(int(cnt), int(k)));
+        if (open && (xc - x0 >= uint32_t(kTileCells) || (pts > 0 && pts + c > kTileTarget))) {
+          emit(x0, xc - x0);
+          open = false;
+        }
+        if (!open) {
+          if (c == 0) continue;
+          open = true;
+          x0 = xc;
+          pts = 0;
+        }
+        pts += c;
+      }
     }
-    tiles[atomicAdd(count, 1u)] = RadiusTile{row, x0, x - x0, 0u};
-  }
+    if (open) emit(x0, dx - x0);
+  };
+  uint32_t n = 0;
+  walk([&](uint32_t, uint32_t) { ++n; });
+  uint32_t slot = 0;
+  if (lane == 0) slot = atomicAdd(count, n);
+  slot = __shfl(slot, 0);
+  walk([&](uint32_t x0, uint32_t len) {
+    if (lane == 0) tiles[slot] = RadiusTile{row, x0, len, 0u};
+    ++slot;
+  });
 }
 
 // kMode 0: radius covariances; 1: grid normals. The grid has a fixed number of
@@ -1262,8 +1292,8 @@ static void make_tiles(const Grid& rg, TileSet& ts, hipStream_t s) {
   ts.tiles.resize(cap);
   ts.count.resize(1);
   fill_u32(ts.count.p, 0u, 1, s);
-  hipLaunchKernelGGL(k_radius_tiles, dim3(uint32_t((rows + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, d,
-                     rg.cell_start.p, ts.tiles.p, ts.count.p);
+  hipLaunchKernelGGL(k_radius_tiles, dim3(uint32_t((rows * kWave + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                     s, d, rg.cell_start.p, ts.tiles.p, ts.count.p);
   // enough workgroups for several per CU; each takes tiles in turn
   ts.grid = uint32_t(std::min<uint64_t>(std::max<uint64_t>((d.npoints + kTileTarget - 1) / kTileTarget, 1), 16384));
 }
