@@ -382,8 +382,9 @@ def _serving(rates, seconds, max_requests, max_active):
 def _radius_leg(w, reps=5):
     """Radius covariances (covariance_radius, COV_GRID_WGSL semantics) and grid
     normals of the workload's target after the voxel filter, radius 0.4 m,
-    with the neighbourhoods read from LDS-staged cell tiles and from device
-    memory per point: kernel times (HIP events) and bit identity."""
+    with the neighbourhoods read from LDS-staged cell tiles (the tile list's
+    construction included) and from device memory per point: kernel times (HIP
+    events) and bit identity."""
     import ctypes
     import spatialrust_amd as sr
     from spatialrust_amd import native

@@ -1303,9 +1303,9 @@ void radius_normals(const Grid& rg, float radius, float eps, float4* nrm_orig, f
   const uint32_t n = rg.desc.npoints;
   if (n == 0) return;
   if (options().radius_tiles.load()) {
+    ProfScope prof("radius_cov", s);  // the tile list included
     TileSet ts;
     make_tiles(rg, ts, s);
-    ProfScope prof("radius_cov", s);
     hipLaunchKernelGGL((k_radius_tiled<0>), dim3(ts.grid), dim3(kTileThreads), 0, s, rg.desc,
                        rg.cell_start.p, rg.pts.p, ts.tiles.p, ts.count.p, radius * radius, eps, nrm_orig,
                        cov6, normal3, 0, 0.0f, 0.0f, 0.0f, nullptr);
@@ -1324,9 +1324,9 @@ void grid_normals(const Grid& rg, float radius, const float* viewpoint, float4* 
   const float vx = viewpoint ? viewpoint[0] : 0.0f, vy = viewpoint ? viewpoint[1] : 0.0f,
               vz = viewpoint ? viewpoint[2] : 0.0f;
   if (options().radius_tiles.load()) {
+    ProfScope prof("grid_normals", s);  // the tile list included
     TileSet ts;
     make_tiles(rg, ts, s);
-    ProfScope prof("grid_normals", s);
     hipLaunchKernelGGL((k_radius_tiled<1>), dim3(ts.grid), dim3(kTileThreads), 0, s, rg.desc,
                        rg.cell_start.p, rg.pts.p, ts.tiles.p, ts.count.p, radius * radius, 0.0f, nullptr,
                        nullptr, nullptr, has_vp, vx, vy, vz, out_orig);
