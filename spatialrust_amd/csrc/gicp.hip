@@ -914,8 +914,10 @@ constexpr uint32_t kHardBlockFlag = 0x80000000u;  // HardQuery.inst: block scann
 // Consecutive queue slices per XCD chunk in the queue stage (see k_gicp_hard).
 constexpr uint32_t kHardChunk = 64;
 
+// The one-kernel first stage at 3 waves per SIMD (its registers then do not
+// spill; cfg1 4.18 -> 4.15 ms against 4).
 #ifndef SR_SEARCH_WAVES
-#define SR_SEARCH_WAVES 4
+#define SR_SEARCH_WAVES 3
 #endif
 // The search stage over marked points (batches) gains from a fifth wave per
 // SIMD although registers then spill: its waves mostly wait on the dependent
